@@ -1,0 +1,787 @@
+// extern "C" entry points (include/ecg.h): context management, the drop-in
+// replacements of MatrixElements / ComputeMatElem(AndDeriv) / GHEPIIS, and
+// the device-resident energy and energy+gradient paths.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "../../include/ecg.h"
+#include "ecg_internal.hpp"
+
+using ecg::zd;
+
+#define HIPCHK(x)                              \
+  do {                                         \
+    hipError_t e_ = (x);                       \
+    if (e_ != hipSuccess) return ECG_ERR_HIP;  \
+  } while (0)
+#define RCCHK(x)                      \
+  do {                                \
+    int r_ = (x);                     \
+    if (r_ != 0) return r_ < 0 ? r_ : ECG_ERR_HIP; \
+  } while (0)
+
+struct ecg_ctx {
+  ecg::Ctx c;
+  ecg::Factor f;
+  hipStream_t stream = nullptr;
+  int npt = 0;
+  double* part = nullptr;
+  size_t part_cap = 0;
+  double* grad = nullptr;
+  size_t grad_cap = 0;
+  zd* ev[3] = {nullptr, nullptr, nullptr};   // iteration vectors; ev[2] = last eigenvector
+  bool have_last = false;
+  int last_K = 0;
+  hipEvent_t e[8] = {};
+  double tms[8] = {};
+  double* pinned = nullptr;   // pinned host staging
+  size_t pinned_cap = 0;
+};
+
+namespace {
+
+template <class T>
+int dev_alloc(T** p, size_t n) {
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if (n == 0) return 0;
+  if (hipMalloc((void**)p, n * sizeof(T)) != hipSuccess) {
+    *p = nullptr;
+    return ECG_ERR_NOMEM;
+  }
+  return 0;
+}
+
+int ensure_capacity(ecg_ctx* x, int K) {
+  ecg::Ctx& c = x->c;
+  if (K <= c.cap) {
+    c.K = K;
+    x->f.K = K;
+    return 0;
+  }
+  const int cap = std::max(K, 64);
+  const int ld = (cap + 63) / 64 * 64;
+  RCCHK(dev_alloc(&c.params, (size_t)x->npt * cap));
+  RCCHK(dev_alloc(&c.bra, (size_t)ecg::bra_stride(c.n) * cap));
+  RCCHK(dev_alloc(&c.ket, (size_t)ecg::ket_stride(c.n) * c.T * cap));
+  RCCHK(dev_alloc(&c.diagS, (size_t)cap));
+  RCCHK(dev_alloc(&c.diagH, (size_t)cap));
+  RCCHK(dev_alloc(&c.H, (size_t)ld * ld));
+  RCCHK(dev_alloc(&c.S, (size_t)ld * ld));
+  ecg::Factor& f = x->f;
+  RCCHK(dev_alloc(&f.F, (size_t)ld * ld));
+  RCCHK(dev_alloc(&f.D, (size_t)ld));
+  RCCHK(dev_alloc(&f.W, (size_t)ld * ecg::NB));
+  RCCHK(dev_alloc(&f.Dinv, (size_t)(ld / ecg::NB) * ecg::NB * ecg::NB));
+  RCCHK(dev_alloc(&f.status, (size_t)4));
+  RCCHK(dev_alloc(&f.vec, (size_t)4 * ld));
+  RCCHK(dev_alloc(&f.tvec, (size_t)ld));
+  RCCHK(dev_alloc(&f.flags, (size_t)2 * (ld / ecg::NB)));
+  HIPCHK(hipMemsetAsync(f.flags, 0, sizeof(int) * 2 * (ld / ecg::NB), x->stream));
+  HIPCHK(hipMemsetAsync(f.status, 0, sizeof(int) * 4, x->stream));
+  f.epoch = 0;
+  RCCHK(dev_alloc(&f.scal, (size_t)16));
+  for (int i = 0; i < 3; ++i) RCCHK(dev_alloc(&x->ev[i], (size_t)ld));
+  HIPCHK(hipMemsetAsync(c.H, 0, sizeof(zd) * ld * ld, x->stream));
+  HIPCHK(hipMemsetAsync(c.S, 0, sizeof(zd) * ld * ld, x->stream));
+  c.cap = cap;
+  c.ld = ld;
+  c.K = K;
+  f.cap = cap;
+  f.ld = ld;
+  f.K = K;
+  x->have_last = false;
+  return 0;
+}
+
+int ensure_pinned(ecg_ctx* x, size_t doubles) {
+  if (doubles <= x->pinned_cap) return 0;
+  if (x->pinned) (void)hipHostFree(x->pinned);
+  x->pinned = nullptr;
+  x->pinned_cap = 0;
+  if (hipHostMalloc((void**)&x->pinned, doubles * sizeof(double), hipHostMallocDefault) != hipSuccess)
+    return ECG_ERR_NOMEM;
+  x->pinned_cap = doubles;
+  return 0;
+}
+
+// Upload parameters and run the per-function set-up for all K functions.
+int upload_params(ecg_ctx* x, int K, const double* params) {
+  RCCHK(ensure_capacity(x, K));
+  HIPCHK(hipMemcpyAsync(x->c.params, params, sizeof(double) * x->npt * K, hipMemcpyHostToDevice, x->stream));
+  RCCHK(ecg::prep(x->c, x->stream));
+  return 0;
+}
+
+bool finite_all(const double* v, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(v[i])) return false;
+  return true;
+}
+
+// Copy device H, S (normalised, both triangles) and diagH/diagS to the
+// caller's arrays with the reference's StoreHS layout for rows r0..r1-1.
+int copy_out_hs(ecg_ctx* x, int r0, int r1, char method, double sigma, double* H, double* S, int ld,
+                double* diagH, double* diagS) {
+  ecg::Ctx& c = x->c;
+  const int K = c.K;
+  const size_t nd = (size_t)2 * c.ld * K;
+  RCCHK(ensure_pinned(x, 2 * nd + 2 * (size_t)K));
+  double* hH = x->pinned;
+  double* hS = x->pinned + nd;
+  double* hdH = x->pinned + 2 * nd;
+  double* hdS = hdH + K;
+  HIPCHK(hipMemcpyAsync(hH, c.H, sizeof(double) * nd, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipMemcpyAsync(hS, c.S, sizeof(double) * nd, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipMemcpyAsync(hdH, c.diagH, sizeof(double) * K, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipMemcpyAsync(hdS, c.diagS, sizeof(double) * K, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  for (int k = r0; k < r1; ++k) {
+    if (!std::isfinite(hdS[k]) || !std::isfinite(hdH[k])) return ECG_ERR_NONFINITE;
+    diagS[k] = hdS[k];
+    if (method == 'G') {
+      diagH[k] = hdH[k];
+    } else {
+      const size_t o = 2 * ((size_t)k * ld + k);
+      S[o] = 1.0;
+      S[o + 1] = 0.0;
+      H[o] = hdH[k] - sigma;
+      H[o + 1] = 0.0;
+    }
+    for (int l = 0; l < k; ++l) {
+      const size_t od = 2 * ((size_t)l * c.ld + k);
+      const size_t oh = 2 * ((size_t)l * ld + k);
+      const double hr = hH[od], hi = hH[od + 1], sr = hS[od], si = hS[od + 1];
+      if (!std::isfinite(hr) || !std::isfinite(hi) || !std::isfinite(sr) || !std::isfinite(si))
+        return ECG_ERR_NONFINITE;
+      S[oh] = sr;
+      S[oh + 1] = si;
+      if (method == 'G') {
+        H[oh] = hr;
+        H[oh + 1] = hi;
+      } else {
+        H[oh] = hr - sigma * sr;
+        H[oh + 1] = hi - sigma * si;
+        const size_t ou = 2 * ((size_t)k * ld + l);
+        S[ou] = sr;
+        S[ou + 1] = -si;
+      }
+    }
+  }
+  return 0;
+}
+
+// Factor H - sigma S on the device; returns the negative-pivot count and
+// the first zero pivot (0 if none).
+int factor_at(ecg_ctx* x, double sigma, int* nneg, int* zero) {
+  RCCHK(ecg::form_shifted(x->c, x->f, sigma, x->stream));
+  RCCHK(ecg::ldlh_factor(x->f, x->stream));
+  int st[2];
+  HIPCHK(hipMemcpyAsync(st, x->f.status, sizeof(st), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  *nneg = st[0];
+  *zero = st[1];
+  return 0;
+}
+
+struct InvitResult {
+  double lambda = 0.0, nd = 0.0, t1 = 0.0;
+  int iters = 0;
+  bool converged = false;
+};
+
+__global__ void k_fill_ones(int K, zd* v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < K) v[i] = ecg::mkz(1.0, 0.0);
+}
+
+// Inverse iteration with the current factorisation (GHEPIIS loop,
+// src/linalg.f90:1892-1917).  Start vector in x->ev[0] unless warm.
+// Stops when the GHEPIIS measure reaches tol, or when it has stopped
+// decreasing below 1e-9 (rounding floor), or after maxiter iterations.
+// Result vector (normalised x^H S x = 1) in x->ev[2].
+int inverse_iteration(ecg_ctx* x, double sigma, double tol, int maxiter, bool warm, InvitResult& r,
+                      bool stop_at_floor) {
+  ecg::Ctx& c = x->c;
+  const int K = c.K;
+  zd* v = x->ev[0];
+  zd* y = x->ev[1];
+  if (warm && x->have_last && x->last_K == K)
+    HIPCHK(hipMemcpyAsync(v, x->ev[2], sizeof(zd) * K, hipMemcpyDeviceToDevice, x->stream));
+  else
+    hipLaunchKernelGGL(k_fill_ones, dim3((K + 255) / 256), dim3(256), 0, x->stream, K, v);
+  double prev = std::numeric_limits<double>::infinity();
+  r = InvitResult();
+  for (int it = 0; it < maxiter; ++it) {
+    RCCHK(ecg::invit_iteration(c, x->f, v, y, x->stream));
+    double sc[2];
+    HIPCHK(hipMemcpyAsync(sc, x->f.scal, sizeof(sc), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    r.nd = sc[0];
+    r.iters = it + 1;
+    std::swap(v, y);   // v now holds the newest iterate
+    if (!std::isfinite(r.nd)) return ECG_ERR_NONFINITE;
+    if (tol > 0.0 ? (r.nd <= tol) : false) {
+      r.converged = true;
+      break;
+    }
+    if (stop_at_floor && it >= 1 && r.nd < 1e-9 && r.nd > 0.5 * prev) break;
+    prev = r.nd;
+  }
+  RCCHK(ecg::rayleigh(c, x->f, v, sigma, x->stream));
+  RCCHK(ecg::normalise(v, K, x->f.scal + 2, x->stream));
+  double q[4];
+  int st[4];
+  HIPCHK(hipMemcpyAsync(q, x->f.scal, sizeof(q), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipMemcpyAsync(st, x->f.status, sizeof(st), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipMemcpyAsync(x->ev[2], v, sizeof(zd) * K, hipMemcpyDeviceToDevice, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  if (st[2]) return ECG_ERR_HIP;   // a triangular-solve wait hit its limit
+  r.t1 = q[2];
+  r.lambda = q[3] / q[2] + sigma;
+  x->have_last = true;
+  x->last_K = K;
+  return 0;
+}
+
+// Root `which` of (H, S) for the assembled device matrices.  With a finite
+// sigma, inverse iteration at that shift (method 'I').  Otherwise the shift
+// is located from inertia counts (number of negative pivots of
+// LDL^H(H - s S) = number of eigenvalues below s), then refined with the
+// Rayleigh quotient.
+int solve_root(ecg_ctx* x, int which, double sigma, double tol, int maxiter, double* E, double* info,
+               int* status) {
+  ecg::Ctx& c = x->c;
+  const int K = c.K;
+  int nfac = 0, nneg = 0, zero = 0;
+  InvitResult r;
+  *status = 0;
+  if (std::isfinite(sigma)) {
+    RCCHK(factor_at(x, sigma, &nneg, &zero));
+    ++nfac;
+    if (zero) {
+      *status = ECG_SINGULAR;
+      return 0;
+    }
+    RCCHK(inverse_iteration(x, sigma, tol, maxiter, true, r, true));
+  } else {
+    std::vector<double> dh(K);
+    HIPCHK(hipMemcpyAsync(dh.data(), c.diagH, sizeof(double) * K, hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    const double s0 = *std::min_element(dh.begin(), dh.end());
+    const double scale = std::max(1.0, std::fabs(s0));
+    auto count = [&](double s, int* nn) -> int {
+      int zz = 0;
+      RCCHK(factor_at(x, s, nn, &zz));
+      ++nfac;
+      if (zz) {   // exactly singular: nudge the shift
+        RCCHK(factor_at(x, s * (1.0 + 1e-13) + 1e-300, nn, &zz));
+        ++nfac;
+      }
+      return 0;
+    };
+    double lo, hi;
+    int nlo, nhi;
+    int n0;
+    RCCHK(count(s0, &n0));
+    double step = 1e-2 * scale;
+    if (n0 >= which) {
+      hi = s0;
+      nhi = n0;
+      for (;;) {
+        double s = s0 - step;
+        int nn;
+        RCCHK(count(s, &nn));
+        if (nn <= which - 1) {
+          lo = s;
+          nlo = nn;
+          break;
+        }
+        hi = s;
+        nhi = nn;
+        step *= 4.0;
+        if (nfac > 60) {
+          *status = ECG_ROOT_NOT_FOUND;
+          return 0;
+        }
+      }
+    } else {
+      lo = s0;
+      nlo = n0;
+      for (;;) {
+        double s = s0 + step;
+        int nn;
+        RCCHK(count(s, &nn));
+        if (nn >= which) {
+          hi = s;
+          nhi = nn;
+          break;
+        }
+        lo = s;
+        nlo = nn;
+        step *= 4.0;
+        if (nfac > 60) {
+          *status = ECG_ROOT_NOT_FOUND;
+          return 0;
+        }
+      }
+    }
+    // isolate: nlo == which-1 and nhi == which; for which > 1 also shrink
+    // the bracket so that lo is nearer to E_which than to E_{which-1}.
+    const double width_goal = (which == 1) ? HUGE_VAL : 1e-6 * scale;
+    while (!(nlo == which - 1 && nhi == which) || (hi - lo) > width_goal) {
+      const double mid = 0.5 * (lo + hi);
+      int nn;
+      RCCHK(count(mid, &nn));
+      if (nn <= which - 1) {
+        lo = mid;
+        nlo = nn;
+      } else {
+        hi = mid;
+        nhi = nn;
+      }
+      if (nfac > 80) {
+        *status = ECG_ROOT_NOT_FOUND;
+        return 0;
+      }
+    }
+    // Refinement.  For the lowest root the Rayleigh quotient is an upper
+    // bound, so the bracket (s, hi) shrinks from both sides: iterate a few
+    // steps at s (below the root), lower hi to the quotient, and move s up
+    // to just below hi whenever the inertia confirms it is still below the
+    // root.  For higher roots the bisection above already made the bracket
+    // narrow; iterate at its lower end.
+    double s = lo;
+    double fac_at = NAN;
+    auto refactor = [&](double at) -> int {
+      if (fac_at == at) return 0;
+      int nn, zz;
+      RCCHK(factor_at(x, at, &nn, &zz));
+      ++nfac;
+      nneg = nn;
+      fac_at = at;
+      return 0;
+    };
+    auto probe = [&](double at, int* nn) -> int {
+      RCCHK(count(at, nn));
+      fac_at = at;
+      nneg = *nn;
+      return 0;
+    };
+    if (which == 1) {
+      bool warm = false;
+      for (int ref = 0; ref < 8; ++ref) {
+        RCCHK(refactor(s));
+        RCCHK(inverse_iteration(x, s, tol, 3, warm, r, false));
+        warm = true;
+        if (r.converged) break;
+        hi = std::min(hi, r.lambda);
+        if (!(hi > s)) break;
+        double cand = hi - 1e-3 * (hi - s);
+        int nn;
+        RCCHK(probe(cand, &nn));
+        if (nn == which - 1) {
+          s = cand;
+          continue;
+        }
+        hi = cand;
+        cand = 0.5 * (s + hi);
+        RCCHK(probe(cand, &nn));
+        if (nn == which - 1) s = cand;
+        else hi = cand;
+      }
+    }
+    RCCHK(refactor(s));
+    if (!r.converged) RCCHK(inverse_iteration(x, s, tol, maxiter, which == 1, r, true));
+    sigma = s;
+  }
+  *E = r.lambda;
+  if (!r.converged && tol > 0.0 && r.nd > tol) *status = ECG_NOT_CONVERGED;
+  if (info) {
+    info[0] = r.iters;
+    info[1] = r.nd;
+    info[2] = sigma;
+    info[3] = nfac;
+    info[4] = nneg;
+    info[5] = r.t1;
+  }
+  return 0;
+}
+
+float ev_ms(hipEvent_t a, hipEvent_t b) {
+  float t = 0.0f;
+  if (hipEventElapsedTime(&t, a, b) != hipSuccess) return -1.0f;
+  return t;
+}
+
+template <int N>
+__global__ void k_one_pair(const ecg::SysConst sc, const double* bra, const double* ket, double* out) {
+  constexpr int m = N * (N + 1) / 2;
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double* bk = bra;                                   // function 0 = k (bra)
+  const double* bl = bra + ecg::BraRec<N>::stride;          // function 1 = l (ket)
+  const double* kl = ket + (size_t)1 * sc.T * ecg::KetRec<N>::stride;
+  const zd* ck = reinterpret_cast<const zd*>(bk + ecg::BraRec<N>::ck);
+  const zd* ct = reinterpret_cast<const zd*>(kl + ecg::KetRec<N>::ct);
+  ecg::PairCore<N> pc;
+  ecg::pair_energy<N>(sc, ck, ct, reinterpret_cast<const zd*>(kl + ecg::KetRec<N>::nl),
+                      bk[ecg::BraRec<N>::adet] * bl[ecg::BraRec<N>::adet], pc);
+  zd gb[m], gk[m], gs[m], g2[m], s2[m];
+  ecg::pair_adjoint<N>(sc, ck, ct, pc, 0.0, gb, gk);
+  for (int e = 0; e < m; ++e) gs[e] = (-1.5) * (pc.S * pc.Ci[e]);
+  ecg::perm_congruence<N>(sc.P[0], gk, g2);
+  ecg::perm_congruence<N>(sc.P[0], gs, s2);
+  const zd H = pc.T + pc.V;
+  zd* o = reinterpret_cast<zd*>(out);
+  o[0] = H;
+  o[1] = pc.S;
+  zd* Dk = o + 2;
+  zd* Dl = o + 2 + 4 * m;
+  const zd aH = 1.5 * H, aS = 1.5 * pc.S;
+  ecg::chain_to_params<N>(gb, aH, bk + ecg::BraRec<N>::lp, -1.0, Dk, Dk + m);
+  ecg::chain_to_params<N>(gs, aS, bk + ecg::BraRec<N>::lp, -1.0, Dk + 2 * m, Dk + 3 * m);
+  ecg::chain_to_params<N>(g2, aH, bl + ecg::BraRec<N>::lp, 1.0, Dl, Dl + m);
+  ecg::chain_to_params<N>(s2, aS, bl + ecg::BraRec<N>::lp, 1.0, Dl + 2 * m, Dl + 3 * m);
+}
+
+int launch_one_pair(const ecg::Ctx& c, const ecg::SysConst& sc, const double* bra, const double* ket, double* out,
+                    hipStream_t s) {
+  switch (c.n) {
+    case 1: hipLaunchKernelGGL(k_one_pair<1>, dim3(1), dim3(64), 0, s, sc, bra, ket, out); break;
+    case 2: hipLaunchKernelGGL(k_one_pair<2>, dim3(1), dim3(64), 0, s, sc, bra, ket, out); break;
+    case 3: hipLaunchKernelGGL(k_one_pair<3>, dim3(1), dim3(64), 0, s, sc, bra, ket, out); break;
+    case 4: hipLaunchKernelGGL(k_one_pair<4>, dim3(1), dim3(64), 0, s, sc, bra, ket, out); break;
+    case 5: hipLaunchKernelGGL(k_one_pair<5>, dim3(1), dim3(64), 0, s, sc, bra, ket, out); break;
+    default: return ECG_ERR_ARG;
+  }
+  return (int)hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" {
+
+int ecg_ctx_create(int n, int T, const int* yhy_mats, const int* yhy_coef, const double* mass, double q0,
+                   const double* q, ecg_ctx** out) {
+  if (!out) return ECG_ERR_ARG;
+  *out = nullptr;
+  if (n < 1 || n > 5 || T < 1 || T > 16 || !yhy_mats || !yhy_coef || !mass || !q) return ECG_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return ECG_ERR_HIP;
+  ecg_ctx* x = new ecg_ctx();
+  ecg::SysConst& sc = x->c.sc;
+  std::memset(&sc, 0, sizeof(sc));
+  sc.n = n;
+  sc.T = T;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) sc.M[i][j] = mass[i + n * j];
+  sc.q0 = q0;
+  for (int i = 0; i < n; ++i) sc.q[i] = q[i];
+  sc.two3n2 = std::pow(2.0, (3 * n) / 2.0);
+  for (int t = 0; t < T; ++t) {
+    sc.coef[t] = (double)yhy_coef[t];
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        const int v = yhy_mats[(size_t)t * n * n + i + n * j];
+        if (v < -127 || v > 127) {
+          delete x;
+          return ECG_ERR_ARG;
+        }
+        sc.P[t][i][j] = (signed char)v;
+      }
+  }
+  x->c.n = n;
+  x->c.T = T;
+  x->npt = n * (n + 1);
+  if (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete x;
+    return ECG_ERR_HIP;
+  }
+  for (int i = 0; i < 8; ++i)
+    if (hipEventCreate(&x->e[i]) != hipSuccess) {
+      delete x;
+      return ECG_ERR_HIP;
+    }
+  *out = x;
+  return ECG_OK;
+}
+
+int ecg_ctx_destroy(ecg_ctx* x) {
+  if (!x) return ECG_ERR_ARG;
+  if (x->stream) (void)hipStreamSynchronize(x->stream);
+  ecg::Ctx& c = x->c;
+  void* ptrs[] = {c.params, c.bra, c.ket, c.diagS, c.diagH, c.H, c.S, x->f.F, x->f.D, x->f.W, x->f.Dinv,
+                  x->f.status, x->f.vec, x->f.tvec, x->f.flags, x->f.scal, x->part, x->grad, x->ev[0],
+                  x->ev[1], x->ev[2]};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (x->pinned) (void)hipHostFree(x->pinned);
+  for (int i = 0; i < 8; ++i)
+    if (x->e[i]) (void)hipEventDestroy(x->e[i]);
+  if (x->stream) (void)hipStreamDestroy(x->stream);
+  delete x;
+  return ECG_OK;
+}
+
+int ecg_set_params(ecg_ctx* x, int K, const double* params) {
+  if (!x || K < 1 || !params) return ECG_ERR_ARG;
+  if (!finite_all(params, (size_t)x->npt * K)) return ECG_ERR_ARG;
+  RCCHK(upload_params(x, K, params));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return ECG_OK;
+}
+
+int ecg_update_params(ecg_ctx* x, int first, int count, const double* params) {
+  if (!x || !params || first < 1 || count < 1 || first - 1 + count > x->c.K) return ECG_ERR_ARG;
+  if (!finite_all(params, (size_t)x->npt * count)) return ECG_ERR_ARG;
+  HIPCHK(hipMemcpyAsync(x->c.params + (size_t)x->npt * (first - 1), params, sizeof(double) * x->npt * count,
+                        hipMemcpyHostToDevice, x->stream));
+  RCCHK(ecg::prep(x->c, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return ECG_OK;
+}
+
+int ecg_matrix_elements(ecg_ctx* x, const double* paramk, const double* paraml, const int* P, int gradflag,
+                        double* Hkl, double* Skl, double* Dk, double* Dl) {
+  if (!x || !paramk || !paraml || !P || !Hkl || !Skl) return ECG_ERR_ARG;
+  const int n = x->c.n, m = n * (n + 1) / 2, npt = x->npt;
+  const bool gk = (gradflag == 1 || gradflag == 3), gl = (gradflag == 2 || gradflag == 3);
+  if ((gk && !Dk) || (gl && !Dl)) return ECG_ERR_ARG;
+  ecg::SysConst sc = x->c.sc;
+  sc.T = 1;
+  sc.coef[0] = 1.0;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) sc.P[0][i][j] = (signed char)P[i + n * j];
+  // private 2-function basis on the device
+  ecg::Ctx c1;
+  c1.sc = sc;
+  c1.n = n;
+  c1.T = 1;
+  c1.K = 2;
+  double *dparams = nullptr, *dbra = nullptr, *dket = nullptr, *dout = nullptr;
+  const size_t nout = 2 * (2 + 8 * (size_t)m);
+  int rc = 0;
+  if (hipMalloc((void**)&dparams, sizeof(double) * 2 * npt) != hipSuccess ||
+      hipMalloc((void**)&dbra, sizeof(double) * 2 * ecg::bra_stride(n)) != hipSuccess ||
+      hipMalloc((void**)&dket, sizeof(double) * 2 * ecg::ket_stride(n)) != hipSuccess ||
+      hipMalloc((void**)&dout, sizeof(double) * nout) != hipSuccess)
+    rc = ECG_ERR_NOMEM;
+  std::vector<double> hout(nout);
+  if (!rc) {
+    std::vector<double> pp(2 * npt);
+    std::memcpy(pp.data(), paramk, sizeof(double) * npt);
+    std::memcpy(pp.data() + npt, paraml, sizeof(double) * npt);
+    c1.params = dparams;
+    c1.bra = dbra;
+    c1.ket = dket;
+    if (hipMemcpyAsync(dparams, pp.data(), sizeof(double) * 2 * npt, hipMemcpyHostToDevice, x->stream) != hipSuccess)
+      rc = ECG_ERR_HIP;
+    if (!rc) rc = ecg::prep(c1, x->stream);
+    if (!rc) rc = launch_one_pair(c1, sc, dbra, dket, dout, x->stream);
+    if (!rc && hipMemcpyAsync(hout.data(), dout, sizeof(double) * nout, hipMemcpyDeviceToHost, x->stream) != hipSuccess)
+      rc = ECG_ERR_HIP;
+    if (!rc && hipStreamSynchronize(x->stream) != hipSuccess) rc = ECG_ERR_HIP;
+  }
+  if (dparams) (void)hipFree(dparams);
+  if (dbra) (void)hipFree(dbra);
+  if (dket) (void)hipFree(dket);
+  if (dout) (void)hipFree(dout);
+  if (rc) return rc < 0 ? rc : ECG_ERR_HIP;
+  if (!finite_all(hout.data(), 4)) return ECG_ERR_NONFINITE;
+  Hkl[0] = hout[0];
+  Hkl[1] = hout[1];
+  Skl[0] = hout[2];
+  Skl[1] = hout[3];
+  if (gk) std::memcpy(Dk, hout.data() + 4, sizeof(double) * 8 * m);
+  if (gl) std::memcpy(Dl, hout.data() + 4 + 8 * m, sizeof(double) * 8 * m);
+  return ECG_OK;
+}
+
+int ecg_compute_mat_elem(ecg_ctx* x, int nmin, int nmax, char method, double approx_energy, double* H,
+                         double* S, int ld, double* diagH, double* diagS) {
+  if (!x || !H || !S || !diagH || !diagS) return ECG_ERR_ARG;
+  if (method != 'G' && method != 'I') return ECG_ERR_ARG;
+  if (x->c.K < 1) return ECG_ERR_STATE;
+  if (nmin < 1 || nmax > x->c.K || nmin > nmax || ld < nmax) return ECG_ERR_ARG;
+  // rows < nmin keep their device values; their diagS comes from the caller
+  if (nmin > 1)
+    HIPCHK(hipMemcpyAsync(x->c.diagS, diagS, sizeof(double) * (nmin - 1), hipMemcpyHostToDevice, x->stream));
+  RCCHK(ecg::assemble(x->c, nmin - 1, nmax, x->stream));
+  return copy_out_hs(x, nmin - 1, nmax, method, approx_energy, H, S, ld, diagH, diagS);
+}
+
+int ecg_compute_mat_elem_and_deriv(ecg_ctx* x, int nfru, char method, double approx_energy, double* H,
+                                   double* S, int ld, double* diagH, double* diagS, double* D) {
+  if (!x || !H || !S || !diagH || !diagS || !D) return ECG_ERR_ARG;
+  if (method != 'G' && method != 'I') return ECG_ERR_ARG;
+  const int K = x->c.K;
+  if (K < 1) return ECG_ERR_STATE;
+  if (nfru < 0 || nfru >= K || ld < K) return ECG_ERR_ARG;
+  const int nfo = K - nfru;
+  const size_t nD = (size_t)2 * x->npt * nfo * K;   // complex entries
+  zd* dD = nullptr;
+  if (hipMalloc((void**)&dD, sizeof(zd) * nD) != hipSuccess) return ECG_ERR_NOMEM;
+  int rc = 0;
+  if (nfru > 0 &&
+      hipMemcpyAsync(x->c.diagS, diagS, sizeof(double) * nfru, hipMemcpyHostToDevice, x->stream) != hipSuccess)
+    rc = ECG_ERR_HIP;
+  if (!rc) rc = ecg::assemble_deriv(x->c, nfru, dD, x->stream);
+  if (!rc && hipMemcpyAsync(D, dD, sizeof(zd) * nD, hipMemcpyDeviceToHost, x->stream) != hipSuccess)
+    rc = ECG_ERR_HIP;
+  if (!rc && hipStreamSynchronize(x->stream) != hipSuccess) rc = ECG_ERR_HIP;
+  (void)hipFree(dD);
+  if (rc) return rc < 0 ? rc : ECG_ERR_HIP;
+  if (!finite_all(D, 2 * nD)) return ECG_ERR_NONFINITE;
+  return copy_out_hs(x, nfru, K, method, approx_energy, H, S, ld, diagH, diagS);
+}
+
+int ecg_ghepiis(ecg_ctx* x, int k, int n, double* M, int nM, double* invD, const double* B, int nB,
+                double apprlambda, const double* v, double tol, int maxiter, int specif_norm, double* lambda,
+                double* xout, double* relacc, int* numiter) {
+  if (!x || !M || !invD || !B || !v || !lambda || !xout || n < 1 || k < 1 || k > n || nM < n || nB < n ||
+      maxiter < 1)
+    return ECG_ERR_ARG;
+  RCCHK(ensure_capacity(x, n));
+  ecg::Ctx& c = x->c;
+  // Device H := lower(M) + sigma S mirrored, S := B, so that the shared
+  // kernels see H - sigma S = M.
+  const size_t nd = (size_t)2 * c.ld * n;
+  RCCHK(ensure_pinned(x, 2 * nd));
+  double* hH = x->pinned;
+  double* hS = x->pinned + nd;
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      const size_t ob = 2 * ((size_t)j * nB + i), od = 2 * ((size_t)j * c.ld + i);
+      hS[od] = B[ob];
+      hS[od + 1] = B[ob + 1];
+      const int a = std::max(i, j), b = std::min(i, j);
+      const size_t om = 2 * ((size_t)b * nM + a);   // lower element (a, b)
+      double mr = M[om], mi = M[om + 1];
+      if (i < j) mi = -mi;                          // upper = conj(lower)
+      if (i == j) mi = 0.0;
+      hH[od] = mr + apprlambda * hS[od];
+      hH[od + 1] = mi + apprlambda * hS[od + 1];
+    }
+  HIPCHK(hipMemcpyAsync(c.H, hH, sizeof(double) * nd, hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(c.S, hS, sizeof(double) * nd, hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(x->ev[2], v, sizeof(zd) * n, hipMemcpyHostToDevice, x->stream));
+  x->have_last = true;
+  x->last_K = n;
+  int nneg = 0, zero = 0;
+  RCCHK(factor_at(x, apprlambda, &nneg, &zero));
+  if (zero) return ECG_SINGULAR;
+  InvitResult r;
+  RCCHK(inverse_iteration(x, apprlambda, tol, maxiter, true, r, false));
+  // x->ev[2] holds x normalised to x^H B x = 1
+  std::vector<double> hx(2 * (size_t)n), hD(n);
+  HIPCHK(hipMemcpyAsync(hx.data(), x->ev[2], sizeof(zd) * n, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipMemcpyAsync(hD.data(), x->f.D, sizeof(double) * n, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipMemcpyAsync(hH, x->f.F, sizeof(double) * nd, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  if (specif_norm == 1 || (specif_norm != 0)) {
+    double nrm = 0.0, mx = 0.0;
+    for (int i = 0; i < n; ++i) {
+      nrm += hx[2 * i] * hx[2 * i] + hx[2 * i + 1] * hx[2 * i + 1];
+      mx = std::max(mx, std::max(std::fabs(hx[2 * i]), std::fabs(hx[2 * i + 1])));
+    }
+    const double s = (specif_norm == 1) ? 1.0 / std::sqrt(nrm) : 1.0 / mx;
+    for (int i = 0; i < 2 * n; ++i) hx[i] *= s;
+  }
+  std::memcpy(xout, hx.data(), sizeof(double) * 2 * n);
+  for (int i = 0; i < n; ++i) {
+    invD[2 * i] = 1.0 / hD[i];
+    invD[2 * i + 1] = 0.0;
+  }
+  // L^H into the strict upper triangle of M
+  for (int j = 1; j < n; ++j)
+    for (int i = 0; i < j; ++i) {
+      const size_t od = 2 * ((size_t)j * c.ld + i), om = 2 * ((size_t)j * nM + i);
+      M[om] = hH[od];
+      M[om + 1] = hH[od + 1];
+    }
+  *lambda = r.lambda;
+  if (relacc) *relacc = r.nd;
+  if (numiter) *numiter = r.iters;
+  if (!r.converged && tol > 0.0) return ECG_NOT_CONVERGED;
+  return ECG_OK;
+}
+
+int ecg_energy(ecg_ctx* x, int which, double sigma, double tol, int maxiter, double* E, double* c, double* info) {
+  if (!x || !E || which < 1 || maxiter < 1) return ECG_ERR_ARG;
+  const int K = x->c.K;
+  if (K < 1) return ECG_ERR_STATE;
+  if (which > K) return ECG_ERR_ARG;
+  HIPCHK(hipEventRecord(x->e[0], x->stream));
+  RCCHK(ecg::assemble(x->c, 0, K, x->stream));
+  HIPCHK(hipEventRecord(x->e[1], x->stream));
+  int status = 0;
+  RCCHK(solve_root(x, which, sigma, tol, maxiter, E, info, &status));
+  HIPCHK(hipEventRecord(x->e[2], x->stream));
+  if (c) HIPCHK(hipMemcpyAsync(c, x->ev[2], sizeof(zd) * K, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  x->tms[0] = ev_ms(x->e[0], x->e[1]);
+  x->tms[1] = ev_ms(x->e[1], x->e[2]);
+  x->tms[2] = 0.0;
+  x->tms[3] = ev_ms(x->e[0], x->e[2]);
+  if (!std::isfinite(*E)) return ECG_ERR_NONFINITE;
+  return status;
+}
+
+int ecg_energy_grad(ecg_ctx* x, int nfru, int which, double sigma, double tol, int maxiter, double* E,
+                    double* grad, double* info) {
+  if (!x || !E || !grad || which < 1 || maxiter < 1) return ECG_ERR_ARG;
+  ecg::Ctx& c = x->c;
+  const int K = c.K;
+  if (K < 1) return ECG_ERR_STATE;
+  if (nfru < 0 || nfru >= K || which > K) return ECG_ERR_ARG;
+  const size_t np = ecg::adjoint_partial_doubles(c.n, K);
+  if (np > x->part_cap) {
+    RCCHK(dev_alloc(&x->part, np));
+    x->part_cap = np;
+  }
+  const size_t ng = (size_t)x->npt * (K - nfru);
+  if (ng > x->grad_cap) {
+    RCCHK(dev_alloc(&x->grad, ng));
+    x->grad_cap = ng;
+  }
+  c.ev_off0 = x->e[5];
+  c.ev_off1 = x->e[6];
+  HIPCHK(hipEventRecord(x->e[0], x->stream));
+  RCCHK(ecg::assemble(c, 0, K, x->stream));
+  HIPCHK(hipEventRecord(x->e[1], x->stream));
+  int status = 0;
+  RCCHK(solve_root(x, which, sigma, tol, maxiter, E, info, &status));
+  HIPCHK(hipEventRecord(x->e[2], x->stream));
+  c.ev_adj0 = x->e[3];
+  c.ev_adj1 = x->e[4];
+  RCCHK(ecg::adjoint(c, nfru, *E, x->ev[2], x->part, x->grad, x->stream));
+  HIPCHK(hipEventRecord(x->e[7], x->stream));
+  HIPCHK(hipMemcpyAsync(grad, x->grad, sizeof(double) * ng, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  c.ev_off0 = c.ev_off1 = c.ev_adj0 = c.ev_adj1 = nullptr;
+  x->tms[0] = ev_ms(x->e[0], x->e[1]);
+  x->tms[1] = ev_ms(x->e[1], x->e[2]);
+  x->tms[2] = ev_ms(x->e[2], x->e[7]);
+  x->tms[3] = ev_ms(x->e[0], x->e[7]);
+  x->tms[4] = ev_ms(x->e[3], x->e[4]);
+  x->tms[5] = ev_ms(x->e[5], x->e[6]);
+  if (!std::isfinite(*E) || !finite_all(grad, ng)) return ECG_ERR_NONFINITE;
+  return status;
+}
+
+int ecg_last_timings(ecg_ctx* x, double* ms, int nmax) {
+  if (!x || !ms || nmax < 1) return ECG_ERR_ARG;
+  const int n = std::min(nmax, 6);
+  for (int i = 0; i < n; ++i) ms[i] = x->tms[i];
+  return n;
+}
+
+}  // extern "C"

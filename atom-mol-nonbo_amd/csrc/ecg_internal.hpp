@@ -1,0 +1,81 @@
+// Internal declarations shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "ecg_math.hpp"
+
+namespace ecg {
+
+constexpr int TILE = 16;   // pair tile edge (TILE x TILE pairs per workgroup)
+constexpr int NB = 64;     // block size of the LDL^H factorisation
+
+// Per-function bra record (doubles): Ck* packed complex, |det L|, L packed.
+template <int N> struct BraRec {
+  static constexpr int m = N * (N + 1) / 2;
+  static constexpr int ck = 0;
+  static constexpr int adet = 2 * m;
+  static constexpr int lp = 2 * m + 1;
+  static constexpr int stride = ((3 * m + 1) + 1) / 2 * 2;
+};
+// Per-(function, term) ket record: P^T (A + iB) P packed complex, M tCl.
+template <int N> struct KetRec {
+  static constexpr int m = N * (N + 1) / 2;
+  static constexpr int ct = 0;
+  static constexpr int nl = 2 * m;
+  static constexpr int stride = 2 * m + 2 * N * N;
+};
+
+inline int bra_stride(int n) {
+  const int m = n * (n + 1) / 2;
+  return ((3 * m + 1) + 1) / 2 * 2;
+}
+inline int ket_stride(int n) { return n * (n + 1) + 2 * n * n; }
+
+// Device state of one basis (owned by the context in ecg_capi.hip).
+struct Ctx {
+  SysConst sc;
+  int n = 0, T = 0, K = 0, ld = 0;
+  int cap = 0;            // allocated basis capacity (functions)
+  double* params = nullptr;   // npt x cap
+  double* bra = nullptr;      // cap x bra_stride
+  double* ket = nullptr;      // cap x T x ket_stride
+  double* diagS = nullptr;    // cap
+  double* diagH = nullptr;    // cap
+  zd* H = nullptr;            // ld x ld
+  zd* S = nullptr;            // ld x ld
+  // optional events bracketing the off-diagonal and adjoint kernels
+  hipEvent_t ev_off0 = nullptr, ev_off1 = nullptr, ev_adj0 = nullptr, ev_adj1 = nullptr;
+};
+
+// pair assembly (ecg_kernels.hip)
+int prep(const Ctx& c, hipStream_t s);
+int assemble(const Ctx& c, int r0, int r1, hipStream_t s);
+int assemble_deriv(const Ctx& c, int nfru, zd* D, hipStream_t s);
+int adjoint(const Ctx& c, int nfru, double E, const zd* cvec, double* part, double* grad, hipStream_t s);
+size_t adjoint_partial_doubles(int n, int K);
+
+// dense Hermitian linear algebra (ecg_linalg.hip)
+struct Factor {
+  int K = 0, ld = 0, cap = 0;
+  zd* F = nullptr;      // ld x ld: L (unit lower, strict part) after factorisation
+  double* D = nullptr;  // K pivots
+  zd* W = nullptr;      // ld x NB panel scratch
+  zd* Dinv = nullptr;   // nblocks x NB x NB inverses of the diagonal blocks of L
+  int* status = nullptr;     // [0] negative pivots, [1] first zero pivot (1-based) or 0,
+                             // [2] triangular-solve wait limit exceeded
+  zd* vec = nullptr;    // 4 x ld work vectors
+  zd* tvec = nullptr;   // ld: D^{-1} L^{-1} b between the two solves
+  double* scal = nullptr;    // 16 scalars
+  int* flags = nullptr;      // 2 x ld/NB block-ready flags of the triangular solves
+  int epoch = 0;             // flag value of the current solve
+};
+int form_shifted(const Ctx& c, Factor& f, double sigma, hipStream_t s);
+int ldlh_factor(Factor& f, hipStream_t s);
+int ldlh_solve(Factor& f, const zd* b, zd* x, hipStream_t s);
+int invit_iteration(const Ctx& c, Factor& f, const zd* v, zd* x, hipStream_t s);
+int rayleigh(const Ctx& c, Factor& f, const zd* x, double sigma, hipStream_t s);
+int normalise(zd* x, int K, const double* t, hipStream_t s);
+
+}  // namespace ecg

@@ -1,0 +1,575 @@
+// Pair-assembly kernels: per-function set-up, diagonal and lower-triangle
+// pair tiles (energy), pair tiles with stored derivatives (drop-in for
+// StoreHSD) and the fused adjoint pass that contracts the gradient with the
+// eigenvector without storing per-pair derivatives.
+//
+// Storage (device, column-major, ld = K rounded up to a multiple of 64):
+//   H, S   complex, normalised off-diagonal elements H_kl / sqrt(S_kk S_ll)
+//          in both triangles (the upper one conjugated), unit diagonal of S,
+//          diagH_k / diagS_k on the diagonal of H.
+//   diagS  real S_kk (un-normalised), diagH real H_kk / S_kk.
+// The reference stores the same numbers in the lower triangles only
+// (src/matform.f90:11-51); the host copy-out restores that layout.
+#include "ecg_internal.hpp"
+
+namespace ecg {
+
+// ----------------------------------------------------------- set-up
+// One thread per function: bra record (Ck* packed, |det L|, L packed) and,
+// for every Y^dagger Y term, the ket record (P^T (A + iB) P packed, M tCl).
+template <int N>
+__global__ void k_prep(const SysConst sc, int K, const double* __restrict__ params, double* __restrict__ bra,
+                       double* __restrict__ ket) {
+  constexpr int m = N * (N + 1) / 2;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  const double* p = params + (size_t)k * 2 * m;
+  double L[N][N], B[N][N], A[N][N];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) L[i][j] = 0.0;
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+#pragma unroll
+    for (int i = j; i < N; ++i) {
+      L[i][j] = p[vidx(N, i, j)];
+      B[i][j] = B[j][i] = p[m + vidx(N, i, j)];
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int c = 0; c <= j; ++c) s = fma(L[i][c], L[j][c], s);
+      A[i][j] = A[j][i] = s;
+    }
+  double det = 1.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) det *= L[i][i];
+  double* br = bra + (size_t)k * BraRec<N>::stride;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      br[BraRec<N>::ck + 2 * sidx(i, j)] = A[i][j];
+      br[BraRec<N>::ck + 2 * sidx(i, j) + 1] = -B[i][j];
+      br[BraRec<N>::lp + sidx(i, j)] = L[i][j];
+    }
+  br[BraRec<N>::adet] = fabs(det);
+  for (int t = 0; t < sc.T; ++t) {
+    double* kr = ket + ((size_t)k * sc.T + t) * KetRec<N>::stride;
+    double tA[N][N], tB[N][N], W1[N][N], W2[N][N];
+    // P^T X P with integer P
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+          a += double(sc.P[t][c][i]) * A[c][j];
+          b += double(sc.P[t][c][i]) * B[c][j];
+        }
+        W1[i][j] = a;
+        W2[i][j] = b;
+      }
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+          a += W1[i][c] * double(sc.P[t][c][j]);
+          b += W2[i][c] * double(sc.P[t][c][j]);
+        }
+        tA[i][j] = a;
+        tB[i][j] = b;
+      }
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j <= i; ++j) {
+        kr[KetRec<N>::ct + 2 * sidx(i, j)] = tA[i][j];
+        kr[KetRec<N>::ct + 2 * sidx(i, j) + 1] = tB[i][j];
+      }
+    // M tCl, row-major
+#pragma unroll
+    for (int a = 0; a < N; ++a)
+#pragma unroll
+      for (int b = 0; b < N; ++b) {
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+          re = fma(sc.M[a][c], tA[c][b], re);
+          im = fma(sc.M[a][c], tB[c][b], im);
+        }
+        kr[KetRec<N>::nl + 2 * (a * N + b)] = re;
+        kr[KetRec<N>::nl + 2 * (a * N + b) + 1] = im;
+      }
+  }
+}
+
+// Sum over Y^dagger Y terms of one pair (energy only).
+template <int N>
+__device__ inline void pair_sum_energy(const SysConst& sc, const double* br, const double* kbase, double adet_l,
+                                       zd& H, zd& S) {
+  H = mkz(0.0, 0.0);
+  S = mkz(0.0, 0.0);
+  const zd* ck = reinterpret_cast<const zd*>(br + BraRec<N>::ck);
+  const double adet = br[BraRec<N>::adet] * adet_l;
+  for (int t = 0; t < sc.T; ++t) {
+    const double* kr = kbase + (size_t)t * KetRec<N>::stride;
+    PairCore<N> pc;
+    pair_energy<N>(sc, ck, reinterpret_cast<const zd*>(kr + KetRec<N>::ct),
+                   reinterpret_cast<const zd*>(kr + KetRec<N>::nl), adet, pc);
+    H = H + sc.coef[t] * (pc.T + pc.V);
+    S = S + sc.coef[t] * pc.S;
+  }
+}
+
+// ----------------------------------------------------- diagonal pairs
+// S_kk, H_kk of rows r0..r1-1; diagS, diagH and the unit/normalised
+// diagonal of the device H, S.
+template <int N>
+__global__ void k_diag(const SysConst sc, int r0, int r1, int ld, const double* __restrict__ bra,
+                       const double* __restrict__ ket, double* __restrict__ diagS,
+                       double* __restrict__ diagH, zd* __restrict__ Hm, zd* __restrict__ Sm) {
+  const int k = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= r1) return;
+  const double* br = bra + (size_t)k * BraRec<N>::stride;
+  zd H, S;
+  pair_sum_energy<N>(sc, br, ket + (size_t)k * sc.T * KetRec<N>::stride, br[BraRec<N>::adet], H, S);
+  const double s = S.x;
+  const double h = H.x / s;
+  diagS[k] = s;
+  diagH[k] = h;
+  Hm[(size_t)k * ld + k] = mkz(h, 0.0);
+  Sm[(size_t)k * ld + k] = mkz(1.0, 0.0);
+}
+
+// Tile coordinates of the b-th lower-triangle tile among tile rows
+// >= tr0 (tile row tr holds tiles tc = 0..tr).
+__device__ inline void tri_tile(int b, int tr0, int& tr, int& tc) {
+  const long long base = (long long)tr0 * (tr0 + 1) / 2;
+  const long long x = base + b;
+  int r = (int)((sqrt(8.0 * (double)x + 1.0) - 1.0) * 0.5);
+  while ((long long)(r + 1) * (r + 2) / 2 <= x) ++r;
+  while ((long long)r * (r + 1) / 2 > x) --r;
+  tr = r;
+  tc = (int)(x - (long long)r * (r + 1) / 2);
+}
+
+// ---------------------------------------------- lower-triangle pair tiles
+// Block = TILE x TILE pairs; thread x -> row k (bra), thread y -> column l
+// (ket).  Bra and ket records of the tile are staged in LDS.
+template <int N>
+__global__ void __launch_bounds__(TILE* TILE) k_offdiag(const SysConst sc, int r0, int r1, int ld, int tr0,
+                                                      const double* __restrict__ bra,
+                                                      const double* __restrict__ ket,
+                                                      const double* __restrict__ diagS,
+                                                      zd* __restrict__ Hm, zd* __restrict__ Sm) {
+  extern __shared__ double lds[];
+  const int T = sc.T;
+  int tr, tc;
+  tri_tile(blockIdx.x, tr0, tr, tc);
+  const int kb = tr * TILE, lb = tc * TILE;
+  double* sbra = lds;
+  double* sket = lds + TILE * BraRec<N>::stride;
+  const int tid = threadIdx.y * TILE + threadIdx.x;
+  for (int i = tid; i < TILE * BraRec<N>::stride; i += TILE * TILE) {
+    const int f = kb + i / BraRec<N>::stride;
+    sbra[i] = (f < r1) ? bra[(size_t)kb * BraRec<N>::stride + i] : 0.0;
+  }
+  const int kstride = T * KetRec<N>::stride;
+  for (int i = tid; i < TILE * kstride; i += TILE * TILE) {
+    const int f = lb + i / kstride;
+    sket[i] = (f < r1) ? ket[(size_t)lb * kstride + i] : 0.0;
+  }
+  __syncthreads();
+  const int k = kb + threadIdx.x, l = lb + threadIdx.y;
+  if (k < r0 || k >= r1 || l >= k) return;
+  const double* br = sbra + threadIdx.x * BraRec<N>::stride;
+  const double* kr = sket + threadIdx.y * kstride;
+  zd H, S;
+  pair_sum_energy<N>(sc, br, kr, bra[(size_t)l * BraRec<N>::stride + BraRec<N>::adet], H, S);
+  const double f = 1.0 / sqrt(diagS[k] * diagS[l]);
+  const zd hn = H * f, sn = S * f;
+  Hm[(size_t)l * ld + k] = hn;
+  Sm[(size_t)l * ld + k] = sn;
+  Hm[(size_t)k * ld + l] = zconj(hn);
+  Sm[(size_t)k * ld + l] = zconj(sn);
+}
+
+// ------------------------------------------- stored derivatives (StoreHSD)
+// Per-pair dH, dS with respect to both functions, summed over terms.
+template <int N>
+__device__ inline void pair_sum_deriv(const SysConst& sc, const double* br, const double* kbase,
+                                      const double* bra_l, bool want_l, zd& H, zd& S, zd* Dk, zd* Dl) {
+  constexpr int m = N * (N + 1) / 2;
+  const zd* ck = reinterpret_cast<const zd*>(br + BraRec<N>::ck);
+  const double adet = br[BraRec<N>::adet] * bra_l[BraRec<N>::adet];
+  zd GHb[m], GHk[m], GSs[m], GSk[m];
+#pragma unroll
+  for (int e = 0; e < m; ++e) GHb[e] = GHk[e] = GSs[e] = GSk[e] = mkz(0.0, 0.0);
+  H = S = mkz(0.0, 0.0);
+  for (int t = 0; t < sc.T; ++t) {
+    const double* kr = kbase + (size_t)t * KetRec<N>::stride;
+    const zd* ct = reinterpret_cast<const zd*>(kr + KetRec<N>::ct);
+    PairCore<N> pc;
+    pair_energy<N>(sc, ck, ct, reinterpret_cast<const zd*>(kr + KetRec<N>::nl), adet, pc);
+    zd gb[m], gk[m];
+    pair_adjoint<N>(sc, ck, ct, pc, 0.0, gb, gk);
+    const double c = sc.coef[t];
+    H = H + c * (pc.T + pc.V);
+    S = S + c * pc.S;
+    const zd cS = (-1.5 * c) * pc.S;
+#pragma unroll
+    for (int e = 0; e < m; ++e) {
+      GHb[e] = GHb[e] + c * gb[e];
+      GSs[e] = zfma(cS, pc.Ci[e], GSs[e]);
+    }
+    if (want_l) {
+      zd g2[m], s2[m], gs[m];
+#pragma unroll
+      for (int e = 0; e < m; ++e) gs[e] = cS * pc.Ci[e];
+      perm_congruence<N>(sc.P[t], gk, g2);
+      perm_congruence<N>(sc.P[t], gs, s2);
+#pragma unroll
+      for (int e = 0; e < m; ++e) {
+        GHk[e] = GHk[e] + c * g2[e];
+        GSk[e] = GSk[e] + s2[e];
+      }
+    }
+  }
+  const double* Lk = br + BraRec<N>::lp;
+  const zd aH = 1.5 * H, aS = 1.5 * S;
+  chain_to_params<N>(GHb, aH, Lk, -1.0, Dk, Dk + m);
+  chain_to_params<N>(GSs, aS, Lk, -1.0, Dk + 2 * m, Dk + 3 * m);
+  if (want_l) {
+    const double* Ll = bra_l + BraRec<N>::lp;
+    chain_to_params<N>(GHk, aH, Ll, 1.0, Dl, Dl + m);
+    chain_to_params<N>(GSk, aS, Ll, 1.0, Dl + 2 * m, Dl + 3 * m);
+  }
+}
+
+// Rows k in [nfru, K) with all l <= k; D layout (2 npt) x nfo x ld complex,
+// as Glob_D (src/globvars.f90:211-227, src/matform.f90:55-115).
+template <int N>
+__global__ void k_deriv(const SysConst sc, int K, int nfru, int ld, const double* __restrict__ bra,
+                        const double* __restrict__ ket, const double* __restrict__ diagS,
+                        zd* __restrict__ Hm, zd* __restrict__ Sm, zd* __restrict__ D) {
+  constexpr int m = N * (N + 1) / 2, np4 = 4 * m;
+  const int T = sc.T;
+  const int nfo = K - nfru;
+  // rows k = nfru..K-1, columns l = 0..k (diagonal included), row by row
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long base = (long long)nfru * (nfru + 1) / 2;
+  const long long x = base + idx;
+  const long long total = (long long)K * (K + 1) / 2;
+  if (x >= total) return;
+  int k = (int)((sqrt(8.0 * (double)x + 1.0) - 1.0) * 0.5);
+  while ((long long)(k + 1) * (k + 2) / 2 <= x) ++k;
+  while ((long long)k * (k + 1) / 2 > x) --k;
+  const int l = (int)(x - (long long)k * (k + 1) / 2);
+  const double* br = bra + (size_t)k * BraRec<N>::stride;
+  const double* brl = bra + (size_t)l * BraRec<N>::stride;
+  const bool want_l = (l >= nfru) && (l != k);
+  zd H, S, Dk[np4], Dl[np4];
+  pair_sum_deriv<N>(sc, br, ket + (size_t)l * T * KetRec<N>::stride, brl, want_l, H, S, Dk, Dl);
+  zd* Dcol = D + (size_t)np4 * ((size_t)(k - nfru) + (size_t)nfo * l);
+  if (k == l) {
+    const double s = diagS[k];
+#pragma unroll
+    for (int e = 0; e < np4; ++e) Dcol[e] = mkz(2.0 * Dk[e].x / s, 0.0);
+    return;
+  }
+  const double f = 1.0 / sqrt(diagS[k] * diagS[l]);
+#pragma unroll
+  for (int e = 0; e < np4; ++e) Dcol[e] = Dk[e] * f;
+  if (want_l) {
+    zd* Dc2 = D + (size_t)np4 * ((size_t)(l - nfru) + (size_t)nfo * k);
+#pragma unroll
+    for (int e = 0; e < np4; ++e) Dc2[e] = zconj(Dl[e] * f);
+  }
+  const zd hn = H * f, sn = S * f;
+  Hm[(size_t)l * ld + k] = hn;
+  Sm[(size_t)l * ld + k] = sn;
+  Hm[(size_t)k * ld + l] = zconj(hn);
+  Sm[(size_t)k * ld + l] = zconj(sn);
+}
+
+// ---------------------------------------------------- fused adjoint pass
+// For a pair (k bra, l ket) with weight w = conj(c_k) c_l / sqrt(S_kk S_ll)
+// the gradient picks up 2 Re(w dh) with h = H - E S.  Per function the
+// contributions reduce to 13 reals: RA (Re of the adjoint, both roles),
+// IB (Im of the adjoint, bra minus ket) and Ra (Re of the det-term scalar).
+template <int N> struct AdjAcc {
+  static constexpr int m = N * (N + 1) / 2;
+  static constexpr int nvals = 2 * m + 1;
+};
+
+template <int N>
+__device__ inline void pair_adjoint_sum(const SysConst& sc, const double* br, const double* kbase,
+                                        double adet_l, double E, bool want_l, zd w, double* vb, double* vk) {
+  constexpr int m = N * (N + 1) / 2;
+  const zd* ck = reinterpret_cast<const zd*>(br + BraRec<N>::ck);
+  const double adet = br[BraRec<N>::adet] * adet_l;
+  zd Gb[m], Gk[m];
+#pragma unroll
+  for (int e = 0; e < m; ++e) Gb[e] = Gk[e] = mkz(0.0, 0.0);
+  zd hs = mkz(0.0, 0.0);
+  for (int t = 0; t < sc.T; ++t) {
+    const double* kr = kbase + (size_t)t * KetRec<N>::stride;
+    const zd* ct = reinterpret_cast<const zd*>(kr + KetRec<N>::ct);
+    PairCore<N> pc;
+    pair_energy<N>(sc, ck, ct, reinterpret_cast<const zd*>(kr + KetRec<N>::nl), adet, pc);
+    zd gb[m], gk[m];
+    pair_adjoint<N>(sc, ck, ct, pc, E, gb, gk);
+    const double c = sc.coef[t];
+    hs = hs + c * (pc.T + pc.V - E * pc.S);
+#pragma unroll
+    for (int e = 0; e < m; ++e) Gb[e] = Gb[e] + c * gb[e];
+    if (want_l) {
+      zd g2[m];
+      perm_congruence<N>(sc.P[t], gk, g2);
+#pragma unroll
+      for (int e = 0; e < m; ++e) Gk[e] = Gk[e] + c * g2[e];
+    }
+  }
+  const zd a = w * (1.5 * hs);
+#pragma unroll
+  for (int e = 0; e < m; ++e) {
+    const zd g = w * Gb[e];
+    vb[e] = g.x;
+    vb[m + e] = g.y;
+  }
+  vb[2 * m] = a.x;
+  if (want_l) {
+#pragma unroll
+    for (int e = 0; e < m; ++e) {
+      const zd g = w * Gk[e];
+      vk[e] = g.x;
+      vk[m + e] = -g.y;
+    }
+    vk[2 * m] = a.x;
+  } else {
+#pragma unroll
+    for (int e = 0; e < 2 * m + 1; ++e) vk[e] = 0.0;
+  }
+}
+
+// Tile pass over all pairs with k >= nfru (rows) and l <= k.  Partials:
+// part[(f * (nt + 1) + slot) * nv + v]; function f in tile row r gets its
+// bra partial from tile (r, c) in slot c and its ket partial from tile
+// (r', r) in slot r' + 1.
+template <int N>
+__global__ void __launch_bounds__(TILE* TILE) k_adjoint(const SysConst sc, int K, int nfru, int tr0, int nt,
+                                                      double E,
+                                                      const double* __restrict__ bra,
+                                                      const double* __restrict__ ket,
+                                                      const double* __restrict__ diagS,
+                                                      const zd* __restrict__ cvec,
+                                                      double* __restrict__ part) {
+  constexpr int nv = AdjAcc<N>::nvals;
+  extern __shared__ double lds[];
+  const int T = sc.T;
+  int tr, tc;
+  tri_tile(blockIdx.x, tr0, tr, tc);
+  const int kb = tr * TILE, lb = tc * TILE;
+  double* sbra = lds;
+  double* sket = lds + TILE * BraRec<N>::stride;
+  double* sred = sket + TILE * T * KetRec<N>::stride;   // TILE*TILE*nv
+  const int tid = threadIdx.y * TILE + threadIdx.x;
+  for (int i = tid; i < TILE * BraRec<N>::stride; i += TILE * TILE) {
+    const int f = kb + i / BraRec<N>::stride;
+    sbra[i] = (f < K) ? bra[(size_t)kb * BraRec<N>::stride + i] : 0.0;
+  }
+  const int kstride = T * KetRec<N>::stride;
+  for (int i = tid; i < TILE * kstride; i += TILE * TILE) {
+    const int f = lb + i / kstride;
+    sket[i] = (f < K) ? ket[(size_t)lb * kstride + i] : 0.0;
+  }
+  __syncthreads();
+  const int k = kb + threadIdx.x, l = lb + threadIdx.y;
+  double vb[nv], vk[nv];
+  const bool active = (k >= nfru) && (k < K) && (l <= k);
+  if (active) {
+    const double* br = sbra + threadIdx.x * BraRec<N>::stride;
+    const double* kr = sket + threadIdx.y * kstride;
+    const zd ckc = zconj(cvec[k]);
+    const zd w = (ckc * cvec[l]) * (1.0 / sqrt(diagS[k] * diagS[l]));
+    const bool want_l = (l != k) && (l >= nfru);
+    pair_adjoint_sum<N>(sc, br, kr, bra[(size_t)l * BraRec<N>::stride + BraRec<N>::adet], E, want_l, w,
+                        vb, vk);
+  } else {
+#pragma unroll
+    for (int e = 0; e < nv; ++e) vb[e] = vk[e] = 0.0;
+  }
+  // reduce bra partials over threadIdx.y (same k), ket partials over x
+  for (int e = 0; e < nv; ++e) sred[(threadIdx.y * TILE + threadIdx.x) * nv + e] = vb[e];
+  __syncthreads();
+  if (threadIdx.y == 0 && kb + (int)threadIdx.x < K) {
+    const int f = kb + threadIdx.x;
+    double* out = part + ((size_t)f * (nt + 1) + tc) * nv;
+    for (int e = 0; e < nv; ++e) {
+      double s = 0.0;
+      for (int y = 0; y < TILE; ++y) s += sred[(y * TILE + threadIdx.x) * nv + e];
+      out[e] = s;
+    }
+  }
+  __syncthreads();
+  for (int e = 0; e < nv; ++e) sred[(threadIdx.x * TILE + threadIdx.y) * nv + e] = vk[e];
+  __syncthreads();
+  if (threadIdx.y == 0 && lb + (int)threadIdx.x < K) {
+    const int f = lb + threadIdx.x;
+    double* out = part + ((size_t)f * (nt + 1) + tr + 1) * nv;
+    for (int e = 0; e < nv; ++e) {
+      double s = 0.0;
+      for (int y = 0; y < TILE; ++y) s += sred[(y * TILE + threadIdx.x) * nv + e];
+      out[e] = s;
+    }
+  }
+}
+
+// Per function f >= nfru: sum its partial slots in fixed order and apply
+// the chain rule to vech L, vech B:
+//   dE/dL(i,j) = 4 (RA L)(i,j) + 2 Ra delta_ij / L_jj
+//   dE/dB(i,i) = 2 IB_ii,   dE/dB(i,j) = 4 IB_ij
+template <int N>
+__global__ void k_adjoint_finish(int K, int nfru, int nt, int tr0, const double* __restrict__ bra,
+                                 const double* __restrict__ part, double* __restrict__ grad) {
+  constexpr int m = N * (N + 1) / 2, nv = AdjAcc<N>::nvals;
+  const int f = nfru + blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= K) return;
+  const int r = f / TILE;
+  double acc[nv];
+#pragma unroll
+  for (int e = 0; e < nv; ++e) acc[e] = 0.0;
+  // valid slots: bra slots c = 0..r (tile rows >= tr0 only), ket slots r'+1 for r' = max(r,tr0)..nt-1
+  for (int s = 0; s <= nt; ++s) {
+    bool valid;
+    if (s <= r) valid = (r >= tr0);
+    else valid = (s - 1 >= tr0);
+    if (!valid) continue;
+    const double* p = part + ((size_t)f * (nt + 1) + s) * nv;
+#pragma unroll
+    for (int e = 0; e < nv; ++e) acc[e] += p[e];
+  }
+  const double* L = bra + (size_t)f * BraRec<N>::stride + BraRec<N>::lp;
+  double* g = grad + (size_t)(f - nfru) * 2 * m;
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+#pragma unroll
+    for (int i = j; i < N; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int c = j; c < N; ++c) s = fma(acc[sidx(i, c)], L[sidx(c, j)], s);
+      double v = 4.0 * s;
+      if (i == j) v += 2.0 * acc[2 * m] / L[sidx(j, j)];
+      g[vidx(N, i, j)] = v;
+      g[m + vidx(N, i, j)] = ((i == j) ? 2.0 : 4.0) * acc[m + sidx(i, j)];
+    }
+}
+
+// ------------------------------------------------------------ launchers
+template <int N>
+static int launch_prep(const Ctx& c, hipStream_t s) {
+  const int K = c.K;
+  hipLaunchKernelGGL(k_prep<N>, dim3((K + 127) / 128), dim3(128), 0, s, c.sc, K, c.params, c.bra, c.ket);
+  return (int)hipGetLastError();
+}
+
+template <int N>
+static int launch_assemble(const Ctx& c, int r0, int r1, hipStream_t s) {
+  if (r1 <= r0) return 0;
+  hipLaunchKernelGGL(k_diag<N>, dim3((r1 - r0 + 127) / 128), dim3(128), 0, s, c.sc, r0, r1, c.ld, c.bra, c.ket,
+                     c.diagS, c.diagH, c.H, c.S);
+  const int tr0 = r0 / TILE, tr1 = (r1 + TILE - 1) / TILE;
+  const long long ntiles = (long long)tr1 * (tr1 + 1) / 2 - (long long)tr0 * (tr0 + 1) / 2;
+  const size_t lds = sizeof(double) * TILE * (BraRec<N>::stride + c.T * KetRec<N>::stride);
+  if (c.ev_off0) (void)hipEventRecord(c.ev_off0, s);
+  hipLaunchKernelGGL(k_offdiag<N>, dim3((unsigned)ntiles), dim3(TILE, TILE), lds, s, c.sc, r0, r1, c.ld, tr0,
+                     c.bra, c.ket, c.diagS, c.H, c.S);
+  if (c.ev_off1) (void)hipEventRecord(c.ev_off1, s);
+  return (int)hipGetLastError();
+}
+
+template <int N>
+static int launch_deriv(const Ctx& c, int nfru, zd* D, hipStream_t s) {
+  const int K = c.K;
+  hipLaunchKernelGGL(k_diag<N>, dim3((K - nfru + 127) / 128), dim3(128), 0, s, c.sc, nfru, K, c.ld, c.bra, c.ket,
+                     c.diagS, c.diagH, c.H, c.S);
+  const long long n = (long long)K * (K + 1) / 2 - (long long)nfru * (nfru + 1) / 2;
+  hipLaunchKernelGGL(k_deriv<N>, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, c.sc, K, nfru, c.ld, c.bra,
+                     c.ket, c.diagS, c.H, c.S, D);
+  return (int)hipGetLastError();
+}
+
+template <int N>
+static int launch_adjoint(const Ctx& c, int nfru, double E, const zd* cvec, double* part, double* grad,
+                          hipStream_t s) {
+  const int K = c.K;
+  const int nt = (K + TILE - 1) / TILE;
+  const int tr0 = nfru / TILE;
+  const long long ntiles = (long long)nt * (nt + 1) / 2 - (long long)tr0 * (tr0 + 1) / 2;
+  const size_t lds = sizeof(double) * TILE *
+                     (BraRec<N>::stride + c.T * KetRec<N>::stride + TILE * AdjAcc<N>::nvals);
+  if (c.ev_adj0) (void)hipEventRecord(c.ev_adj0, s);
+  hipLaunchKernelGGL(k_adjoint<N>, dim3((unsigned)ntiles), dim3(TILE, TILE), lds, s, c.sc, K, nfru, tr0, nt, E,
+                     c.bra, c.ket, c.diagS, cvec, part);
+  if (c.ev_adj1) (void)hipEventRecord(c.ev_adj1, s);
+  hipLaunchKernelGGL(k_adjoint_finish<N>, dim3((K - nfru + 127) / 128), dim3(128), 0, s, K, nfru, nt, tr0,
+                     c.bra, part, grad);
+  return (int)hipGetLastError();
+}
+
+int prep(const Ctx& c, hipStream_t s) {
+  switch (c.n) {
+    case 1: return launch_prep<1>(c, s);
+    case 2: return launch_prep<2>(c, s);
+    case 3: return launch_prep<3>(c, s);
+    case 4: return launch_prep<4>(c, s);
+    case 5: return launch_prep<5>(c, s);
+  }
+  return -1;
+}
+int assemble(const Ctx& c, int r0, int r1, hipStream_t s) {
+  switch (c.n) {
+    case 1: return launch_assemble<1>(c, r0, r1, s);
+    case 2: return launch_assemble<2>(c, r0, r1, s);
+    case 3: return launch_assemble<3>(c, r0, r1, s);
+    case 4: return launch_assemble<4>(c, r0, r1, s);
+    case 5: return launch_assemble<5>(c, r0, r1, s);
+  }
+  return -1;
+}
+int assemble_deriv(const Ctx& c, int nfru, zd* D, hipStream_t s) {
+  switch (c.n) {
+    case 1: return launch_deriv<1>(c, nfru, D, s);
+    case 2: return launch_deriv<2>(c, nfru, D, s);
+    case 3: return launch_deriv<3>(c, nfru, D, s);
+    case 4: return launch_deriv<4>(c, nfru, D, s);
+    case 5: return launch_deriv<5>(c, nfru, D, s);
+  }
+  return -1;
+}
+int adjoint(const Ctx& c, int nfru, double E, const zd* cvec, double* part, double* grad, hipStream_t s) {
+  switch (c.n) {
+    case 1: return launch_adjoint<1>(c, nfru, E, cvec, part, grad, s);
+    case 2: return launch_adjoint<2>(c, nfru, E, cvec, part, grad, s);
+    case 3: return launch_adjoint<3>(c, nfru, E, cvec, part, grad, s);
+    case 4: return launch_adjoint<4>(c, nfru, E, cvec, part, grad, s);
+    case 5: return launch_adjoint<5>(c, nfru, E, cvec, part, grad, s);
+  }
+  return -1;
+}
+size_t adjoint_partial_doubles(int n, int K) {
+  const int nt = (K + TILE - 1) / TILE;
+  const int m = n * (n + 1) / 2;
+  return (size_t)K * (nt + 1) * (2 * m + 1);
+}
+
+}  // namespace ecg

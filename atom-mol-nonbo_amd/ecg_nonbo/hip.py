@@ -1,0 +1,230 @@
+"""ctypes binding of libecg_hip.so (the C-ABI in include/ecg.h).
+
+Mirrors the reference's subroutine interfaces on the Python side:
+``MatrixElements`` (src/matelem.f90:12), ``ComputeMatElem`` /
+``ComputeMatElemAndDeriv`` (src/matform.f90:119, 234), ``GHEPIIS``
+(src/linalg.f90:1781) and the energy evaluators ``EnergyGA``/``EnergyIA``
+and ``EnergyGB``/``EnergyIB`` (src/workproc.f90:1697-2154).
+
+There is no fallback: if the shared library is missing or no GPU is
+present, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_HERE, "libecg_hip.so")
+
+ECG_OK = 0
+ECG_ERR_ARG = -1
+ECG_ERR_HIP = -2
+ECG_ERR_NOMEM = -3
+ECG_ERR_NONFINITE = -4
+ECG_ERR_STATE = -5
+ECG_SINGULAR = 1
+ECG_NOT_CONVERGED = 2
+ECG_ROOT_NOT_FOUND = 3
+
+_NAMES = {
+    ECG_ERR_ARG: "invalid argument", ECG_ERR_HIP: "HIP runtime error", ECG_ERR_NOMEM: "device allocation failed",
+    ECG_ERR_NONFINITE: "non-finite pair element", ECG_ERR_STATE: "call order violated",
+    ECG_SINGULAR: "H - sigma S singular", ECG_NOT_CONVERGED: "inverse iteration not converged",
+    ECG_ROOT_NOT_FOUND: "requested root not isolated",
+}
+
+EXPORTS = [
+    "ecg_ctx_create", "ecg_ctx_destroy", "ecg_set_params", "ecg_update_params", "ecg_matrix_elements",
+    "ecg_compute_mat_elem", "ecg_compute_mat_elem_and_deriv", "ecg_ghepiis", "ecg_energy", "ecg_energy_grad",
+    "ecg_last_timings",
+]
+
+
+class EcgError(RuntimeError):
+    def __init__(self, func, code):
+        super().__init__("%s returned %d (%s)" % (func, code, _NAMES.get(code, "?")))
+        self.code = code
+
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i = ctypes.POINTER(ctypes.c_int)
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the library and declare the prototypes (no GPU is touched)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError("%s is not built; run atom-mol-nonbo_amd/build.py" % path)
+    L = ctypes.CDLL(path)
+    c_int, c_double, c_char = ctypes.c_int, ctypes.c_double, ctypes.c_char
+    vp = ctypes.c_void_p
+    L.ecg_ctx_create.argtypes = [c_int, c_int, _i, _i, _d, c_double, _d, ctypes.POINTER(vp)]
+    L.ecg_ctx_destroy.argtypes = [vp]
+    L.ecg_set_params.argtypes = [vp, c_int, _d]
+    L.ecg_update_params.argtypes = [vp, c_int, c_int, _d]
+    L.ecg_matrix_elements.argtypes = [vp, _d, _d, _i, c_int, _d, _d, _d, _d]
+    L.ecg_compute_mat_elem.argtypes = [vp, c_int, c_int, c_char, c_double, _d, _d, c_int, _d, _d]
+    L.ecg_compute_mat_elem_and_deriv.argtypes = [vp, c_int, c_char, c_double, _d, _d, c_int, _d, _d, _d]
+    L.ecg_ghepiis.argtypes = [vp, c_int, c_int, _d, c_int, _d, _d, c_int, c_double, _d, c_double, c_int, c_int,
+                              _d, _d, _d, _i]
+    L.ecg_energy.argtypes = [vp, c_int, c_double, c_double, c_int, _d, _d, _d]
+    L.ecg_energy_grad.argtypes = [vp, c_int, c_int, c_double, c_double, c_int, _d, _d, _d]
+    L.ecg_last_timings.argtypes = [vp, _d, c_int]
+    for name in EXPORTS:
+        getattr(L, name).restype = c_int
+    _lib = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(_d)
+
+
+def _check(func, rc, ok=(ECG_OK,)):
+    if rc not in ok:
+        raise EcgError(func, rc)
+    return rc
+
+
+class EcgContext:
+    """Device state of one system and basis (one ecg_ctx)."""
+
+    def __init__(self, system):
+        L = load()
+        self.system = system
+        n = system.n
+        self.n = n
+        self.npt = system.npt
+        P = np.ascontiguousarray(np.transpose(system.yhy_mats, (0, 2, 1)).reshape(-1), dtype=np.int32)
+        coef = np.ascontiguousarray(system.yhy_coef, dtype=np.int32)
+        M = np.ascontiguousarray(system.mass_matrix.T.reshape(-1), dtype=np.float64)
+        q = np.ascontiguousarray(system.q, dtype=np.float64)
+        h = ctypes.c_void_p()
+        _check("ecg_ctx_create", L.ecg_ctx_create(n, system.T, P.ctypes.data_as(_i), coef.ctypes.data_as(_i),
+                                                   _p(M), float(system.q0), _p(q), ctypes.byref(h)))
+        self._h = h
+        self.K = 0
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            load().ecg_ctx_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --------------------------------------------------------- parameters
+    def set_params(self, params):
+        """params: (K, npt) array, row k = (vech L_k, vech B_k)."""
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        K = p.shape[0]
+        _check("ecg_set_params", load().ecg_set_params(self._h, K, _p(p)))
+        self.K = K
+
+    def update_params(self, first, params):
+        p = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+        _check("ecg_update_params", load().ecg_update_params(self._h, first, p.shape[0], _p(p)))
+
+    # --------------------------------------------------- drop-in routines
+    def MatrixElements(self, paramk, paraml, P, gradflag=0):
+        """(Hkl, Skl, Dk, Dl) of one pair and one symmetry matrix P."""
+        n = self.n
+        pk = np.ascontiguousarray(paramk, dtype=np.float64)
+        pl = np.ascontiguousarray(paraml, dtype=np.float64)
+        Pf = np.ascontiguousarray(np.asarray(P).T.reshape(-1), dtype=np.int32)
+        H = np.zeros(1, np.complex128)
+        S = np.zeros(1, np.complex128)
+        Dk = np.zeros(2 * n * (n + 1), np.complex128)
+        Dl = np.zeros(2 * n * (n + 1), np.complex128)
+        _check("ecg_matrix_elements", load().ecg_matrix_elements(
+            self._h, _p(pk), _p(pl), Pf.ctypes.data_as(_i), gradflag, _p(H.view(np.float64)),
+            _p(S.view(np.float64)), _p(Dk.view(np.float64)), _p(Dl.view(np.float64))))
+        return H[0], S[0], Dk, Dl
+
+    def ComputeMatElem(self, nmin=1, nmax=None, method="G", approx_energy=0.0, H=None, S=None, diagH=None,
+                       diagS=None):
+        K = self.K
+        nmax = K if nmax is None else nmax
+        if H is None:
+            H = np.zeros((K, K), np.complex128, order="F")
+            S = np.zeros((K, K), np.complex128, order="F")
+            diagH = np.zeros(K)
+            diagS = np.zeros(K)
+        assert H.flags.f_contiguous and S.flags.f_contiguous
+        _check("ecg_compute_mat_elem", load().ecg_compute_mat_elem(
+            self._h, nmin, nmax, method.encode(), float(approx_energy), H.ctypes.data_as(_d),
+            S.ctypes.data_as(_d), H.shape[0], _p(diagH), _p(diagS)))
+        return H, S, diagH, diagS
+
+    def ComputeMatElemAndDeriv(self, nfru=0, method="G", approx_energy=0.0, H=None, S=None, diagH=None,
+                               diagS=None):
+        """Rows nfru+1..K with derivatives.  Rows <= nfru are not recomputed:
+        pass the arrays of an earlier ComputeMatElem (their diagS normalises
+        the new rows), as the reference's global state provides."""
+        K = self.K
+        if H is None:
+            if nfru > 0:
+                H, S, diagH, diagS = self.ComputeMatElem(1, K, method, approx_energy)
+            else:
+                H = np.zeros((K, K), np.complex128, order="F")
+                S = np.zeros((K, K), np.complex128, order="F")
+                diagH = np.zeros(K)
+                diagS = np.zeros(K)
+        D = np.zeros((2 * self.npt, K - nfru, K), np.complex128, order="F")
+        _check("ecg_compute_mat_elem_and_deriv", load().ecg_compute_mat_elem_and_deriv(
+            self._h, nfru, method.encode(), float(approx_energy), H.ctypes.data_as(_d), S.ctypes.data_as(_d),
+            K, _p(diagH), _p(diagS), D.ctypes.data_as(_d)))
+        return H, S, diagH, diagS, D
+
+    def GHEPIIS(self, k, M, invD, B, apprlambda, v, tol, maxiter=20, specif_norm=0):
+        """Returns (ErrorCode, lambda, x, RelAcc, NumIter); M and invD are
+        updated in place like the reference's arguments."""
+        n = M.shape[0]
+        assert M.flags.f_contiguous and B.flags.f_contiguous
+        v = np.ascontiguousarray(v, dtype=np.complex128)
+        x = np.zeros(n, np.complex128)
+        lam = ctypes.c_double()
+        rel = ctypes.c_double()
+        nit = ctypes.c_int()
+        rc = load().ecg_ghepiis(self._h, k, n, M.ctypes.data_as(_d), M.shape[0], invD.ctypes.data_as(_d),
+                                B.ctypes.data_as(_d), B.shape[0], float(apprlambda), v.ctypes.data_as(_d),
+                                float(tol), maxiter, specif_norm, ctypes.byref(lam), x.ctypes.data_as(_d),
+                                ctypes.byref(rel), ctypes.byref(nit))
+        _check("ecg_ghepiis", rc, ok=(ECG_OK, ECG_SINGULAR, ECG_NOT_CONVERGED))
+        return rc, lam.value, x, rel.value, nit.value
+
+    # --------------------------------------------------- device-resident
+    def energy(self, which=1, sigma=float("nan"), tol=1e-12, maxiter=20, want_vector=False):
+        """(status, E, c or None, info) for root `which`."""
+        E = ctypes.c_double()
+        info = np.zeros(8)
+        c = np.zeros(self.K, np.complex128) if want_vector else None
+        rc = load().ecg_energy(self._h, which, float(sigma), float(tol), maxiter, ctypes.byref(E),
+                               c.ctypes.data_as(_d) if c is not None else None, _p(info))
+        _check("ecg_energy", rc, ok=(ECG_OK, ECG_NOT_CONVERGED))
+        return rc, E.value, c, info
+
+    def energy_grad(self, nfru=0, which=1, sigma=float("nan"), tol=1e-12, maxiter=20):
+        """(status, E, grad, info); grad ordered per function (dE/dvechL,
+        dE/dvechB) for functions nfru+1..K."""
+        E = ctypes.c_double()
+        info = np.zeros(8)
+        g = np.zeros(self.npt * (self.K - nfru))
+        rc = load().ecg_energy_grad(self._h, nfru, which, float(sigma), float(tol), maxiter, ctypes.byref(E),
+                                    _p(g), _p(info))
+        _check("ecg_energy_grad", rc, ok=(ECG_OK, ECG_NOT_CONVERGED))
+        return rc, E.value, g, info
+
+    def timings(self):
+        t = np.zeros(8)
+        n = load().ecg_last_timings(self._h, _p(t), 8)
+        return t[:n]

@@ -1,0 +1,197 @@
+"""Benchmark of the ECG variational hot path on one MI355X.
+
+Workload (one step): a full energy plus analytic gradient for the
+4-particle H2-like non-BO system (two protons, two electrons; n = 3
+pseudo-particles; Y = (1+P12)(1+P34), T = 4 simplified Y^dagger Y terms) with
+K = 2000 seeded complex ECGs (numpy PCG64, seed 20261203 + 3; L_ii =
+10**U(-2,2), L_ij ~ N(0, (0.1 L_jj)^2), B_ij ~ N(0, 0.05^2)), FP64 complex:
+  assembly of all K(K+1)/2 = 2,001,000 pair elements H_kl, S_kl (x T terms)
+  -> LDL^H of H - sigma S and inverse iteration for the lowest root
+  -> fused adjoint pass: dE/d(vech L_k, vech B_k) for all 12 K parameters.
+Each step starts from new parameters (every parameter scaled by 1 + 1e-7 u,
+u ~ U(-1,1), as an optimiser step would) and uses the previous step's
+energy for the shift (sigma = E_prev - 1e-4 |E_prev|, the reference's
+method 'I' with its Glob_ApproxEnergy); nothing of a previous step's H, S,
+factor or gradient is reused.
+
+Metric: pair H,S+grad elements per second = K(K+1)/2 / (seconds per E+grad).
+Prints one JSON line (see the driver contract) with a `roofline` object for
+the dominant kernel (the fused adjoint pass, timed with HIP events on the
+library's stream) and a `cpu_baseline` object (the oracle's pair algebra on
+the host cores over a bounded random sample of the same pairs, plus the
+host eigen step at full size).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "atom-mol-nonbo_amd"))
+sys.path.insert(0, ROOT)
+
+from ecg_nonbo import named_system, seeded_basis, SEED_BASE  # noqa: E402
+from ecg_nonbo.hip import EcgContext  # noqa: E402
+
+FP64_PEAK_TFLOPS = 78.6       # MI355X dense FP64 (vector and matrix), AMD datasheet
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_flops(system, params, K):
+    """Reference-algorithm flops (oracle operation count, convention of the
+    flop table: add/sub/mul/div/sqrt = 1, per (pair, term), per-function set-up
+    and integer permutation products excluded) of one full E+grad pair pass:
+    T x [K(K-1)/2 x f(gradflag 3) + K x f(gradflag 1)], with f averaged over
+    the T symmetry matrices."""
+    from oracle import oracle
+    f3 = np.mean([oracle.flops_per_pair(system, params[0], params[1], 3, term=t) for t in range(system.T)])
+    f1 = np.mean([oracle.flops_per_pair(system, params[0], params[0], 1, term=t) for t in range(system.T)])
+    f0 = np.mean([oracle.flops_per_pair(system, params[0], params[1], 0, term=t) for t in range(system.T)])
+    return f0, f1, f3, system.T * (K * (K - 1) / 2 * f3 + K * f1)
+
+
+def cpu_baseline(system, params, K, seconds_target=15.0):
+    """Oracle pair algebra on all host threads over a random sample of the
+    workload's pairs, then the host eigen step (LAPACK zhegvx via scipy) at
+    the full K on the oracle's own H, S."""
+    from oracle import oracle
+    import scipy.linalg.lapack as lapack
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    rng = np.random.default_rng(7)
+    n_probe = 4000
+    kk = rng.integers(0, K, n_probe)
+    ll = (rng.random(n_probe) * (kk + 1)).astype(np.int64)
+    t0 = time.perf_counter()
+    oracle.pair_batch(system, params, kk, ll, threads)
+    rate = n_probe / (time.perf_counter() - t0)
+    n_s = int(min(K * (K + 1) // 2, max(20000, rate * seconds_target * 0.7)))
+    kk = rng.integers(0, K, n_s)
+    ll = (rng.random(n_s) * (kk + 1)).astype(np.int64)
+    t0 = time.perf_counter()
+    oracle.pair_batch(system, params, kk, ll, threads)
+    t_pairs = time.perf_counter() - t0
+    rate = n_s / t_pairs
+    H, S, dH, dS = oracle.assemble(system, params, "G", nthreads=threads)
+    Hf, Sf = oracle.full_hermitian(H, S, dH)
+    t0 = time.perf_counter()
+    out = lapack.zhegvx(Hf, Sf, itype=1, jobz="V", uplo="U", il=1, iu=1, abstol=2 * np.finfo(float).tiny,
+                        **{"range": "I"})
+    t_eig = time.perf_counter() - t0
+    npairs = K * (K + 1) // 2
+    s_per_eg = npairs / rate + t_eig
+    return {
+        "value": npairs / s_per_eg,
+        "unit": "pair elements/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "%d random pairs of the K=%d H2 basis with gradients (T=4 terms each) in %.1f s, "
+                  "plus one LAPACK zhegvx (lowest root, eigenvector) at K=%d in %.1f s; "
+                  "s per E+grad = %.2f" % (n_s, K, t_pairs, K, t_eig, s_per_eg),
+        "s_per_energy_grad": s_per_eg,
+        "pair_rate": rate,
+        "zhegvx_E": float(out[0][0]) if out[-1] == 0 else None,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--K", type=int, default=2000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the path on a single MI355X "
+              "(see DESIGN.md for how it would shard); --gpus %d is not supported" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    K = args.K
+    system = named_system("h2")
+    base = seeded_basis(system.n, K, SEED_BASE + 3)
+    rng = np.random.default_rng(SEED_BASE + 33)
+    ctx = EcgContext(system)
+    ctx.set_params(base)
+    # cold start (untimed): locate the lowest root without a shift
+    st, E, _, info = ctx.energy(1, float("nan"), 1e-12, 60)
+    sigma = E - 1e-4 * abs(E)
+    print("# cold start E=%.15f factorisations=%d iterations=%d" % (E, info[3], info[0]), file=sys.stderr,
+          flush=True)
+
+    def step(i):
+        nonlocal sigma
+        p = base * (1.0 + 1e-7 * rng.uniform(-1, 1, base.shape))
+        ctx.set_params(p)
+        st, e, g, inf = ctx.energy_grad(0, 1, sigma, 1e-12, 30)
+        sigma = e - 1e-4 * abs(e)
+        return e, g, inf
+
+    for i in range(args.warmup):
+        e, g, inf = step(i)
+        print("# warmup %d E=%.15f iters=%d" % (i, e, inf[0]), file=sys.stderr, flush=True)
+    times = []
+    adj_ms = []
+    off_ms = []
+    eig_ms = []
+    iters = []
+    t_all0 = time.perf_counter()
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        e, g, inf = step(i)
+        times.append(time.perf_counter() - t0)
+        tm = ctx.timings()
+        adj_ms.append(tm[4])
+        off_ms.append(tm[5])
+        eig_ms.append(tm[1])
+        iters.append(inf[0])
+        print("# step %d E=%.15f |g|=%.3e ms=%.3f (assembly %.3f eigen %.3f adjoint %.3f)"
+              % (i, e, np.linalg.norm(g), 1e3 * times[-1], tm[0], tm[1], tm[2]), file=sys.stderr, flush=True)
+    t_all = time.perf_counter() - t_all0
+    s_per = t_all / args.steps
+    npairs = K * (K + 1) // 2
+    f0, f1, f3, flops_adj = algorithmic_flops(system, base, K)
+    adj_s = float(np.mean(adj_ms)) * 1e-3
+    achieved = flops_adj / adj_s / 1e12
+    out = {
+        "metric": "FP64 pair H,S+grad elements/s (E+grad, K=2000, n=3)",
+        "value": npairs / s_per,
+        "unit": "pair elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * s_per,
+        "s_per_energy_grad": s_per,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded ECG basis)",
+        "config": {"workload": "H2-like n=3 (T=4) energy+gradient, K=2000 complex ECGs, seed %d" % (SEED_BASE + 3),
+                   "K": K, "n": 3, "T": system.T, "pairs": npairs},
+        "phases_ms": {"offdiag_kernel": float(np.mean(off_ms)), "eigen_step": float(np.mean(eig_ms)),
+                      "adjoint_kernel": float(np.mean(adj_ms)), "inverse_iterations": float(np.mean(iters))},
+        "roofline": {
+            "bound": "mfma",
+            "kernel": "k_adjoint<3> (fused E+grad pair pass)",
+            "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": None,
+            "flops_per_launch": flops_adj,
+            "flops_per_pair_term": {"energy": f0, "grad_bra": f1, "grad_both": f3},
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(system, base, K)
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,136 @@
+/*
+ * C-ABI of the MI355X ECG pair path (libecg_hip.so).
+ *
+ * Conventions (the reference's Fortran storage, so a Fortran host can pass
+ * its arrays through ISO_C_BINDING unchanged):
+ *   - matrices are column-major with an explicit leading dimension;
+ *   - complex values are two interleaved doubles (re, im), i.e.
+ *     complex(8) / double _Complex / hipDoubleComplex, passed as double*;
+ *   - function indices (nmin, nmax, nfru, which, k) are 1-based;
+ *   - the parameter column of function k is (vech L_k, vech B_k), npt =
+ *     n(n+1) doubles, vech running down the columns of the lower triangle
+ *     (src/matelem.f90:109-122);
+ *   - every entry point returns an int status (ECG_OK = 0) and never exits.
+ *
+ * The caller owns every host array; the library owns the device state of a
+ * context (parameters, H, S, factor, work buffers) so repeated calls do not
+ * re-upload.  A context is not re-entrant: calls on one context must not
+ * overlap.  All entry points are synchronous: they return after the results
+ * are in the caller's arrays.
+ */
+#ifndef ECG_H
+#define ECG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define ECG_OK 0
+#define ECG_ERR_ARG (-1)         /* invalid argument */
+#define ECG_ERR_HIP (-2)         /* HIP runtime error (no device, launch failure) */
+#define ECG_ERR_NOMEM (-3)       /* device allocation failed */
+#define ECG_ERR_NONFINITE (-4)   /* a pair element is Inf/NaN (pseudo-Cholesky breakdown);
+                                    the reference does not flag this (src/matelem.f90:211-214) */
+#define ECG_ERR_STATE (-5)       /* call order violated (e.g. no parameters set) */
+/* positive codes mirror the reference:
+ *   GHEPIIS / EnergyIA: 1 = H - sigma S singular (zero pivot), 2 = inverse
+ *   iteration not converged to Tol within MaxIter (src/linalg.f90:1911);
+ *   energy with root search: 3 = the requested root could not be isolated. */
+#define ECG_SINGULAR 1
+#define ECG_NOT_CONVERGED 2
+#define ECG_ROOT_NOT_FOUND 3
+
+typedef struct ecg_ctx ecg_ctx;
+
+/* Create a context for one physical system.  Replaces the set-up the
+ * reference does in ProgramDataInit (src/workproc.f90:575-1202):
+ *   n          pseudo-particles (1..5)
+ *   T          number of simplified Y^dagger Y terms (1..16)
+ *   yhy_mats   T integer n x n matrices, column-major, term after term
+ *              (Glob_YHYMatr, src/globvars.f90)
+ *   yhy_coef   T integer coefficients (Glob_YHYCoeff)
+ *   mass       n x n mass matrix, column-major (Glob_MassMatrix)
+ *   q0, q      charge of particle 1 and the n pseudo-charges
+ *              (Glob_PseudoCharge0, Glob_PseudoCharge) */
+int ecg_ctx_create(int n, int T, const int* yhy_mats, const int* yhy_coef, const double* mass, double q0,
+                   const double* q, ecg_ctx** out);
+int ecg_ctx_destroy(ecg_ctx* ctx);
+
+/* Upload the nonlinear parameters of K functions (Glob_NonlinParam(npt, K),
+ * src/globvars.f90:141-146).  Functions first..first+count-1 (1-based) are
+ * replaced when first > 1 (the rest stay resident). */
+int ecg_set_params(ecg_ctx* ctx, int K, const double* params);
+int ecg_update_params(ecg_ctx* ctx, int first, int count, const double* params);
+
+/* One pair and one symmetry matrix P on the device: MatrixElements
+ * (src/matelem.f90:12-750).  Dk, Dl: 4*np complex each
+ * (dH/dvechL, dH/dvechB, dS/dvechL, dS/dvechB), written when gradflag asks
+ * for them (1: k, 2: l, 3: both).  The system constants of ctx are used. */
+int ecg_matrix_elements(ecg_ctx* ctx, const double* paramk, const double* paraml, const int* P, int gradflag,
+                        double* Hkl, double* Skl, double* Dk, double* Dl);
+
+/* ComputeMatElem(Nmin, Nmax) + StoreHS (src/matform.f90:11-51, 119-230):
+ * rows nmin..nmax, all l <= k, summed over the Y^dagger Y terms, written
+ * with the reference's normalisation and layout:
+ *   method 'G': diagS(k) = Re S_kk, diagH(k) = Re H_kk / diagS(k); lower
+ *               triangles of H, S hold H_kl f, S_kl f, f = 1/sqrt(diagS_k diagS_l)
+ *   method 'I': S(k,k) = 1, H(k,k) = diagH(k) - approx_energy; S in both
+ *               triangles; lower H holds (H_kl - approx_energy S_kl) f
+ * Elements of rows < nmin are neither computed nor written, but their
+ * diagS must be the values of an earlier call (as in the reference). */
+int ecg_compute_mat_elem(ecg_ctx* ctx, int nmin, int nmax, char method, double approx_energy, double* H,
+                         double* S, int ld, double* diagH, double* diagS);
+
+/* ComputeMatElemAndDeriv(nfru+1, K) + StoreHSD (src/matform.f90:55-115,
+ * 234-360).  D: (2*npt) x (K - nfru) x ld complex (Glob_D).  H, S, diagH,
+ * diagS as ecg_compute_mat_elem for rows nfru+1..K. */
+int ecg_compute_mat_elem_and_deriv(ecg_ctx* ctx, int nfru, char method, double approx_energy, double* H,
+                                   double* S, int ld, double* diagH, double* diagS, double* D);
+
+/* GHEPIIS (src/linalg.f90:1781-1926) on host arrays: inverse iteration for
+ * the eigenvalue of (A, B) nearest apprlambda.  M: lower triangle holds
+ * A - apprlambda B (rows >= k; rows < k as well, the factor is recomputed
+ * from row 1 on the device), the strict upper triangle receives L^H;
+ * invD receives 1/D (complex, imaginary part zero); B full Hermitian.
+ * v: start vector; x: eigenvector (normalised x^H B x = 1 when
+ * specif_norm = 0, x^H x = 1 when 1, max |Re|,|Im| = 1 otherwise).
+ * Returns 0, ECG_SINGULAR or ECG_NOT_CONVERGED like the reference's
+ * ErrorCode. */
+int ecg_ghepiis(ecg_ctx* ctx, int k, int n, double* M, int nM, double* invD, const double* B, int nB,
+                double apprlambda, const double* v, double tol, int maxiter, int specif_norm, double* lambda,
+                double* x, double* relacc, int* numiter);
+
+/* Device-resident energy: assembles H, S for all K functions, then finds
+ * root number `which` (1 = lowest) of H c = E S c by LDL^H inverse
+ * iteration.  If sigma is finite it is used as the shift (the reference's
+ * method 'I', Glob_ApproxEnergy); otherwise the shift is found from
+ * Sylvester inertia counts of LDL^H(H - sigma S) (replacing ZHEGVX's
+ * RANGE='I' selection, src/workproc.f90:1750).  c (nullable): K complex,
+ * c^H S c = 1 for the normalised S.  info (nullable, 8 doubles):
+ *   [0] iterations, [1] convergence measure, [2] final shift,
+ *   [3] factorisations, [4] negative pivots at the final shift,
+ *   [5] x^H S x before normalisation. */
+int ecg_energy(ecg_ctx* ctx, int which, double sigma, double tol, int maxiter, double* E, double* c, double* info);
+
+/* Device-resident energy and analytic gradient (EnergyGB/EnergyIB,
+ * src/workproc.f90:1841-1950, 2063-2154): the gradient with respect to the
+ * parameters of functions nfru+1..K, ordered (dE/dvechL_k, dE/dvechB_k)
+ * per function (npt*(K-nfru) doubles).  The per-pair derivatives are
+ * recomputed inside one fused adjoint pass and contracted with the
+ * eigenvector on the device; nothing of size K^2*npt is stored. */
+int ecg_energy_grad(ecg_ctx* ctx, int nfru, int which, double sigma, double tol, int maxiter, double* E,
+                    double* grad, double* info);
+
+/* Device time (ms, HIP events on the context's stream) of the phases of
+ * the last ecg_energy / ecg_energy_grad call: [0] assembly, [1] eigen step
+ * (all factorisations and iterations), [2] adjoint pass, [3] whole call,
+ * [4] adjoint kernel alone, [5] off-diagonal assembly kernel alone.
+ * Returns the number of values written. */
+int ecg_last_timings(ecg_ctx* ctx, double* ms, int nmax);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ECG_H */

@@ -1,0 +1,386 @@
+// C entry points of the CPU oracle (test infrastructure only; see the
+// header of ecg_oracle.hpp).  Matrices are Fortran column-major, complex
+// values are interleaved (re, im) doubles, function indices are 1-based
+// where the reference's are.
+#include <omp.h>
+
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <vector>
+
+#include "ecg_oracle.hpp"
+
+namespace ecgo {
+
+// ------------------------------------------------ operation-counting type
+// Every +, -, *, / and sqrt on this type adds one to the thread's counter
+// while counting is on.  Used only to count the reference algorithm's work.
+struct Fl {
+  double v;
+  Fl() : v(0) {}
+  Fl(double x) : v(x) {}
+};
+thread_local long long g_flops = 0;
+thread_local int g_pause = 0;
+inline void tick() { if (!g_pause) ++g_flops; }
+inline Fl operator+(Fl a, Fl b) { tick(); return Fl(a.v + b.v); }
+inline Fl operator-(Fl a, Fl b) { tick(); return Fl(a.v - b.v); }
+inline Fl operator*(Fl a, Fl b) { tick(); return Fl(a.v * b.v); }
+inline Fl operator/(Fl a, Fl b) { tick(); return Fl(a.v / b.v); }
+inline Fl operator-(Fl a) { return Fl(-a.v); }
+inline bool operator>=(Fl a, Fl b) { return a.v >= b.v; }
+inline bool operator==(Fl a, Fl b) { return a.v == b.v; }
+inline Fl fabs(Fl a) { return Fl(std::fabs(a.v)); }
+inline Fl sqrt(Fl a) { tick(); return Fl(std::sqrt(a.v)); }
+inline Fl hypot(Fl a, Fl b) { tick(); tick(); tick(); tick(); return Fl(std::hypot(a.v, b.v)); }
+inline Fl copysign(Fl a, Fl b) { return Fl(std::copysign(a.v, b.v)); }
+template <> struct Pause<Fl> {
+  Pause() { ++g_pause; }
+  ~Pause() { --g_pause; }
+};
+
+template <class R>
+static System<R> make_sys(int n, int T, const int* P, const int* coef, const double* M, double q0,
+                          const double* q) {
+  System<R> sy;
+  sy.n = n;
+  sy.T = T;
+  for (int i = 0; i < NN; ++i)
+    for (int j = 0; j < NN; ++j) sy.M[i][j] = R((i < n && j < n) ? M[i + n * j] : 0.0);
+  sy.q0 = R(q0);
+  for (int i = 0; i < NN; ++i) sy.q[i] = R(i < n ? q[i] : 0.0);
+  sy.two3n2 = R(std::pow(2.0, (3 * n) / 2.0));
+  sy.P = P;
+  sy.coef = coef;
+  return sy;
+}
+
+}  // namespace ecgo
+
+using namespace ecgo;
+typedef cx<double> cd;
+
+static inline cd ld(const double* a, long i) { return cd(a[2 * i], a[2 * i + 1]); }
+static inline void st(double* a, long i, cd v) { a[2 * i] = v.re; a[2 * i + 1] = v.im; }
+
+extern "C" {
+
+// Number of Y^dagger Y-summed pair elements of MatrixElements-work counted
+// for one (pair, term) in the reference algorithm (src/matelem.f90:12-750),
+// excluding per-function set-up and integer permutation products.
+long long oracle_flops_per_pair(int n, const int* P, const double* M, double q0, const double* q,
+                                const double* pk, const double* pl, int gradflag) {
+  int coef = 1;
+  System<Fl> sy = make_sys<Fl>(n, 1, P, &coef, M, q0, q);
+  Fl pkf[2 * NNP], plf[2 * NNP];
+  for (int i = 0; i < n * (n + 1); ++i) { pkf[i] = Fl(pk[i]); plf[i] = Fl(pl[i]); }
+  cx<Fl> H, S, Dk[4 * NNP], Dl[4 * NNP];
+  g_flops = 0;
+  g_pause = 0;
+  matrix_elements(sy, pkf, plf, P, gradflag, H, S, Dk, Dl);
+  return g_flops;
+}
+
+// One pair summed over the T terms (MatrixElements x YHY coefficients).
+// H, S: 1 complex each; Dk, Dl: 4*np complex each (may be null when the
+// gradflag does not ask for them).
+void oracle_pair(int n, int T, const int* P, const int* coef, const double* M, double q0,
+                 const double* q, const double* pk, const double* pl, int gradflag, double* H,
+                 double* S, double* Dk, double* Dl) {
+  System<double> sy = make_sys<double>(n, T, P, coef, M, q0, q);
+  cd h, s, dk[4 * NNP], dl[4 * NNP];
+  pair_sum(sy, pk, pl, gradflag, h, s, dk, dl);
+  st(H, 0, h);
+  st(S, 0, s);
+  const int np4 = 2 * n * (n + 1);
+  if (Dk && (gradflag == 1 || gradflag == 3))
+    for (int m = 0; m < np4; ++m) st(Dk, m, dk[m]);
+  if (Dl && (gradflag == 2 || gradflag == 3))
+    for (int m = 0; m < np4; ++m) st(Dl, m, dl[m]);
+}
+
+// Throughput probe for the CPU baseline: evaluates the pairs (kidx[i],
+// lidx[i]) (0-based, bra k >= ket l) with gradflag 3 (1 on the diagonal),
+// summed over the T terms, on nthreads OpenMP threads.  Returns the sum of
+// Re(H) over the pairs (so the work cannot be optimised away).
+double oracle_pair_batch(int n, int T, const int* P, const int* coef, const double* M, double q0,
+                         const double* q, const double* params, long npairs, const int* kidx,
+                         const int* lidx, int nthreads) {
+  System<double> sy = make_sys<double>(n, T, P, coef, M, q0, q);
+  const int npt = n * (n + 1);
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  double acc = 0.0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : acc)
+  for (long i = 0; i < npairs; ++i) {
+    cd h, s, dk[4 * NNP], dl[4 * NNP];
+    const int k = kidx[i], l = lidx[i];
+    pair_sum(sy, params + (long)npt * k, params + (long)npt * l, k == l ? 1 : 3, h, s, dk, dl);
+    acc += h.re + dk[0].re + (k == l ? 0.0 : dl[0].re);
+  }
+  return acc;
+}
+
+// ComputeMatElem(Nmin, Nmax) + StoreHS (src/matform.f90:11-51, 119-230) for
+// method 'G' or 'I'.  params: npt x K column-major (column k = function k).
+// H, S: ld x ld complex column-major; diagH, diagS: K reals.  The rows
+// nmin..nmax (1-based) are (re)computed; diagS of rows < nmin must already
+// be present.  Diagonals are stored first, then the off-diagonal elements,
+// which reproduces the reference's store order dependence.
+int oracle_assemble(int n, int T, const int* P, const int* coef, const double* M, double q0,
+                    const double* q, int K, const double* params, int nmin, int nmax, char method,
+                    double sigma, double* H, double* S, int ld, double* diagH, double* diagS,
+                    int nthreads) {
+  if (n < 1 || n > NN || nmin < 1 || nmax > K || nmin > nmax || ld < nmax) return -1;
+  if (method != 'G' && method != 'I') return -2;
+  System<double> sy = make_sys<double>(n, T, P, coef, M, q0, q);
+  const int npt = n * (n + 1);
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int k = nmin; k <= nmax; ++k) {
+    cd h, s;
+    pair_sum(sy, params + (long)npt * (k - 1), params + (long)npt * (k - 1), 0, h, s, (cd*)nullptr, (cd*)nullptr);
+    const long kk = (long)(k - 1) * ld + (k - 1);
+    diagS[k - 1] = s.re;
+    if (method == 'G') {
+      diagH[k - 1] = h.re / diagS[k - 1];
+    } else {
+      st(S, kk, cd(1.0, 0.0));
+      st(H, kk, cd(h.re / diagS[k - 1] - sigma, 0.0));
+    }
+  }
+  const long npairs_rows = nmax - nmin + 1;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (long r = 0; r < npairs_rows; ++r) {
+    const int k = nmin + (int)r;
+    for (int l = k - 1; l >= 1; --l) {
+      cd h, s;
+      pair_sum(sy, params + (long)npt * (k - 1), params + (long)npt * (l - 1), 0, h, s, (cd*)nullptr, (cd*)nullptr);
+      const double f = 1.0 / std::sqrt(diagS[k - 1] * diagS[l - 1]);
+      const long kl = (long)(l - 1) * ld + (k - 1), lk = (long)(k - 1) * ld + (l - 1);
+      if (method == 'G') {
+        st(S, kl, s * f);
+        st(H, kl, h * f);
+      } else {
+        cd sn = s * f;
+        st(S, kl, sn);
+        st(S, lk, conj(sn));
+        st(H, kl, (h - sigma * s) * f);
+      }
+    }
+  }
+  return 0;
+}
+
+// ComputeMatElemAndDeriv(nfru+1, K) + StoreHSD (src/matform.f90:55-115,
+// 234-360).  D: (2*npt) x nfo x ld complex, as Glob_D.
+int oracle_assemble_deriv(int n, int T, const int* P, const int* coef, const double* M, double q0,
+                          const double* q, int K, const double* params, int nfru, char method,
+                          double sigma, double* H, double* S, int ld, double* diagH, double* diagS,
+                          double* D, int nthreads) {
+  if (n < 1 || n > NN || nfru < 0 || nfru >= K || ld < K) return -1;
+  if (method != 'G' && method != 'I') return -2;
+  System<double> sy = make_sys<double>(n, T, P, coef, M, q0, q);
+  const int npt = n * (n + 1), np4 = 2 * npt, nfo = K - nfru;
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  auto Didx = [&](int m, int kf, int j) { return (long)m + (long)np4 * ((long)kf + (long)nfo * j); };
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int k = nfru + 1; k <= K; ++k) {
+    cd h, s, dk[4 * NNP];
+    pair_sum(sy, params + (long)npt * (k - 1), params + (long)npt * (k - 1), 1, h, s, dk, (cd*)nullptr);
+    const long kk = (long)(k - 1) * ld + (k - 1);
+    diagS[k - 1] = s.re;
+    if (method == 'G') {
+      diagH[k - 1] = h.re / diagS[k - 1];
+    } else {
+      st(S, kk, cd(1.0, 0.0));
+      st(H, kk, cd(h.re / diagS[k - 1] - sigma, 0.0));
+    }
+    for (int m = 0; m < np4; ++m) st(D, Didx(m, k - 1 - nfru, k - 1), cd(2.0 * dk[m].re / diagS[k - 1], 0.0));
+  }
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int k = nfru + 1; k <= K; ++k) {
+    for (int l = k - 1; l >= 1; --l) {
+      const int gf = (l > nfru) ? 3 : 1;
+      cd h, s, dk[4 * NNP], dl[4 * NNP];
+      pair_sum(sy, params + (long)npt * (k - 1), params + (long)npt * (l - 1), gf, h, s, dk, dl);
+      const double f = 1.0 / std::sqrt(diagS[k - 1] * diagS[l - 1]);
+      const long kl = (long)(l - 1) * ld + (k - 1), lk = (long)(k - 1) * ld + (l - 1);
+      if (method == 'G') {
+        st(S, kl, s * f);
+        st(H, kl, h * f);
+      } else {
+        cd sn = s * f;
+        st(S, kl, sn);
+        st(S, lk, conj(sn));
+        st(H, kl, (h - sigma * s) * f);
+      }
+      for (int m = 0; m < np4; ++m) st(D, Didx(m, k - 1 - nfru, l - 1), dk[m] * f);
+      if (l > nfru)
+        for (int m = 0; m < np4; ++m) st(D, Didx(m, l - 1 - nfru, k - 1), conj(dl[m] * f));
+    }
+  }
+  return 0;
+}
+
+// Gradient contraction of EnergyGB/EnergyIB (src/workproc.f90:1925-1940):
+// grad((k-1)*npt+m) = 2 Re(conj(c_k') W_m) - |c_k'|^2 (D_H - E D_S)(m,k,k')
+// with W_m = sum_l c_l (D_H - E D_S)(m,k,l), k' = k + nfru.
+void oracle_gradient(int npt, int nfa, int nfru, const double* D, const double* c, double E,
+                     double* grad) {
+  const int nfo = nfa - nfru, np4 = 2 * npt;
+  auto Dv = [&](int m, int kf, int j) { return ld(D, (long)m + (long)np4 * ((long)kf + (long)nfo * j)); };
+  for (int k = 0; k < nfo; ++k) {
+    std::vector<cd> W(npt);
+    for (int l = 0; l < nfa; ++l) {
+      cd t = ld(c, l);
+      for (int m = 0; m < npt; ++m) W[m] = W[m] + t * (Dv(m, k, l) - E * Dv(m + npt, k, l));
+    }
+    cd t = ld(c, k + nfru);
+    for (int m = 0; m < npt; ++m) {
+      double g = 2.0 * (conj(t) * W[m]).re;
+      g -= (t * conj(t) * (Dv(m, k, k + nfru) - E * Dv(m + npt, k, k + nfru))).re;
+      grad[(long)k * npt + m] = g;
+    }
+  }
+}
+
+// LDLHF (src/linalg.f90:316-455, single-process branch): update the LDL^H
+// factorisation of the Hermitian matrix held in A's lower triangle from
+// row m (1-based) to n.  L^H goes to the strict upper triangle, 1/D to invD.
+int oracle_ldlhf(int m, int n, double* A, int nA, double* invD) {
+  std::vector<cd> w(n + 1);
+  auto a = [&](int i, int j) { return ld(A, (long)(i - 1) + (long)nA * (j - 1)); };
+  auto sa = [&](int i, int j, cd v) { st(A, (long)(i - 1) + (long)nA * (j - 1), v); };
+  for (int i = m; i <= n; ++i) {
+    for (int j = 1; j <= i; ++j) {
+      const int jm = j - 1;
+      if (i == j) {
+        for (int q = 1; q <= jm; ++q) w[q] = a(q, i) * ld(invD, q - 1);
+        cd x = a(i, j);
+        for (int q = 1; q <= jm; ++q) x = x - conj(a(q, i)) * w[q];
+        x = conj(x);
+        for (int q = 1; q <= jm; ++q) sa(q, i, w[q]);
+        if (x.re == 0.0 && x.im == 0.0) return 1;
+        st(invD, j - 1, cd(1.0) / x);
+      } else {
+        cd x = a(i, j);
+        for (int q = 1; q <= jm; ++q) x = x - conj(a(q, i)) * a(q, j);
+        sa(j, i, conj(x));
+      }
+    }
+  }
+  return 0;
+}
+
+// LDLHS (src/linalg.f90:618-775, plain branch): x = A^{-1} b using L^H in
+// the strict upper triangle of A and invD.
+void oracle_ldlhs(int n, const double* A, int nA, const double* invD, const double* b, double* x) {
+  auto a = [&](int i, int j) { return ld(A, (long)(i - 1) + (long)nA * (j - 1)); };
+  std::vector<cd> v(n + 1);
+  for (int j = 1; j <= n; ++j) v[j] = ld(b, j - 1);
+  for (int j = 1; j <= n; ++j) {
+    cd t = v[j];
+    for (int i = 1; i < j; ++i) t = t - conj(a(i, j)) * v[i];
+    v[j] = t;
+  }
+  for (int j = 1; j <= n; ++j) v[j] = v[j] * ld(invD, j - 1);
+  for (int j = n; j >= 1; --j) {
+    cd t = v[j];
+    for (int i = j - 1; i >= 1; --i) v[i] = v[i] - t * a(i, j);
+  }
+  for (int j = 1; j <= n; ++j) st(x, j - 1, v[j]);
+}
+
+// x^H A x from the lower triangle of A (VMMHMV, src/linalg.f90:1180-1255).
+static double vmmhmv(int n, const double* A, int nA, const std::vector<cd>& x) {
+  double r = 0.0;
+  for (int k = 0; k < n; ++k) {
+    cd akk = ld(A, (long)k + (long)nA * k);
+    r += (x[k].re * x[k].re + x[k].im * x[k].im) * akk.re;
+    double sr = 0.0, si = 0.0;
+    for (int j = k + 1; j < n; ++j) {
+      cd ajk = ld(A, (long)j + (long)nA * k);
+      sr += x[j].re * ajk.re + x[j].im * ajk.im;
+      si += x[j].im * ajk.re - x[j].re * ajk.im;
+    }
+    r += 2.0 * (x[k].re * sr + x[k].im * si);
+  }
+  return r;
+}
+
+// GHEPIIS (src/linalg.f90:1781-1926): inverse iteration for the eigenvalue
+// of (A, B) nearest apprlambda, with M holding A - apprlambda*B in its lower
+// triangle and the known L^H rows < k in its upper triangle.  B is full.
+// Returns ErrorCode (0, 1 singular, 2 not converged).  out3 = (lambda,
+// RelAcc, NumIter).
+int oracle_ghepiis(int k, int n, double* M, int nM, double* invD, const double* B, int nB,
+                   double apprlambda, double* v, double Tol, int MaxIter, int SpecifNorm,
+                   double* x_out, double* out3) {
+  int err = oracle_ldlhf(k, n, M, nM, invD);
+  out3[0] = 0.0; out3[1] = HUGE_VAL; out3[2] = 0.0;
+  if (err) return err;
+  const double sqrtn = std::sqrt(2.0 * n);
+  std::vector<cd> vv(n), w(n), x(n);
+  for (int i = 0; i < n; ++i) vv[i] = ld(v, i);
+  std::vector<double> wb(2 * n), xb(2 * n);
+  bool notconv = true;
+  int it = 0;
+  double nd = HUGE_VAL, ndprev = HUGE_VAL;
+  while (notconv && it < MaxIter) {
+    if (it != 0) vv = x;
+    // w = B^H v (MHMV)
+    for (int i = 0; i < n; ++i) {
+      cd t;
+      for (int j = 0; j < n; ++j) t = t + conj(ld(B, (long)j + (long)nB * i)) * vv[j];
+      w[i] = t;
+    }
+    for (int i = 0; i < n; ++i) { wb[2 * i] = w[i].re; wb[2 * i + 1] = w[i].im; }
+    oracle_ldlhs(n, M, nM, invD, wb.data(), xb.data());
+    for (int i = 0; i < n; ++i) x[i] = cd(xb[2 * i], xb[2 * i + 1]);
+    double t1 = 0.0;
+    for (int i = 0; i < n; ++i) {
+      t1 = std::fmax(t1, std::fabs(x[i].re));
+      t1 = std::fmax(t1, std::fabs(x[i].im));
+    }
+    for (int i = 0; i < n; ++i) x[i] = x[i] * cd(1.0 / t1, 0.0);
+    cd a;
+    double r = 0.0;
+    for (int i = 0; i < n; ++i) {
+      a = a + conj(x[i]) * vv[i];
+      r += vv[i].re * vv[i].re + vv[i].im * vv[i].im;
+    }
+    cd tc(a.re / r, a.im / r);
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) {
+      cd d = x[i] - tc * vv[i];
+      s += d.re * d.re + d.im * d.im;
+    }
+    nd = sqrtn * std::sqrt(s);
+    if (Tol > 0.0) {
+      if (nd <= Tol) notconv = false;
+    } else {
+      if (nd > ndprev && nd <= std::fabs(Tol)) notconv = false;
+      ndprev = nd;
+    }
+    ++it;
+  }
+  if (notconv) err = 2;
+  const double t1 = vmmhmv(n, B, nB, x);
+  const double t2 = vmmhmv(n, M, nM, x);
+  const double lambda = t2 / t1 + apprlambda;
+  if (SpecifNorm == 0) {
+    for (int i = 0; i < n; ++i) x[i] = x[i] * cd(1.0 / std::sqrt(t1), 0.0);
+  } else if (SpecifNorm == 1) {
+    double q = 0.0;
+    for (int i = 0; i < n; ++i) q += x[i].re * x[i].re + x[i].im * x[i].im;
+    for (int i = 0; i < n; ++i) x[i] = x[i] * cd(1.0 / std::sqrt(q), 0.0);
+  }
+  for (int i = 0; i < n; ++i) st(x_out, i, x[i]);
+  out3[0] = lambda;
+  out3[1] = nd;
+  out3[2] = it;
+  return err;
+}
+
+}  // extern "C"

@@ -1,0 +1,299 @@
+"""GPU parity: the HIP path through the C-ABI against the oracle.
+
+Each test prints the case it runs (system, K, seed, entry point) before the
+GPU call, so a failing run names its case.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from ecg_nonbo import named_system, read_inout, seeded_basis, SEED_BASE
+from ecg_nonbo.hip import EcgContext, ECG_OK, ECG_NOT_CONVERGED
+from helpers import pair_sensitivity, close
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.mark.parametrize("name", ["hminus", "h2", "li"])
+def test_matrix_elements_single_pair(name):
+    sy = named_system(name)
+    B = seeded_basis(sy.n, 12, SEED_BASE + 101)
+    say("ecg_matrix_elements", name)
+    ctx = EcgContext(sy)
+    worst = 0.0
+    for t in range(sy.T):
+        P = sy.yhy_mats[t]
+        for i in range(0, 12, 2):
+            for j in range(1, 12, 3):
+                (H0, S0, Dk0, Dl0), sens, mag = pair_sensitivity(sy, B[i], B[j], 3, term=t)
+                H, S, Dk, Dl = ctx.MatrixElements(B[i], B[j], P, 3)
+                for q, (got, want) in enumerate(((H, H0), (S, S0), (Dk, Dk0), (Dl, Dl0))):
+                    ok, d, tol = close(got, want, mag[q], sens[q])
+                    assert ok, (name, t, i, j, q, d, tol)
+                    worst = max(worst, d / max(mag[q], 1e-300))
+    print("worst relative deviation", worst)
+    ctx.close()
+
+
+def test_matrix_elements_gradflag_subsets():
+    sy = named_system("h2")
+    B = seeded_basis(sy.n, 4, SEED_BASE + 7)
+    ctx = EcgContext(sy)
+    say("ecg_matrix_elements gradflag subsets")
+    H3, S3, Dk3, Dl3 = ctx.MatrixElements(B[0], B[1], sy.yhy_mats[2], 3)
+    H1, S1, Dk1, _ = ctx.MatrixElements(B[0], B[1], sy.yhy_mats[2], 1)
+    H2, S2, _, Dl2 = ctx.MatrixElements(B[0], B[1], sy.yhy_mats[2], 2)
+    H0, S0, _, _ = ctx.MatrixElements(B[0], B[1], sy.yhy_mats[2], 0)
+    assert H0 == H1 == H2 == H3 and S0 == S1 == S2 == S3
+    np.testing.assert_array_equal(Dk1, Dk3)
+    np.testing.assert_array_equal(Dl2, Dl3)
+    ctx.close()
+
+
+def _assembled_tolerances(sy, params, H0, S0, diagS):
+    """Per-element error bars for the normalised lower triangles: the pair's
+    own rounding sensitivity, plus the sensitivity of the normalisation
+    f = 1/sqrt(S_kk S_ll) carried by the diagonal elements."""
+    K = params.shape[0]
+    tolH = np.zeros((K, K))
+    tolS = np.zeros((K, K))
+    rel_diag = np.zeros(K)
+    dh = np.zeros(K)
+    for k in range(K):
+        (h, s, _, _), sens, mag = pair_sensitivity(sy, params[k], params[k], 0, seed=k * K + k)
+        rel_diag[k] = max(1e-12, 16 * sens[1] / abs(s))
+        dh[k] = abs(h) / abs(s)
+    for k in range(K):
+        for l in range(k + 1):
+            (h, s, _, _), sens, mag = pair_sensitivity(sy, params[k], params[l], 0, seed=k * K + l)
+            f = 1.0 / np.sqrt(diagS[k] * diagS[l])
+            fn = 0.5 * (rel_diag[k] + rel_diag[l])
+            tolH[k, l] = max(1e-12 * mag[0], 16 * sens[0]) * f + fn * abs(h) * f
+            tolS[k, l] = max(1e-12 * mag[1], 16 * sens[1]) * f + fn * abs(s) * f
+    # absolute floor: a few ulps of the normalised matrices' natural scale
+    # (|S_kl| <= 1; |H_kl| <= sqrt(|H_kk H_ll|) up to the Cauchy-Schwarz bound)
+    tolS += 4 * np.finfo(float).eps
+    tolH += 4 * np.finfo(float).eps * np.sqrt(np.outer(dh, dh))
+    np.fill_diagonal(tolS, rel_diag)
+    return tolH, tolS
+
+
+@pytest.mark.parametrize("name,K,method", [("hminus", 48, "G"), ("h2", 40, "G"), ("li", 24, "I"),
+                                          ("h2", 33, "I")])
+def test_compute_mat_elem(name, K, method):
+    sy = named_system(name)
+    params = seeded_basis(sy.n, K, SEED_BASE + 200 + K)
+    sigma = -1.1
+    say("ecg_compute_mat_elem", name, K, method)
+    ctx = EcgContext(sy)
+    ctx.set_params(params)
+    H, S, dH, dS = ctx.ComputeMatElem(1, K, method, sigma)
+    Ho, So, dHo, dSo = oracle.assemble(sy, params, method, sigma)
+    tolH, tolS = _assembled_tolerances(sy, params, Ho, So, dSo)
+    dtol = np.diag(tolS) * dSo          # diagonal: relative error bar of S_kk
+    assert np.all(np.abs(dS - dSo) <= dtol), np.max(np.abs(dS - dSo) / dtol)
+    if method == "G":
+        assert np.all(np.abs(dH - dHo) <= np.diag(tolH) + np.abs(dHo) * np.diag(tolS)), "diagH"
+    low = np.tril_indices(K, -1)
+    dHv = np.abs(H - Ho)[low]
+    dSv = np.abs(S - So)[low]
+    tol_shift = 0.0 if method == "G" else abs(sigma) * tolS[low]
+    assert np.all(dSv <= tolS[low]), np.max(dSv / tolS[low])
+    assert np.all(dHv <= tolH[low] + tol_shift), np.max(dHv / (tolH[low] + tol_shift))
+    if method == "I":
+        np.testing.assert_allclose(np.diag(H).real, np.diag(Ho).real, rtol=1e-12, atol=1e-12)
+        np.testing.assert_array_equal(np.diag(S), np.ones(K))
+        np.testing.assert_allclose(S, S.conj().T, rtol=0, atol=0)
+    ctx.close()
+
+
+def test_compute_mat_elem_incremental_rows():
+    """Rows nmin..nmax only, with diagS of earlier rows supplied by the
+    caller, as in the candidate loops of the reference."""
+    sy = named_system("h2")
+    K = 30
+    params = seeded_basis(sy.n, K, SEED_BASE + 301)
+    say("ecg_compute_mat_elem incremental", K)
+    ctx = EcgContext(sy)
+    ctx.set_params(params)
+    Ho, So, dHo, dSo = oracle.assemble(sy, params, "G")
+    H = np.zeros((K, K), np.complex128, order="F")
+    S = np.zeros((K, K), np.complex128, order="F")
+    dH = np.zeros(K)
+    dS = np.zeros(K)
+    dS[:25] = dSo[:25]
+    ctx.ComputeMatElem(26, K, "G", 0.0, H, S, dH, dS)
+    assert np.all(H[:25, :] == 0)     # rows < nmin untouched
+    tolH, tolS = _assembled_tolerances(sy, params, Ho, So, dSo)
+    assert np.all(np.abs(dS[25:] - dSo[25:]) <= np.diag(tolS)[25:] * dSo[25:])
+    for k in range(25, K):
+        for l in range(k):
+            assert abs(H[k, l] - Ho[k, l]) <= tolH[k, l]
+            assert abs(S[k, l] - So[k, l]) <= tolS[k, l]
+    ctx.close()
+
+
+@pytest.mark.parametrize("name,K,nfru", [("hminus", 20, 0), ("h2", 16, 0), ("h2", 18, 7), ("li", 12, 3)])
+def test_compute_mat_elem_and_deriv(name, K, nfru):
+    sy = named_system(name)
+    params = seeded_basis(sy.n, K, SEED_BASE + 400 + K + nfru)
+    say("ecg_compute_mat_elem_and_deriv", name, K, nfru)
+    ctx = EcgContext(sy)
+    ctx.set_params(params)
+    # rows <= nfru come from an earlier assembly, as in the reference
+    Hp, Sp, dHp, dSp = oracle.assemble(sy, params, "G")
+    H, S, dH, dS, D = ctx.ComputeMatElemAndDeriv(nfru, "G", 0.0, Hp.copy(order="F"), Sp.copy(order="F"),
+                                                  dHp.copy(), dSp.copy())
+    Ho, So, dHo, dSo, Do = oracle.assemble_deriv(sy, params, nfru, "G", 0.0, 0, Hp.copy(order="F"),
+                                                 Sp.copy(order="F"), dHp.copy(), dSp.copy())
+    assert np.all(np.abs(dS - dSo) <= 1e-9 * dSo)
+    # per (function, partner) column of D: error bar from the pair's sensitivity
+    npt = sy.npt
+    dscale = np.max(np.abs(Do))
+    rel_diag = np.zeros(K)
+    for a in range(K):
+        (_, s, _, _), sens, _ = pair_sensitivity(sy, params[a], params[a], 0, seed=a)
+        rel_diag[a] = max(1e-12, 16 * sens[1] / abs(s))
+    for kf in range(K - nfru):
+        k = kf + nfru
+        for j in range(K):
+            a, b = (k, j) if j <= k else (j, k)
+            _, sens, mag = pair_sensitivity(sy, params[a], params[b], 3, seed=a * K + b)
+            err = np.max(np.abs(D[:, kf, j] - Do[:, kf, j]))
+            f = (2.0 if j == k else 1.0) / np.sqrt(dSo[a] * dSo[b])
+            q = 2 if j <= k else 3
+            tol = (max(1e-12 * mag[q], 16 * sens[q]) * f + 4 * np.finfo(float).eps * dscale
+                   + 0.5 * (rel_diag[a] + rel_diag[b]) * np.max(np.abs(Do[:, kf, j])))
+            assert err <= tol, (k, j, err, tol)
+    ctx.close()
+
+
+def _fixture(golden_dir, name):
+    import os
+    return read_inout(os.path.join(golden_dir, "hdplus", name + ".txt"))
+
+
+@pytest.mark.parametrize("fname", ["inout_1_500", "inout_2_500", "inout_1_1000", "inout_2_1500"])
+def test_fixture_energy_with_shift(golden_dir, fname):
+    """Known answers of the reference (CURRENT_ENERGY line of each file):
+    E at the shift CURRENT_ENERGY * INVITPARAMETER, as the reference's
+    method 'I' does."""
+    f = _fixture(golden_dir, fname)
+    say("ecg_energy shifted", fname, f.basis_size)
+    ctx = EcgContext(f.system)
+    ctx.set_params(f.params)
+    sigma = f.current_energy * f.invit_parameter
+    rc, E, c, info = ctx.energy(f.which_eigenvalue, sigma, f.eigval_tolerance, 20)
+    print(fname, "E", repr(E), "file", repr(f.current_energy), "diff", E - f.current_energy, "info", info[:5])
+    assert abs(E - f.current_energy) <= 1e-10
+    assert int(info[4]) == f.which_eigenvalue - 1     # inertia below the shift
+    ctx.close()
+
+
+@pytest.mark.parametrize("fname", ["inout_1_500", "inout_2_500"])
+def test_fixture_energy_root_search(golden_dir, fname):
+    """No shift given: the root WHICH_EIGENVALUE is located from inertia
+    counts (the ZHEGVX RANGE='I' use of the reference)."""
+    f = _fixture(golden_dir, fname)
+    say("ecg_energy search", fname, f.basis_size, f.which_eigenvalue)
+    ctx = EcgContext(f.system)
+    ctx.set_params(f.params)
+    rc, E, c, info = ctx.energy(f.which_eigenvalue, float("nan"), 1e-12, 40)
+    print(fname, "E", repr(E), "diff", E - f.current_energy, "info", info[:5])
+    assert abs(E - f.current_energy) <= 1e-10
+    ctx.close()
+
+
+def test_ghepiis_dropin(golden_dir):
+    f = _fixture(golden_dir, "inout_1_500")
+    K = f.basis_size
+    sigma = f.current_energy * f.invit_parameter
+    Ho, So, _, _ = oracle.assemble(f.system, f.params, "I", sigma)
+    M1 = Ho.copy(order="F")
+    inv1 = np.zeros(K, np.complex128)
+    err1, lam1, x1, _, _ = oracle.ghepiis(1, M1, inv1, So.copy(order="F"), sigma, np.ones(K, np.complex128),
+                                          1e-12)
+    say("ecg_ghepiis", K)
+    ctx = EcgContext(f.system)
+    M2 = Ho.copy(order="F")
+    inv2 = np.zeros(K, np.complex128)
+    err2, lam2, x2, rel2, nit2 = ctx.GHEPIIS(1, M2, inv2, So.copy(order="F"), sigma,
+                                             np.ones(K, np.complex128), 1e-12)
+    print("lambda", lam1, lam2, lam2 - lam1, "iters", nit2, "err", err2)
+    assert abs(lam2 - lam1) <= 1e-12
+    assert abs(lam2 - f.current_energy) <= 1e-10
+    # same eigenvector up to a phase, normalised x^H S x = 1
+    ph = np.vdot(x1, x2)
+    ph /= abs(ph)
+    assert np.max(np.abs(x2 - ph * x1)) <= 1e-6 * np.max(np.abs(x1))
+    # the factor: invD and L^H in the upper triangle
+    np.testing.assert_allclose(inv2.real, inv1.real, rtol=1e-8)
+    iu = np.triu_indices(K, 1)
+    np.testing.assert_allclose(M2[iu], M1[iu], rtol=1e-6, atol=1e-9)
+    ctx.close()
+
+
+def _oracle_energy_grad(sy, params, sigma, nfru=0):
+    """E and gradient the reference way: stored D, inverse iteration at
+    sigma, EnergyGB contraction."""
+    K = params.shape[0]
+    H, S, dH, dS, D = oracle.assemble_deriv(sy, params, nfru, "I", sigma)   # rows <= nfru assembled first
+    err, lam, x, _, _ = oracle.ghepiis(1, H.copy(order="F"), np.zeros(K, np.complex128), S.copy(order="F"),
+                                       sigma, np.ones(K, np.complex128), 1e-13, 40)
+    g = oracle.gradient(D, x, lam, nfru)
+    return lam, g
+
+
+@pytest.mark.parametrize("name,K,nfru", [("hminus", 40, 0), ("h2", 36, 0), ("h2", 30, 12), ("li", 24, 0)])
+def test_energy_grad_vs_oracle(name, K, nfru):
+    sy = named_system(name)
+    params = seeded_basis(sy.n, K, SEED_BASE + 500 + K)
+    say("ecg_energy_grad", name, K, nfru)
+    ctx = EcgContext(sy)
+    ctx.set_params(params)
+    rc, E0, _, info0 = ctx.energy(1, float("nan"), 1e-13, 60)
+    sigma = E0 * (1 + 1e-7) if E0 < 0 else E0 * (1 - 1e-7)
+    rc, E, g, info = ctx.energy_grad(nfru, 1, sigma, 1e-13, 40)
+    Eo, go = _oracle_energy_grad(sy, params, sigma, nfru)
+    print(name, "E", E, Eo, E - Eo, "max|g|", np.max(np.abs(go)), "max dev", np.max(np.abs(g - go)))
+    assert abs(E - Eo) <= 1e-10 * max(1.0, abs(Eo))
+    np.testing.assert_allclose(g, go, rtol=1e-7, atol=1e-8 * np.max(np.abs(go)))
+    ctx.close()
+
+
+def test_energy_grad_finite_differences():
+    """Central differences of E on random parameters (the reference's
+    validation method, src/workproc.f90:5766-5826; relative step 1e-4 |p|
+    with a floor, so that the eigen step's rounding does not dominate)."""
+    sy = named_system("hminus")
+    K = 30
+    params = seeded_basis(sy.n, K, SEED_BASE + 600)
+    say("energy_grad FD", K)
+    ctx = EcgContext(sy)
+    ctx.set_params(params)
+    rc, E0, _, _ = ctx.energy(1, float("nan"), 1e-13, 60)
+    sigma = E0 * (1 + 1e-6)
+    rc, E, g, _ = ctx.energy_grad(0, 1, sigma, 1e-13, 40)
+    rng = np.random.default_rng(5)
+    idx = rng.choice(params.size, 12, replace=False)
+    for q in idx:
+        k, m = divmod(int(q), sy.npt)
+        h = 1e-4 * max(abs(params[k, m]), 1e-2)
+        p1 = params.copy()
+        p1[k, m] += h
+        p2 = params.copy()
+        p2[k, m] -= h
+        ctx.set_params(p1)
+        _, e1, _, _ = ctx.energy(1, sigma, 1e-13, 40)
+        ctx.set_params(p2)
+        _, e2, _, _ = ctx.energy(1, sigma, 1e-13, 40)
+        fd = (e1 - e2) / (2 * h)
+        an = g[k * sy.npt + m]
+        assert abs(fd - an) <= 1e-6 * max(1.0, abs(an)) + 1e-12 / h, (k, m, fd, an)
+    ctx.close()
