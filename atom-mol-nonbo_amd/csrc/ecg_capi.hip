@@ -80,7 +80,7 @@ int ensure_capacity(ecg_ctx* x, int K) {
   RCCHK(dev_alloc(&f.F, (size_t)ld * ld));
   RCCHK(dev_alloc(&f.D, (size_t)ld));
   RCCHK(dev_alloc(&f.W, (size_t)ld * ecg::NB));
-  RCCHK(dev_alloc(&f.Dinv, (size_t)(ld / ecg::NB) * ecg::NB * ecg::NB));
+  RCCHK(dev_alloc(&f.Dinv, (size_t)(ld / ecg::NB) * 2 * ecg::NB * ecg::NB));
   RCCHK(dev_alloc(&f.status, (size_t)4));
   RCCHK(dev_alloc(&f.vec, (size_t)4 * ld));
   RCCHK(dev_alloc(&f.tvec, (size_t)ld));

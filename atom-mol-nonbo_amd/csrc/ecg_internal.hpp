@@ -62,7 +62,7 @@ struct Factor {
   zd* F = nullptr;      // ld x ld: L (unit lower, strict part) after factorisation
   double* D = nullptr;  // K pivots
   zd* W = nullptr;      // ld x NB panel scratch
-  zd* Dinv = nullptr;   // nblocks x NB x NB inverses of the diagonal blocks of L
+  zd* Dinv = nullptr;   // nblocks x 2 x NB x NB: inverse X of each diagonal block of L, then X^H
   int* status = nullptr;     // [0] negative pivots, [1] first zero pivot (1-based) or 0,
                              // [2] triangular-solve wait limit exceeded
   zd* vec = nullptr;    // 4 x ld work vectors

@@ -134,7 +134,7 @@ __device__ inline void pair_sum_energy(const SysConst& sc, const double* br, con
 // S_kk, H_kk of rows r0..r1-1; diagS, diagH and the unit/normalised
 // diagonal of the device H, S.
 template <int N>
-__global__ void k_diag(const SysConst sc, int r0, int r1, int ld, const double* __restrict__ bra,
+__global__ void __launch_bounds__(128) k_diag(const SysConst sc, int r0, int r1, int ld, const double* __restrict__ bra,
                        const double* __restrict__ ket, double* __restrict__ diagS,
                        double* __restrict__ diagH, zd* __restrict__ Hm, zd* __restrict__ Sm) {
   const int k = r0 + blockIdx.x * blockDim.x + threadIdx.x;
@@ -258,7 +258,7 @@ __device__ inline void pair_sum_deriv(const SysConst& sc, const double* br, cons
 // Rows k in [nfru, K) with all l <= k; D layout (2 npt) x nfo x ld complex,
 // as Glob_D (src/globvars.f90:211-227, src/matform.f90:55-115).
 template <int N>
-__global__ void k_deriv(const SysConst sc, int K, int nfru, int ld, const double* __restrict__ bra,
+__global__ void __launch_bounds__(128) k_deriv(const SysConst sc, int K, int nfru, int ld, const double* __restrict__ bra,
                         const double* __restrict__ ket, const double* __restrict__ diagS,
                         zd* __restrict__ Hm, zd* __restrict__ Sm, zd* __restrict__ D) {
   constexpr int m = N * (N + 1) / 2, np4 = 4 * m;

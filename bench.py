@@ -10,7 +10,7 @@ K = 2000 seeded complex ECGs (numpy PCG64, seed 20261203 + 3; L_ii =
   -> fused adjoint pass: dE/d(vech L_k, vech B_k) for all 12 K parameters.
 Each step starts from new parameters (every parameter scaled by 1 + 1e-7 u,
 u ~ U(-1,1), as an optimiser step would) and uses the previous step's
-energy for the shift (sigma = E_prev - 1e-4 |E_prev|, the reference's
+energy for the shift (sigma = E_prev - 1e-6 |E_prev|, the reference's
 method 'I' with its Glob_ApproxEnergy); nothing of a previous step's H, S,
 factor or gradient is reused.
 
@@ -119,7 +119,7 @@ def main():
     ctx.set_params(base)
     # cold start (untimed): locate the lowest root without a shift
     st, E, _, info = ctx.energy(1, float("nan"), 1e-12, 60)
-    sigma = E - 1e-4 * abs(E)
+    sigma = E - 1e-6 * abs(E)
     print("# cold start E=%.15f factorisations=%d iterations=%d" % (E, info[3], info[0]), file=sys.stderr,
           flush=True)
 
@@ -128,7 +128,7 @@ def main():
         p = base * (1.0 + 1e-7 * rng.uniform(-1, 1, base.shape))
         ctx.set_params(p)
         st, e, g, inf = ctx.energy_grad(0, 1, sigma, 1e-12, 30)
-        sigma = e - 1e-4 * abs(e)
+        sigma = e - 1e-6 * abs(e)
         return e, g, inf
 
     for i in range(args.warmup):

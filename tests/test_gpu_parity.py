@@ -107,7 +107,7 @@ def test_compute_mat_elem(name, K, method):
     assert np.all(dSv <= tolS[low]), np.max(dSv / tolS[low])
     assert np.all(dHv <= tolH[low] + tol_shift), np.max(dHv / (tolH[low] + tol_shift))
     if method == "I":
-        np.testing.assert_allclose(np.diag(H).real, np.diag(Ho).real, rtol=1e-12, atol=1e-12)
+        assert np.all(np.abs(np.diag(H).real - np.diag(Ho).real) <= np.diag(tolH) + 1e-15 * abs(sigma)), "diag"
         np.testing.assert_array_equal(np.diag(S), np.ones(K))
         np.testing.assert_allclose(S, S.conj().T, rtol=0, atol=0)
     ctx.close()
