@@ -30,25 +30,31 @@ int form_shifted(const Ctx& c, Factor& f, double sigma, hipStream_t s) {
 }
 
 // --------------------------------------------------- diagonal block LDL^H
-// One workgroup of 256 threads factors the jb x jb diagonal block at
-// (j0, j0), held in one 64 x 64 LDS array, blocked by 16:
-//   for each 16-wide sub-block s: wave 0 factors the 16 x 16 diagonal piece
-//   and inverts its unit-lower factor (wave-synchronous, no workgroup
-//   barriers); all waves then form the sub-panel (W = A X_ss^H, L = W D^-1)
-//   and update the trailing part;
-//   afterwards wave s builds block column s of X = L^{-1}:
-//   X_ts = -X_tt sum_{u=s}^{t-1} L_tu X_us.
-// LDS use: lower triangle = the block, then L (pivots on the diagonal while
-// factoring); strict upper triangle = X transposed (X(i, j), i > j, at
-// A[j][i]; X has a unit diagonal), and while factoring, block row 0 of the
-// upper triangle holds the current sub-panel W transposed.
+// One workgroup of 256 threads (4 waves) factors the jb x jb diagonal block
+// at (j0, j0), held in one 64 x 64 LDS array, in 16-wide sub-blocks s:
+//   A. wave 0 factors the 16 x 16 diagonal piece with the piece in
+//      registers (lane = row i, column quad g) and one LDS broadcast of two
+//      pivot rows per pivot pair; the same row operations applied to the
+//      identity give X_ss = L_ss^{-1} (Gauss-Jordan);
+//   B. the sub-panel below: L = A X_ss^H D^{-1}, one 16-row tile per wave
+//      on MFMA;
+//   C. the trailing update A -= (L D) L^H on 16 x 16 lower tiles (MFMA).
+// Then X = L^{-1} is completed by 2 x 2 block recursion on MFMA tiles:
+//   X_10 = -X_11 (L_10 X_00), X_32 = -X_33 (L_32 X_22), then the 32 x 32
+//   block X_21 = -X_22 (L_21 X_11).
+// LDS use: lower triangle = the block, then L (the pivots on the diagonal
+// while factoring); strict upper triangle = X transposed (X(i, j), i > j,
+// at A[j][i]; X has a unit diagonal).
 // Outputs: unit-lower L (strict part, and L^H mirrored into the upper
 // triangle of F), D, and X and X^H (column-major 64 x 64 each, used by the
 // panel solve and the triangular solves).
 constexpr int SB = 16;   // sub-block width
+typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 struct DiagLds {
   zd A[NB][NB + 1];
+  zd rowA[2][2][SB];   // pivot-pair rows broadcast (double-buffered by pair parity)
+  zd rowX[2][2][SB];
   double d[NB];
 };
 
@@ -56,6 +62,51 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// 16 x 16 complex tile product on MFMA: acc(r, c) += sum_{k<16} a(r, k) b(k, c),
+// with a(r, k) and b(k, c) supplied by functors (LDS reads).  Accumulator
+// element v of a lane is (row 4 v + lane / 16, column lane % 16).
+struct Acc16 {
+  dbl4 re, im;
+};
+__device__ inline void acc16_zero(Acc16& a) {
+  a.re = dbl4{0.0, 0.0, 0.0, 0.0};
+  a.im = dbl4{0.0, 0.0, 0.0, 0.0};
+}
+template <class FA, class FB>
+__device__ __attribute__((always_inline)) inline void mfma16(const FA& fa, const FB& fb, Acc16& acc) {
+  const int lane = threadIdx.x & 63, r = lane & 15, k = lane >> 4;
+  zd av[4], bv[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    av[s] = fa(r, 4 * s + k);
+    bv[s] = fb(4 * s + k, r);
+  }
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s].x, bv[s].x, acc.re, 0, 0, 0);
+    acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[s].y, bv[s].y, acc.re, 0, 0, 0);
+    acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s].x, bv[s].y, acc.im, 0, 0, 0);
+    acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s].y, bv[s].x, acc.im, 0, 0, 0);
+  }
+}
+__device__ inline int acc16_row(int v) { return 4 * v + ((threadIdx.x & 63) >> 4); }
+__device__ inline int acc16_col() { return threadIdx.x & 15; }
+
+// X(i, j) of the block from the transposed store (unit diagonal, zero above)
+__device__ inline zd x_at(const DiagLds& L, int i, int j) {
+  const zd v = L.A[j][i];
+  return mkz(i > j ? v.x : (i == j ? 1.0 : 0.0), i > j ? v.y : 0.0);
+}
+
+// X_ts (t > s, 16 x 16) = -X_tt T where T = sum_{u=s}^{t-1} L_tu X_us has
+// been staged at X_ts's own place; result returned in acc (not yet stored).
+__device__ __attribute__((always_inline)) inline void x_block_finish(const DiagLds& L, int t, int s, Acc16& acc) {
+  acc16_zero(acc);
+  const int ti = t * SB, si = s * SB;
+  mfma16([&](int r, int k) { return x_at(L, ti + r, ti + k); },
+         [&](int k, int c) { return L.A[si + c][ti + k]; }, acc);
 }
 
 __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int ld, int j0, zd* __restrict__ F,
@@ -66,9 +117,16 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
   const int jb = min(NB, K - j0);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (!a_in_lds) {
-    for (int e = tid; e < NB * NB; e += blockDim.x) {
-      const int j = e >> 6, i = e & (NB - 1);
-      L.A[i][j] = (i < jb && j < jb && i >= j) ? F[(size_t)(j0 + j) * ld + j0 + i] : mkz(0.0, 0.0);
+    zd v[NB * NB / 256];
+#pragma unroll
+    for (int u = 0; u < NB * NB / 256; ++u) {
+      const int e = tid + 256 * u, j = e >> 6, i = e & (NB - 1);
+      v[u] = (i < jb && j < jb && i >= j) ? F[(size_t)(j0 + j) * ld + j0 + i] : mkz(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < NB * NB / 256; ++u) {
+      const int e = tid + 256 * u;
+      L.A[e & (NB - 1)][e >> 6] = v[u];
     }
   }
   __syncthreads();
@@ -76,131 +134,237 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
   for (int s = 0; s < nsub; ++s) {
     const int c = s * SB;
     const int sb = min(SB, jb - c);
+    // ---- A: 16 x 16 LDL^H and X_ss, wave 0, two pivots (p, q = p + 1) per
+    // broadcast.  With a = A(p,p), b = A(q,p), c' = A(q,q) and
+    // det = a c' - |b|^2: d_p = a, L(q,p) = b / a, d_q = det / a, and both
+    // reciprocals come from one division, r = 1 / (a det): 1/a = det r,
+    // 1/d_q = a^2 r.
     if (wave == 0) {
-      // 16 x 16 LDL^H of A[c.., c..]
-      for (int p = 0; p < sb; ++p) {
-        const double d = L.A[c + p][c + p].x;
-        const double rd = (d != 0.0) ? 1.0 / d : 0.0;
+      const int i = lane & 15, g = lane >> 4;
+      zd a[4], x[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int e = lane + 64 * q;
-          const int i = e >> 4, j = e & (SB - 1);
-          if (j > p && j <= i && i < sb)
-            L.A[c + i][c + j] = zfms(L.A[c + i][c + p] * rd, zconj(L.A[c + j][c + p]), L.A[c + i][c + j]);
-        }
-        wave_sync();
-        if (lane > p && lane < sb) L.A[c + lane][c + p] = L.A[c + lane][c + p] * rd;
-        if (lane == 0) {
-          L.d[c + p] = d;
-          if (d < 0.0) atomicAdd(&status[0], 1);
-          if (d == 0.0) atomicCAS(&status[1], 0, j0 + c + p + 1);
-        }
-        wave_sync();
+      for (int t = 0; t < 4; ++t) {
+        const int j = 4 * g + t;
+        const zd lo = L.A[c + max(i, j)][c + min(i, j)];
+        const bool in = (i < sb && j < sb);
+        const double id = (i == j) ? 1.0 : 0.0;
+        a[t] = in ? mkz(lo.x, j <= i ? lo.y : -lo.y) : mkz(id, 0.0);
+        x[t] = mkz(id, 0.0);
       }
-      // X_ss = inverse of the unit-lower piece, by row operations; X_ss(i, j)
-      // (i > j) lives at A[c + j][c + i], which starts at zero
-      for (int p = 0; p < sb; ++p) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int e = lane + 64 * q;
-          const int i = e >> 4, j = e & (SB - 1);
-          if (i > p && i < sb && j <= p) {
-            const zd a = L.A[c + j][c + p];
-            const zd xpj = mkz(j == p ? 1.0 : a.x, j == p ? 0.0 : a.y);
-            L.A[c + j][c + i] = zfms(L.A[c + i][c + p], xpj, L.A[c + j][c + i]);
+      for (int pp = 0; pp < SB / 2; ++pp) {
+        const int p = 2 * pp, q = p + 1;
+        zd* rP = L.rowA[pp & 1][0];
+        zd* rQ = L.rowA[pp & 1][1];
+        zd* xP = L.rowX[pp & 1][0];
+        zd* xQ = L.rowX[pp & 1][1];
+        if (i == p || i == q) {
+          zd* ra = (i == p) ? rP : rQ;
+          zd* rx = (i == p) ? xP : xQ;
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            ra[4 * g + t] = a[t];
+            rx[4 * g + t] = x[t];
           }
         }
         wave_sync();
+        const double ap = rP[p].x, cq = rQ[q].x;
+        const zd bq = rQ[p];
+        const zd api = rP[i], aqi0 = rQ[i];
+        zd apj[4], aqj[4], xpj[4], xqj[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          apj[t] = rP[4 * g + t];
+          aqj[t] = rQ[4 * g + t];
+          xpj[t] = xP[4 * g + t];
+          xqj[t] = xQ[4 * g + t];
+        }
+        const double det = fma(ap, cq, -fma(bq.x, bq.x, bq.y * bq.y));
+        const double prod = ap * det;
+        double rd1, rd2, dq;
+        if (prod != 0.0) {
+          const double r = 1.0 / prod;
+          rd1 = det * r;
+          rd2 = ap * ap * r;
+          dq = det * rd1;
+        } else {   // a zero pivot: recorded below, its column left unscaled
+          rd1 = (ap != 0.0) ? 1.0 / ap : 0.0;
+          dq = (ap != 0.0) ? det * rd1 : cq;
+          rd2 = (dq != 0.0) ? 1.0 / dq : 0.0;
+        }
+        const zd l = bq * rd1;   // L(q, p)
+        // row q after pivot p: A'(q, .) = A(q, .) - L(q,p) A(p, .), same for X
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          aqj[t] = zfms(l, apj[t], aqj[t]);
+          xqj[t] = zfms(l, xpj[t], xqj[t]);
+        }
+        if (i > q) {
+          const zd lp = zconj(api) * rd1;                   // L(i, p)
+          const zd lq = zconj(zfms(l, api, aqi0)) * rd2;    // L(i, q) = conj(A'(q, i)) / d_q
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const int j = 4 * g + t;
+            if (j > q) a[t] = zfms(lq, aqj[t], zfms(lp, apj[t], a[t]));
+            if (j == p) a[t] = lp;
+            if (j == q) a[t] = lq;
+            x[t] = zfms(lq, xqj[t], zfms(lp, xpj[t], x[t]));
+          }
+        } else if (i == q) {
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            if (4 * g + t == p) a[t] = l;
+            x[t] = xqj[t];
+          }
+        }
+        if (lane == 0) {
+          if (p < sb) {
+            L.d[c + p] = ap;
+            if (ap < 0.0) atomicAdd(&status[0], 1);
+            if (ap == 0.0) atomicCAS(&status[1], 0, j0 + c + p + 1);
+          }
+          if (q < sb) {
+            L.d[c + q] = dq;
+            if (dq < 0.0) atomicAdd(&status[0], 1);
+            if (dq == 0.0) atomicCAS(&status[1], 0, j0 + c + q + 1);
+          }
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int j = 4 * g + t;
+        if (i < sb && j < i) {
+          L.A[c + i][c + j] = a[t];
+          L.A[c + j][c + i] = x[t];
+        }
       }
     }
     __syncthreads();
-    // sub-panel: W(ii, j) = sum_{p <= j} A(c+sb+ii, c+p) conj(X_ss(j, p)), kept at A[j][SB + ii]
     const int rows = jb - c - sb;
-    for (int e = tid; e < rows * SB; e += blockDim.x) {
-      const int ii = e >> 4, j = e & (SB - 1);
-      const int i = c + sb + ii;
-      if (j < sb) {
-        zd w = L.A[i][c + j];
-        for (int p = 0; p < j; ++p) w = zfma(L.A[i][c + p], zconj(L.A[c + p][c + j]), w);
-        L.A[j][SB + ii] = w;
+    const int nt = (rows + SB - 1) / SB;
+    // ---- B: sub-panel L(r, c+j) = sum_{k <= j} A(r, c+k) conj(X_ss(j, k)) / d_j
+    if (wave < nt) {
+      const int r0 = c + SB * (wave + 1);
+      Acc16 acc;
+      acc16_zero(acc);
+      mfma16([&](int r, int k) { return (r0 + r < jb) ? L.A[r0 + r][c + k] : mkz(0.0, 0.0); },
+             [&](int k, int cc) { return zconj(x_at(L, c + cc, c + k)); }, acc);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = r0 + acc16_row(v), cc = acc16_col();
+        if (r < jb) L.A[r][c + cc] = mkz(acc.re[v], acc.im[v]) * (1.0 / L.d[c + cc]);
       }
     }
     __syncthreads();
-    for (int e = tid; e < rows * SB; e += blockDim.x) {
-      const int ii = e >> 4, j = e & (SB - 1);
-      if (j < sb) L.A[c + sb + ii][c + j] = L.A[j][SB + ii] * (1.0 / L.d[c + j]);
-    }
-    __syncthreads();
-    // trailing update: A(i, j) -= sum_p W(i, p) conj(L(j, c+p)), c+sb <= j <= i < jb
-    for (int e = tid; e < rows * rows; e += blockDim.x) {
-      const int ii = e / rows, jj = e - ii * rows;
-      if (jj <= ii) {
-        const int i = c + sb + ii, j = c + sb + jj;
-        zd acc = L.A[i][j];
-        for (int p = 0; p < sb; ++p) acc = zfms(L.A[p][SB + ii], zconj(L.A[j][c + p]), acc);
-        L.A[i][j] = acc;
+    // ---- C: trailing update on lower 16 x 16 tiles (ta >= tb)
+    for (int tt = wave; tt < nt * (nt + 1) / 2; tt += 4) {
+      int ta = 0;
+      while ((ta + 1) * (ta + 2) / 2 <= tt) ++ta;
+      const int tb = tt - ta * (ta + 1) / 2;
+      const int ri = c + SB * (ta + 1), rj = c + SB * (tb + 1);
+      Acc16 acc;
+      acc16_zero(acc);
+      mfma16([&](int r, int k) { return (ri + r < jb) ? L.A[ri + r][c + k] * L.d[c + k] : mkz(0.0, 0.0); },
+             [&](int k, int cc) { return (rj + cc < jb) ? zconj(L.A[rj + cc][c + k]) : mkz(0.0, 0.0); }, acc);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = ri + acc16_row(v), cc = rj + acc16_col();
+        if (r < jb && r >= cc) {
+          const zd o = L.A[r][cc];
+          L.A[r][cc] = mkz(o.x - acc.re[v], o.y - acc.im[v]);
+        }
       }
     }
     __syncthreads();
   }
-  // X = L^{-1}: wave s fills the strict upper block row s (X block column s
-  // transposed).  Only wave s writes block row s, and it reads block row s,
-  // the lower triangle and the diagonal blocks' upper parts.
-  if (wave < nsub) {
-    const int s = wave;
-    for (int t = s + 1; t < nsub; ++t) {
-      zd T[4];
-      // T(i, j) = sum_{k = m}^{t*SB-1} L(t*SB+i, k) X(k, m), m = s*SB + j (X(m, m) = 1)
+  // ---- X = L^{-1}: 2 x 2 block recursion.  T is staged in X's own place.
+  // level 1: X_10 (wave 0) and X_32 (wave 1)
+  {
+    const int t = 2 * wave + 1, s = 2 * wave;
+    const bool act = (wave < 2 && t < nsub);
+    Acc16 acc;
+    if (act) {
+      acc16_zero(acc);
+      mfma16([&](int r, int k) { return L.A[t * SB + r][s * SB + k]; },
+             [&](int k, int cc) { return x_at(L, s * SB + k, s * SB + cc); }, acc);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int e = lane + 64 * q;
-        const int i = e >> 4, j = e & (SB - 1);
-        const int m = s * SB + j, row = t * SB + i;
-        zd acc = L.A[row][m];
-        for (int k = m + 1; k < t * SB; ++k) acc = zfma(L.A[row][k], L.A[m][k], acc);
-        T[q] = acc;
-      }
+      for (int v = 0; v < 4; ++v) L.A[s * SB + acc16_col()][t * SB + acc16_row(v)] = mkz(acc.re[v], acc.im[v]);
+    }
+    wave_sync();
+    if (act) x_block_finish(L, t, s, acc);
+    wave_sync();
+    if (act) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int e = lane + 64 * q;
-        L.A[s * SB + (e & (SB - 1))][t * SB + (e >> 4)] = T[q];
-      }
-      wave_sync();
-      // X_ts(i, j) = -(T(i, j) + sum_{k < i} X_tt(i, k) T(k, j))
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int e = lane + 64 * q;
-        const int i = e >> 4, j = e & (SB - 1);
-        const int m = s * SB + j;
-        zd acc = L.A[m][t * SB + i];
-        for (int k = 0; k < i; ++k) acc = zfma(L.A[t * SB + k][t * SB + i], L.A[m][t * SB + k], acc);
-        T[q] = mkz(-acc.x, -acc.y);
-      }
-      wave_sync();
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int e = lane + 64 * q;
-        L.A[s * SB + (e & (SB - 1))][t * SB + (e >> 4)] = T[q];
-      }
-      wave_sync();
+      for (int v = 0; v < 4; ++v)
+        L.A[s * SB + acc16_col()][t * SB + acc16_row(v)] = mkz(-acc.re[v], -acc.im[v]);
     }
   }
   __syncthreads();
-  // outputs
+  // level 2: X_ts, t in {2, 3}, s in {0, 1}; wave = 2 (t - 2) + s
+  {
+    const int t = 2 + (wave >> 1), s = wave & 1;
+    const bool act = (t < nsub);
+    Acc16 acc;
+    if (act) {
+      // T_ts = sum_{u = s}^{1} L_tu X_us
+      acc16_zero(acc);
+      for (int u = s; u < 2; ++u)
+        mfma16([&](int r, int k) { return L.A[t * SB + r][u * SB + k]; },
+               [&](int k, int cc) { return x_at(L, u * SB + k, s * SB + cc); }, acc);
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) L.A[s * SB + acc16_col()][t * SB + acc16_row(v)] = mkz(acc.re[v], acc.im[v]);
+    }
+    __syncthreads();
+    if (act) {
+      // X_ts = -sum_{u = 2}^{t} X_tu T_us
+      acc16_zero(acc);
+      for (int u = 2; u <= t; ++u)
+        mfma16([&](int r, int k) { return x_at(L, t * SB + r, u * SB + k); },
+               [&](int k, int cc) { return L.A[s * SB + cc][u * SB + k]; }, acc);
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        L.A[s * SB + acc16_col()][t * SB + acc16_row(v)] = mkz(-acc.re[v], -acc.im[v]);
+    }
+  }
+  __syncthreads();
+  // ---- outputs
   zd* Xg = Dinv + (size_t)(j0 / NB) * 2 * NB * NB;
   zd* XHg = Xg + NB * NB;
-  for (int e = tid; e < NB * NB; e += blockDim.x) {
-    const int j = e >> 6, i = e & (NB - 1);       // column-major walk: coalesced writes
-    const bool in = (i < jb && j < jb);
-    if (in && i > j) F[(size_t)(j0 + j) * ld + j0 + i] = L.A[i][j];
-    const zd xu = L.A[j][i], xl = L.A[i][j];
-    const double dg = (in && i == j) ? 1.0 : 0.0;
-    Xg[(size_t)j * NB + i] = (in && i > j) ? xu : mkz(dg, 0.0);
-    XHg[(size_t)j * NB + i] = (in && i < j) ? zconj(xl) : mkz(dg, 0.0);
-  }
-  for (int e = tid; e < NB * NB; e += blockDim.x) {
-    const int i = e >> 6, j = e & (NB - 1);       // L^H into the upper triangle: row-major walk
-    if (i < jb && j < jb && i > j) F[(size_t)(j0 + i) * ld + j0 + j] = zconj(L.A[i][j]);
+  {
+    zd lo[NB * NB / 256], up[NB * NB / 256];
+#pragma unroll
+    for (int u = 0; u < NB * NB / 256; ++u) {
+      const int e = tid + 256 * u, j = e >> 6, i = e & (NB - 1);   // column-major walk: coalesced
+      lo[u] = L.A[i][j];
+      up[u] = L.A[j][i];
+    }
+#pragma unroll
+    for (int u = 0; u < NB * NB / 256; ++u) {
+      const int e = tid + 256 * u, j = e >> 6, i = e & (NB - 1);
+      const bool in = (i < jb && j < jb);
+      if (in && i > j) F[(size_t)(j0 + j) * ld + j0 + i] = lo[u];
+      const double dg = (in && i == j) ? 1.0 : 0.0;
+      Xg[(size_t)j * NB + i] = (in && i > j) ? up[u] : mkz(dg, 0.0);
+      // X^H(i, j) = conj(X(j, i)), X(j, i) (j > i) at A[i][j]
+      XHg[(size_t)j * NB + i] = (in && i < j) ? zconj(lo[u]) : mkz(dg, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < NB * NB / 256; ++u) {
+      const int e = tid + 256 * u, i = e >> 6, j = e & (NB - 1);   // row-major walk: coalesced
+      lo[u] = L.A[i][j];
+    }
+#pragma unroll
+    for (int u = 0; u < NB * NB / 256; ++u) {
+      const int e = tid + 256 * u, i = e >> 6, j = e & (NB - 1);
+      // L^H(j, i) = conj(L(i, j)) goes to F(j0 + j, j0 + i) for i > j
+      if (i < jb && j < jb && i > j) F[(size_t)(j0 + i) * ld + j0 + j] = zconj(lo[u]);
+    }
   }
   for (int p = tid; p < jb; p += blockDim.x) D[j0 + p] = L.d[p];
 }
@@ -219,8 +383,6 @@ __global__ void __launch_bounds__(256) k_ldl_diag(int K, int ld, int j0, zd* __r
 // four real products per complex one.  Accumulator v of a lane holds row
 // 4*v + lane/16, column lane%16 of its 16x16 tile (measured lane layout of
 // the instruction on gfx950).
-typedef double dbl4 __attribute__((ext_vector_type(4)));
-
 struct Tile64 {
   dbl4 re[2][2], im[2][2];
 };
@@ -282,33 +444,46 @@ __device__ inline void tile_pos(int a, int q, int v, int& r, int& c) {
 }
 
 // ------------------------------------------------------------ panel solve
-// Row block i0..i0+ib-1 below the diagonal block: W = A X^H (so that
+// 16-row strip i0..i0+15 below the diagonal block: W = A X^H (so that
 // W L_JJ^H = A), L = W D^{-1}; L goes to F (and L^H to its upper
-// triangle), W to the panel scratch used by the trailing update.
+// triangle), W to the panel scratch used by the trailing update.  One
+// workgroup per strip, wave w computes columns 16w..16w+15 as one 16 x 16
+// MFMA tile (k = 64 in four chunks), which keeps the panel's latency at a
+// single tile's.
 __global__ void __launch_bounds__(256) k_ldl_panel(int K, int ld, int j0, zd* __restrict__ F,
                                                    const double* __restrict__ D,
                                                    const zd* __restrict__ Dinv, zd* __restrict__ W) {
   const int jb = min(NB, K - j0);
-  const int i0 = j0 + jb + blockIdx.x * NB;
-  const int ib = min(NB, K - i0);
-  Tile64 t;
-  mfma_tile_abh(F + (size_t)j0 * ld + i0, ld, ib, Dinv + (size_t)(j0 / NB) * 2 * NB * NB, NB, jb, jb, t);
+  const int i0 = j0 + jb + blockIdx.x * SB;
+  const int wave = threadIdx.x >> 6;
+  const int c0 = SB * wave;
+  if (c0 >= jb) return;
+  const zd* X = Dinv + (size_t)(j0 / NB) * 2 * NB * NB;   // X(i, j) at X[j * NB + i]
+  const zd* A = F + (size_t)j0 * ld + i0;                  // A(r, k) at A[k * ld + r]
+  Acc16 acc;
+  acc16_zero(acc);
+  for (int k0 = 0; k0 < jb; k0 += SB) {
+    mfma16([&](int r, int k) {
+             const int kk = k0 + k;
+             return (i0 + r < K && kk < jb) ? A[(size_t)kk * ld + r] : mkz(0.0, 0.0);
+           },
+           [&](int k, int cc) {
+             const int kk = k0 + k;
+             return (kk < jb) ? zconj(X[(size_t)kk * NB + c0 + cc]) : mkz(0.0, 0.0);
+           },
+           acc);
+  }
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int q = 0; q < 2; ++q)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        int r, c;
-        tile_pos(a, q, v, r, c);
-        if (r < ib && c < jb) {
-          const zd w = mkz(t.re[a][q][v], t.im[a][q][v]);
-          const zd l = w * (1.0 / D[j0 + c]);
-          W[(size_t)c * ld + i0 + r] = w;
-          F[(size_t)(j0 + c) * ld + i0 + r] = l;
-          F[(size_t)(i0 + r) * ld + j0 + c] = zconj(l);
-        }
-      }
+  for (int v = 0; v < 4; ++v) {
+    const int r = i0 + acc16_row(v), c = c0 + acc16_col();
+    if (r < K && c < jb) {
+      const zd w = mkz(acc.re[v], acc.im[v]);
+      const zd l = w * (1.0 / D[j0 + c]);
+      W[(size_t)c * ld + r] = w;
+      F[(size_t)(j0 + c) * ld + r] = l;
+      F[(size_t)r * ld + j0 + c] = zconj(l);
+    }
+  }
 }
 
 // -------------------------------------------------------- trailing update
@@ -372,7 +547,8 @@ int ldlh_factor(Factor& f, hipStream_t s) {
     const int rest = K - (j0 + jb);
     if (rest <= 0) break;
     const int nrb = (rest + NB - 1) / NB;
-    hipLaunchKernelGGL(k_ldl_panel, dim3(nrb), dim3(256), 0, s, K, f.ld, j0, f.F, f.D, f.Dinv, f.W);
+    hipLaunchKernelGGL(k_ldl_panel, dim3((rest + SB - 1) / SB), dim3(256), 0, s, K, f.ld, j0, f.F, f.D, f.Dinv,
+                       f.W);
     hipLaunchKernelGGL(k_ldl_update, dim3(nrb * (nrb + 1) / 2), dim3(256), 0, s, K, f.ld, j0, f.F, f.W, f.D,
                        f.Dinv, f.status);
   }
@@ -523,6 +699,13 @@ __global__ void __launch_bounds__(256) k_trsv_bwd(int K, int ld, const zd* __res
   zd acc = mkz(0.0, 0.0);
   int done = 0;                  // blocks nb-1, nb-2, ... already folded
   const int need = nb - 1 - bl;
+  zd pre[NB / 4];   // prefetched block of L^H for the next dependency
+  if (need > 0 && r < bs) {
+    const int J = nb - 1, js = min(NB, K - J * NB);
+    const zd* col = F + (size_t)(J * NB) * ld + b0 + r;
+#pragma unroll
+    for (int u = 0; u < NB / 4; ++u) pre[u] = (4 * u + g < js) ? col[(size_t)(4 * u + g) * ld] : mkz(0.0, 0.0);
+  }
   while (done < need) {
     if (g == 0) {
       const int m = await_run(fin, nb - 1 - done, -1, need - done, epoch, status);
@@ -535,13 +718,21 @@ __global__ void __launch_bounds__(256) k_trsv_bwd(int K, int ld, const zd* __res
     }
     __syncthreads();
     const int m = run;
-    if (r < bs)
-      for (int t = 0; t < m; ++t) {
-        const int J = nb - 1 - done - t;
-        const int js = min(NB, K - J * NB);
+    if (r < bs) {
+#pragma unroll
+      for (int u = 0; u < NB / 4; ++u) acc = zfma(pre[u], xs[0][4 * u + g], acc);
+      for (int t = 1; t < m; ++t) {
+        const int J = nb - 1 - done - t;   // a full block: only block nb-1 can be partial
         const zd* col = F + (size_t)(J * NB) * ld + b0 + r;
-        for (int c = g; c < js; c += 4) acc = zfma(col[(size_t)c * ld], xs[t][c], acc);
+        for (int c = g; c < NB; c += 4) acc = zfma(col[(size_t)c * ld], xs[t][c], acc);
       }
+      if (done + m < need) {
+        const int J = nb - 1 - done - m;
+        const zd* col = F + (size_t)(J * NB) * ld + b0 + r;
+#pragma unroll
+        for (int u = 0; u < NB / 4; ++u) pre[u] = col[(size_t)(4 * u + g) * ld];
+      }
+    }
     done += m;
     __syncthreads();
   }
