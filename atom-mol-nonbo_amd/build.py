@@ -45,7 +45,32 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    build_fortran()
     return LIB
+
+
+FLANG = "/opt/rocm/lib/llvm/bin/flang"
+FDIR = os.path.join(HERE, "fortran")
+FBUILD = os.path.join(FDIR, "_build")
+DRIVER_SRC = os.path.join(os.path.dirname(HERE), "tests", "fortran", "drive_fixture.f90")
+DRIVER = os.path.join(os.path.dirname(HERE), "tests", "fortran", "drive_fixture")
+
+
+def build_fortran():
+    """Compile the Fortran host layer (fortran/ecg_gpu.f90) and the test
+    driver that links it against libecg_hip.so.  Skipped (returns None)
+    when no Fortran compiler is installed."""
+    fc = os.environ.get("FC", FLANG)
+    if not os.path.exists(fc):
+        return None
+    os.makedirs(FBUILD, exist_ok=True)
+    obj = os.path.join(FBUILD, "ecg_gpu.o")
+    subprocess.run([fc, "-O2", "-c", os.path.join(FDIR, "ecg_gpu.f90"), "-module-dir", FBUILD, "-o", obj],
+                   check=True)
+    if os.path.exists(DRIVER_SRC):
+        subprocess.run([fc, "-O2", "-I", FBUILD, DRIVER_SRC, obj, "-L" + HERE, "-lecg_hip",
+                        "-Wl,-rpath,$ORIGIN/../../atom-mol-nonbo_amd", "-o", DRIVER], check=True)
+    return obj
 
 
 if __name__ == "__main__":

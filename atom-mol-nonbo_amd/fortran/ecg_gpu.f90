@@ -1,0 +1,285 @@
+! Fortran host layer of the MI355X ECG path (libecg_hip.so).
+!
+! Three modules:
+!   ecg_capi     ISO_C_BINDING interface blocks for every entry point of
+!                include/ecg.h (plain pointers, sizes and scalars).
+!   ecg_globals  the part of the reference's global state (src/globvars.f90)
+!                that the path reads and writes, with the reference's names
+!                and types.  Inside ATOM-MOL-nonBO this module is replaced by
+!                "use globvars" (see the line marked GLOBALS below).
+!   ecg_hostpath the routines the path replaces, with the reference's names
+!                and argument lists:
+!                  MatrixElements          src/matelem.f90:12
+!                  ComputeMatElem          src/matform.f90:119
+!                  ComputeMatElemAndDeriv  src/matform.f90:234
+!                  GHEPIIS                 src/linalg.f90:1781
+!                  EnergyGA / EnergyIA     src/workproc.f90:1697, 1954
+!                  EnergyGB / EnergyIB     src/workproc.f90:1841, 2063
+!                plus EcgHostInit / EcgHostFinish (context set-up, no
+!                counterpart in the reference: ProgramDataInit ends by
+!                calling EcgHostInit).
+! Errors come back in ErrorCode with the reference's meaning (0 = fine,
+! 1 = singular, 2 = not converged) or the library's negative codes; nothing
+! here stops the program.
+
+module ecg_capi
+  use, intrinsic :: iso_c_binding
+  implicit none
+  integer(c_int), parameter :: ECG_OK = 0, ECG_ERR_ARG = -1, ECG_ERR_HIP = -2, ECG_ERR_NOMEM = -3, &
+                               ECG_ERR_NONFINITE = -4, ECG_ERR_STATE = -5, ECG_SINGULAR = 1, &
+                               ECG_NOT_CONVERGED = 2, ECG_ROOT_NOT_FOUND = 3
+  interface
+    integer(c_int) function ecg_ctx_create(n, T, yhy_mats, yhy_coef, mass, q0, q, ctx) &
+        bind(C, name="ecg_ctx_create")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: n, T
+      integer(c_int), intent(in) :: yhy_mats(*), yhy_coef(*)
+      real(c_double), intent(in) :: mass(*)
+      real(c_double), value :: q0
+      real(c_double), intent(in) :: q(*)
+      type(c_ptr), intent(out) :: ctx
+    end function
+    integer(c_int) function ecg_ctx_destroy(ctx) bind(C, name="ecg_ctx_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function ecg_set_params(ctx, K, params) bind(C, name="ecg_set_params")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: K
+      real(c_double), intent(in) :: params(*)
+    end function
+    integer(c_int) function ecg_update_params(ctx, first, count, params) bind(C, name="ecg_update_params")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: first, count
+      real(c_double), intent(in) :: params(*)
+    end function
+    integer(c_int) function ecg_matrix_elements(ctx, paramk, paraml, P, gradflag, Hkl, Skl, Dk, Dl) &
+        bind(C, name="ecg_matrix_elements")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: paramk(*), paraml(*)
+      integer(c_int), intent(in) :: P(*)
+      integer(c_int), value :: gradflag
+      complex(c_double_complex), intent(out) :: Hkl, Skl
+      complex(c_double_complex), intent(inout) :: Dk(*), Dl(*)
+    end function
+    integer(c_int) function ecg_compute_mat_elem(ctx, nmin, nmax, method, approx_energy, H, S, ld, &
+        diagH, diagS) bind(C, name="ecg_compute_mat_elem")
+      import :: c_int, c_double, c_double_complex, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nmin, nmax, ld
+      character(kind=c_char), value :: method
+      real(c_double), value :: approx_energy
+      complex(c_double_complex), intent(inout) :: H(*), S(*)
+      real(c_double), intent(inout) :: diagH(*), diagS(*)
+    end function
+    integer(c_int) function ecg_compute_mat_elem_and_deriv(ctx, nfru, method, approx_energy, H, S, ld, &
+        diagH, diagS, D) bind(C, name="ecg_compute_mat_elem_and_deriv")
+      import :: c_int, c_double, c_double_complex, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nfru, ld
+      character(kind=c_char), value :: method
+      real(c_double), value :: approx_energy
+      complex(c_double_complex), intent(inout) :: H(*), S(*), D(*)
+      real(c_double), intent(inout) :: diagH(*), diagS(*)
+    end function
+    integer(c_int) function ecg_ghepiis(ctx, k, n, M, nM, invD, B, nB, apprlambda, v, tol, maxiter, &
+        specif_norm, lambda, x, relacc, numiter) bind(C, name="ecg_ghepiis")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: k, n, nM, nB, maxiter, specif_norm
+      complex(c_double_complex), intent(inout) :: M(*), invD(*)
+      complex(c_double_complex), intent(in) :: B(*), v(*)
+      real(c_double), value :: apprlambda, tol
+      real(c_double), intent(out) :: lambda, relacc
+      complex(c_double_complex), intent(out) :: x(*)
+      integer(c_int), intent(out) :: numiter
+    end function
+    integer(c_int) function ecg_energy(ctx, which, sigma, tol, maxiter, E, c, info) bind(C, name="ecg_energy")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: which, maxiter
+      real(c_double), value :: sigma, tol
+      real(c_double), intent(out) :: E
+      complex(c_double_complex), intent(out) :: c(*)
+      real(c_double), intent(out) :: info(*)
+    end function
+    integer(c_int) function ecg_energy_grad(ctx, nfru, which, sigma, tol, maxiter, E, grad, info) &
+        bind(C, name="ecg_energy_grad")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nfru, which, maxiter
+      real(c_double), value :: sigma, tol
+      real(c_double), intent(out) :: E
+      real(c_double), intent(out) :: grad(*), info(*)
+    end function
+    integer(c_int) function ecg_last_timings(ctx, ms, nmax) bind(C, name="ecg_last_timings")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: ms(*)
+      integer(c_int), value :: nmax
+    end function
+  end interface
+end module ecg_capi
+
+! The reference's global state used by the path (names and types of
+! src/globvars.f90).  Replaced by the reference's own module on integration.
+module ecg_globals
+  implicit none
+  integer, parameter :: dprec = kind(1.0d0)
+  integer :: Glob_n = 0, Glob_np = 0, Glob_npt = 0
+  integer :: Glob_NumYHYTerms = 0
+  integer, allocatable :: Glob_YHYMatr(:,:,:), Glob_YHYCoeff(:)
+  real(dprec), allocatable :: Glob_MassMatrix(:,:), Glob_PseudoCharge(:)
+  real(dprec) :: Glob_PseudoCharge0 = 0
+  integer :: Glob_CurrBasisSize = 0, Glob_HSLeadDim = 0, Glob_nfru = 0, Glob_nfo = 0
+  real(dprec), allocatable :: Glob_NonlinParam(:,:)
+  complex(dprec), allocatable :: Glob_H(:,:), Glob_S(:,:), Glob_D(:,:,:), Glob_c(:), Glob_invD(:)
+  real(dprec), allocatable :: Glob_diagH(:), Glob_diagS(:)
+  character :: Glob_GSEPSolutionMethod = 'G'
+  real(dprec) :: Glob_ApproxEnergy = 0
+  integer :: Glob_WhichEigenvalue = 1
+  real(dprec) :: Glob_EigvalTol = 1.0d-12
+  integer :: Glob_InvItMaxIter = 20
+end module ecg_globals
+
+module ecg_hostpath
+  use, intrinsic :: iso_c_binding
+  use, intrinsic :: ieee_arithmetic
+  use ecg_capi
+  use ecg_globals   ! GLOBALS: "use globvars" inside ATOM-MOL-nonBO
+  implicit none
+  private
+  public :: EcgHostInit, EcgHostFinish, MatrixElements, ComputeMatElem, ComputeMatElemAndDeriv, GHEPIIS, &
+            EnergyGA, EnergyIA, EnergyGB, EnergyIB
+  type(c_ptr), save :: ctx = c_null_ptr
+
+contains
+
+  ! Create the device context from the system globals (after the reference's
+  ! ProgramDataInit has filled Glob_MassMatrix, Glob_PseudoCharge*, Glob_YHYMatr).
+  subroutine EcgHostInit(ErrorCode)
+    integer, intent(out) :: ErrorCode
+    if (c_associated(ctx)) ErrorCode = ecg_ctx_destroy(ctx)
+    ErrorCode = ecg_ctx_create(Glob_n, Glob_NumYHYTerms, Glob_YHYMatr, Glob_YHYCoeff, Glob_MassMatrix, &
+                               Glob_PseudoCharge0, Glob_PseudoCharge, ctx)
+  end subroutine
+
+  subroutine EcgHostFinish()
+    integer :: rc
+    if (c_associated(ctx)) rc = ecg_ctx_destroy(ctx)
+    ctx = c_null_ptr
+  end subroutine
+
+  ! parameters of functions 1..K to the device
+  integer function sync_params(K)
+    integer, intent(in) :: K
+    sync_params = ecg_set_params(ctx, K, Glob_NonlinParam)
+  end function
+
+  ! src/matelem.f90:12 — one pair, one symmetry matrix, normalised elements
+  subroutine MatrixElements(Paramk, Paraml, P, Hkl, Skl, Dk, Dl, gradflag)
+    real(dprec), intent(in) :: Paramk(2*Glob_np), Paraml(2*Glob_np)
+    integer, intent(in) :: P(Glob_n, Glob_n)
+    complex(dprec), intent(out) :: Skl, Hkl
+    complex(dprec), intent(out) :: Dk(4*Glob_np), Dl(4*Glob_np)
+    integer, intent(in) :: gradflag
+    integer :: rc
+    rc = ecg_matrix_elements(ctx, Paramk, Paraml, P, gradflag, Hkl, Skl, Dk, Dl)
+    if (rc /= ECG_OK) then
+      Hkl = cmplx(ieee_value(1.0_dprec, ieee_quiet_nan), 0.0_dprec, dprec)
+      Skl = Hkl
+    end if
+  end subroutine
+
+  ! src/matform.f90:119 — rows Nmin..Nmax into Glob_H, Glob_S, Glob_diagH,
+  ! Glob_diagS with StoreHS's normalisation for Glob_GSEPSolutionMethod
+  subroutine ComputeMatElem(Nmin, Nmax)
+    integer, intent(in) :: Nmin, Nmax
+    integer :: rc
+    rc = sync_params(Nmax)
+    if (rc == ECG_OK) rc = ecg_compute_mat_elem(ctx, Nmin, Nmax, Glob_GSEPSolutionMethod, Glob_ApproxEnergy, &
+                                                Glob_H, Glob_S, Glob_HSLeadDim, Glob_diagH, Glob_diagS)
+    if (rc /= ECG_OK) Glob_diagS(Nmin) = ieee_value(1.0_dprec, ieee_quiet_nan)
+  end subroutine
+
+  ! src/matform.f90:234 — as ComputeMatElem plus Glob_D in StoreHSD's
+  ! layout for the optimised functions Glob_nfru+1..Nmax (Nmin = Glob_nfru+1)
+  subroutine ComputeMatElemAndDeriv(Nmin, Nmax)
+    integer, intent(in) :: Nmin, Nmax
+    integer :: rc
+    rc = sync_params(Nmax)
+    if (rc == ECG_OK) rc = ecg_compute_mat_elem_and_deriv(ctx, Nmin - 1, Glob_GSEPSolutionMethod, &
+                                                          Glob_ApproxEnergy, Glob_H, Glob_S, Glob_HSLeadDim, &
+                                                          Glob_diagH, Glob_diagS, Glob_D)
+    if (rc /= ECG_OK) Glob_diagS(Nmin) = ieee_value(1.0_dprec, ieee_quiet_nan)
+  end subroutine
+
+  ! src/linalg.f90:1781 — inverse iteration on host arrays; the LDL^H
+  ! factor (L^H in the upper triangle of M, 1/D in invD) is recomputed on
+  ! the device from row 1, so rows < k need not hold a factor; w (a work
+  ! array of the reference routine) is not used.
+  subroutine GHEPIIS(k, n, M, nM, invD, B, nB, apprlambda, v, w, Tol, lambda, x, RelAcc, MaxIter, SpecifNorm, &
+                     NumIter, ErrorCode)
+    integer :: k, n, nM, nB, MaxIter, SpecifNorm, NumIter, ErrorCode
+    complex(dprec) :: M(nM, n), invD(n), B(nB, n), v(n), w(n), x(n)
+    real(dprec) :: apprlambda, Tol, lambda, RelAcc
+    ErrorCode = ecg_ghepiis(ctx, k, n, M, nM, invD, B, nB, apprlambda, v, Tol, MaxIter, SpecifNorm, lambda, x, &
+                            RelAcc, NumIter)
+  end subroutine
+
+  ! src/workproc.f90:1697 — energy of root Glob_WhichEigenvalue of functions
+  ! 1..Nmax (root selected by inertia counts instead of ZHEGVX RANGE='I');
+  ! eigenvector to Glob_c.  H and S are assembled and kept on the device
+  ! whatever Nmin and AreMatElemNeeded say (a full assembly costs less
+  ! than the eigen step), and Glob_H, Glob_S are not written.
+  real(dprec) function EnergyGA(Nmin, Nmax, AreMatElemNeeded, ErrorCode)
+    integer :: Nmin, Nmax, ErrorCode
+    logical :: AreMatElemNeeded
+    real(dprec) :: info(8)
+    ErrorCode = sync_params(Nmax)
+    if (ErrorCode == ECG_OK) ErrorCode = ecg_energy(ctx, Glob_WhichEigenvalue, ieee_value(1.0_dprec, ieee_quiet_nan), &
+                                                    Glob_EigvalTol, 60, EnergyGA, Glob_c, info)
+  end function
+
+  ! src/workproc.f90:1954 — as EnergyGA with the shift Glob_ApproxEnergy
+  ! (method 'I')
+  real(dprec) function EnergyIA(Nmin, Nmax, AreMatElemNeeded, ErrorCode)
+    integer :: Nmin, Nmax, ErrorCode
+    logical :: AreMatElemNeeded
+    real(dprec) :: info(8)
+    ErrorCode = sync_params(Nmax)
+    if (ErrorCode == ECG_OK) ErrorCode = ecg_energy(ctx, Glob_WhichEigenvalue, Glob_ApproxEnergy, Glob_EigvalTol, &
+                                                    Glob_InvItMaxIter, EnergyIA, Glob_c, info)
+  end function
+
+  ! src/workproc.f90:1841 — energy and gradient w.r.t. the parameters of
+  ! functions Glob_nfru+1..Glob_CurrBasisSize, (dE/dvechL_k, dE/dvechB_k)
+  ! per function
+  subroutine EnergyGB(Evalue, Gradient, AreMatElemNeeded, ErrorCode)
+    real(dprec) :: Evalue
+    real(dprec) :: Gradient(Glob_npt*Glob_nfo)
+    logical :: AreMatElemNeeded
+    integer :: ErrorCode
+    real(dprec) :: info(8)
+    ErrorCode = sync_params(Glob_CurrBasisSize)
+    if (ErrorCode == ECG_OK) ErrorCode = ecg_energy_grad(ctx, Glob_nfru, Glob_WhichEigenvalue, &
+                                                         ieee_value(1.0_dprec, ieee_quiet_nan), Glob_EigvalTol, 60, &
+                                                         Evalue, Gradient, info)
+  end subroutine
+
+  ! src/workproc.f90:2063 — as EnergyGB with the shift Glob_ApproxEnergy
+  subroutine EnergyIB(Evalue, Gradient, AreMatElemNeeded, ErrorCode)
+    real(dprec) :: Evalue
+    real(dprec) :: Gradient(Glob_npt*Glob_nfo)
+    logical :: AreMatElemNeeded
+    integer :: ErrorCode
+    real(dprec) :: info(8)
+    ErrorCode = sync_params(Glob_CurrBasisSize)
+    if (ErrorCode == ECG_OK) ErrorCode = ecg_energy_grad(ctx, Glob_nfru, Glob_WhichEigenvalue, Glob_ApproxEnergy, &
+                                                         Glob_EigvalTol, Glob_InvItMaxIter, Evalue, Gradient, info)
+  end subroutine
+
+end module ecg_hostpath

@@ -297,3 +297,87 @@ def test_energy_grad_finite_differences():
         an = g[k * sy.npt + m]
         assert abs(fd - an) <= 1e-6 * max(1.0, abs(an)) + 1e-12 / h, (k, m, fd, an)
     ctx.close()
+
+
+def _write_fortran_input(path, system, params, which, approx):
+    n, T = system.n, system.T
+    with open(path, "w") as fh:
+        fh.write("%d %d\n" % (n, T))
+        fh.write(" ".join(str(int(v)) for t in range(T) for v in np.asarray(system.yhy_mats[t]).T.reshape(-1)) + "\n")
+        fh.write(" ".join(str(int(v)) for v in system.yhy_coef) + "\n")
+        fh.write(" ".join(repr(float(v)) for v in np.asarray(system.mass_matrix).T.reshape(-1)) + "\n")
+        fh.write(repr(float(system.q0)) + "\n")
+        fh.write(" ".join(repr(float(v)) for v in system.q) + "\n")
+        fh.write("%d %d %r\n" % (params.shape[0], which, float(approx)))
+        for row in params:
+            fh.write(" ".join(repr(float(v)) for v in row) + "\n")
+
+
+def test_fortran_host_layer_fixture(golden_dir, tmp_path):
+    """The Fortran host layer (module ecg_hostpath: ComputeMatElem + GHEPIIS
+    on host arrays, EnergyIA, EnergyIB) reproduces the fixture energy, and
+    its gradient equals the C-ABI's."""
+    import os
+    import subprocess
+    drv = os.path.join(os.path.dirname(__file__), "fortran", "drive_fixture")
+    if not os.path.exists(drv):
+        pytest.skip("Fortran driver not built (no Fortran compiler where build() ran)")
+    f = _fixture(golden_dir, "inout_1_500")
+    approx = f.current_energy * f.invit_parameter
+    inp = tmp_path / "system.txt"
+    _write_fortran_input(str(inp), f.system, f.params, f.which_eigenvalue, approx)
+    say("fortran host layer", "inout_1_500", f.basis_size)
+    out = subprocess.run([drv, str(inp)], capture_output=True, text=True, timeout=120)
+    print(out.stdout, out.stderr)
+    assert out.returncode == 0
+    rec = {ln.split()[0]: ln.split()[1:] for ln in out.stdout.splitlines() if ln.strip()}
+    e_gh, err_gh = float(rec["GHEPIIS"][0]), int(rec["GHEPIIS"][1])
+    e_ia, err_ia = float(rec["ENERGYIA"][0]), int(rec["ENERGYIA"][1])
+    e_ib, err_ib = float(rec["ENERGYIB"][0]), int(rec["ENERGYIB"][1])
+    gnorm, g1, glast = (float(v) for v in rec["ENERGYIB"][2:5])
+    assert err_gh == 0 and err_ia == 0 and err_ib == 0
+    for e in (e_gh, e_ia, e_ib):
+        assert abs(e - f.current_energy) <= 1e-10
+    ctx = EcgContext(f.system)
+    ctx.set_params(f.params)
+    rc, E, g, info = ctx.energy_grad(0, f.which_eigenvalue, approx, f.eigval_tolerance, 20)
+    ctx.close()
+    assert rc == ECG_OK
+    # the fixture basis is optimised (|grad| ~ 3e-8): absolute bound at the
+    # gradient's rounding level plus a relative one
+    assert abs(gnorm - np.linalg.norm(g)) <= 1e-12 + 1e-7 * np.linalg.norm(g)
+    assert abs(g1 - g[0]) <= 1e-12 + 1e-7 * abs(g[0]) and abs(glast - g[-1]) <= 1e-12 + 1e-7 * abs(g[-1])
+
+
+@pytest.mark.parametrize("name,K,seed,grad", [("hminus", 500, 0, False), ("h2", 1000, 2, True)])
+def test_seeded_configuration_vs_oracle(name, K, seed, grad):
+    """Full-size seeded configurations: H^- (n=2, T=2) K=500 assembly +
+    lowest root; H2 (n=3, T=4) K=1000 energy + gradient (12 parameters per
+    function).  The oracle runs the reference algorithm on the host (stored
+    derivatives, GHEPIIS restatement, EnergyGB contraction)."""
+    sy = named_system(name)
+    params = seeded_basis(sy.n, K, SEED_BASE + seed)
+    say("seeded configuration", name, K, "seed", SEED_BASE + seed, "grad", grad)
+    ctx = EcgContext(sy)
+    ctx.set_params(params)
+    rc, E0, _, info0 = ctx.energy(1, float("nan"), 1e-12, 60)
+    assert rc == ECG_OK and int(info0[4]) == 0
+    sigma = E0 - 1e-6 * abs(E0)
+    if grad:
+        rc, E, g, info = ctx.energy_grad(0, 1, sigma, 1e-12, 40)
+        assert rc == ECG_OK
+        Eo, go = _oracle_energy_grad(sy, params, sigma, 0)
+        print(name, K, "E", E, Eo, E - Eo, "max|g|", np.max(np.abs(go)), "max dev", np.max(np.abs(g - go)))
+        assert abs(E - Eo) <= 1e-10 * max(1.0, abs(Eo))
+        np.testing.assert_allclose(g, go, rtol=1e-7, atol=1e-8 * np.max(np.abs(go)))
+    else:
+        rc, E, _, info = ctx.energy(1, sigma, 1e-12, 40)
+        assert rc == ECG_OK
+        H, S, dH, dS = oracle.assemble(sy, params, "I", sigma)
+        err, lam, x, _, _ = oracle.ghepiis(1, H.copy(order="F"), np.zeros(K, np.complex128), S.copy(order="F"),
+                                           sigma, np.ones(K, np.complex128), 1e-12, 40)
+        print(name, K, "E", E, lam, E - lam, "err", err, "cold E", E0)
+        assert err in (0, 2)   # 2: the criterion is below the rounding floor (kappa(S) = 4.5e8)
+        assert abs(E - lam) <= 1e-10 * max(1.0, abs(lam))
+        assert abs(E0 - lam) <= 1e-10 * max(1.0, abs(lam))
+    ctx.close()
