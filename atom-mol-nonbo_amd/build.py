@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libecg_hip.so")
-SOURCES = ["ecg_kernels.hip", "ecg_linalg.hip", "ecg_capi.hip"]
+SOURCES = ["ecg_kernels.hip", "ecg_linalg.hip", "ecg_screen.hip", "ecg_capi.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 instructions stay off (the path is FP64 throughout)

@@ -42,6 +42,7 @@ struct ecg_ctx {
   double tms[8] = {};
   double* pinned = nullptr;   // pinned host staging
   size_t pinned_cap = 0;
+  ecg::Screen scr;            // candidate screening session
 };
 
 namespace {
@@ -111,6 +112,58 @@ int ensure_pinned(ecg_ctx* x, size_t doubles) {
     return ECG_ERR_NOMEM;
   x->pinned_cap = doubles;
   return 0;
+}
+
+// Screening buffers: ld x ld matrices for the current basis size, C
+// candidate columns, nsteps loop steps.
+int ensure_screen(ecg_ctx* x, int C, int nsteps) {
+  ecg::Screen& sc = x->scr;
+  const ecg::Ctx& c = x->c;
+  const int ld = c.ld;
+  if (sc.ld != ld) {
+    RCCHK(dev_alloc(&sc.G, (size_t)ld * ld));
+    RCCHK(dev_alloc(&sc.P, (size_t)ld * ld));
+    RCCHK(dev_alloc(&sc.Z, (size_t)ld * ld));
+    RCCHK(dev_alloc(&sc.v0, (size_t)ld));
+    RCCHK(dev_alloc(&sc.state, (size_t)2));
+    sc.ld = ld;
+    sc.ccap = 0;
+    sc.ready = false;
+  }
+  if (C > sc.ccap) {
+    const int cc = std::max(C, 16);
+    RCCHK(dev_alloc(&sc.cparams, (size_t)x->npt * cc));
+    RCCHK(dev_alloc(&sc.cbra, (size_t)ecg::bra_stride(c.n) * cc));
+    RCCHK(dev_alloc(&sc.cket, (size_t)ecg::ket_stride(c.n) * c.T * cc));
+    RCCHK(dev_alloc(&sc.cdiag, (size_t)2 * cc));
+    RCCHK(dev_alloc(&sc.Hb, (size_t)ld * cc));
+    RCCHK(dev_alloc(&sc.Bm, (size_t)ld * 2 * cc));
+    RCCHK(dev_alloc(&sc.Wm, (size_t)ld * 2 * cc));
+    RCCHK(dev_alloc(&sc.V, (size_t)ld * cc));
+    RCCHK(dev_alloc(&sc.U, (size_t)ld * cc));
+    RCCHK(dev_alloc(&sc.Y1, (size_t)ld * cc));
+    RCCHK(dev_alloc(&sc.Y2, (size_t)ld * cc));
+    RCCHK(dev_alloc(&sc.vK, (size_t)cc));
+    RCCHK(dev_alloc(&sc.cs, (size_t)8 * cc));
+    sc.ccap = cc;
+  }
+  if (nsteps > 0 && (size_t)x->npt * C * nsteps > sc.callcap) {
+    RCCHK(dev_alloc(&sc.call, (size_t)x->npt * C * nsteps));
+    sc.callcap = (size_t)x->npt * C * nsteps;
+  }
+  if (nsteps > sc.scap) {
+    RCCHK(dev_alloc(&sc.rec, (size_t)2 * nsteps));
+    sc.scap = nsteps;
+  }
+  return 0;
+}
+
+void free_screen(ecg::Screen& sc) {
+  void* ptrs[] = {sc.G, sc.P, sc.Z, sc.call, sc.cparams, sc.cbra, sc.cket, sc.cdiag, sc.Hb, sc.Bm, sc.Wm,
+                  sc.V, sc.U, sc.Y1, sc.Y2, sc.vK, sc.cs, sc.v0, sc.state, sc.rec};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  sc = ecg::Screen();
 }
 
 // Upload parameters and run the per-function set-up for all K functions.
@@ -524,6 +577,7 @@ int ecg_ctx_destroy(ecg_ctx* x) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (x->pinned) (void)hipHostFree(x->pinned);
+  free_screen(x->scr);
   for (int i = 0; i < 8; ++i)
     if (x->e[i]) (void)hipEventDestroy(x->e[i]);
   if (x->stream) (void)hipStreamDestroy(x->stream);
@@ -775,6 +829,136 @@ int ecg_energy_grad(ecg_ctx* x, int nfru, int which, double sigma, double tol, i
   x->tms[5] = ev_ms(x->e[5], x->e[6]);
   if (!std::isfinite(*E) || !finite_all(grad, ng)) return ECG_ERR_NONFINITE;
   return status;
+}
+
+int ecg_get_params(ecg_ctx* x, double* params) {
+  if (!x || !params) return ECG_ERR_ARG;
+  if (x->c.K < 1) return ECG_ERR_STATE;
+  HIPCHK(hipMemcpyAsync(params, x->c.params, sizeof(double) * x->npt * x->c.K, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return ECG_OK;
+}
+
+int ecg_screen_prepare(ecg_ctx* x, double sigma, double tol, int maxiter, double* info) {
+  if (!x || !std::isfinite(sigma) || maxiter < 1) return ECG_ERR_ARG;
+  ecg::Ctx& c = x->c;
+  const int K = c.K;
+  if (K < 2) return ECG_ERR_STATE;
+  RCCHK(ensure_screen(x, 1, 0));
+  ecg::Screen& sc = x->scr;
+  sc.ready = false;
+  RCCHK(ecg::prep(c, x->stream));
+  RCCHK(ecg::assemble(c, 0, K, x->stream));
+  RCCHK(ecg::screen_prepare(c, x->f, sc, sigma, x->stream));
+  std::vector<double> D(K);
+  int st[4];
+  HIPCHK(hipMemcpyAsync(D.data(), x->f.D, sizeof(double) * K, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipMemcpyAsync(st, x->f.status, sizeof(st), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  int nneg = 0, zero = 0;
+  for (int i = 0; i < K - 1; ++i) {
+    if (D[i] < 0.0) ++nneg;
+    if (D[i] == 0.0 && !zero) zero = i + 1;
+  }
+  sc.nneg = nneg;
+  if (zero || st[1]) return ECG_SINGULAR;
+  // energy and eigenvector of the current basis at this shift: start
+  // vector of every candidate and the energy a candidate must beat
+  InvitResult r;
+  RCCHK(inverse_iteration(x, sigma, tol, maxiter, true, r, true));
+  HIPCHK(hipMemcpyAsync(sc.v0, x->ev[2], sizeof(zd) * K, hipMemcpyDeviceToDevice, x->stream));
+  const double state[2] = {r.lambda, 0.0};
+  HIPCHK(hipMemcpyAsync(sc.state, state, sizeof(state), hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  if (info) {
+    info[0] = r.lambda;
+    info[1] = nneg;
+    info[2] = r.iters;
+    info[3] = r.nd;
+  }
+  sc.ready = true;
+  return (r.converged || r.nd < 1e-9) ? ECG_OK : ECG_NOT_CONVERGED;
+}
+
+int ecg_screen(ecg_ctx* x, int ncand, const double* cand, int iters, double* E, double* info) {
+  if (!x || ncand < 1 || !cand || iters < 1 || !E) return ECG_ERR_ARG;
+  if (!x->scr.ready || x->scr.ld != x->c.ld || x->scr.Kp != x->c.K - 1) return ECG_ERR_STATE;
+  if (!finite_all(cand, (size_t)x->npt * ncand)) return ECG_ERR_ARG;
+  RCCHK(ensure_screen(x, ncand, 0));
+  ecg::Screen& sc = x->scr;
+  HIPCHK(hipMemcpyAsync(sc.cparams, cand, sizeof(double) * x->npt * ncand, hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipEventRecord(x->e[0], x->stream));
+  RCCHK(ecg::screen_batch(x->c, x->f, sc, ncand, iters, false, x->npt, x->stream));
+  HIPCHK(hipEventRecord(x->e[1], x->stream));
+  std::vector<double> cs((size_t)8 * ncand);
+  HIPCHK(hipMemcpyAsync(cs.data(), sc.cs, sizeof(double) * 8 * ncand, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  x->tms[0] = ev_ms(x->e[0], x->e[1]);
+  bool fin = true;
+  for (int j = 0; j < ncand; ++j) {
+    E[j] = cs[8 * j + 0];
+    fin = fin && std::isfinite(E[j]);
+    if (info) {
+      info[4 * j + 0] = cs[8 * j + 1];
+      info[4 * j + 1] = sc.nneg + (cs[8 * j + 2] < 0.0 ? 1 : 0);
+      info[4 * j + 2] = cs[8 * j + 2];
+      info[4 * j + 3] = cs[8 * j + 3];
+    }
+  }
+  return fin ? ECG_OK : ECG_ERR_NONFINITE;
+}
+
+int ecg_screen_loop(ecg_ctx* x, int nsteps, int ncand, const double* cand, int iters, double* Ebest, int* ibest) {
+  if (!x || nsteps < 1 || ncand < 1 || !cand || iters < 1 || !Ebest || !ibest) return ECG_ERR_ARG;
+  if (!x->scr.ready || x->scr.ld != x->c.ld || x->scr.Kp != x->c.K - 1) return ECG_ERR_STATE;
+  if (!finite_all(cand, (size_t)x->npt * ncand * nsteps)) return ECG_ERR_ARG;
+  RCCHK(ensure_screen(x, ncand, nsteps));
+  ecg::Screen& sc = x->scr;
+  const int npt = x->npt;
+  HIPCHK(hipMemcpyAsync(sc.call, cand, sizeof(double) * npt * ncand * nsteps, hipMemcpyHostToDevice, x->stream));
+  double st1 = 0.0;
+  HIPCHK(hipMemcpyAsync(sc.state + 1, &st1, sizeof(double), hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  // one step as a graph: fetch, rows, Schur complements, iterations,
+  // Rayleigh quotients, selection; replayed nsteps times
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
+  int rc = ecg::screen_batch(x->c, x->f, sc, ncand, iters, true, npt, x->stream);
+  if (!rc) rc = ecg::screen_select(x->c, sc, ncand, npt, x->stream);
+  const hipError_t ec = hipStreamEndCapture(x->stream, &g);
+  if (rc || ec != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc < 0 ? rc : ECG_ERR_HIP;
+  }
+  if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return ECG_ERR_HIP;
+  }
+  HIPCHK(hipEventRecord(x->e[0], x->stream));
+  for (int i = 0; i < nsteps; ++i) {
+    if (hipGraphLaunch(ge, x->stream) != hipSuccess) {
+      (void)hipStreamSynchronize(x->stream);
+      (void)hipGraphExecDestroy(ge);
+      (void)hipGraphDestroy(g);
+      return ECG_ERR_HIP;
+    }
+  }
+  HIPCHK(hipEventRecord(x->e[1], x->stream));
+  std::vector<double> rec(2 * (size_t)nsteps);
+  HIPCHK(hipMemcpyAsync(rec.data(), sc.rec, sizeof(double) * 2 * nsteps, hipMemcpyDeviceToHost, x->stream));
+  // accepted candidates changed function K: refresh its pair records
+  RCCHK(ecg::prep(x->c, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  (void)hipGraphExecDestroy(ge);
+  (void)hipGraphDestroy(g);
+  x->tms[0] = ev_ms(x->e[0], x->e[1]);
+  x->have_last = false;
+  for (int i = 0; i < nsteps; ++i) {
+    Ebest[i] = rec[2 * i];
+    ibest[i] = (int)rec[2 * i + 1];
+  }
+  return ECG_OK;
 }
 
 int ecg_last_timings(ecg_ctx* x, double* ms, int nmax) {

@@ -78,4 +78,53 @@ int invit_iteration(const Ctx& c, Factor& f, const zd* v, zd* x, hipStream_t s);
 int rayleigh(const Ctx& c, Factor& f, const zd* x, double sigma, hipStream_t s);
 int normalise(zd* x, int K, const double* t, hipStream_t s);
 
+// candidate screening (ecg_screen.hip; pair rows in ecg_kernels.hip)
+int candidate_rows(const Ctx& c, int Kp, int C, const double* cparams, double* cbra, double* cket,
+                   double* cdiag, zd* Hb, zd* Sb, int ldb, hipStream_t s);
+
+// Device state of a screening session: functions 1..Kp (Kp = K-1) fixed,
+// candidates replace function K.  M' = H' - sigma S' (leading Kp block),
+// G = M'^{-1}, P = G S'.  Per-candidate columns are ld x ccap complex.
+struct Screen {
+  int Kp = 0, ld = 0, ccap = 0, scap = 0, nneg = 0;
+  size_t callcap = 0;   // doubles in call
+  double sigma = 0.0;
+  bool ready = false;
+  zd* G = nullptr;       // ld x ld
+  zd* P = nullptr;       // ld x ld
+  zd* Z = nullptr;       // ld x ld set-up scratch
+  double* call = nullptr;    // npt x ccap x scap candidate parameters of a loop
+  double* cparams = nullptr; // npt x ccap
+  double* cbra = nullptr;
+  double* cket = nullptr;
+  double* cdiag = nullptr;   // 2 ccap
+  zd* Hb = nullptr;      // ld x ccap border h
+  zd* Bm = nullptr;      // ld x 2 ccap: [m | s] (m = h - sigma s)
+  zd* Wm = nullptr;      // ld x 2 ccap: G [m | s]
+  zd* V = nullptr;       // ld x ccap iterates (first Kp entries)
+  zd* U = nullptr;       // ld x ccap
+  zd* Y1 = nullptr;      // ld x ccap
+  zd* Y2 = nullptr;      // ld x ccap
+  zd* vK = nullptr;      // ccap last entries of the iterates
+  double* cs = nullptr;  // 8 x ccap per-candidate scalars
+  zd* v0 = nullptr;      // ld start vector (current eigenvector)
+  double* state = nullptr;   // loop state: [0] current E, [1] step
+  double* rec = nullptr;     // 2 x scap: best E and index per step
+};
+// 0 or a status; computes the factor of H - sigma S (whole basis), G and P.
+int screen_prepare(const Ctx& c, Factor& f, Screen& sc, double sigma, hipStream_t s);
+// One batch of C candidates (parameters already in sc.cparams, or, when
+// from_loop, taken from sc.call at the step counter): rows, Schur
+// complements, `iters` inverse iterations from sc.v0, Rayleigh quotients.
+// Per-candidate results in sc.cs: [0] E, [1] convergence measure, [2] delta
+// (Schur complement of the candidate's pivot), [3] x^H S x.
+int screen_batch(const Ctx& c, const Factor& f, Screen& sc, int C, int iters, bool from_loop, int npt,
+                 hipStream_t s);
+// Loop bookkeeping kernel: pick the best candidate of the step, accept it
+// when it lowers the current energy (parameters into function K of c,
+// start vector := its eigenvector), record, advance the step counter.
+int screen_select(const Ctx& c, Screen& sc, int C, int npt, hipStream_t s);
+int zgemm(int trans, int M, int N, int Kd, zd alpha, const zd* A, int lda, const zd* B, int ldb, zd beta, zd* Cm,
+          int ldc, hipStream_t s);
+
 }  // namespace ecg

@@ -475,6 +475,59 @@ __global__ void k_adjoint_finish(int K, int nfru, int nt, int tr0, const double*
 }
 
 // ------------------------------------------------------------ launchers
+// ------------------------------------------------------ screening rows
+// Candidate functions for the last basis position (the reference's trial
+// loops, src/workproc.f90:1375-1487, 3397-3413): the candidate is the bra of
+// row K (it is the reference's row index k >= l), the fixed functions are
+// the kets.
+// Diagonal pair of candidate j: cdiag[2j] = Re S_jj, cdiag[2j+1] = Re H_jj / S_jj.
+template <int N>
+__global__ void __launch_bounds__(128) k_cand_diag(const SysConst sc, int C, const double* __restrict__ cbra,
+                                                   const double* __restrict__ cket, double* __restrict__ cdiag) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= C) return;
+  const double* br = cbra + (size_t)j * BraRec<N>::stride;
+  zd H, S;
+  pair_sum_energy<N>(sc, br, cket + (size_t)j * sc.T * KetRec<N>::stride, br[BraRec<N>::adet], H, S);
+  cdiag[2 * j] = S.x;
+  cdiag[2 * j + 1] = H.x / S.x;
+}
+
+// Border columns of candidate j against the fixed functions l < Kp, as the
+// upper-right column of the bordered matrices (conjugate of row K):
+//   Hb(l, j) = conj(H_Kl) f, Sb(l, j) = conj(S_Kl) f, f = 1/sqrt(S_jj diagS_l).
+template <int N>
+__global__ void __launch_bounds__(128) k_cand_rows(const SysConst sc, int Kp, int ld,
+                                                   const double* __restrict__ bra, const double* __restrict__ ket,
+                                                   const double* __restrict__ diagS,
+                                                   const double* __restrict__ cbra,
+                                                   const double* __restrict__ cdiag, zd* __restrict__ Hb,
+                                                   zd* __restrict__ Sb) {
+  __shared__ double sbra[BraRec<N>::stride];
+  const int j = blockIdx.y;
+  for (int i = threadIdx.x; i < BraRec<N>::stride; i += blockDim.x)
+    sbra[i] = cbra[(size_t)j * BraRec<N>::stride + i];
+  __syncthreads();
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= Kp) return;
+  zd H, S;
+  pair_sum_energy<N>(sc, sbra, ket + (size_t)l * sc.T * KetRec<N>::stride,
+                     bra[(size_t)l * BraRec<N>::stride + BraRec<N>::adet], H, S);
+  const double f = 1.0 / sqrt(cdiag[2 * j] * diagS[l]);
+  Hb[(size_t)j * ld + l] = zconj(H) * f;
+  Sb[(size_t)j * ld + l] = zconj(S) * f;
+}
+
+template <int N>
+static int launch_cand(const Ctx& c, int Kp, int C, const double* cparams, double* cbra, double* cket,
+                       double* cdiag, zd* Hb, zd* Sb, int ldb, hipStream_t s) {
+  hipLaunchKernelGGL(k_prep<N>, dim3((C + 127) / 128), dim3(128), 0, s, c.sc, C, cparams, cbra, cket);
+  hipLaunchKernelGGL(k_cand_diag<N>, dim3((C + 127) / 128), dim3(128), 0, s, c.sc, C, cbra, cket, cdiag);
+  hipLaunchKernelGGL(k_cand_rows<N>, dim3((Kp + 127) / 128, C), dim3(128), 0, s, c.sc, Kp, ldb, c.bra, c.ket,
+                     c.diagS, cbra, cdiag, Hb, Sb);
+  return (int)hipGetLastError();
+}
+
 template <int N>
 static int launch_prep(const Ctx& c, hipStream_t s) {
   const int K = c.K;
@@ -570,6 +623,18 @@ size_t adjoint_partial_doubles(int n, int K) {
   const int nt = (K + TILE - 1) / TILE;
   const int m = n * (n + 1) / 2;
   return (size_t)K * (nt + 1) * (2 * m + 1);
+}
+
+int candidate_rows(const Ctx& c, int Kp, int C, const double* cparams, double* cbra, double* cket,
+                   double* cdiag, zd* Hb, zd* Sb, int ldb, hipStream_t s) {
+  switch (c.n) {
+    case 1: return launch_cand<1>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, s);
+    case 2: return launch_cand<2>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, s);
+    case 3: return launch_cand<3>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, s);
+    case 4: return launch_cand<4>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, s);
+    case 5: return launch_cand<5>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, s);
+  }
+  return -1;
 }
 
 }  // namespace ecg

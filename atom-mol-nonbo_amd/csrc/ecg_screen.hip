@@ -1,0 +1,446 @@
+// Batched candidate screening for the last basis position.
+//
+// The reference scores trial functions one at a time (GenerateTrialParam
+// and the trial loops, src/workproc.f90:1375-1487, 3397-3413, 4054-4070):
+// for each trial it recomputes row K of H and S, updates the LDL^H factor
+// from row K (LDLHF, src/linalg.f90:316) and runs inverse iteration
+// (GHEPIIS, src/linalg.f90:1781).  Here all candidates of a step share the
+// fixed leading block M' = H' - sigma S' (functions 1..K-1):
+//   set-up (once per session):  LDL^H of H - sigma S, Z = L'^{-1} by blocked
+//     substitution on GEMMs with the stored 64 x 64 block inverses,
+//     G = Z^H D'^{-1} Z = M'^{-1}, P = G S';
+//   per step, for C candidates (columns of one matrix):
+//     border columns m = h - sigma s, s of each candidate (pair kernel),
+//     [w_m | w_s] = G [m | s]                     (one GEMM, N = 2C),
+//     Schur complement delta = mu - m^H w_m        (the candidate's pivot),
+//     inverse iteration x = M_j^{-1} S_j v with the bordered solve
+//       u = P v' + w_s v_K,  x_K = (s^H v' + v_K - m^H u) / delta,
+//       x' = u - w_m x_K                            (one GEMM per iteration),
+//     GHEPIIS's scaling and convergence measure per candidate,
+//     Rayleigh quotient E = x^H H_j x / x^H S_j x (two GEMMs).
+// Every GEMM is Kp x Kp times Kp x C on MFMA; nothing in a step synchronises
+// with the host, so a step can be captured in a hipGraph and replayed.
+#include "ecg_internal.hpp"
+
+namespace ecg {
+
+// ------------------------------------------------------------------ ZGEMM
+// C = alpha op(A) B + beta C, complex, column-major; op(A) = A (TRANS = 0)
+// or A^H (TRANS = 1).  Workgroup tile 64 x 32, k-chunks of 16 staged in
+// LDS with coalesced loads; wave w computes rows 32 (w & 1) + 16 a (a = 0,
+// 1) and columns 16 (w >> 1) as two 16 x 16 f64 MFMA tiles, four real
+// products per complex one.
+typedef double gdbl4 __attribute__((ext_vector_type(4)));
+
+template <int TRANS>
+__global__ void __launch_bounds__(256) k_zgemm(int M, int N, int Kd, zd alpha, const zd* __restrict__ A, int lda,
+                                               const zd* __restrict__ B, int ldb, zd beta, zd* __restrict__ Cm,
+                                               int ldc) {
+  __shared__ zd As[16][64 + 1];   // As[p][r] = op(A)(r0 + r, p0 + p)
+  __shared__ zd Bs[32][16 + 1];   // Bs[c][p] = B(p0 + p, c0 + c)
+  const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 32;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int rt = (wave & 1) * 32, ct = (wave >> 1) * 16;
+  gdbl4 re[2], im[2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    re[a] = gdbl4{0.0, 0.0, 0.0, 0.0};
+    im[a] = gdbl4{0.0, 0.0, 0.0, 0.0};
+  }
+  for (int p0 = 0; p0 < Kd; p0 += 16) {
+    zd va[4], vb[2];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + 256 * u;
+      int r, p;
+      if (TRANS == 0) {
+        r = e & 63;
+        p = e >> 6;
+      } else {
+        p = e & 15;
+        r = e >> 4;
+      }
+      const bool in = (r0 + r < M) && (p0 + p < Kd);
+      zd v = mkz(0.0, 0.0);
+      if (in) {
+        if (TRANS == 0) v = A[(size_t)(p0 + p) * lda + r0 + r];
+        else v = zconj(A[(size_t)(r0 + r) * lda + p0 + p]);
+      }
+      va[u] = v;
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int e = tid + 256 * u, p = e & 15, c = e >> 4;
+      vb[u] = (c0 + c < N && p0 + p < Kd) ? B[(size_t)(c0 + c) * ldb + p0 + p] : mkz(0.0, 0.0);
+    }
+    __syncthreads();   // the previous chunk's reads are done
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + 256 * u;
+      if (TRANS == 0) As[e >> 6][e & 63] = va[u];
+      else As[e & 15][e >> 4] = va[u];
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int e = tid + 256 * u;
+      Bs[e >> 4][e & 15] = vb[u];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const int p = 4 * kk + lk;
+      const zd b = Bs[ct + lr][p];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const zd x = As[p][rt + 16 * a + lr];
+        re[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, b.x, re[a], 0, 0, 0);
+        re[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.y, b.y, re[a], 0, 0, 0);
+        im[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, b.y, im[a], 0, 0, 0);
+        im[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, b.x, im[a], 0, 0, 0);
+      }
+    }
+  }
+  const bool bz = (beta.x == 0.0 && beta.y == 0.0);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = r0 + rt + 16 * a + 4 * v + lk, c = c0 + ct + lr;
+      if (r < M && c < N) {
+        const zd acc = mkz(re[a][v], im[a][v]);
+        zd* o = Cm + (size_t)c * ldc + r;
+        zd out = alpha * acc;
+        if (!bz) out = zfma(beta, *o, out);
+        *o = out;
+      }
+    }
+}
+
+int zgemm(int trans, int M, int N, int Kd, zd alpha, const zd* A, int lda, const zd* B, int ldb, zd beta, zd* Cm,
+          int ldc, hipStream_t s) {
+  if (M <= 0 || N <= 0) return 0;
+  const dim3 grid((M + 63) / 64, (N + 31) / 32);
+  if (trans) hipLaunchKernelGGL(k_zgemm<1>, grid, dim3(256), 0, s, M, N, Kd, alpha, A, lda, B, ldb, beta, Cm, ldc);
+  else hipLaunchKernelGGL(k_zgemm<0>, grid, dim3(256), 0, s, M, N, Kd, alpha, A, lda, B, ldb, beta, Cm, ldc);
+  return (int)hipGetLastError();
+}
+
+// ------------------------------------------------------------ set-up
+__global__ void k_identity(int n, int ld, zd* __restrict__ Z) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i < n) Z[(size_t)j * ld + i] = mkz(i == j ? 1.0 : 0.0, 0.0);
+}
+
+// rows r0..r0+rows-1 of Z (columns 0..ncol-1) := T (rows x ncol, leading dimension 64)
+__global__ void k_copy_rows(int rows, int ncol, int r0, int ld, const zd* __restrict__ T, zd* __restrict__ Z) {
+  const int i = threadIdx.x & 63, j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i < rows && j < ncol) Z[(size_t)j * ld + r0 + i] = T[(size_t)j * 64 + i];
+}
+
+// Y(i, j) = Z(i, j) / D_i
+__global__ void k_row_scale(int n, int ld, const double* __restrict__ D, const zd* __restrict__ Z,
+                            zd* __restrict__ Y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i < n) Y[(size_t)j * ld + i] = Z[(size_t)j * ld + i] * (1.0 / D[i]);
+}
+
+int screen_prepare(const Ctx& c, Factor& f, Screen& sc, double sigma, hipStream_t s) {
+  const int Kp = c.K - 1, ld = c.ld;
+  sc.Kp = Kp;
+  sc.sigma = sigma;
+  int rc = form_shifted(c, f, sigma, s);
+  if (!rc) rc = ldlh_factor(f, s);
+  if (rc) return rc;
+  // Z = L'^{-1}: block b: Z_b = X_b R_b, then R_below -= L_below,b Z_b.
+  // Only columns < b0 + bs of block rows >= b are nonzero.
+  hipLaunchKernelGGL(k_identity, dim3((Kp + 255) / 256, Kp), dim3(256), 0, s, Kp, ld, sc.Z);
+  zd* T = sc.G;   // 64 x ld scratch before G is formed
+  for (int b0 = 0; b0 < Kp; b0 += NB) {
+    const int bs = (Kp - b0 < NB) ? Kp - b0 : NB;
+    const int ncol = b0 + bs;
+    const zd* Xb = f.Dinv + (size_t)(b0 / NB) * 2 * NB * NB;
+    rc = zgemm(0, bs, ncol, bs, mkz(1.0, 0.0), Xb, NB, sc.Z + b0, ld, mkz(0.0, 0.0), T, NB, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_copy_rows, dim3((ncol + 3) / 4), dim3(256), 0, s, bs, ncol, b0, ld, T, sc.Z);
+    const int rest = Kp - ncol;
+    if (rest > 0) {
+      rc = zgemm(0, rest, ncol, bs, mkz(-1.0, 0.0), f.F + (size_t)b0 * ld + ncol, ld, sc.Z + b0, ld,
+                 mkz(1.0, 0.0), sc.Z + ncol, ld, s);
+      if (rc) return rc;
+    }
+  }
+  // G = Z^H (D^{-1} Z), P = G S'
+  hipLaunchKernelGGL(k_row_scale, dim3((Kp + 255) / 256, Kp), dim3(256), 0, s, Kp, ld, f.D, sc.Z, sc.P);
+  rc = zgemm(1, Kp, Kp, Kp, mkz(1.0, 0.0), sc.Z, ld, sc.P, ld, mkz(0.0, 0.0), sc.G, ld, s);
+  if (!rc) rc = zgemm(0, Kp, Kp, Kp, mkz(1.0, 0.0), sc.G, ld, c.S, ld, mkz(0.0, 0.0), sc.P, ld, s);
+  return rc;
+}
+
+// ------------------------------------------------- per-candidate kernels
+// per-candidate scalars cs[8 j + .]
+enum { CS_E = 0, CS_ND = 1, CS_DELTA = 2, CS_XSX = 3, CS_MU = 4, CS_HKK = 5 };
+
+__device__ inline double wg_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += sh[i];
+  return t;
+}
+__device__ inline double wg_max(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t = fmax(t, sh[i]);
+  return t;
+}
+
+// loop mode: candidates of the current step into cparams
+__global__ void k_cand_fetch(int n, const double* __restrict__ call, const double* __restrict__ state,
+                             double* __restrict__ cparams) {
+  const size_t step = (size_t)state[1];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    cparams[i] = call[step * n + i];
+}
+
+// m = h - sigma s, mu = H_KK - sigma; start vectors
+__global__ void __launch_bounds__(256) k_cand_border(int Kp, int ld, int C, double sigma,
+                                                     const zd* __restrict__ Hb, zd* __restrict__ Bm,
+                                                     const double* __restrict__ cdiag, double* __restrict__ cs,
+                                                     const zd* __restrict__ v0, zd* __restrict__ V,
+                                                     zd* __restrict__ vK) {
+  const int j = blockIdx.x;
+  const zd* h = Hb + (size_t)j * ld;
+  const zd* sv = Bm + (size_t)(C + j) * ld;
+  zd* m = Bm + (size_t)j * ld;
+  zd* v = V + (size_t)j * ld;
+  for (int l = threadIdx.x; l < Kp; l += blockDim.x) {
+    m[l] = zfms(mkz(sigma, 0.0), sv[l], h[l]);
+    v[l] = v0[l];
+  }
+  if (threadIdx.x == 0) {
+    cs[8 * j + CS_HKK] = cdiag[2 * j + 1];
+    cs[8 * j + CS_MU] = cdiag[2 * j + 1] - sigma;
+    vK[j] = v0[Kp];
+  }
+}
+
+// delta_j = mu_j - Re(m_j^H w_m,j)
+__global__ void __launch_bounds__(256) k_cand_schur(int Kp, int ld, const zd* __restrict__ Bm,
+                                                    const zd* __restrict__ Wm, double* __restrict__ cs) {
+  __shared__ double sh[8];
+  const int j = blockIdx.x;
+  const zd* m = Bm + (size_t)j * ld;
+  const zd* w = Wm + (size_t)j * ld;
+  double a = 0.0;
+  for (int l = threadIdx.x; l < Kp; l += blockDim.x) a += m[l].x * w[l].x + m[l].y * w[l].y;
+  a = wg_sum(a, sh);
+  if (threadIdx.x == 0) cs[8 * j + CS_DELTA] = cs[8 * j + CS_MU] - a;
+}
+
+// One bordered inverse-iteration update per candidate (U = P V on entry):
+// x = M_j^{-1} S_j v, scaled by max(|Re|, |Im|), GHEPIIS's convergence
+// measure sqrt(2K) ||x - tc v||, tc = x^H v / v^H v; V, vK := x.
+__global__ void __launch_bounds__(256) k_cand_iter(int Kp, int ld, int C, zd* __restrict__ U,
+                                                   const zd* __restrict__ Wm, const zd* __restrict__ Bm,
+                                                   zd* __restrict__ V, zd* __restrict__ vK,
+                                                   double* __restrict__ cs) {
+  __shared__ double sh[8];
+  const int j = blockIdx.x;
+  const zd* m = Bm + (size_t)j * ld;
+  const zd* sv = Bm + (size_t)(C + j) * ld;
+  const zd* wm = Wm + (size_t)j * ld;
+  const zd* ws = Wm + (size_t)(C + j) * ld;
+  zd* u = U + (size_t)j * ld;
+  zd* v = V + (size_t)j * ld;
+  const zd vk = vK[j];
+  // r_K = s^H v' + v_K;  u = P v' + w_s v_K;  m^H u
+  double rr = 0.0, ri = 0.0, mr = 0.0, mi = 0.0;
+  for (int l = threadIdx.x; l < Kp; l += blockDim.x) {
+    const zd vl = v[l], sl = sv[l];
+    rr += sl.x * vl.x + sl.y * vl.y;
+    ri += sl.x * vl.y - sl.y * vl.x;
+    const zd ul = zfma(ws[l], vk, u[l]);
+    u[l] = ul;
+    const zd ml = m[l];
+    mr += ml.x * ul.x + ml.y * ul.y;
+    mi += ml.x * ul.y - ml.y * ul.x;
+  }
+  rr = wg_sum(rr, sh);
+  ri = wg_sum(ri, sh);
+  mr = wg_sum(mr, sh);
+  mi = wg_sum(mi, sh);
+  const double delta = cs[8 * j + CS_DELTA];
+  const zd xk = mkz((rr + vk.x - mr) / delta, (ri + vk.y - mi) / delta);
+  double mx = fmax(fabs(xk.x), fabs(xk.y));
+  for (int l = threadIdx.x; l < Kp; l += blockDim.x) {
+    const zd xl = zfms(wm[l], xk, u[l]);
+    u[l] = xl;
+    mx = fmax(mx, fmax(fabs(xl.x), fabs(xl.y)));
+  }
+  const double t1 = wg_max(mx, sh);
+  const double inv = 1.0 / t1;
+  // tc = (x^H v)/(v^H v) over all K entries
+  double ar = 0.0, ai = 0.0, vv = 0.0;
+  for (int l = threadIdx.x; l < Kp; l += blockDim.x) {
+    const zd xl = u[l] * inv, vl = v[l];
+    ar += xl.x * vl.x + xl.y * vl.y;
+    ai += xl.x * vl.y - xl.y * vl.x;
+    vv += vl.x * vl.x + vl.y * vl.y;
+  }
+  ar = wg_sum(ar, sh);
+  ai = wg_sum(ai, sh);
+  vv = wg_sum(vv, sh);
+  const zd xks = xk * inv;
+  ar += xks.x * vk.x + xks.y * vk.y;
+  ai += xks.x * vk.y - xks.y * vk.x;
+  vv += vk.x * vk.x + vk.y * vk.y;
+  const zd tc = mkz(ar / vv, ai / vv);
+  double nd = 0.0;
+  for (int l = threadIdx.x; l < Kp; l += blockDim.x) {
+    const zd xl = u[l] * inv;
+    const zd d = zfms(tc, v[l], xl);
+    nd += d.x * d.x + d.y * d.y;
+    v[l] = xl;
+  }
+  nd = wg_sum(nd, sh);
+  if (threadIdx.x == 0) {
+    const zd d = zfms(tc, vk, xks);
+    nd += d.x * d.x + d.y * d.y;
+    cs[8 * j + CS_ND] = sqrt(2.0 * (Kp + 1)) * sqrt(nd);
+    vK[j] = xks;
+  }
+}
+
+// E_j = x^H H_j x / x^H S_j x with Y1 = S' x', Y2 = H' x'
+__global__ void __launch_bounds__(256) k_cand_rayleigh(int Kp, int ld, int C, const zd* __restrict__ V,
+                                                       const zd* __restrict__ vK, const zd* __restrict__ Y1,
+                                                       const zd* __restrict__ Y2, const zd* __restrict__ Hb,
+                                                       const zd* __restrict__ Bm, double* __restrict__ cs) {
+  __shared__ double sh[8];
+  const int j = blockIdx.x;
+  const zd* x = V + (size_t)j * ld;
+  const zd* y1 = Y1 + (size_t)j * ld;
+  const zd* y2 = Y2 + (size_t)j * ld;
+  const zd* h = Hb + (size_t)j * ld;
+  const zd* sv = Bm + (size_t)(C + j) * ld;
+  const zd xk = vK[j];
+  double s1 = 0.0, h1 = 0.0, sb = 0.0, hb = 0.0;
+  for (int l = threadIdx.x; l < Kp; l += blockDim.x) {
+    const zd xl = x[l];
+    s1 += xl.x * y1[l].x + xl.y * y1[l].y;
+    h1 += xl.x * y2[l].x + xl.y * y2[l].y;
+    // Re(conj(x_l) s_l x_K), Re(conj(x_l) h_l x_K)
+    const zd sx = sv[l] * xk, hx = h[l] * xk;
+    sb += xl.x * sx.x + xl.y * sx.y;
+    hb += xl.x * hx.x + xl.y * hx.y;
+  }
+  s1 = wg_sum(s1, sh);
+  h1 = wg_sum(h1, sh);
+  sb = wg_sum(sb, sh);
+  hb = wg_sum(hb, sh);
+  if (threadIdx.x == 0) {
+    const double xk2 = xk.x * xk.x + xk.y * xk.y;
+    const double xsx = s1 + 2.0 * sb + xk2;
+    const double xhx = h1 + 2.0 * hb + cs[8 * j + CS_HKK] * xk2;
+    cs[8 * j + CS_E] = xhx / xsx;
+    cs[8 * j + CS_XSX] = xsx;
+  }
+}
+
+// Loop bookkeeping (one workgroup): best candidate of the step; when it
+// lowers the current energy its parameters become function K and its
+// eigenvector (normalised x^H S x = 1) the next start vector.
+__global__ void __launch_bounds__(256) k_cand_select(int Kp, int ld, int C, int npt,
+                                                     const double* __restrict__ cs,
+                                                     const double* __restrict__ cparams,
+                                                     double* __restrict__ params, const zd* __restrict__ V,
+                                                     const zd* __restrict__ vK, zd* __restrict__ v0,
+                                                     double* __restrict__ state, double* __restrict__ rec) {
+  __shared__ double shE[256];
+  __shared__ int shI[256];
+  double e = INFINITY;
+  int bi = -1;
+  for (int j = threadIdx.x; j < C; j += blockDim.x) {
+    const double ej = cs[8 * j + CS_E];
+    if (ej < e) {   // NaN never wins
+      e = ej;
+      bi = j;
+    }
+  }
+  shE[threadIdx.x] = e;
+  shI[threadIdx.x] = bi;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      const double e2 = shE[threadIdx.x + o];
+      const int i2 = shI[threadIdx.x + o];
+      if (i2 >= 0 && (e2 < shE[threadIdx.x] || (e2 == shE[threadIdx.x] && i2 < shI[threadIdx.x]))) {
+        shE[threadIdx.x] = e2;
+        shI[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  const double eb = shE[0];
+  const int ib = shI[0];
+  const double ecur = state[0];
+  const bool accept = (ib >= 0) && (eb < ecur);
+  if (accept) {
+    for (int i = threadIdx.x; i < npt; i += blockDim.x) params[(size_t)Kp * npt + i] = cparams[(size_t)ib * npt + i];
+    const double sc = 1.0 / sqrt(cs[8 * ib + CS_XSX]);
+    for (int l = threadIdx.x; l < Kp; l += blockDim.x) v0[l] = V[(size_t)ib * ld + l] * sc;
+    if (threadIdx.x == 0) v0[Kp] = vK[ib] * sc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int step = (int)state[1];
+    rec[2 * step] = eb;
+    rec[2 * step + 1] = accept ? (double)ib : -1.0 - (double)(ib < 0 ? 0 : ib);
+    if (accept) state[0] = eb;
+    state[1] = step + 1.0;
+  }
+}
+
+int screen_batch(const Ctx& c, const Factor& f, Screen& sc, int C, int iters, bool from_loop, int npt,
+                 hipStream_t s) {
+  const int Kp = sc.Kp, ld = sc.ld;
+  const zd one = mkz(1.0, 0.0), zero = mkz(0.0, 0.0);
+  if (from_loop)
+    hipLaunchKernelGGL(k_cand_fetch, dim3((npt * C + 255) / 256), dim3(256), 0, s, npt * C, sc.call, sc.state,
+                       sc.cparams);
+  int rc = candidate_rows(c, Kp, C, sc.cparams, sc.cbra, sc.cket, sc.cdiag, sc.Hb, sc.Bm + (size_t)C * ld, ld, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_cand_border, dim3(C), dim3(256), 0, s, Kp, ld, C, sc.sigma, sc.Hb, sc.Bm, sc.cdiag, sc.cs,
+                     sc.v0, sc.V, sc.vK);
+  rc = zgemm(0, Kp, 2 * C, Kp, one, sc.G, ld, sc.Bm, ld, zero, sc.Wm, ld, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_cand_schur, dim3(C), dim3(256), 0, s, Kp, ld, sc.Bm, sc.Wm, sc.cs);
+  for (int it = 0; it < iters; ++it) {
+    rc = zgemm(0, Kp, C, Kp, one, sc.P, ld, sc.V, ld, zero, sc.U, ld, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_cand_iter, dim3(C), dim3(256), 0, s, Kp, ld, C, sc.U, sc.Wm, sc.Bm, sc.V, sc.vK, sc.cs);
+  }
+  rc = zgemm(0, Kp, C, Kp, one, c.S, ld, sc.V, ld, zero, sc.Y1, ld, s);
+  if (!rc) rc = zgemm(0, Kp, C, Kp, one, c.H, ld, sc.V, ld, zero, sc.Y2, ld, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_cand_rayleigh, dim3(C), dim3(256), 0, s, Kp, ld, C, sc.V, sc.vK, sc.Y1, sc.Y2, sc.Hb, sc.Bm,
+                     sc.cs);
+  (void)f;
+  return (int)hipGetLastError();
+}
+
+int screen_select(const Ctx& c, Screen& sc, int C, int npt, hipStream_t s) {
+  hipLaunchKernelGGL(k_cand_select, dim3(1), dim3(256), 0, s, sc.Kp, sc.ld, C, npt, sc.cs, sc.cparams, c.params,
+                     sc.V, sc.vK, sc.v0, sc.state, sc.rec);
+  return (int)hipGetLastError();
+}
+
+}  // namespace ecg

@@ -39,7 +39,7 @@ _NAMES = {
 EXPORTS = [
     "ecg_ctx_create", "ecg_ctx_destroy", "ecg_set_params", "ecg_update_params", "ecg_matrix_elements",
     "ecg_compute_mat_elem", "ecg_compute_mat_elem_and_deriv", "ecg_ghepiis", "ecg_energy", "ecg_energy_grad",
-    "ecg_last_timings",
+    "ecg_get_params", "ecg_screen_prepare", "ecg_screen", "ecg_screen_loop", "ecg_last_timings",
 ]
 
 
@@ -75,6 +75,10 @@ def load(path: str = LIB_PATH):
                               _d, _d, _d, _i]
     L.ecg_energy.argtypes = [vp, c_int, c_double, c_double, c_int, _d, _d, _d]
     L.ecg_energy_grad.argtypes = [vp, c_int, c_int, c_double, c_double, c_int, _d, _d, _d]
+    L.ecg_get_params.argtypes = [vp, _d]
+    L.ecg_screen_prepare.argtypes = [vp, c_double, c_double, c_int, _d]
+    L.ecg_screen.argtypes = [vp, c_int, _d, c_int, _d, _d]
+    L.ecg_screen_loop.argtypes = [vp, c_int, c_int, _d, c_int, _d, _i]
     L.ecg_last_timings.argtypes = [vp, _d, c_int]
     for name in EXPORTS:
         getattr(L, name).restype = c_int
@@ -223,6 +227,40 @@ class EcgContext:
                                     _p(g), _p(info))
         _check("ecg_energy_grad", rc, ok=(ECG_OK, ECG_NOT_CONVERGED))
         return rc, E.value, g, info
+
+    def get_params(self):
+        p = np.zeros((self.K, self.npt))
+        _check("ecg_get_params", load().ecg_get_params(self._h, _p(p)))
+        return p
+
+    # ---------------------------------------------------------- screening
+    def screen_prepare(self, sigma, tol=1e-12, maxiter=40):
+        """Fix functions 1..K-1 and the shift; (status, info) with info =
+        [current E, negative pivots of H'-sigma S', iterations, measure]."""
+        info = np.zeros(4)
+        rc = load().ecg_screen_prepare(self._h, float(sigma), float(tol), maxiter, _p(info))
+        _check("ecg_screen_prepare", rc, ok=(ECG_OK, ECG_NOT_CONVERGED))
+        return rc, info
+
+    def screen(self, cand, iters=4):
+        """Energies of candidates (ncand x npt) for function K;
+        (E, info) with info rows [measure, negative pivots, pivot, x^H S x]."""
+        cand = np.ascontiguousarray(cand, dtype=np.float64)
+        nc = cand.shape[0]
+        E = np.zeros(nc)
+        info = np.zeros((nc, 4))
+        _check("ecg_screen", load().ecg_screen(self._h, nc, _p(cand), iters, _p(E), _p(info)))
+        return E, info
+
+    def screen_loop(self, cand, iters=4):
+        """cand: (nsteps, ncand, npt); (Ebest, ibest) per step."""
+        cand = np.ascontiguousarray(cand, dtype=np.float64)
+        ns, nc = cand.shape[0], cand.shape[1]
+        Eb = np.zeros(ns)
+        ib = np.zeros(ns, np.int32)
+        _check("ecg_screen_loop", load().ecg_screen_loop(self._h, ns, nc, _p(cand), iters, _p(Eb),
+                                                         ib.ctypes.data_as(_i)))
+        return Eb, ib
 
     def timings(self):
         t = np.zeros(8)

@@ -115,6 +115,34 @@ module ecg_capi
       real(c_double), intent(out) :: E
       real(c_double), intent(out) :: grad(*), info(*)
     end function
+    integer(c_int) function ecg_get_params(ctx, params) bind(C, name="ecg_get_params")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: params(*)
+    end function
+    integer(c_int) function ecg_screen_prepare(ctx, sigma, tol, maxiter, info) bind(C, name="ecg_screen_prepare")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), value :: sigma, tol
+      integer(c_int), value :: maxiter
+      real(c_double), intent(out) :: info(*)
+    end function
+    integer(c_int) function ecg_screen(ctx, ncand, cand, iters, E, info) bind(C, name="ecg_screen")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ncand, iters
+      real(c_double), intent(in) :: cand(*)
+      real(c_double), intent(out) :: E(*), info(*)
+    end function
+    integer(c_int) function ecg_screen_loop(ctx, nsteps, ncand, cand, iters, Ebest, ibest) &
+        bind(C, name="ecg_screen_loop")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsteps, ncand, iters
+      real(c_double), intent(in) :: cand(*)
+      real(c_double), intent(out) :: Ebest(*)
+      integer(c_int), intent(out) :: ibest(*)
+    end function
     integer(c_int) function ecg_last_timings(ctx, ms, nmax) bind(C, name="ecg_last_timings")
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx

@@ -98,6 +98,73 @@ def cpu_baseline(system, params, K, seconds_target=15.0):
     }
 
 
+def screening_leg(system, base, K, ncand=256, nsteps=100, iters=4, plain_sample=32):
+    """Batched candidate screening for the last function (K functions, the
+    first K-1 fixed) against the plain per-candidate GPU path.
+
+    Batched: nsteps optimisation steps of ncand seeded candidates each,
+    one hipGraph replay per step (rows, Schur complements, `iters`
+    inverse iterations, Rayleigh quotients, accept-if-better).
+    Plain: the same candidates one at a time (ecg_screen with one
+    candidate: the per-candidate GEMMs become GEMVs, one launch sequence
+    and one host round trip per candidate), on a sample; and the full
+    per-candidate recomputation (new parameters, full assembly,
+    factorisation, inverse iteration) on a smaller sample."""
+    ctx = EcgContext(system)
+    ctx.set_params(base)
+    st, E0, _, _ = ctx.energy(1, float("nan"), 1e-12, 60)
+    sigma = E0 - 1e-6 * abs(E0)
+    t0 = time.perf_counter()
+    ctx.screen_prepare(sigma, 1e-12, 40)
+    t_prep = time.perf_counter() - t0
+    cand = np.stack([seeded_basis(system.n, ncand, SEED_BASE + 4 + 1000 * (s + 1)) for s in range(nsteps)])
+    rng = np.random.default_rng(SEED_BASE + 4)
+    for s in range(nsteps):   # a few near-copies of the current function, so that steps can improve
+        cand[s, :4] = base[-1] * (1 + 1e-4 * rng.standard_normal((4, system.npt)))
+    ctx.screen_loop(cand[:2], iters=iters)        # warm-up (graph build, first launches)
+    ctx.set_params(base)
+    ctx.screen_prepare(sigma, 1e-12, 40)
+    t0 = time.perf_counter()
+    Eb, ib = ctx.screen_loop(cand, iters=iters)
+    t_loop = time.perf_counter() - t0
+    dev_loop = ctx.timings()[0] * 1e-3
+    # accuracy of `iters` iterations: one step's candidates at iters and at 4 x iters
+    ctx.set_params(base)
+    ctx.screen_prepare(sigma, 1e-12, 40)
+    Ea, ia = ctx.screen(cand[0], iters=iters)
+    Ec, ic = ctx.screen(cand[0], iters=4 * iters)
+    err = float(np.max(np.abs(Ea - Ec)))
+    # plain per-candidate path
+    t0 = time.perf_counter()
+    for j in range(plain_sample):
+        ctx.screen(cand[0, j:j + 1], iters=iters)
+    t_plain = (time.perf_counter() - t0) / plain_sample
+    full_sample = 4
+    t0 = time.perf_counter()
+    for j in range(full_sample):
+        p = base.copy()
+        p[-1] = cand[0, j]
+        ctx.set_params(p)
+        ctx.energy(1, sigma, 1e-12, 40)
+    t_full = (time.perf_counter() - t0) / full_sample
+    ctx.close()
+    rate = ncand * nsteps / t_loop
+    return {
+        "workload": "H2-like n=3 (T=4), K=%d, %d seeded candidates for function K per step, %d steps in one "
+                    "hipGraph loop, %d inverse iterations per candidate" % (K, ncand, nsteps, iters),
+        "candidates_per_s": rate,
+        "ms_per_step": 1e3 * t_loop / nsteps,
+        "device_ms_per_step": 1e3 * dev_loop / nsteps,
+        "prepare_ms": 1e3 * t_prep,
+        "plain_candidates_per_s": 1.0 / t_plain,
+        "speedup_vs_plain": rate * t_plain,
+        "full_recompute_candidates_per_s": 1.0 / t_full,
+        "accepted_steps": int(np.sum(ib >= 0)),
+        "E_start": float(E0), "E_end": float(Eb[ib >= 0][-1]) if np.any(ib >= 0) else float(E0),
+        "max_dE_vs_%d_iterations" % (4 * iters): err,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +172,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--K", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-screen", action="store_true", help="skip the candidate-screening leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the path on a single MI355X "
@@ -187,6 +255,8 @@ def main():
             "flops_per_pair_term": {"energy": f0, "grad_bra": f1, "grad_both": f3},
         },
     }
+    if not args.no_screen:
+        out["screening"] = screening_leg(system, base, K)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(system, base, K)
     ctx.close()

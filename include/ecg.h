@@ -122,10 +122,47 @@ int ecg_energy(ecg_ctx* ctx, int which, double sigma, double tol, int maxiter, d
 int ecg_energy_grad(ecg_ctx* ctx, int nfru, int which, double sigma, double tol, int maxiter, double* E,
                     double* grad, double* info);
 
+/* Current parameters of the K functions (npt x K), e.g. after
+ * ecg_screen_loop accepted candidates into function K. */
+int ecg_get_params(ecg_ctx* ctx, double* params);
+
+/* Candidate screening for the last basis position: the batched
+ * replacement of the reference's trial loops (GenerateTrialParam and the
+ * trial loops of BasisEnl / OptCycle, src/workproc.f90:1375-1487,
+ * 3397-3413, 4054-4070), each trial of which recomputes row K of H, S,
+ * updates the LDL^H factor from row K (LDLHF, src/linalg.f90:316) and runs
+ * GHEPIIS (src/linalg.f90:1781) at the shift Glob_ApproxEnergy.
+ *
+ * ecg_screen_prepare fixes functions 1..K-1 of the current parameters and
+ * the shift sigma: factor of H - sigma S, G = (H' - sigma S')^{-1} and
+ * G S' of the leading K-1 block on the device, and the energy and
+ * eigenvector of the current basis at sigma (inverse iteration, tol and
+ * maxiter as GHEPIIS).  info (nullable, 4): [0] current energy, [1]
+ * negative pivots of H' - sigma S', [2] iterations, [3] convergence
+ * measure.  Returns ECG_SINGULAR if H' - sigma S' has a zero pivot. */
+int ecg_screen_prepare(ecg_ctx* ctx, double sigma, double tol, int maxiter, double* info);
+
+/* Energies of ncand candidates for function K (cand: npt x ncand; each
+ * replaces function K alone): `iters` inverse iterations at sigma from the
+ * current eigenvector on the bordered matrices, Rayleigh quotients.
+ * info (nullable, 4 per candidate): [0] convergence measure of the last
+ * iteration (GHEPIIS's), [1] negative pivots of H - sigma S with the
+ * candidate, [2] the candidate's pivot (Schur complement), [3] x^H S x. */
+int ecg_screen(ecg_ctx* ctx, int ncand, const double* cand, int iters, double* E, double* info);
+
+/* nsteps optimisation steps on the device (cand: npt x ncand x nsteps):
+ * each step screens its ncand candidates and keeps the best one as
+ * function K when it lowers the current energy.  One step is captured in
+ * a hipGraph and replayed nsteps times with no host synchronisation.
+ * Ebest[s]: best energy of step s; ibest[s]: accepted candidate index, or
+ * -1 - index of the best candidate when it was not accepted. */
+int ecg_screen_loop(ecg_ctx* ctx, int nsteps, int ncand, const double* cand, int iters, double* Ebest, int* ibest);
+
 /* Device time (ms, HIP events on the context's stream) of the phases of
  * the last ecg_energy / ecg_energy_grad call: [0] assembly, [1] eigen step
  * (all factorisations and iterations), [2] adjoint pass, [3] whole call,
  * [4] adjoint kernel alone, [5] off-diagonal assembly kernel alone.
+ * After ecg_screen / ecg_screen_loop, [0] is the device time of the call.
  * Returns the number of values written. */
 int ecg_last_timings(ecg_ctx* ctx, double* ms, int nmax);
 
