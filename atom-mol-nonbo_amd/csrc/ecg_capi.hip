@@ -43,6 +43,8 @@ struct ecg_ctx {
   double* pinned = nullptr;   // pinned host staging
   size_t pinned_cap = 0;
   ecg::Screen scr;            // candidate screening session
+  zd* dD = nullptr;           // stored derivatives (2 npt x nfo x K) of ecg_energy_grad_stored
+  size_t dD_cap = 0;
 };
 
 namespace {
@@ -80,13 +82,15 @@ int ensure_capacity(ecg_ctx* x, int K) {
   ecg::Factor& f = x->f;
   RCCHK(dev_alloc(&f.F, (size_t)ld * ld));
   RCCHK(dev_alloc(&f.D, (size_t)ld));
-  RCCHK(dev_alloc(&f.W, (size_t)ld * ecg::NB));
+  RCCHK(dev_alloc(&f.W, (size_t)ld * 2 * ecg::NB));
   RCCHK(dev_alloc(&f.Dinv, (size_t)(ld / ecg::NB) * 2 * ecg::NB * ecg::NB));
   RCCHK(dev_alloc(&f.status, (size_t)4));
   RCCHK(dev_alloc(&f.vec, (size_t)4 * ld));
   RCCHK(dev_alloc(&f.tvec, (size_t)ld));
   RCCHK(dev_alloc(&f.flags, (size_t)2 * (ld / ecg::NB)));
   HIPCHK(hipMemsetAsync(f.flags, 0, sizeof(int) * 2 * (ld / ecg::NB), x->stream));
+  RCCHK(dev_alloc(&f.pflag, (size_t)(ld / ecg::NB) + 2));
+  HIPCHK(hipMemsetAsync(f.pflag, 0, sizeof(int) * ((ld / ecg::NB) + 2), x->stream));
   HIPCHK(hipMemsetAsync(f.status, 0, sizeof(int) * 4, x->stream));
   f.epoch = 0;
   RCCHK(dev_alloc(&f.scal, (size_t)16));
@@ -520,6 +524,43 @@ int launch_one_pair(const ecg::Ctx& c, const ecg::SysConst& sc, const double* br
 
 }  // namespace
 
+// EnergyGB's contraction on the device (src/workproc.f90:1925-1940), for the
+// stored-derivative mode: one workgroup per optimised function k,
+//   W = sum_l c_l (D_H(:,k,l) - E D_S(:,k,l)),
+//   grad(:,k) = 2 Re(conj(c_k) W) - Re(|c_k|^2 (D_H - E D_S)(:,k,k+nfru)).
+// D(m, k, l) at m + 2 npt (k + nfo l); thread -> (m, l-group) so that a
+// group of npt threads reads one contiguous column chunk.  Fixed-order
+// reduction over the l-groups.
+__global__ void __launch_bounds__(256) k_contract_stored(int npt, int nfo, int nfa, int nfru,
+                                                         const zd* __restrict__ D, const zd* __restrict__ c,
+                                                         double E, double* __restrict__ grad) {
+  __shared__ zd part[256];
+  const int k = blockIdx.x;
+  const int groups = blockDim.x / npt;
+  const int t = threadIdx.x;
+  const int m = t % npt, g = t / npt;
+  zd w = ecg::mkz(0.0, 0.0);
+  if (g < groups) {
+    for (int l = g; l < nfa; l += groups) {
+      const zd* col = D + (size_t)2 * npt * ((size_t)k + (size_t)nfo * l);
+      const zd d = ecg::zfms(ecg::mkz(E, 0.0), col[m + npt], col[m]);
+      w = ecg::zfma(c[l], d, w);
+    }
+  }
+  part[t] = w;
+  __syncthreads();
+  if (t < npt) {
+    zd W = ecg::mkz(0.0, 0.0);
+    for (int q = 0; q < groups; ++q) W = W + part[q * npt + t];
+    const zd ck = c[k + nfru];
+    const zd* col = D + (size_t)2 * npt * ((size_t)k + (size_t)nfo * (k + nfru));
+    const zd d = ecg::zfms(ecg::mkz(E, 0.0), col[t + npt], col[t]);
+    const double g2 = 2.0 * (ck.x * W.x + ck.y * W.y);
+    const double cc = ck.x * ck.x + ck.y * ck.y;
+    grad[(size_t)k * npt + t] = g2 - cc * d.x;
+  }
+}
+
 extern "C" {
 
 int ecg_ctx_create(int n, int T, const int* yhy_mats, const int* yhy_coef, const double* mass, double q0,
@@ -572,12 +613,13 @@ int ecg_ctx_destroy(ecg_ctx* x) {
   if (x->stream) (void)hipStreamSynchronize(x->stream);
   ecg::Ctx& c = x->c;
   void* ptrs[] = {c.params, c.bra, c.ket, c.diagS, c.diagH, c.H, c.S, x->f.F, x->f.D, x->f.W, x->f.Dinv,
-                  x->f.status, x->f.vec, x->f.tvec, x->f.flags, x->f.scal, x->part, x->grad, x->ev[0],
+                  x->f.status, x->f.vec, x->f.tvec, x->f.flags, x->f.pflag, x->f.scal, x->part, x->grad, x->ev[0],
                   x->ev[1], x->ev[2]};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (x->pinned) (void)hipHostFree(x->pinned);
   free_screen(x->scr);
+  if (x->dD) (void)hipFree(x->dD);
   for (int i = 0; i < 8; ++i)
     if (x->e[i]) (void)hipEventDestroy(x->e[i]);
   if (x->stream) (void)hipStreamDestroy(x->stream);
@@ -827,6 +869,47 @@ int ecg_energy_grad(ecg_ctx* x, int nfru, int which, double sigma, double tol, i
   x->tms[3] = ev_ms(x->e[0], x->e[7]);
   x->tms[4] = ev_ms(x->e[3], x->e[4]);
   x->tms[5] = ev_ms(x->e[5], x->e[6]);
+  if (!std::isfinite(*E) || !finite_all(grad, ng)) return ECG_ERR_NONFINITE;
+  return status;
+}
+
+int ecg_energy_grad_stored(ecg_ctx* x, int nfru, int which, double sigma, double tol, int maxiter, double* E,
+                           double* grad, double* info) {
+  if (!x || !E || !grad || which < 1 || maxiter < 1) return ECG_ERR_ARG;
+  ecg::Ctx& c = x->c;
+  const int K = c.K;
+  if (K < 1) return ECG_ERR_STATE;
+  if (nfru < 0 || nfru >= K || which > K) return ECG_ERR_ARG;
+  const int nfo = K - nfru;
+  const size_t nD = (size_t)2 * x->npt * nfo * K;
+  if (nD > x->dD_cap) {
+    RCCHK(dev_alloc(&x->dD, nD));
+    x->dD_cap = nD;
+  }
+  const size_t ng = (size_t)x->npt * nfo;
+  if (ng > x->grad_cap) {
+    RCCHK(dev_alloc(&x->grad, ng));
+    x->grad_cap = ng;
+  }
+  HIPCHK(hipEventRecord(x->e[0], x->stream));
+  if (nfru > 0) RCCHK(ecg::assemble(c, 0, nfru, x->stream));
+  RCCHK(ecg::assemble_deriv(c, nfru, x->dD, x->stream));
+  HIPCHK(hipEventRecord(x->e[1], x->stream));
+  int status = 0;
+  RCCHK(solve_root(x, which, sigma, tol, maxiter, E, info, &status));
+  HIPCHK(hipEventRecord(x->e[2], x->stream));
+  hipLaunchKernelGGL(k_contract_stored, dim3(nfo), dim3(256), 0, x->stream, x->npt, nfo, K, nfru, x->dD, x->ev[2],
+                     *E, x->grad);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(x->e[7], x->stream));
+  HIPCHK(hipMemcpyAsync(grad, x->grad, sizeof(double) * ng, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  x->tms[0] = ev_ms(x->e[0], x->e[1]);
+  x->tms[1] = ev_ms(x->e[1], x->e[2]);
+  x->tms[2] = ev_ms(x->e[2], x->e[7]);
+  x->tms[3] = ev_ms(x->e[0], x->e[7]);
+  x->tms[4] = x->tms[2];
+  x->tms[5] = 0.0;
   if (!std::isfinite(*E) || !finite_all(grad, ng)) return ECG_ERR_NONFINITE;
   return status;
 }

@@ -61,7 +61,7 @@ struct Factor {
   int K = 0, ld = 0, cap = 0;
   zd* F = nullptr;      // ld x ld: L (unit lower, strict part) after factorisation
   double* D = nullptr;  // K pivots
-  zd* W = nullptr;      // ld x NB panel scratch
+  zd* W = nullptr;      // ld x 2 NB panel scratch (double-buffered by block column)
   zd* Dinv = nullptr;   // nblocks x 2 x NB x NB: inverse X of each diagonal block of L, then X^H
   int* status = nullptr;     // [0] negative pivots, [1] first zero pivot (1-based) or 0,
                              // [2] triangular-solve wait limit exceeded
@@ -69,6 +69,7 @@ struct Factor {
   zd* tvec = nullptr;   // ld: D^{-1} L^{-1} b between the two solves
   double* scal = nullptr;    // 16 scalars
   int* flags = nullptr;      // 2 x ld/NB block-ready flags of the triangular solves
+  int* pflag = nullptr;      // ld/NB + 1 ready flags of the factorisation (diagonal block, column tiles)
   int epoch = 0;             // flag value of the current solve
 };
 int form_shifted(const Ctx& c, Factor& f, double sigma, hipStream_t s);

@@ -29,12 +29,36 @@ int form_shifted(const Ctx& c, Factor& f, double sigma, hipStream_t s) {
   return (int)hipGetLastError();
 }
 
+// ------------------------------------------------ workgroup flags
+constexpr long FACT_SPIN_LIMIT = 1L << 20;
+
+__device__ inline void publish_wg(int* flag, int epoch) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline void wait_wg(const int* f1, const int* f2, int epoch, int* status) {
+  if (threadIdx.x == 0) {
+    long n = 0;
+    while (__hip_atomic_load(f1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch ||
+           __hip_atomic_load(f2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+      __builtin_amdgcn_s_sleep(2);
+      if (++n > FACT_SPIN_LIMIT) {
+        atomicExch(&status[2], 1);
+        break;
+      }
+    }
+  }
+  __syncthreads();
+}
+
 // --------------------------------------------------- diagonal block LDL^H
 // One workgroup of 256 threads (4 waves) factors the jb x jb diagonal block
 // at (j0, j0), held in one 64 x 64 LDS array, in 16-wide sub-blocks s:
-//   A. wave 0 factors the 16 x 16 diagonal piece with the piece in
-//      registers (lane = row i, column quad g) and one LDS broadcast of two
-//      pivot rows per pivot pair; the same row operations applied to the
+//   A. the four waves factor the 16 x 16 diagonal piece with the piece in
+//      registers (one element per lane) and one LDS broadcast of two pivot
+//      rows per pivot pair; the same row operations applied to the
 //      identity give X_ss = L_ss^{-1} (Gauss-Jordan);
 //   B. the sub-panel below: L = A X_ss^H D^{-1}, one 16-row tile per wave
 //      on MFMA;
@@ -113,7 +137,8 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
                                                                      double* __restrict__ D,
                                                                      zd* __restrict__ Dinv,
                                                                      int* __restrict__ status, DiagLds& L,
-                                                                     bool a_in_lds) {
+                                                                     bool a_in_lds, int* pub = nullptr,
+                                                                     int epoch = 0) {
   const int jb = min(NB, K - j0);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (!a_in_lds) {
@@ -134,22 +159,21 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
   for (int s = 0; s < nsub; ++s) {
     const int c = s * SB;
     const int sb = min(SB, jb - c);
-    // ---- A: 16 x 16 LDL^H and X_ss, wave 0, two pivots (p, q = p + 1) per
-    // broadcast.  With a = A(p,p), b = A(q,p), c' = A(q,q) and
+    // ---- A: 16 x 16 LDL^H and X_ss, all four waves (lane = one element:
+    // row 4 wave + lane / 16, column lane % 16), two pivots (p, q = p + 1)
+    // per broadcast.  With a = A(p,p), b = A(q,p), c' = A(q,q) and
     // det = a c' - |b|^2: d_p = a, L(q,p) = b / a, d_q = det / a, and both
     // reciprocals come from one division, r = 1 / (a det): 1/a = det r,
     // 1/d_q = a^2 r.
-    if (wave == 0) {
-      const int i = lane & 15, g = lane >> 4;
-      zd a[4], x[4];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int j = 4 * g + t;
+    {
+      const int i = 4 * wave + (lane >> 4), j = lane & 15;
+      zd a, x;
+      {
         const zd lo = L.A[c + max(i, j)][c + min(i, j)];
         const bool in = (i < sb && j < sb);
         const double id = (i == j) ? 1.0 : 0.0;
-        a[t] = in ? mkz(lo.x, j <= i ? lo.y : -lo.y) : mkz(id, 0.0);
-        x[t] = mkz(id, 0.0);
+        a = in ? mkz(lo.x, j <= i ? lo.y : -lo.y) : mkz(id, 0.0);
+        x = mkz(id, 0.0);
       }
 #pragma unroll
       for (int pp = 0; pp < SB / 2; ++pp) {
@@ -158,27 +182,19 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
         zd* rQ = L.rowA[pp & 1][1];
         zd* xP = L.rowX[pp & 1][0];
         zd* xQ = L.rowX[pp & 1][1];
-        if (i == p || i == q) {
-          zd* ra = (i == p) ? rP : rQ;
-          zd* rx = (i == p) ? xP : xQ;
-#pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            ra[4 * g + t] = a[t];
-            rx[4 * g + t] = x[t];
-          }
+        if (i == p) {
+          rP[j] = a;
+          xP[j] = x;
+        } else if (i == q) {
+          rQ[j] = a;
+          xQ[j] = x;
         }
-        wave_sync();
+        __syncthreads();
         const double ap = rP[p].x, cq = rQ[q].x;
         const zd bq = rQ[p];
         const zd api = rP[i], aqi0 = rQ[i];
-        zd apj[4], aqj[4], xpj[4], xqj[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          apj[t] = rP[4 * g + t];
-          aqj[t] = rQ[4 * g + t];
-          xpj[t] = xP[4 * g + t];
-          xqj[t] = xQ[4 * g + t];
-        }
+        const zd apj = rP[j], xpj = xP[j];
+        zd aqj = rQ[j], xqj = xQ[j];
         const double det = fma(ap, cq, -fma(bq.x, bq.x, bq.y * bq.y));
         const double prod = ap * det;
         double rd1, rd2, dq;
@@ -194,30 +210,20 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
         }
         const zd l = bq * rd1;   // L(q, p)
         // row q after pivot p: A'(q, .) = A(q, .) - L(q,p) A(p, .), same for X
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          aqj[t] = zfms(l, apj[t], aqj[t]);
-          xqj[t] = zfms(l, xpj[t], xqj[t]);
-        }
+        aqj = zfms(l, apj, aqj);
+        xqj = zfms(l, xpj, xqj);
         if (i > q) {
           const zd lp = zconj(api) * rd1;                   // L(i, p)
           const zd lq = zconj(zfms(l, api, aqi0)) * rd2;    // L(i, q) = conj(A'(q, i)) / d_q
-#pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            const int j = 4 * g + t;
-            if (j > q) a[t] = zfms(lq, aqj[t], zfms(lp, apj[t], a[t]));
-            if (j == p) a[t] = lp;
-            if (j == q) a[t] = lq;
-            x[t] = zfms(lq, xqj[t], zfms(lp, xpj[t], x[t]));
-          }
+          if (j > q) a = zfms(lq, aqj, zfms(lp, apj, a));
+          if (j == p) a = lp;
+          if (j == q) a = lq;
+          x = zfms(lq, xqj, zfms(lp, xpj, x));
         } else if (i == q) {
-#pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            if (4 * g + t == p) a[t] = l;
-            x[t] = xqj[t];
-          }
+          if (j == p) a = l;
+          x = xqj;
         }
-        if (lane == 0) {
+        if (tid == 0) {
           if (p < sb) {
             L.d[c + p] = ap;
             if (ap < 0.0) atomicAdd(&status[0], 1);
@@ -230,13 +236,9 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
           }
         }
       }
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int j = 4 * g + t;
-        if (i < sb && j < i) {
-          L.A[c + i][c + j] = a[t];
-          L.A[c + j][c + i] = x[t];
-        }
+      if (i < sb && j < i) {
+        L.A[c + i][c + j] = a;
+        L.A[c + j][c + i] = x;
       }
     }
     __syncthreads();
@@ -333,7 +335,8 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
     }
   }
   __syncthreads();
-  // ---- outputs
+  // ---- outputs: X, X^H and D first (all the next panel needs), then the
+  // flag (look-ahead use), then L and its mirror in F
   zd* Xg = Dinv + (size_t)(j0 / NB) * 2 * NB * NB;
   zd* XHg = Xg + NB * NB;
   {
@@ -348,11 +351,17 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
     for (int u = 0; u < NB * NB / 256; ++u) {
       const int e = tid + 256 * u, j = e >> 6, i = e & (NB - 1);
       const bool in = (i < jb && j < jb);
-      if (in && i > j) F[(size_t)(j0 + j) * ld + j0 + i] = lo[u];
       const double dg = (in && i == j) ? 1.0 : 0.0;
       Xg[(size_t)j * NB + i] = (in && i > j) ? up[u] : mkz(dg, 0.0);
       // X^H(i, j) = conj(X(j, i)), X(j, i) (j > i) at A[i][j]
       XHg[(size_t)j * NB + i] = (in && i < j) ? zconj(lo[u]) : mkz(dg, 0.0);
+    }
+    for (int p = tid; p < jb; p += blockDim.x) D[j0 + p] = L.d[p];
+    if (pub) publish_wg(pub, epoch);
+#pragma unroll
+    for (int u = 0; u < NB * NB / 256; ++u) {
+      const int e = tid + 256 * u, j = e >> 6, i = e & (NB - 1);
+      if (i < jb && j < jb && i > j) F[(size_t)(j0 + j) * ld + j0 + i] = lo[u];
     }
 #pragma unroll
     for (int u = 0; u < NB * NB / 256; ++u) {
@@ -366,7 +375,6 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
       if (i < jb && j < jb && i > j) F[(size_t)(j0 + i) * ld + j0 + j] = zconj(lo[u]);
     }
   }
-  for (int p = tid; p < jb; p += blockDim.x) D[j0 + p] = L.d[p];
 }
 
 __global__ void __launch_bounds__(256) k_ldl_diag(int K, int ld, int j0, zd* __restrict__ F,
@@ -444,39 +452,40 @@ __device__ inline void tile_pos(int a, int q, int v, int& r, int& c) {
 }
 
 // ------------------------------------------------------------ panel solve
-// 16-row strip i0..i0+15 below the diagonal block: W = A X^H (so that
-// W L_JJ^H = A), L = W D^{-1}; L goes to F (and L^H to its upper
-// triangle), W to the panel scratch used by the trailing update.  One
-// workgroup per strip, wave w computes columns 16w..16w+15 as one 16 x 16
-// MFMA tile (k = 64 in four chunks), which keeps the panel's latency at a
-// single tile's.
-__global__ void __launch_bounds__(256) k_ldl_panel(int K, int ld, int j0, zd* __restrict__ F,
-                                                   const double* __restrict__ D,
-                                                   const zd* __restrict__ Dinv, zd* __restrict__ W) {
-  const int jb = min(NB, K - j0);
-  const int i0 = j0 + jb + blockIdx.x * SB;
+// 16-row strip i0..i0+15 below the diagonal block at j0: W = A X^H (so
+// that W L_JJ^H = A), L = W D^{-1}; L goes to F (and L^H to its upper
+// triangle), W to the panel scratch used by the trailing update.  Wave w
+// computes columns 16w..16w+15 as one 16 x 16 MFMA tile: all 16 k-steps'
+// operand fragments are loaded first (one memory latency), then the 64
+// MFMAs.
+__device__ __attribute__((always_inline)) inline void panel_strip(int K, int ld, int j0, int i0, zd* __restrict__ F,
+                                                                  const double* __restrict__ D,
+                                                                  const zd* __restrict__ Dinv, zd* __restrict__ W) {
   const int wave = threadIdx.x >> 6;
   const int c0 = SB * wave;
-  if (c0 >= jb) return;
   const zd* X = Dinv + (size_t)(j0 / NB) * 2 * NB * NB;   // X(i, j) at X[j * NB + i]
   const zd* A = F + (size_t)j0 * ld + i0;                  // A(r, k) at A[k * ld + r]
+  const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+  zd av[NB / 4], bv[NB / 4];
+#pragma unroll
+  for (int t = 0; t < NB / 4; ++t) {
+    const int kk = 4 * t + lk;
+    av[t] = (i0 + lr < K) ? A[(size_t)kk * ld + lr] : mkz(0.0, 0.0);
+    bv[t] = zconj(X[(size_t)kk * NB + c0 + lr]);
+  }
   Acc16 acc;
   acc16_zero(acc);
-  for (int k0 = 0; k0 < jb; k0 += SB) {
-    mfma16([&](int r, int k) {
-             const int kk = k0 + k;
-             return (i0 + r < K && kk < jb) ? A[(size_t)kk * ld + r] : mkz(0.0, 0.0);
-           },
-           [&](int k, int cc) {
-             const int kk = k0 + k;
-             return (kk < jb) ? zconj(X[(size_t)kk * NB + c0 + cc]) : mkz(0.0, 0.0);
-           },
-           acc);
+#pragma unroll
+  for (int t = 0; t < NB / 4; ++t) {
+    acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t].x, bv[t].x, acc.re, 0, 0, 0);
+    acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[t].y, bv[t].y, acc.re, 0, 0, 0);
+    acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t].x, bv[t].y, acc.im, 0, 0, 0);
+    acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t].y, bv[t].x, acc.im, 0, 0, 0);
   }
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int r = i0 + acc16_row(v), c = c0 + acc16_col();
-    if (r < K && c < jb) {
+    if (r < K) {
       const zd w = mkz(acc.re[v], acc.im[v]);
       const zd l = w * (1.0 / D[j0 + c]);
       W[(size_t)c * ld + r] = w;
@@ -486,18 +495,48 @@ __global__ void __launch_bounds__(256) k_ldl_panel(int K, int ld, int j0, zd* __
   }
 }
 
+// Panel of the first block column (later panels are computed inside the
+// trailing update of the previous column).  Needs a full 64-wide block.
+__global__ void __launch_bounds__(256) k_ldl_panel(int K, int ld, int j0, zd* __restrict__ F,
+                                                   const double* __restrict__ D,
+                                                   const zd* __restrict__ Dinv, zd* __restrict__ W) {
+  panel_strip(K, ld, j0, j0 + NB + blockIdx.x * SB, F, D, Dinv, W);
+}
+
 // -------------------------------------------------------- trailing update
-// Lower block pairs (I >= Kb) of the trailing matrix: C_IK -= W_I L_K^H.
-// The workgroup that updates the first trailing diagonal block then
-// factors it (look-ahead), so the next block column needs no separate
-// diagonal launch.
+// One launch per block column j (diagonal block at j0, trailing blocks
+// from t0 = j + 1):
+//   workgroups 0..ntiles-1: lower block pairs (I >= Kb) of the trailing
+//     matrix, C_IK -= W_I L_K^H.  The one that updates the first trailing
+//     diagonal block then factors it (look-ahead).
+//   workgroups ntiles..: 16-row strips of the next block column's panel.
+//     Each waits until the tile holding its rows and the look-ahead factor
+//     are published, then runs the panel solve for its strip.
+// Publication: the producer's threads fence at agent scope, the workgroup
+// synchronises, thread 0 stores the flag (release).  A waiting workgroup's
+// thread 0 polls the flags (relaxed, bounded: after SPIN_LIMIT rounds it
+// records status[2] and proceeds, so every workgroup exits); the data it
+// then reads was written in this launch by the producer only, and caches
+// hold no older copy of it (they are invalidated at kernel start).
+// Waits only point to lower workgroup indices, dispatched no later.
+// W is double-buffered by column parity: the tiles read column j's, the
+// strips write column j+1's.
 __global__ void __launch_bounds__(256) k_ldl_update(int K, int ld, int j0, zd* __restrict__ F,
-                                                    const zd* __restrict__ W, double* __restrict__ D,
-                                                    zd* __restrict__ Dinv, int* __restrict__ status) {
+                                                    const zd* __restrict__ Wcur, zd* __restrict__ Wnext,
+                                                    double* __restrict__ D, zd* __restrict__ Dinv,
+                                                    int* __restrict__ status, int* __restrict__ pflag,
+                                                    int epoch, int ntiles) {
   __shared__ DiagLds Ld;
   const int jb = min(NB, K - j0);
   const int t0 = (j0 + jb) / NB;
   const int b = blockIdx.x;
+  if (b >= ntiles) {
+    const int i0 = (t0 + 1) * NB + (b - ntiles) * SB;
+    const int bi = (i0 - t0 * NB) / NB;
+    wait_wg(&pflag[0], &pflag[1 + bi], epoch, status);
+    panel_strip(K, ld, t0 * NB, i0, F, D, Dinv, Wnext);
+    return;
+  }
   int bi = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
   while ((bi + 1) * (bi + 2) / 2 <= b) ++bi;
   while (bi * (bi + 1) / 2 > b) --bi;
@@ -505,7 +544,7 @@ __global__ void __launch_bounds__(256) k_ldl_update(int K, int ld, int j0, zd* _
   const int I0 = (t0 + bi) * NB, K0 = (t0 + bk) * NB;
   const int ib = min(NB, K - I0), kb = min(NB, K - K0);
   Tile64 t;
-  mfma_tile_abh(W + I0, ld, ib, F + (size_t)j0 * ld + K0, ld, kb, jb, t);
+  mfma_tile_abh(Wcur + I0, ld, ib, F + (size_t)j0 * ld + K0, ld, kb, jb, t);
   const bool lookahead = (bi == 0 && bk == 0);
   if (lookahead)
     for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) Ld.A[e >> 6][e & (NB - 1)] = mkz(0.0, 0.0);
@@ -530,7 +569,9 @@ __global__ void __launch_bounds__(256) k_ldl_update(int K, int ld, int j0, zd* _
     // this workgroup owns the next diagonal block (its values are in LDS):
     // factor it now instead of in a separate launch
     __syncthreads();
-    ldl_diag_block(K, ld, I0, F, D, Dinv, status, Ld, true);
+    ldl_diag_block(K, ld, I0, F, D, Dinv, status, Ld, true, &pflag[0], epoch);
+  } else if (bk == 0) {
+    publish_wg(&pflag[1 + bi], epoch);
   }
 }
 
@@ -542,15 +583,20 @@ int ldlh_factor(Factor& f, hipStream_t s) {
   const int K = f.K;
   hipLaunchKernelGGL(k_clear_status, dim3(1), dim3(64), 0, s, f.status);
   hipLaunchKernelGGL(k_ldl_diag, dim3(1), dim3(256), 0, s, K, f.ld, 0, f.F, f.D, f.Dinv, f.status);
-  for (int j0 = 0; j0 < K; j0 += NB) {
-    const int jb = (K - j0 < NB) ? K - j0 : NB;
-    const int rest = K - (j0 + jb);
-    if (rest <= 0) break;
-    const int nrb = (rest + NB - 1) / NB;
-    hipLaunchKernelGGL(k_ldl_panel, dim3((rest + SB - 1) / SB), dim3(256), 0, s, K, f.ld, j0, f.F, f.D, f.Dinv,
+  if (K > NB)
+    hipLaunchKernelGGL(k_ldl_panel, dim3((K - NB + SB - 1) / SB), dim3(256), 0, s, K, f.ld, 0, f.F, f.D, f.Dinv,
                        f.W);
-    hipLaunchKernelGGL(k_ldl_update, dim3(nrb * (nrb + 1) / 2), dim3(256), 0, s, K, f.ld, j0, f.F, f.W, f.D,
-                       f.Dinv, f.status);
+  for (int j0 = 0, col = 0; j0 + NB < K; j0 += NB, ++col) {
+    const int rest = K - (j0 + NB);   // rows below block column col
+    const int nrb = (rest + NB - 1) / NB;
+    const int ntiles = nrb * (nrb + 1) / 2;
+    const int rest2 = rest - NB;      // rows below the next block column
+    const int nstrip = rest2 > 0 ? (rest2 + SB - 1) / SB : 0;
+    zd* Wcur = f.W + (size_t)(col & 1) * f.ld * NB;
+    zd* Wnext = f.W + (size_t)((col + 1) & 1) * f.ld * NB;
+    const int epoch = ++f.epoch;
+    hipLaunchKernelGGL(k_ldl_update, dim3(ntiles + nstrip), dim3(256), 0, s, K, f.ld, j0, f.F, Wcur, Wnext, f.D,
+                       f.Dinv, f.status, f.pflag, epoch, ntiles);
   }
   return (int)hipGetLastError();
 }

@@ -39,7 +39,7 @@ _NAMES = {
 EXPORTS = [
     "ecg_ctx_create", "ecg_ctx_destroy", "ecg_set_params", "ecg_update_params", "ecg_matrix_elements",
     "ecg_compute_mat_elem", "ecg_compute_mat_elem_and_deriv", "ecg_ghepiis", "ecg_energy", "ecg_energy_grad",
-    "ecg_get_params", "ecg_screen_prepare", "ecg_screen", "ecg_screen_loop", "ecg_last_timings",
+    "ecg_energy_grad_stored", "ecg_get_params", "ecg_screen_prepare", "ecg_screen", "ecg_screen_loop", "ecg_last_timings",
 ]
 
 
@@ -75,6 +75,7 @@ def load(path: str = LIB_PATH):
                               _d, _d, _d, _i]
     L.ecg_energy.argtypes = [vp, c_int, c_double, c_double, c_int, _d, _d, _d]
     L.ecg_energy_grad.argtypes = [vp, c_int, c_int, c_double, c_double, c_int, _d, _d, _d]
+    L.ecg_energy_grad_stored.argtypes = [vp, c_int, c_int, c_double, c_double, c_int, _d, _d, _d]
     L.ecg_get_params.argtypes = [vp, _d]
     L.ecg_screen_prepare.argtypes = [vp, c_double, c_double, c_int, _d]
     L.ecg_screen.argtypes = [vp, c_int, _d, c_int, _d, _d]
@@ -226,6 +227,17 @@ class EcgContext:
         rc = load().ecg_energy_grad(self._h, nfru, which, float(sigma), float(tol), maxiter, ctypes.byref(E),
                                     _p(g), _p(info))
         _check("ecg_energy_grad", rc, ok=(ECG_OK, ECG_NOT_CONVERGED))
+        return rc, E.value, g, info
+
+    def energy_grad_stored(self, nfru=0, which=1, sigma=float("nan"), tol=1e-12, maxiter=20):
+        """As energy_grad, with the per-pair derivatives stored on the device
+        and contracted afterwards (the reference's scheme)."""
+        E = ctypes.c_double()
+        info = np.zeros(8)
+        g = np.zeros(self.npt * (self.K - nfru))
+        rc = load().ecg_energy_grad_stored(self._h, nfru, which, float(sigma), float(tol), maxiter,
+                                           ctypes.byref(E), _p(g), _p(info))
+        _check("ecg_energy_grad_stored", rc, ok=(ECG_OK, ECG_NOT_CONVERGED))
         return rc, E.value, g, info
 
     def get_params(self):

@@ -115,6 +115,15 @@ module ecg_capi
       real(c_double), intent(out) :: E
       real(c_double), intent(out) :: grad(*), info(*)
     end function
+    integer(c_int) function ecg_energy_grad_stored(ctx, nfru, which, sigma, tol, maxiter, E, grad, info) &
+        bind(C, name="ecg_energy_grad_stored")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nfru, which, maxiter
+      real(c_double), value :: sigma, tol
+      real(c_double), intent(out) :: E
+      real(c_double), intent(out) :: grad(*), info(*)
+    end function
     integer(c_int) function ecg_get_params(ctx, params) bind(C, name="ecg_get_params")
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx

@@ -165,6 +165,41 @@ def screening_leg(system, base, K, ncand=256, nsteps=100, iters=4, plain_sample=
     }
 
 
+def memory_modes_leg(system, Ks=(1000, 4000), reps=3):
+    """E+grad at K=1000 and K=4000 in the two memory modes: recompute (the
+    fused adjoint pass; O(K) extra device memory) and stored (per-pair
+    derivatives in Glob_D's layout, 2 npt x K x K complex, then the
+    contraction pass)."""
+    out = []
+    for K in Ks:
+        base = seeded_basis(system.n, K, SEED_BASE + 3 + K)
+        ctx = EcgContext(system)
+        ctx.set_params(base)
+        st, E0, _, _ = ctx.energy(1, float("nan"), 1e-12, 60)
+        sigma = E0 - 1e-6 * abs(E0)
+        row = {"K": K, "pairs": K * (K + 1) // 2, "D_bytes": 2 * system.npt * K * K * 16}
+        for mode in ("recompute", "stored"):
+            fn = ctx.energy_grad if mode == "recompute" else ctx.energy_grad_stored
+            fn(0, 1, sigma, 1e-12, 30)   # warm-up (allocation)
+            ts, tm = [], []
+            for r in range(reps):
+                t0 = time.perf_counter()
+                st, E, g, inf = fn(0, 1, sigma, 1e-12, 30)
+                ts.append(time.perf_counter() - t0)
+                tm.append(ctx.timings())
+            tm = np.mean(tm, axis=0)
+            row[mode] = {"ms": 1e3 * float(np.mean(ts)), "pair_elements_per_s": row["pairs"] / float(np.mean(ts)),
+                         "assembly_ms": float(tm[0]), "eigen_ms": float(tm[1]), "gradient_pass_ms": float(tm[2]),
+                         "E": float(E)}
+            if mode == "recompute":
+                g_ref = g
+            else:
+                row["max_grad_diff"] = float(np.max(np.abs(g - g_ref)))
+        ctx.close()
+        out.append(row)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +208,7 @@ def main():
     ap.add_argument("--K", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-screen", action="store_true", help="skip the candidate-screening leg")
+    ap.add_argument("--no-modes", action="store_true", help="skip the stored/recompute memory-mode leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the path on a single MI355X "
@@ -257,6 +293,8 @@ def main():
     }
     if not args.no_screen:
         out["screening"] = screening_leg(system, base, K)
+    if not args.no_modes:
+        out["memory_modes"] = memory_modes_leg(system)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(system, base, K)
     ctx.close()

@@ -122,6 +122,16 @@ int ecg_energy(ecg_ctx* ctx, int which, double sigma, double tol, int maxiter, d
 int ecg_energy_grad(ecg_ctx* ctx, int nfru, int which, double sigma, double tol, int maxiter, double* E,
                     double* grad, double* info);
 
+/* The same energy and gradient in the reference's stored-derivative
+ * scheme (ComputeMatElemAndDeriv + StoreHSD + EnergyGB's contraction,
+ * src/matform.f90:234-360, src/workproc.f90:1925-1940): the per-pair
+ * derivatives of rows nfru+1..K are stored on the device in Glob_D's layout
+ * (2 npt x (K-nfru) x K complex: 1.5 GB at K=2000, n=3) and contracted with
+ * the eigenvector in a second pass.  Timings: [0] assembly with stored
+ * derivatives, [1] eigen step, [2] contraction, [3] whole call. */
+int ecg_energy_grad_stored(ecg_ctx* ctx, int nfru, int which, double sigma, double tol, int maxiter, double* E,
+                           double* grad, double* info);
+
 /* Current parameters of the K functions (npt x K), e.g. after
  * ecg_screen_loop accepted candidates into function K. */
 int ecg_get_params(ecg_ctx* ctx, double* params);

@@ -381,3 +381,26 @@ def test_seeded_configuration_vs_oracle(name, K, seed, grad):
         assert abs(E - lam) <= 1e-10 * max(1.0, abs(lam))
         assert abs(E0 - lam) <= 1e-10 * max(1.0, abs(lam))
     ctx.close()
+
+
+@pytest.mark.parametrize("name,K,nfru,seed", [("h2", 36, 12, 700), ("li", 30, 0, 701), ("h2", 1000, 0, 2)])
+def test_energy_grad_stored_mode(name, K, nfru, seed):
+    """Stored-derivative mode (D in Glob_D's layout + EnergyGB contraction
+    on the device) against the recompute mode and, at small K, the oracle."""
+    sy = named_system(name)
+    params = seeded_basis(sy.n, K, SEED_BASE + seed)
+    say("ecg_energy_grad_stored", name, K, nfru)
+    ctx = EcgContext(sy)
+    ctx.set_params(params)
+    rc, E0, _, _ = ctx.energy(1, float("nan"), 1e-12, 60)
+    sigma = E0 - 1e-6 * abs(E0)
+    rc1, E1, g1, _ = ctx.energy_grad(nfru, 1, sigma, 1e-12, 40)
+    rc2, E2, g2, _ = ctx.energy_grad_stored(nfru, 1, sigma, 1e-12, 40)
+    print(name, K, "E", E1, E2, E1 - E2, "max|g|", np.max(np.abs(g1)), "max dev", np.max(np.abs(g1 - g2)))
+    assert abs(E1 - E2) <= 1e-12 * max(1.0, abs(E1))
+    np.testing.assert_allclose(g2, g1, rtol=1e-8, atol=1e-9 * np.max(np.abs(g1)))
+    if K <= 64:
+        Eo, go = _oracle_energy_grad(sy, params, sigma, nfru)
+        assert abs(E2 - Eo) <= 1e-10 * max(1.0, abs(Eo))
+        np.testing.assert_allclose(g2, go, rtol=1e-7, atol=1e-8 * np.max(np.abs(go)))
+    ctx.close()
