@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libecg_hip.so")
 SOURCES = ["ecg_kernels.hip", "ecg_linalg.hip", "ecg_screen.hip", "ecg_capi.hip"]
 FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-pthread",
     # packed FP32 instructions stay off (the path is FP64 throughout)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
@@ -41,7 +41,7 @@ def build(verbose: bool = False) -> str:
     if failed:
         raise RuntimeError("hipcc failed")
     tmp = LIB + ".tmp"
-    subprocess.run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs], check=True)
+    subprocess.run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs], check=True)
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)

@@ -4,10 +4,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <thread>
 #include <vector>
 
 #include "../../include/ecg.h"
@@ -45,6 +48,24 @@ struct ecg_ctx {
   ecg::Screen scr;            // candidate screening session
   zd* dD = nullptr;           // stored derivatives (2 npt x nfo x K) of ecg_energy_grad_stored
   size_t dD_cap = 0;
+};
+
+// A packed batch of independent problems (one context each).
+struct ecg_batch {
+  std::vector<ecg_ctx*> ctx;
+  hipStream_t lead = nullptr;
+  hipEvent_t ev_asm = nullptr;
+  std::vector<hipEvent_t> ev_eig;
+  struct Class {
+    int n = 0, maxT = 0, nf = 0;
+    std::vector<int> probs;
+    std::vector<ecg::ProbDesc> h;
+    ecg::ProbDesc* d = nullptr;
+    long long ntiles = 0;
+  };
+  std::vector<Class> classes;
+  int packed = 1;     // 1: pair phases packed across problems; 0: per problem
+  int nstreams = 4;   // problems whose eigen steps run concurrently
 };
 
 namespace {
@@ -912,6 +933,204 @@ int ecg_energy_grad_stored(ecg_ctx* x, int nfru, int which, double sigma, double
   x->tms[5] = 0.0;
   if (!std::isfinite(*E) || !finite_all(grad, ng)) return ECG_ERR_NONFINITE;
   return status;
+}
+
+int ecg_batch_create(int nprob, ecg_ctx** ctxs, ecg_batch** out) {
+  if (!out || nprob < 1 || !ctxs) return ECG_ERR_ARG;
+  *out = nullptr;
+  for (int i = 0; i < nprob; ++i)
+    if (!ctxs[i] || ctxs[i]->c.K < 1) return ECG_ERR_STATE;
+  ecg_batch* b = new ecg_batch();
+  b->ctx.assign(ctxs, ctxs + nprob);
+  if (hipStreamCreateWithFlags(&b->lead, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_asm, hipEventDisableTiming) != hipSuccess) {
+    ecg_batch_destroy(b);
+    return ECG_ERR_HIP;
+  }
+  b->ev_eig.resize(nprob, nullptr);
+  for (int i = 0; i < nprob; ++i)
+    if (hipEventCreateWithFlags(&b->ev_eig[i], hipEventDisableTiming) != hipSuccess) {
+      ecg_batch_destroy(b);
+      return ECG_ERR_HIP;
+    }
+  for (int i = 0; i < nprob; ++i) {
+    const int n = ctxs[i]->c.n;
+    auto it = std::find_if(b->classes.begin(), b->classes.end(), [&](const ecg_batch::Class& c) { return c.n == n; });
+    if (it == b->classes.end()) {
+      b->classes.emplace_back();
+      it = b->classes.end() - 1;
+      it->n = n;
+    }
+    it->probs.push_back(i);
+  }
+  for (auto& cl : b->classes) {
+    cl.h.resize(cl.probs.size());
+    if (hipMalloc((void**)&cl.d, sizeof(ecg::ProbDesc) * cl.probs.size()) != hipSuccess) {
+      ecg_batch_destroy(b);
+      return ECG_ERR_NOMEM;
+    }
+  }
+  *out = b;
+  return ECG_OK;
+}
+
+int ecg_batch_destroy(ecg_batch* b) {
+  if (!b) return ECG_ERR_ARG;
+  if (b->lead) (void)hipStreamSynchronize(b->lead);
+  for (auto& cl : b->classes)
+    if (cl.d) (void)hipFree(cl.d);
+  for (hipEvent_t e : b->ev_eig)
+    if (e) (void)hipEventDestroy(e);
+  if (b->ev_asm) (void)hipEventDestroy(b->ev_asm);
+  if (b->lead) (void)hipStreamDestroy(b->lead);
+  delete b;
+  return ECG_OK;
+}
+
+int ecg_batch_set_config(ecg_batch* b, int packed, int nstreams) {
+  if (!b || (packed != 0 && packed != 1) || nstreams < 1 || nstreams > 16) return ECG_ERR_ARG;
+  b->packed = packed;
+  b->nstreams = nstreams;
+  return ECG_OK;
+}
+
+namespace {
+// descriptors of every class from the contexts' current state
+int batch_descriptors(ecg_batch* b, const double* E) {
+  HIPCHK(hipStreamSynchronize(b->lead));   // the previous upload has consumed the host tables
+  for (auto& cl : b->classes) {
+    long long t0 = 0;
+    int f0 = 0;
+    cl.maxT = 0;
+    for (size_t q = 0; q < cl.probs.size(); ++q) {
+      const int i = cl.probs[q];
+      ecg_ctx* x = b->ctx[i];
+      const ecg::Ctx& c = x->c;
+      ecg::ProbDesc& d = cl.h[q];
+      d.sc = c.sc;
+      d.K = c.K;
+      d.ld = c.ld;
+      d.nt = (c.K + ecg::TILE - 1) / ecg::TILE;
+      d.f0 = f0;
+      d.tile0_off = t0;
+      d.tile0_adj = t0;
+      t0 += (long long)d.nt * (d.nt + 1) / 2;
+      f0 += c.K;
+      d.bra = c.bra;
+      d.ket = c.ket;
+      d.diagS = c.diagS;
+      d.H = c.H;
+      d.S = c.S;
+      d.cvec = x->ev[2];
+      d.part = x->part;
+      d.grad = x->grad;
+      d.E = E ? E[i] : 0.0;
+      cl.maxT = std::max(cl.maxT, c.T);
+    }
+    cl.ntiles = t0;
+    cl.nf = f0;
+    HIPCHK(hipMemcpyAsync(cl.d, cl.h.data(), sizeof(ecg::ProbDesc) * cl.h.size(), hipMemcpyHostToDevice, b->lead));
+  }
+  return 0;
+}
+
+int ensure_grad_buffers(ecg_ctx* x) {
+  const size_t np = ecg::adjoint_partial_doubles(x->c.n, x->c.K);
+  if (np > x->part_cap) {
+    RCCHK(dev_alloc(&x->part, np));
+    x->part_cap = np;
+  }
+  const size_t ng = (size_t)x->npt * x->c.K;
+  if (ng > x->grad_cap) {
+    RCCHK(dev_alloc(&x->grad, ng));
+    x->grad_cap = ng;
+  }
+  return 0;
+}
+}  // namespace
+
+int ecg_batch_energy_grad(ecg_batch* b, const double* sigma, double tol, int maxiter, double* E, double* grad,
+                          double* lat_ms) {
+  if (!b || !sigma || !E || !grad || maxiter < 1) return ECG_ERR_ARG;
+  const int np = (int)b->ctx.size();
+  std::vector<size_t> goff(np + 1, 0);
+  for (int i = 0; i < np; ++i) {
+    RCCHK(ensure_grad_buffers(b->ctx[i]));
+    goff[i + 1] = goff[i] + (size_t)b->ctx[i]->npt * b->ctx[i]->c.K;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  auto since = [&]() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  };
+  // contexts' own streams must have finished earlier work before the lead stream touches their buffers
+  for (ecg_ctx* x : b->ctx) HIPCHK(hipStreamSynchronize(x->stream));
+  if (b->packed) {
+    RCCHK(batch_descriptors(b, nullptr));
+    for (ecg_ctx* x : b->ctx) RCCHK(ecg::assemble_diag(x->c, b->lead));
+    for (auto& cl : b->classes)
+      RCCHK(ecg::packed_pairs(cl.n, false, cl.d, (int)cl.h.size(), cl.ntiles, cl.nf, cl.maxT, b->lead));
+    HIPCHK(hipEventRecord(b->ev_asm, b->lead));
+  }
+  // eigen steps (and, unpacked, the whole problem) on nstreams worker
+  // threads, problems taken largest first
+  std::vector<int> order(np);
+  for (int i = 0; i < np; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return b->ctx[a]->c.K > b->ctx[c]->c.K; });
+  std::atomic<int> next(0);
+  std::vector<int> st(np, 0);
+  auto worker = [&]() {
+    for (;;) {
+      const int q = next.fetch_add(1);
+      if (q >= np) return;
+      const int i = order[q];
+      ecg_ctx* x = b->ctx[i];
+      double info[8];
+      int status = 0, rc = 0;
+      if (b->packed) {
+        if (hipStreamWaitEvent(x->stream, b->ev_asm, 0) != hipSuccess) rc = ECG_ERR_HIP;
+      } else {
+        rc = ecg::assemble(x->c, 0, x->c.K, x->stream);
+      }
+      if (!rc) rc = solve_root(x, 1, sigma[i], tol, maxiter, &E[i], info, &status);
+      if (!rc && !b->packed) {
+        rc = ecg::adjoint(x->c, 0, E[i], x->ev[2], x->part, x->grad, x->stream);
+        if (!rc && hipMemcpyAsync(grad + goff[i], x->grad, sizeof(double) * (goff[i + 1] - goff[i]),
+                                  hipMemcpyDeviceToHost, x->stream) != hipSuccess)
+          rc = ECG_ERR_HIP;
+        if (!rc && hipStreamSynchronize(x->stream) != hipSuccess) rc = ECG_ERR_HIP;
+        if (lat_ms) lat_ms[i] = since();
+      }
+      if (!rc && b->packed && hipEventRecord(b->ev_eig[i], x->stream) != hipSuccess) rc = ECG_ERR_HIP;
+      st[i] = rc ? rc : status;
+    }
+  };
+  {
+    std::vector<std::thread> pool;
+    const int nt = std::min(b->nstreams, np);
+    for (int t = 0; t < nt; ++t) pool.emplace_back(worker);
+    for (auto& t : pool) t.join();
+  }
+  for (int i = 0; i < np; ++i)
+    if (st[i] < 0) return st[i];
+  if (b->packed) {
+    RCCHK(batch_descriptors(b, E));
+    for (int i = 0; i < np; ++i) HIPCHK(hipStreamWaitEvent(b->lead, b->ev_eig[i], 0));
+    for (auto& cl : b->classes)
+      RCCHK(ecg::packed_pairs(cl.n, true, cl.d, (int)cl.h.size(), cl.ntiles, cl.nf, cl.maxT, b->lead));
+    for (int i = 0; i < np; ++i)
+      HIPCHK(hipMemcpyAsync(grad + goff[i], b->ctx[i]->grad, sizeof(double) * (goff[i + 1] - goff[i]),
+                            hipMemcpyDeviceToHost, b->lead));
+    HIPCHK(hipStreamSynchronize(b->lead));
+    const double t = since();
+    if (lat_ms)
+      for (int i = 0; i < np; ++i) lat_ms[i] = t;
+  }
+  int worst = 0;
+  for (int i = 0; i < np; ++i) {
+    if (!std::isfinite(E[i]) || !finite_all(grad + goff[i], goff[i + 1] - goff[i])) return ECG_ERR_NONFINITE;
+    worst = std::max(worst, st[i]);
+  }
+  return worst;
 }
 
 int ecg_get_params(ecg_ctx* x, double* params) {

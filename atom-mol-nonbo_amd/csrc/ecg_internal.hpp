@@ -79,6 +79,26 @@ int invit_iteration(const Ctx& c, Factor& f, const zd* v, zd* x, hipStream_t s);
 int rayleigh(const Ctx& c, Factor& f, const zd* x, double sigma, hipStream_t s);
 int normalise(zd* x, int K, const double* t, hipStream_t s);
 
+// packed multi-problem pair launches (ecg_kernels.hip): one descriptor per
+// problem, problems of one launch share n; tile ranges ascending.
+struct ProbDesc {
+  SysConst sc;
+  int K = 0, ld = 0, nt = 0, f0 = 0;     // f0: first function in the batch's concatenation
+  long long tile0_off = 0, tile0_adj = 0;
+  const double* bra = nullptr;
+  const double* ket = nullptr;
+  const double* diagS = nullptr;
+  zd* H = nullptr;
+  zd* S = nullptr;
+  const zd* cvec = nullptr;
+  double* part = nullptr;
+  double* grad = nullptr;
+  double E = 0.0;
+};
+int assemble_diag(const Ctx& c, hipStream_t s);   // diagonal pairs of all K functions
+int packed_pairs(int n, bool adjoint, const ProbDesc* dpd, int np, long long ntiles, int nf, int maxT,
+                 hipStream_t s);
+
 // candidate screening (ecg_screen.hip; pair rows in ecg_kernels.hip)
 int candidate_rows(const Ctx& c, int Kp, int C, const double* cparams, double* cbra, double* cket,
                    double* cdiag, zd* Hb, zd* Sb, int ldb, hipStream_t s);

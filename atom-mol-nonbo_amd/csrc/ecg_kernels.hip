@@ -10,6 +10,8 @@
 //   diagS  real S_kk (un-normalised), diagH real H_kk / S_kk.
 // The reference stores the same numbers in the lower triangles only
 // (src/matform.f90:11-51); the host copy-out restores that layout.
+#include <type_traits>
+
 #include "ecg_internal.hpp"
 
 namespace ecg {
@@ -166,15 +168,13 @@ __device__ inline void tri_tile(int b, int tr0, int& tr, int& tc) {
 // Block = TILE x TILE pairs; thread x -> row k (bra), thread y -> column l
 // (ket).  Bra and ket records of the tile are staged in LDS.
 template <int N>
-__global__ void __launch_bounds__(TILE* TILE) k_offdiag(const SysConst sc, int r0, int r1, int ld, int tr0,
-                                                      const double* __restrict__ bra,
-                                                      const double* __restrict__ ket,
-                                                      const double* __restrict__ diagS,
-                                                      zd* __restrict__ Hm, zd* __restrict__ Sm) {
-  extern __shared__ double lds[];
+__device__ __attribute__((always_inline)) inline void offdiag_tile(const SysConst& sc, int r0, int r1, int ld, int tr,
+                                                                   int tc, const double* __restrict__ bra,
+                                                                   const double* __restrict__ ket,
+                                                                   const double* __restrict__ diagS,
+                                                                   zd* __restrict__ Hm, zd* __restrict__ Sm,
+                                                                   double* lds) {
   const int T = sc.T;
-  int tr, tc;
-  tri_tile(blockIdx.x, tr0, tr, tc);
   const int kb = tr * TILE, lb = tc * TILE;
   double* sbra = lds;
   double* sket = lds + TILE * BraRec<N>::stride;
@@ -201,6 +201,18 @@ __global__ void __launch_bounds__(TILE* TILE) k_offdiag(const SysConst sc, int r
   Sm[(size_t)l * ld + k] = sn;
   Hm[(size_t)k * ld + l] = zconj(hn);
   Sm[(size_t)k * ld + l] = zconj(sn);
+}
+
+template <int N>
+__global__ void __launch_bounds__(TILE* TILE) k_offdiag(const SysConst sc, int r0, int r1, int ld, int tr0,
+                                                      const double* __restrict__ bra,
+                                                      const double* __restrict__ ket,
+                                                      const double* __restrict__ diagS,
+                                                      zd* __restrict__ Hm, zd* __restrict__ Sm) {
+  extern __shared__ double lds[];
+  int tr, tc;
+  tri_tile(blockIdx.x, tr0, tr, tc);
+  offdiag_tile<N>(sc, r0, r1, ld, tr, tc, bra, ket, diagS, Hm, Sm, lds);
 }
 
 // ------------------------------------------- stored derivatives (StoreHSD)
@@ -366,18 +378,15 @@ __device__ inline void pair_adjoint_sum(const SysConst& sc, const double* br, co
 // bra partial from tile (r, c) in slot c and its ket partial from tile
 // (r', r) in slot r' + 1.
 template <int N>
-__global__ void __launch_bounds__(TILE* TILE) k_adjoint(const SysConst sc, int K, int nfru, int tr0, int nt,
-                                                      double E,
-                                                      const double* __restrict__ bra,
-                                                      const double* __restrict__ ket,
-                                                      const double* __restrict__ diagS,
-                                                      const zd* __restrict__ cvec,
-                                                      double* __restrict__ part) {
+__device__ __attribute__((always_inline)) inline void adjoint_tile(const SysConst& sc, int K, int nfru, int nt,
+                                                                   int tr, int tc, double E,
+                                                                   const double* __restrict__ bra,
+                                                                   const double* __restrict__ ket,
+                                                                   const double* __restrict__ diagS,
+                                                                   const zd* __restrict__ cvec,
+                                                                   double* __restrict__ part, double* lds) {
   constexpr int nv = AdjAcc<N>::nvals;
-  extern __shared__ double lds[];
   const int T = sc.T;
-  int tr, tc;
-  tri_tile(blockIdx.x, tr0, tr, tc);
   const int kb = tr * TILE, lb = tc * TILE;
   double* sbra = lds;
   double* sket = lds + TILE * BraRec<N>::stride;
@@ -434,16 +443,30 @@ __global__ void __launch_bounds__(TILE* TILE) k_adjoint(const SysConst sc, int K
   }
 }
 
+template <int N>
+__global__ void __launch_bounds__(TILE* TILE) k_adjoint(const SysConst sc, int K, int nfru, int tr0, int nt,
+                                                      double E,
+                                                      const double* __restrict__ bra,
+                                                      const double* __restrict__ ket,
+                                                      const double* __restrict__ diagS,
+                                                      const zd* __restrict__ cvec,
+                                                      double* __restrict__ part) {
+  extern __shared__ double lds[];
+  int tr, tc;
+  tri_tile(blockIdx.x, tr0, tr, tc);
+  adjoint_tile<N>(sc, K, nfru, nt, tr, tc, E, bra, ket, diagS, cvec, part, lds);
+}
+
 // Per function f >= nfru: sum its partial slots in fixed order and apply
 // the chain rule to vech L, vech B:
 //   dE/dL(i,j) = 4 (RA L)(i,j) + 2 Ra delta_ij / L_jj
 //   dE/dB(i,i) = 2 IB_ii,   dE/dB(i,j) = 4 IB_ij
 template <int N>
-__global__ void k_adjoint_finish(int K, int nfru, int nt, int tr0, const double* __restrict__ bra,
-                                 const double* __restrict__ part, double* __restrict__ grad) {
+__device__ __attribute__((always_inline)) inline void adjoint_finish_one(int f, int nfru, int nt, int tr0,
+                                                                         const double* __restrict__ bra,
+                                                                         const double* __restrict__ part,
+                                                                         double* __restrict__ grad) {
   constexpr int m = N * (N + 1) / 2, nv = AdjAcc<N>::nvals;
-  const int f = nfru + blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= K) return;
   const int r = f / TILE;
   double acc[nv];
 #pragma unroll
@@ -472,6 +495,55 @@ __global__ void k_adjoint_finish(int K, int nfru, int nt, int tr0, const double*
       g[vidx(N, i, j)] = v;
       g[m + vidx(N, i, j)] = ((i == j) ? 2.0 : 4.0) * acc[m + sidx(i, j)];
     }
+}
+
+template <int N>
+__global__ void k_adjoint_finish(int K, int nfru, int nt, int tr0, const double* __restrict__ bra,
+                                 const double* __restrict__ part, double* __restrict__ grad) {
+  const int f = nfru + blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= K) return;
+  adjoint_finish_one<N>(f, nfru, nt, tr0, bra, part, grad);
+}
+
+// ------------------------------------------------- packed problem batches
+// One launch covers the pair tiles of every problem of a batch that has
+// the same n: workgroup b belongs to the problem whose tile range
+// [tile0, tile0 + tiles) holds b (ranges are ascending in the table).
+__device__ inline int find_prob(const ProbDesc* __restrict__ pd, int np, long long b, bool adj) {
+  int p = 0;
+  while (p + 1 < np && (adj ? pd[p + 1].tile0_adj : pd[p + 1].tile0_off) <= b) ++p;
+  return p;
+}
+
+template <int N>
+__global__ void __launch_bounds__(TILE* TILE) k_offdiag_multi(const ProbDesc* __restrict__ pd, int np) {
+  extern __shared__ double lds[];
+  const int p = find_prob(pd, np, blockIdx.x, false);
+  const ProbDesc& P = pd[p];
+  int tr, tc;
+  tri_tile((int)(blockIdx.x - P.tile0_off), 0, tr, tc);
+  offdiag_tile<N>(P.sc, 0, P.K, P.ld, tr, tc, P.bra, P.ket, P.diagS, P.H, P.S, lds);
+}
+
+template <int N>
+__global__ void __launch_bounds__(TILE* TILE) k_adjoint_multi(const ProbDesc* __restrict__ pd, int np) {
+  extern __shared__ double lds[];
+  const int p = find_prob(pd, np, blockIdx.x, true);
+  const ProbDesc& P = pd[p];
+  int tr, tc;
+  tri_tile((int)(blockIdx.x - P.tile0_adj), 0, tr, tc);
+  adjoint_tile<N>(P.sc, P.K, 0, P.nt, tr, tc, P.E, P.bra, P.ket, P.diagS, P.cvec, P.part, lds);
+}
+
+// thread -> function f0 + t of the batch's concatenated functions
+template <int N>
+__global__ void k_adjoint_finish_multi(const ProbDesc* __restrict__ pd, int np, int nf) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nf) return;
+  int p = 0;
+  while (p + 1 < np && pd[p + 1].f0 <= g) ++p;
+  const ProbDesc& P = pd[p];
+  adjoint_finish_one<N>(g - P.f0, 0, P.nt, 0, P.bra, P.part, P.grad);
 }
 
 // ------------------------------------------------------------ launchers
@@ -633,6 +705,48 @@ int candidate_rows(const Ctx& c, int Kp, int C, const double* cparams, double* c
     case 3: return launch_cand<3>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, s);
     case 4: return launch_cand<4>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, s);
     case 5: return launch_cand<5>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, s);
+  }
+  return -1;
+}
+
+int packed_pairs(int n, bool adjoint, const ProbDesc* dpd, int np, long long ntiles, int nf, int maxT,
+                 hipStream_t s) {
+  auto go = [&](auto tag) -> int {
+    constexpr int N = decltype(tag)::value;
+    if (!adjoint) {
+      const size_t lds = sizeof(double) * TILE * (BraRec<N>::stride + maxT * KetRec<N>::stride);
+      hipLaunchKernelGGL(k_offdiag_multi<N>, dim3((unsigned)ntiles), dim3(TILE, TILE), lds, s, dpd, np);
+    } else {
+      const size_t lds = sizeof(double) * TILE *
+                         (BraRec<N>::stride + maxT * KetRec<N>::stride + TILE * AdjAcc<N>::nvals);
+      hipLaunchKernelGGL(k_adjoint_multi<N>, dim3((unsigned)ntiles), dim3(TILE, TILE), lds, s, dpd, np);
+      hipLaunchKernelGGL(k_adjoint_finish_multi<N>, dim3((nf + 127) / 128), dim3(128), 0, s, dpd, np, nf);
+    }
+    return (int)hipGetLastError();
+  };
+  switch (n) {
+    case 1: return go(std::integral_constant<int, 1>());
+    case 2: return go(std::integral_constant<int, 2>());
+    case 3: return go(std::integral_constant<int, 3>());
+    case 4: return go(std::integral_constant<int, 4>());
+    case 5: return go(std::integral_constant<int, 5>());
+  }
+  return -1;
+}
+
+int assemble_diag(const Ctx& c, hipStream_t s) {
+  auto go = [&](auto tag) -> int {
+    constexpr int N = decltype(tag)::value;
+    hipLaunchKernelGGL(k_diag<N>, dim3((c.K + 127) / 128), dim3(128), 0, s, c.sc, 0, c.K, c.ld, c.bra, c.ket,
+                       c.diagS, c.diagH, c.H, c.S);
+    return (int)hipGetLastError();
+  };
+  switch (c.n) {
+    case 1: return go(std::integral_constant<int, 1>());
+    case 2: return go(std::integral_constant<int, 2>());
+    case 3: return go(std::integral_constant<int, 3>());
+    case 4: return go(std::integral_constant<int, 4>());
+    case 5: return go(std::integral_constant<int, 5>());
   }
   return -1;
 }

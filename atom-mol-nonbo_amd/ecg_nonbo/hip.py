@@ -39,7 +39,8 @@ _NAMES = {
 EXPORTS = [
     "ecg_ctx_create", "ecg_ctx_destroy", "ecg_set_params", "ecg_update_params", "ecg_matrix_elements",
     "ecg_compute_mat_elem", "ecg_compute_mat_elem_and_deriv", "ecg_ghepiis", "ecg_energy", "ecg_energy_grad",
-    "ecg_energy_grad_stored", "ecg_get_params", "ecg_screen_prepare", "ecg_screen", "ecg_screen_loop", "ecg_last_timings",
+    "ecg_energy_grad_stored", "ecg_batch_create", "ecg_batch_destroy", "ecg_batch_set_config",
+    "ecg_batch_energy_grad", "ecg_get_params", "ecg_screen_prepare", "ecg_screen", "ecg_screen_loop", "ecg_last_timings",
 ]
 
 
@@ -76,6 +77,10 @@ def load(path: str = LIB_PATH):
     L.ecg_energy.argtypes = [vp, c_int, c_double, c_double, c_int, _d, _d, _d]
     L.ecg_energy_grad.argtypes = [vp, c_int, c_int, c_double, c_double, c_int, _d, _d, _d]
     L.ecg_energy_grad_stored.argtypes = [vp, c_int, c_int, c_double, c_double, c_int, _d, _d, _d]
+    L.ecg_batch_create.argtypes = [c_int, ctypes.POINTER(vp), ctypes.POINTER(vp)]
+    L.ecg_batch_destroy.argtypes = [vp]
+    L.ecg_batch_set_config.argtypes = [vp, c_int, c_int]
+    L.ecg_batch_energy_grad.argtypes = [vp, _d, c_double, c_int, _d, _d, _d]
     L.ecg_get_params.argtypes = [vp, _d]
     L.ecg_screen_prepare.argtypes = [vp, c_double, c_double, c_int, _d]
     L.ecg_screen.argtypes = [vp, c_int, _d, c_int, _d, _d]
@@ -278,3 +283,42 @@ class EcgContext:
         t = np.zeros(8)
         n = load().ecg_last_timings(self._h, _p(t), 8)
         return t[:n]
+
+
+class EcgBatch:
+    """A packed batch of independent problems (one EcgContext each, with
+    parameters set): ecg_batch_create / ecg_batch_energy_grad."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        arr = (ctypes.c_void_p * len(self.contexts))(*[c._h.value for c in self.contexts])
+        h = ctypes.c_void_p()
+        _check("ecg_batch_create", load().ecg_batch_create(len(self.contexts), arr, ctypes.byref(h)))
+        self._h = h
+
+    def set_config(self, packed=1, nstreams=4):
+        _check("ecg_batch_set_config", load().ecg_batch_set_config(self._h, int(packed), int(nstreams)))
+
+    def energy_grad(self, sigmas, tol=1e-12, maxiter=20):
+        """(status, E per problem, list of gradients, latency ms per problem)."""
+        sig = np.ascontiguousarray(sigmas, dtype=np.float64)
+        n = len(self.contexts)
+        E = np.zeros(n)
+        sizes = [c.npt * c.K for c in self.contexts]
+        g = np.zeros(int(np.sum(sizes)))
+        lat = np.zeros(n)
+        rc = load().ecg_batch_energy_grad(self._h, _p(sig), float(tol), maxiter, _p(E), _p(g), _p(lat))
+        _check("ecg_batch_energy_grad", rc, ok=(ECG_OK, ECG_NOT_CONVERGED))
+        offs = np.concatenate([[0], np.cumsum(sizes)])
+        return rc, E, [g[offs[i]:offs[i + 1]] for i in range(n)], lat
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            load().ecg_batch_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

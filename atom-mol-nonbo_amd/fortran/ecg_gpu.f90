@@ -124,6 +124,30 @@ module ecg_capi
       real(c_double), intent(out) :: E
       real(c_double), intent(out) :: grad(*), info(*)
     end function
+    integer(c_int) function ecg_batch_create(nprob, ctxs, batch) bind(C, name="ecg_batch_create")
+      import :: c_int, c_ptr
+      integer(c_int), value :: nprob
+      type(c_ptr), intent(in) :: ctxs(*)
+      type(c_ptr), intent(out) :: batch
+    end function
+    integer(c_int) function ecg_batch_destroy(batch) bind(C, name="ecg_batch_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+    end function
+    integer(c_int) function ecg_batch_set_config(batch, packed, nstreams) bind(C, name="ecg_batch_set_config")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int), value :: packed, nstreams
+    end function
+    integer(c_int) function ecg_batch_energy_grad(batch, sigma, tol, maxiter, E, grad, lat_ms) &
+        bind(C, name="ecg_batch_energy_grad")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: batch
+      real(c_double), intent(in) :: sigma(*)
+      real(c_double), value :: tol
+      integer(c_int), value :: maxiter
+      real(c_double), intent(out) :: E(*), grad(*), lat_ms(*)
+    end function
     integer(c_int) function ecg_get_params(ctx, params) bind(C, name="ecg_get_params")
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx

@@ -200,6 +200,65 @@ def memory_modes_leg(system, Ks=(1000, 4000), reps=3):
     return out
 
 
+BATCH_SYSTEMS = ("hminus", "h2", "li")
+
+
+def batch_leg(nprob=32, reps=3):
+    """32 independent problems (n=2 H^- and n=3 H2 / Li, round robin), K
+    drawn uniformly from [200, 2000] (seeded), energy + full gradient each,
+    as one packed batch.  A small autotuner times the launch structures
+    (packed or per-problem pair launches x 1/2/4 concurrent eigen streams)
+    on this batch and keeps the fastest; the timed runs use it."""
+    from ecg_nonbo.hip import EcgBatch
+    rng = np.random.default_rng(SEED_BASE + 5)
+    Ks = rng.integers(200, 2001, nprob)
+    ctxs, sig, pairs = [], [], 0
+    for i in range(nprob):
+        sy = named_system(BATCH_SYSTEMS[i % 3])
+        K = int(Ks[i])
+        c = EcgContext(sy)
+        c.set_params(seeded_basis(sy.n, K, SEED_BASE + 5 + 100 * (i + 1)))
+        st, E0, _, _ = c.energy(1, float("nan"), 1e-12, 60)
+        sig.append(E0 - 1e-6 * abs(E0))
+        ctxs.append(c)
+        pairs += K * (K + 1) // 2
+    b = EcgBatch(ctxs)
+    tune = []
+    for packed in (1, 0):
+        for ns in (1, 2, 4):
+            b.set_config(packed, ns)
+            ts = []
+            for r in range(2):
+                t0 = time.perf_counter()
+                b.energy_grad(sig, 1e-12, 30)
+                ts.append(time.perf_counter() - t0)
+            tune.append({"packed": packed, "streams": ns, "ms": 1e3 * min(ts)})
+    best = min(tune, key=lambda d: d["ms"])
+    b.set_config(best["packed"], best["streams"])
+    ts, lats = [], []
+    for r in range(reps):
+        t0 = time.perf_counter()
+        st, E, grads, lat = b.energy_grad(sig, 1e-12, 30)
+        ts.append(time.perf_counter() - t0)
+        lats.append(lat)
+    b.close()
+    for c in ctxs:
+        c.close()
+    lat = np.concatenate(lats)
+    t = float(np.mean(ts))
+    return {
+        "workload": "%d problems (H- n=2, H2 and Li n=3, round robin), K ~ U[200, 2000] seed %d, energy + full "
+                    "gradient each" % (nprob, SEED_BASE + 5),
+        "pairs": int(pairs),
+        "batch_ms": 1e3 * t,
+        "pair_elements_per_s": pairs / t,
+        "latency_ms_p50": float(np.percentile(lat, 50)),
+        "latency_ms_p99": float(np.percentile(lat, 99)),
+        "config": best,
+        "autotune": tune,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,6 +268,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-screen", action="store_true", help="skip the candidate-screening leg")
     ap.add_argument("--no-modes", action="store_true", help="skip the stored/recompute memory-mode leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the multi-problem batch leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the path on a single MI355X "
@@ -295,6 +355,8 @@ def main():
         out["screening"] = screening_leg(system, base, K)
     if not args.no_modes:
         out["memory_modes"] = memory_modes_leg(system)
+    if not args.no_batch:
+        out["batch"] = batch_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(system, base, K)
     ctx.close()

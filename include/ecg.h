@@ -132,6 +132,25 @@ int ecg_energy_grad(ecg_ctx* ctx, int nfru, int which, double sigma, double tol,
 int ecg_energy_grad_stored(ecg_ctx* ctx, int nfru, int which, double sigma, double tol, int maxiter, double* E,
                            double* grad, double* info);
 
+/* Packed batches of independent problems (uneven K, mixed n), no
+ * counterpart in the reference (which runs one problem per MPI job):
+ * ecg_batch_create binds nprob contexts whose parameters are set.
+ * ecg_batch_energy_grad computes, for every problem i, the root nearest
+ * sigma[i] and the gradient of all its functions (as ecg_energy_grad with
+ * nfru = 0): the pair phases of all problems with the same n run as one
+ * launch each (assembly, adjoint), the eigen steps of up to nstreams
+ * problems run concurrently (largest first).  grad: the problems' gradients
+ * concatenated in order; lat_ms (nullable): time from the call's start
+ * until each problem's results were on the host.
+ * ecg_batch_set_config selects packed (1) or per-problem (0) pair launches
+ * and nstreams (an autotuner times the combinations; see bench.py). */
+typedef struct ecg_batch ecg_batch;
+int ecg_batch_create(int nprob, ecg_ctx** ctxs, ecg_batch** out);
+int ecg_batch_destroy(ecg_batch* b);
+int ecg_batch_set_config(ecg_batch* b, int packed, int nstreams);
+int ecg_batch_energy_grad(ecg_batch* b, const double* sigma, double tol, int maxiter, double* E, double* grad,
+                          double* lat_ms);
+
 /* Current parameters of the K functions (npt x K), e.g. after
  * ecg_screen_loop accepted candidates into function K. */
 int ecg_get_params(ecg_ctx* ctx, double* params);
