@@ -48,6 +48,7 @@ struct ecg_ctx {
   ecg::Screen scr;            // candidate screening session
   zd* dD = nullptr;           // stored derivatives (2 npt x nfo x K) of ecg_energy_grad_stored
   size_t dD_cap = 0;
+  ecg::Tracer tr;             // per-kernel tracing (off by default)
 };
 
 // A packed batch of independent problems (one context each).
@@ -199,6 +200,10 @@ int upload_params(ecg_ctx* x, int K, const double* params) {
   return 0;
 }
 
+void trace_collect(ecg_ctx* x) {
+  if (x->tr.on) x->tr.collect();
+}
+
 bool finite_all(const double* v, size_t n) {
   for (size_t i = 0; i < n; ++i)
     if (!std::isfinite(v[i])) return false;
@@ -262,9 +267,10 @@ int copy_out_hs(ecg_ctx* x, int r0, int r1, char method, double sigma, double* H
 int factor_at(ecg_ctx* x, double sigma, int* nneg, int* zero) {
   RCCHK(ecg::form_shifted(x->c, x->f, sigma, x->stream));
   RCCHK(ecg::ldlh_factor(x->f, x->stream));
-  int st[2];
+  int st[3];
   HIPCHK(hipMemcpyAsync(st, x->f.status, sizeof(st), hipMemcpyDeviceToHost, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
+  if (st[2]) return ECG_ERR_SYNC;   // a device-side wait of the factorisation hit its limit
   *nneg = st[0];
   *zero = st[1];
   return 0;
@@ -322,7 +328,7 @@ int inverse_iteration(ecg_ctx* x, double sigma, double tol, int maxiter, bool wa
   HIPCHK(hipMemcpyAsync(st, x->f.status, sizeof(st), hipMemcpyDeviceToHost, x->stream));
   HIPCHK(hipMemcpyAsync(x->ev[2], v, sizeof(zd) * K, hipMemcpyDeviceToDevice, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
-  if (st[2]) return ECG_ERR_HIP;   // a triangular-solve wait hit its limit
+  if (st[2]) return ECG_ERR_SYNC;   // a device-side wait (factor or solve) hit its limit
   r.t1 = q[2];
   r.lambda = q[3] / q[2] + sigma;
   x->have_last = true;
@@ -641,6 +647,7 @@ int ecg_ctx_destroy(ecg_ctx* x) {
   if (x->pinned) (void)hipHostFree(x->pinned);
   free_screen(x->scr);
   if (x->dD) (void)hipFree(x->dD);
+  for (hipEvent_t e : x->tr.pool) (void)hipEventDestroy(e);
   for (int i = 0; i < 8; ++i)
     if (x->e[i]) (void)hipEventDestroy(x->e[i]);
   if (x->stream) (void)hipStreamDestroy(x->stream);
@@ -649,6 +656,7 @@ int ecg_ctx_destroy(ecg_ctx* x) {
 }
 
 int ecg_set_params(ecg_ctx* x, int K, const double* params) {
+  (void)hipGetLastError();   // start from a clean runtime error state
   if (!x || K < 1 || !params) return ECG_ERR_ARG;
   if (!finite_all(params, (size_t)x->npt * K)) return ECG_ERR_ARG;
   RCCHK(upload_params(x, K, params));
@@ -668,6 +676,7 @@ int ecg_update_params(ecg_ctx* x, int first, int count, const double* params) {
 
 int ecg_matrix_elements(ecg_ctx* x, const double* paramk, const double* paraml, const int* P, int gradflag,
                         double* Hkl, double* Skl, double* Dk, double* Dl) {
+  (void)hipGetLastError();   // start from a clean runtime error state
   if (!x || !paramk || !paraml || !P || !Hkl || !Skl) return ECG_ERR_ARG;
   const int n = x->c.n, m = n * (n + 1) / 2, npt = x->npt;
   const bool gk = (gradflag == 1 || gradflag == 3), gl = (gradflag == 2 || gradflag == 3);
@@ -724,6 +733,7 @@ int ecg_matrix_elements(ecg_ctx* x, const double* paramk, const double* paraml, 
 
 int ecg_compute_mat_elem(ecg_ctx* x, int nmin, int nmax, char method, double approx_energy, double* H,
                          double* S, int ld, double* diagH, double* diagS) {
+  (void)hipGetLastError();   // start from a clean runtime error state
   if (!x || !H || !S || !diagH || !diagS) return ECG_ERR_ARG;
   if (method != 'G' && method != 'I') return ECG_ERR_ARG;
   if (x->c.K < 1) return ECG_ERR_STATE;
@@ -737,6 +747,7 @@ int ecg_compute_mat_elem(ecg_ctx* x, int nmin, int nmax, char method, double app
 
 int ecg_compute_mat_elem_and_deriv(ecg_ctx* x, int nfru, char method, double approx_energy, double* H,
                                    double* S, int ld, double* diagH, double* diagS, double* D) {
+  (void)hipGetLastError();   // start from a clean runtime error state
   if (!x || !H || !S || !diagH || !diagS || !D) return ECG_ERR_ARG;
   if (method != 'G' && method != 'I') return ECG_ERR_ARG;
   const int K = x->c.K;
@@ -763,6 +774,7 @@ int ecg_compute_mat_elem_and_deriv(ecg_ctx* x, int nfru, char method, double app
 int ecg_ghepiis(ecg_ctx* x, int k, int n, double* M, int nM, double* invD, const double* B, int nB,
                 double apprlambda, const double* v, double tol, int maxiter, int specif_norm, double* lambda,
                 double* xout, double* relacc, int* numiter) {
+  (void)hipGetLastError();   // start from a clean runtime error state
   if (!x || !M || !invD || !B || !v || !lambda || !xout || n < 1 || k < 1 || k > n || nM < n || nB < n ||
       maxiter < 1)
     return ECG_ERR_ARG;
@@ -832,6 +844,7 @@ int ecg_ghepiis(ecg_ctx* x, int k, int n, double* M, int nM, double* invD, const
 }
 
 int ecg_energy(ecg_ctx* x, int which, double sigma, double tol, int maxiter, double* E, double* c, double* info) {
+  (void)hipGetLastError();   // start from a clean runtime error state
   if (!x || !E || which < 1 || maxiter < 1) return ECG_ERR_ARG;
   const int K = x->c.K;
   if (K < 1) return ECG_ERR_STATE;
@@ -848,12 +861,14 @@ int ecg_energy(ecg_ctx* x, int which, double sigma, double tol, int maxiter, dou
   x->tms[1] = ev_ms(x->e[1], x->e[2]);
   x->tms[2] = 0.0;
   x->tms[3] = ev_ms(x->e[0], x->e[2]);
+  trace_collect(x);
   if (!std::isfinite(*E)) return ECG_ERR_NONFINITE;
   return status;
 }
 
 int ecg_energy_grad(ecg_ctx* x, int nfru, int which, double sigma, double tol, int maxiter, double* E,
                     double* grad, double* info) {
+  (void)hipGetLastError();   // start from a clean runtime error state
   if (!x || !E || !grad || which < 1 || maxiter < 1) return ECG_ERR_ARG;
   ecg::Ctx& c = x->c;
   const int K = c.K;
@@ -890,12 +905,14 @@ int ecg_energy_grad(ecg_ctx* x, int nfru, int which, double sigma, double tol, i
   x->tms[3] = ev_ms(x->e[0], x->e[7]);
   x->tms[4] = ev_ms(x->e[3], x->e[4]);
   x->tms[5] = ev_ms(x->e[5], x->e[6]);
+  trace_collect(x);
   if (!std::isfinite(*E) || !finite_all(grad, ng)) return ECG_ERR_NONFINITE;
   return status;
 }
 
 int ecg_energy_grad_stored(ecg_ctx* x, int nfru, int which, double sigma, double tol, int maxiter, double* E,
                            double* grad, double* info) {
+  (void)hipGetLastError();   // start from a clean runtime error state
   if (!x || !E || !grad || which < 1 || maxiter < 1) return ECG_ERR_ARG;
   ecg::Ctx& c = x->c;
   const int K = c.K;
@@ -931,6 +948,7 @@ int ecg_energy_grad_stored(ecg_ctx* x, int nfru, int which, double sigma, double
   x->tms[3] = ev_ms(x->e[0], x->e[7]);
   x->tms[4] = x->tms[2];
   x->tms[5] = 0.0;
+  trace_collect(x);
   if (!std::isfinite(*E) || !finite_all(grad, ng)) return ECG_ERR_NONFINITE;
   return status;
 }
@@ -1051,6 +1069,7 @@ int ensure_grad_buffers(ecg_ctx* x) {
 
 int ecg_batch_energy_grad(ecg_batch* b, const double* sigma, double tol, int maxiter, double* E, double* grad,
                           double* lat_ms) {
+  (void)hipGetLastError();   // start from a clean runtime error state
   if (!b || !sigma || !E || !grad || maxiter < 1) return ECG_ERR_ARG;
   const int np = (int)b->ctx.size();
   std::vector<size_t> goff(np + 1, 0);
@@ -1133,6 +1152,37 @@ int ecg_batch_energy_grad(ecg_batch* b, const double* sigma, double tol, int max
   return worst;
 }
 
+int ecg_trace(ecg_ctx* x, int enable) {
+  if (!x) return ECG_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(x->stream));
+  ecg::Tracer& t = x->tr;
+  if (enable && t.pool.empty()) {
+    t.pool.resize(4096, nullptr);
+    for (auto& e : t.pool) HIPCHK(hipEventCreate(&e));
+  }
+  t.collect();
+  for (int i = 0; i < ecg::TR_COUNT; ++i) {
+    t.sum_ms[i] = 0.0;
+    t.count[i] = 0;
+  }
+  t.on = enable != 0;
+  x->c.tr = t.on ? &t : nullptr;
+  x->f.tr = t.on ? &t : nullptr;
+  return ECG_OK;
+}
+
+int ecg_trace_read(ecg_ctx* x, double* count, double* total_ms, int nmax) {
+  if (!x || !count || !total_ms || nmax < 1) return ECG_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(x->stream));
+  x->tr.collect();
+  const int n = std::min(nmax, (int)ecg::TR_COUNT);
+  for (int i = 0; i < n; ++i) {
+    count[i] = (double)x->tr.count[i];
+    total_ms[i] = x->tr.sum_ms[i];
+  }
+  return n;
+}
+
 int ecg_get_params(ecg_ctx* x, double* params) {
   if (!x || !params) return ECG_ERR_ARG;
   if (x->c.K < 1) return ECG_ERR_STATE;
@@ -1142,6 +1192,7 @@ int ecg_get_params(ecg_ctx* x, double* params) {
 }
 
 int ecg_screen_prepare(ecg_ctx* x, double sigma, double tol, int maxiter, double* info) {
+  (void)hipGetLastError();   // start from a clean runtime error state
   if (!x || !std::isfinite(sigma) || maxiter < 1) return ECG_ERR_ARG;
   ecg::Ctx& c = x->c;
   const int K = c.K;
@@ -1163,6 +1214,7 @@ int ecg_screen_prepare(ecg_ctx* x, double sigma, double tol, int maxiter, double
     if (D[i] == 0.0 && !zero) zero = i + 1;
   }
   sc.nneg = nneg;
+  if (st[2]) return ECG_ERR_SYNC;
   if (zero || st[1]) return ECG_SINGULAR;
   // energy and eigenvector of the current basis at this shift: start
   // vector of every candidate and the energy a candidate must beat
@@ -1183,6 +1235,7 @@ int ecg_screen_prepare(ecg_ctx* x, double sigma, double tol, int maxiter, double
 }
 
 int ecg_screen(ecg_ctx* x, int ncand, const double* cand, int iters, double* E, double* info) {
+  (void)hipGetLastError();   // start from a clean runtime error state
   if (!x || ncand < 1 || !cand || iters < 1 || !E) return ECG_ERR_ARG;
   if (!x->scr.ready || x->scr.ld != x->c.ld || x->scr.Kp != x->c.K - 1) return ECG_ERR_STATE;
   if (!finite_all(cand, (size_t)x->npt * ncand)) return ECG_ERR_ARG;
@@ -1211,6 +1264,7 @@ int ecg_screen(ecg_ctx* x, int ncand, const double* cand, int iters, double* E, 
 }
 
 int ecg_screen_loop(ecg_ctx* x, int nsteps, int ncand, const double* cand, int iters, double* Ebest, int* ibest) {
+  (void)hipGetLastError();   // start from a clean runtime error state
   if (!x || nsteps < 1 || ncand < 1 || !cand || iters < 1 || !Ebest || !ibest) return ECG_ERR_ARG;
   if (!x->scr.ready || x->scr.ld != x->c.ld || x->scr.Kp != x->c.K - 1) return ECG_ERR_STATE;
   if (!finite_all(cand, (size_t)x->npt * ncand * nsteps)) return ECG_ERR_ARG;

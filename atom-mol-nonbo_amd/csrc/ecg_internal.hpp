@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <vector>
 
 #include "ecg_math.hpp"
 
@@ -33,6 +34,46 @@ inline int bra_stride(int n) {
 }
 inline int ket_stride(int n) { return n * (n + 1) + 2 * n * n; }
 
+// Per-kernel tracing: event pairs around launches, aggregated per label
+// after the stream has synchronised.  Off unless enabled on the context.
+enum TraceLabel {
+  TR_OFFDIAG = 0, TR_ADJOINT, TR_LDL_UPDATE, TR_LDL_DIAG, TR_LDL_PANEL, TR_TRSV_FWD, TR_TRSV_BWD, TR_GEMV,
+  TR_FORM_SHIFTED, TR_COUNT
+};
+struct Tracer {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<int> label;       // per recorded pair
+  int used = 0;                 // events used
+  bool pending = false;         // a begin() awaits its end()
+  double sum_ms[TR_COUNT] = {};
+  long long count[TR_COUNT] = {};
+  void begin(int lab, hipStream_t s) {
+    if (!on || used + 2 > (int)pool.size()) return;
+    (void)hipEventRecord(pool[used], s);
+    label.push_back(lab);
+    used += 2;
+    pending = true;
+  }
+  void end(hipStream_t s) {
+    if (!pending) return;
+    (void)hipEventRecord(pool[used - 1], s);
+    pending = false;
+  }
+  // call after the stream has synchronised
+  void collect() {
+    for (size_t i = 0; i < label.size(); ++i) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, pool[2 * i], pool[2 * i + 1]) == hipSuccess) {
+        sum_ms[label[i]] += ms;
+        count[label[i]] += 1;
+      }
+    }
+    label.clear();
+    used = 0;
+  }
+};
+
 // Device state of one basis (owned by the context in ecg_capi.hip).
 struct Ctx {
   SysConst sc;
@@ -47,6 +88,7 @@ struct Ctx {
   zd* S = nullptr;            // ld x ld
   // optional events bracketing the off-diagonal and adjoint kernels
   hipEvent_t ev_off0 = nullptr, ev_off1 = nullptr, ev_adj0 = nullptr, ev_adj1 = nullptr;
+  Tracer* tr = nullptr;
 };
 
 // pair assembly (ecg_kernels.hip)
@@ -71,6 +113,7 @@ struct Factor {
   int* flags = nullptr;      // 2 x ld/NB block-ready flags of the triangular solves
   int* pflag = nullptr;      // ld/NB + 1 ready flags of the factorisation (diagonal block, column tiles)
   int epoch = 0;             // flag value of the current solve
+  Tracer* tr = nullptr;
 };
 int form_shifted(const Ctx& c, Factor& f, double sigma, hipStream_t s);
 int ldlh_factor(Factor& f, hipStream_t s);

@@ -616,8 +616,10 @@ static int launch_assemble(const Ctx& c, int r0, int r1, hipStream_t s) {
   const long long ntiles = (long long)tr1 * (tr1 + 1) / 2 - (long long)tr0 * (tr0 + 1) / 2;
   const size_t lds = sizeof(double) * TILE * (BraRec<N>::stride + c.T * KetRec<N>::stride);
   if (c.ev_off0) (void)hipEventRecord(c.ev_off0, s);
+  if (c.tr) c.tr->begin(TR_OFFDIAG, s);
   hipLaunchKernelGGL(k_offdiag<N>, dim3((unsigned)ntiles), dim3(TILE, TILE), lds, s, c.sc, r0, r1, c.ld, tr0,
                      c.bra, c.ket, c.diagS, c.H, c.S);
+  if (c.tr) c.tr->end(s);
   if (c.ev_off1) (void)hipEventRecord(c.ev_off1, s);
   return (int)hipGetLastError();
 }
@@ -643,8 +645,10 @@ static int launch_adjoint(const Ctx& c, int nfru, double E, const zd* cvec, doub
   const size_t lds = sizeof(double) * TILE *
                      (BraRec<N>::stride + c.T * KetRec<N>::stride + TILE * AdjAcc<N>::nvals);
   if (c.ev_adj0) (void)hipEventRecord(c.ev_adj0, s);
+  if (c.tr) c.tr->begin(TR_ADJOINT, s);
   hipLaunchKernelGGL(k_adjoint<N>, dim3((unsigned)ntiles), dim3(TILE, TILE), lds, s, c.sc, K, nfru, tr0, nt, E,
                      c.bra, c.ket, c.diagS, cvec, part);
+  if (c.tr) c.tr->end(s);
   if (c.ev_adj1) (void)hipEventRecord(c.ev_adj1, s);
   hipLaunchKernelGGL(k_adjoint_finish<N>, dim3((K - nfru + 127) / 128), dim3(128), 0, s, K, nfru, nt, tr0,
                      c.bra, part, grad);

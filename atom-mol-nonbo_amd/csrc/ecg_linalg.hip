@@ -24,13 +24,19 @@ __global__ void k_form_shifted(int K, int ld, const zd* __restrict__ H, const zd
 }
 
 int form_shifted(const Ctx& c, Factor& f, double sigma, hipStream_t s) {
+  if (c.tr) c.tr->begin(TR_FORM_SHIFTED, s);
   hipLaunchKernelGGL(k_form_shifted, dim3((c.K + 255) / 256, c.K), dim3(256), 0, s, c.K, c.ld, c.H, c.S,
                      sigma, f.F);
+  if (c.tr) c.tr->end(s);
   return (int)hipGetLastError();
 }
 
 // ------------------------------------------------ workgroup flags
-constexpr long FACT_SPIN_LIMIT = 1L << 20;
+// Device-side waits are bounded in time: after WAIT_LIMIT_TICKS of the
+// constant-rate wall clock (2 s at its 100 MHz) the waiting wave records
+// status[2] and proceeds, so every wave exits; the host reports
+// ECG_ERR_SYNC.
+constexpr unsigned long long WAIT_LIMIT_TICKS = 200000000ull;
 
 __device__ inline void publish_wg(int* flag, int epoch) {
   __threadfence();
@@ -40,11 +46,11 @@ __device__ inline void publish_wg(int* flag, int epoch) {
 
 __device__ inline void wait_wg(const int* f1, const int* f2, int epoch, int* status) {
   if (threadIdx.x == 0) {
-    long n = 0;
+    const unsigned long long t0 = wall_clock64();
     while (__hip_atomic_load(f1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch ||
            __hip_atomic_load(f2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
       __builtin_amdgcn_s_sleep(2);
-      if (++n > FACT_SPIN_LIMIT) {
+      if (wall_clock64() - t0 > WAIT_LIMIT_TICKS) {
         atomicExch(&status[2], 1);
         break;
       }
@@ -514,7 +520,7 @@ __global__ void __launch_bounds__(256) k_ldl_panel(int K, int ld, int j0, zd* __
 //     are published, then runs the panel solve for its strip.
 // Publication: the producer's threads fence at agent scope, the workgroup
 // synchronises, thread 0 stores the flag (release).  A waiting workgroup's
-// thread 0 polls the flags (relaxed, bounded: after SPIN_LIMIT rounds it
+// thread 0 polls the flags (relaxed, bounded in time: past the limit it
 // records status[2] and proceeds, so every workgroup exits); the data it
 // then reads was written in this launch by the producer only, and caches
 // hold no older copy of it (they are invalidated at kernel start).
@@ -582,10 +588,15 @@ __global__ void k_clear_status(int* status) {
 int ldlh_factor(Factor& f, hipStream_t s) {
   const int K = f.K;
   hipLaunchKernelGGL(k_clear_status, dim3(1), dim3(64), 0, s, f.status);
+  if (f.tr) f.tr->begin(TR_LDL_DIAG, s);
   hipLaunchKernelGGL(k_ldl_diag, dim3(1), dim3(256), 0, s, K, f.ld, 0, f.F, f.D, f.Dinv, f.status);
-  if (K > NB)
+  if (f.tr) f.tr->end(s);
+  if (K > NB) {
+    if (f.tr) f.tr->begin(TR_LDL_PANEL, s);
     hipLaunchKernelGGL(k_ldl_panel, dim3((K - NB + SB - 1) / SB), dim3(256), 0, s, K, f.ld, 0, f.F, f.D, f.Dinv,
                        f.W);
+    if (f.tr) f.tr->end(s);
+  }
   for (int j0 = 0, col = 0; j0 + NB < K; j0 += NB, ++col) {
     const int rest = K - (j0 + NB);   // rows below block column col
     const int nrb = (rest + NB - 1) / NB;
@@ -595,8 +606,10 @@ int ldlh_factor(Factor& f, hipStream_t s) {
     zd* Wcur = f.W + (size_t)(col & 1) * f.ld * NB;
     zd* Wnext = f.W + (size_t)((col + 1) & 1) * f.ld * NB;
     const int epoch = ++f.epoch;
+    if (f.tr) f.tr->begin(TR_LDL_UPDATE, s);
     hipLaunchKernelGGL(k_ldl_update, dim3(ntiles + nstrip), dim3(256), 0, s, K, f.ld, j0, f.F, Wcur, Wnext, f.D,
                        f.Dinv, f.status, f.pflag, epoch, ntiles);
+    if (f.tr) f.tr->end(s);
   }
   return (int)hipGetLastError();
 }
@@ -612,11 +625,10 @@ int ldlh_factor(Factor& f, hipStream_t s) {
 // poll returned.  No acquire fence is used: it would invalidate the L2 that
 // holds the factor being streamed.  The next L block is prefetched into
 // registers before polling, and X_b (the inverse of the diagonal block) is
-// staged in LDS at kernel start.  Polling is bounded: after SPIN_LIMIT
-// rounds the wave records an error in status[2] and proceeds, so every wave
-// always exits.  Dependencies only point to lower workgroup indices, which
+// staged in LDS at kernel start.  Polling is bounded in time
+// (WAIT_LIMIT_TICKS): past it the wave records an error in status[2] and
+// proceeds, so every wave always exits.  Dependencies only point to lower workgroup indices, which
 // are dispatched no later than the consumer.
-constexpr long SPIN_LIMIT = 1L << 20;
 constexpr int RUN_MAX = 8;   // blocks folded per round
 
 __device__ inline zd load_published(const zd* p) {
@@ -642,7 +654,7 @@ __device__ inline void publish_flag(int* flag, int epoch) {
 __device__ inline int await_run(const int* flags, int first, int dir, int left, int epoch, int* status) {
   const int lane = threadIdx.x & 63;
   const int want = min(left, RUN_MAX);
-  long n = 0;
+  const unsigned long long t0 = wall_clock64();
   for (;;) {
     bool ready = false;
     if (lane < want)
@@ -651,7 +663,7 @@ __device__ inline int await_run(const int* flags, int first, int dir, int left, 
     const int m = (~mask == 0ull) ? 64 : (__ffsll((long long)~mask) - 1);
     if (m > 0) return min(m, want);
     __builtin_amdgcn_s_sleep(1);
-    if (++n > SPIN_LIMIT) {
+    if (wall_clock64() - t0 > WAIT_LIMIT_TICKS) {
       if (lane == 0) atomicExch(&status[2], 1);
       return want;
     }
@@ -805,10 +817,14 @@ int ldlh_solve(Factor& f, const zd* b, zd* x, hipStream_t s) {
   int* ff = f.flags;
   int* fb = f.flags + f.ld / NB;
   const int e1 = ++f.epoch;
+  if (f.tr) f.tr->begin(TR_TRSV_FWD, s);
   hipLaunchKernelGGL(k_trsv_fwd, dim3(nb), dim3(256), 0, s, f.K, f.ld, f.F, f.D, f.Dinv, b, y, z, ff, ff, e1,
                      f.status);
+  if (f.tr) f.tr->end(s);
   const int e2 = ++f.epoch;
+  if (f.tr) f.tr->begin(TR_TRSV_BWD, s);
   hipLaunchKernelGGL(k_trsv_bwd, dim3(nb), dim3(256), 0, s, f.K, f.ld, f.F, f.Dinv, z, x, fb, fb, e2, f.status);
+  if (f.tr) f.tr->end(s);
   return (int)hipGetLastError();
 }
 

@@ -25,6 +25,7 @@ ECG_ERR_HIP = -2
 ECG_ERR_NOMEM = -3
 ECG_ERR_NONFINITE = -4
 ECG_ERR_STATE = -5
+ECG_ERR_SYNC = -6
 ECG_SINGULAR = 1
 ECG_NOT_CONVERGED = 2
 ECG_ROOT_NOT_FOUND = 3
@@ -32,6 +33,7 @@ ECG_ROOT_NOT_FOUND = 3
 _NAMES = {
     ECG_ERR_ARG: "invalid argument", ECG_ERR_HIP: "HIP runtime error", ECG_ERR_NOMEM: "device allocation failed",
     ECG_ERR_NONFINITE: "non-finite pair element", ECG_ERR_STATE: "call order violated",
+    ECG_ERR_SYNC: "device-side wait timed out",
     ECG_SINGULAR: "H - sigma S singular", ECG_NOT_CONVERGED: "inverse iteration not converged",
     ECG_ROOT_NOT_FOUND: "requested root not isolated",
 }
@@ -40,7 +42,7 @@ EXPORTS = [
     "ecg_ctx_create", "ecg_ctx_destroy", "ecg_set_params", "ecg_update_params", "ecg_matrix_elements",
     "ecg_compute_mat_elem", "ecg_compute_mat_elem_and_deriv", "ecg_ghepiis", "ecg_energy", "ecg_energy_grad",
     "ecg_energy_grad_stored", "ecg_batch_create", "ecg_batch_destroy", "ecg_batch_set_config",
-    "ecg_batch_energy_grad", "ecg_get_params", "ecg_screen_prepare", "ecg_screen", "ecg_screen_loop", "ecg_last_timings",
+    "ecg_batch_energy_grad", "ecg_trace", "ecg_trace_read", "ecg_get_params", "ecg_screen_prepare", "ecg_screen", "ecg_screen_loop", "ecg_last_timings",
 ]
 
 
@@ -81,6 +83,8 @@ def load(path: str = LIB_PATH):
     L.ecg_batch_destroy.argtypes = [vp]
     L.ecg_batch_set_config.argtypes = [vp, c_int, c_int]
     L.ecg_batch_energy_grad.argtypes = [vp, _d, c_double, c_int, _d, _d, _d]
+    L.ecg_trace.argtypes = [vp, c_int]
+    L.ecg_trace_read.argtypes = [vp, _d, _d, c_int]
     L.ecg_get_params.argtypes = [vp, _d]
     L.ecg_screen_prepare.argtypes = [vp, c_double, c_double, c_int, _d]
     L.ecg_screen.argtypes = [vp, c_int, _d, c_int, _d, _d]
@@ -244,6 +248,22 @@ class EcgContext:
                                            ctypes.byref(E), _p(g), _p(info))
         _check("ecg_energy_grad_stored", rc, ok=(ECG_OK, ECG_NOT_CONVERGED))
         return rc, E.value, g, info
+
+    TRACE_LABELS = ("offdiag", "adjoint", "ldl_update", "ldl_diag", "ldl_panel", "trsv_fwd", "trsv_bwd", "gemv",
+                    "form_shifted")
+
+    def trace(self, enable=True):
+        _check("ecg_trace", load().ecg_trace(self._h, int(bool(enable))))
+
+    def trace_read(self):
+        """{kernel: (launches, total device ms)} since tracing was enabled."""
+        n = len(self.TRACE_LABELS)
+        cnt = np.zeros(n)
+        tot = np.zeros(n)
+        m = load().ecg_trace_read(self._h, _p(cnt), _p(tot), n)
+        if m < 0:
+            raise EcgError("ecg_trace_read", m)
+        return {self.TRACE_LABELS[i]: (int(cnt[i]), float(tot[i])) for i in range(m)}
 
     def get_params(self):
         p = np.zeros((self.K, self.npt))

@@ -26,7 +26,7 @@ module ecg_capi
   use, intrinsic :: iso_c_binding
   implicit none
   integer(c_int), parameter :: ECG_OK = 0, ECG_ERR_ARG = -1, ECG_ERR_HIP = -2, ECG_ERR_NOMEM = -3, &
-                               ECG_ERR_NONFINITE = -4, ECG_ERR_STATE = -5, ECG_SINGULAR = 1, &
+                               ECG_ERR_NONFINITE = -4, ECG_ERR_STATE = -5, ECG_ERR_SYNC = -6, ECG_SINGULAR = 1, &
                                ECG_NOT_CONVERGED = 2, ECG_ROOT_NOT_FOUND = 3
   interface
     integer(c_int) function ecg_ctx_create(n, T, yhy_mats, yhy_coef, mass, q0, q, ctx) &
@@ -147,6 +147,17 @@ module ecg_capi
       real(c_double), value :: tol
       integer(c_int), value :: maxiter
       real(c_double), intent(out) :: E(*), grad(*), lat_ms(*)
+    end function
+    integer(c_int) function ecg_trace(ctx, enable) bind(C, name="ecg_trace")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: enable
+    end function
+    integer(c_int) function ecg_trace_read(ctx, count, total_ms, nmax) bind(C, name="ecg_trace_read")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: count(*), total_ms(*)
+      integer(c_int), value :: nmax
     end function
     integer(c_int) function ecg_get_params(ctx, params) bind(C, name="ecg_get_params")
       import :: c_int, c_double, c_ptr

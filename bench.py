@@ -55,6 +55,74 @@ def algorithmic_flops(system, params, K):
     return f0, f1, f3, system.T * (K * (K - 1) / 2 * f3 + K * f1)
 
 
+def factor_flops(K, nb=64):
+    """Algorithmic flops (complex FMA = 8) of each trailing-update launch of
+    the blocked LDL^H (one per block column j but the last): the rank-64
+    Hermitian update of the m x m trailing lower triangle (m = K - 64 (j+1)),
+    the look-ahead LDL^H and unit-lower inverse of the next b x b diagonal
+    block (b = min(64, m)), and the next panel W = A X^H over the m - b rows
+    below it (X lower triangular)."""
+    out = []
+    for j0 in range(0, K - nb, nb):
+        m = K - (j0 + nb)
+        b = min(nb, m)
+        upd = 8.0 * nb * m * (m + 1) / 2
+        diag = 8.0 * b ** 3 / 6 + 8.0 * b ** 3 / 6
+        panel = 8.0 * (m - b) * b * (b + 1) / 2
+        out.append(upd + diag + panel)
+    return out
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (profiles/pmc_traffic.txt, lines 'kernel bytes_per_launch'),
+    or None when it has not been measured."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 2 and parts[0] == kernel and not line.startswith("#"):
+            try:
+                return float(parts[1])
+            except ValueError:
+                return None
+    return None
+
+
+def kernel_table(system, base, K, trace, nsteps, flops_adj):
+    """Per-kernel device time per E+grad step (traced launches) and each
+    kernel's roofline figures; the dominant kernel is the one with the
+    largest share."""
+    upd = factor_flops(K)
+    nf = max(1, trace["ldl_update"][0] // len(upd)) if trace["ldl_update"][0] else 1
+    rows = {}
+    for name, (cnt, tot) in trace.items():
+        if cnt == 0:
+            continue
+        avg = tot / cnt * 1e-3   # s per launch
+        r = {"launches_per_step": cnt / nsteps, "ms_per_step": tot / nsteps, "avg_us": avg * 1e6}
+        if name == "adjoint":
+            r.update(bound="mfma", work=flops_adj, unit="TFLOP/s", achieved=flops_adj / avg / 1e12,
+                     peak=FP64_PEAK_TFLOPS)
+        elif name == "ldl_update":
+            w = float(np.mean(upd))
+            r.update(bound="mfma", work=w, unit="TFLOP/s", achieved=w / avg / 1e12, peak=FP64_PEAK_TFLOPS)
+        elif name in ("trsv_fwd", "trsv_bwd"):
+            w = 16.0 * K * (K + 1) / 2     # the triangle of L streamed once
+            r.update(bound="hbm", work=w, unit="GB/s", achieved=w / avg / 1e9, peak=HBM_PEAK_GBS)
+        elif name == "offdiag":
+            f0 = np.mean([__import__("oracle").oracle.flops_per_pair(system, base[0], base[1], 0, term=t)
+                          for t in range(system.T)])
+            w = system.T * f0 * K * (K - 1) / 2
+            r.update(bound="mfma", work=w, unit="TFLOP/s", achieved=w / avg / 1e12, peak=FP64_PEAK_TFLOPS)
+        if "achieved" in r:
+            r["frac"] = r["achieved"] / r["peak"]
+        rows[name] = r
+    dominant = max(rows, key=lambda k: rows[k]["ms_per_step"])
+    return rows, dominant, nf
+
+
 def cpu_baseline(system, params, K, seconds_target=15.0):
     """Oracle pair algebra on all host threads over a random sample of the
     workload's pairs, then the host eigen step (LAPACK zhegvx via scipy) at
@@ -319,8 +387,20 @@ def main():
     s_per = t_all / args.steps
     npairs = K * (K + 1) // 2
     f0, f1, f3, flops_adj = algorithmic_flops(system, base, K)
-    adj_s = float(np.mean(adj_ms)) * 1e-3
-    achieved = flops_adj / adj_s / 1e12
+    # traced pass (not timed): per-kernel device time of the same steps
+    ctx.trace(True)
+    ntr = 3
+    for i in range(ntr):
+        step(i)
+    trace = ctx.trace_read()
+    ctx.trace(False)
+    rows, dominant, _ = kernel_table(system, base, K, trace, ntr, flops_adj)
+    dk = rows[dominant]
+    names = {"ldl_update": "k_ldl_update (trailing update + look-ahead diagonal block + next panel)",
+             "adjoint": "k_adjoint<3> (fused E+grad pair pass)", "trsv_fwd": "k_trsv_fwd", "trsv_bwd": "k_trsv_bwd",
+             "offdiag": "k_offdiag<3>"}
+    rocname = {"ldl_update": "k_ldl_update", "adjoint": "k_adjoint<3>", "trsv_fwd": "k_trsv_fwd",
+               "trsv_bwd": "k_trsv_bwd", "offdiag": "k_offdiag<3>"}
     out = {
         "metric": "FP64 pair H,S+grad elements/s (E+grad, K=2000, n=3)",
         "value": npairs / s_per,
@@ -340,16 +420,19 @@ def main():
         "phases_ms": {"offdiag_kernel": float(np.mean(off_ms)), "eigen_step": float(np.mean(eig_ms)),
                       "adjoint_kernel": float(np.mean(adj_ms)), "inverse_iterations": float(np.mean(iters))},
         "roofline": {
-            "bound": "mfma",
-            "kernel": "k_adjoint<3> (fused E+grad pair pass)",
-            "achieved": achieved,
-            "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
-            "flops_per_launch": flops_adj,
-            "flops_per_pair_term": {"energy": f0, "grad_bra": f1, "grad_both": f3},
+            "bound": dk["bound"],
+            "kernel": names.get(dominant, dominant),
+            "achieved": dk["achieved"],
+            "peak": dk["peak"],
+            "unit": dk["unit"],
+            "frac": dk["frac"],
+            "traffic": pmc_traffic(rocname.get(dominant, dominant)),
+            "work_per_launch": dk["work"],
+            "avg_launch_us": dk["avg_us"],
+            "share_of_step": dk["ms_per_step"] / (1e3 * s_per),
         },
+        "kernels": rows,
+        "flops_per_pair_term": {"energy": f0, "grad_bra": f1, "grad_both": f3},
     }
     if not args.no_screen:
         out["screening"] = screening_leg(system, base, K)

@@ -33,6 +33,7 @@ extern "C" {
 #define ECG_ERR_NONFINITE (-4)   /* a pair element is Inf/NaN (pseudo-Cholesky breakdown);
                                     the reference does not flag this (src/matelem.f90:211-214) */
 #define ECG_ERR_STATE (-5)       /* call order violated (e.g. no parameters set) */
+#define ECG_ERR_SYNC (-6)        /* a device-side wait between workgroups exceeded its time limit */
 /* positive codes mirror the reference:
  *   GHEPIIS / EnergyIA: 1 = H - sigma S singular (zero pivot), 2 = inverse
  *   iteration not converged to Tol within MaxIter (src/linalg.f90:1911);
@@ -150,6 +151,25 @@ int ecg_batch_destroy(ecg_batch* b);
 int ecg_batch_set_config(ecg_batch* b, int packed, int nstreams);
 int ecg_batch_energy_grad(ecg_batch* b, const double* sigma, double tol, int maxiter, double* E, double* grad,
                           double* lat_ms);
+
+/* Per-kernel tracing of one context: with tracing on, every launch of the
+ * kernels below is bracketed by HIP events on the context's stream and its
+ * device time is accumulated per kernel (the reference's own timing is
+ * MPI_WTIME around whole routines).  ecg_trace(ctx, 1) enables and resets;
+ * ecg_trace_read writes launch counts and total milliseconds per label and
+ * returns the number of labels written. */
+#define ECG_TRACE_OFFDIAG 0       /* off-diagonal pair assembly */
+#define ECG_TRACE_ADJOINT 1       /* fused adjoint pair pass */
+#define ECG_TRACE_LDL_UPDATE 2    /* trailing update + look-ahead diagonal block + next panel */
+#define ECG_TRACE_LDL_DIAG 3      /* first diagonal block */
+#define ECG_TRACE_LDL_PANEL 4     /* first panel */
+#define ECG_TRACE_TRSV_FWD 5      /* forward triangular solve */
+#define ECG_TRACE_TRSV_BWD 6      /* backward triangular solve */
+#define ECG_TRACE_GEMV 7          /* (reserved) */
+#define ECG_TRACE_FORM_SHIFTED 8  /* H - sigma S */
+#define ECG_TRACE_COUNT 9
+int ecg_trace(ecg_ctx* ctx, int enable);
+int ecg_trace_read(ecg_ctx* ctx, double* count, double* total_ms, int nmax);
 
 /* Current parameters of the K functions (npt x K), e.g. after
  * ecg_screen_loop accepted candidates into function K. */
