@@ -327,6 +327,23 @@ def batch_leg(nprob=32, reps=3):
     }
 
 
+def copy_calibration(nbytes=256 << 20, reps=3):
+    """Device-to-device copies of a known size through the HIP runtime (its
+    copy kernel reads and writes nbytes each): the reference point that
+    scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print("# copy calibration: %d x %d bytes" % (reps, nbytes), file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -337,12 +354,16 @@ def main():
     ap.add_argument("--no-screen", action="store_true", help="skip the candidate-screening leg")
     ap.add_argument("--no-modes", action="store_true", help="skip the stored/recompute memory-mode leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the multi-problem batch leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first issue device copies of a known size (reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the path on a single MI355X "
               "(see DESIGN.md for how it would shard); --gpus %d is not supported" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
     K = args.K
     system = named_system("h2")
     base = seeded_basis(system.n, K, SEED_BASE + 3)
