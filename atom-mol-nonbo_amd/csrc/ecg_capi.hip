@@ -108,6 +108,8 @@ int ensure_capacity(ecg_ctx* x, int K) {
   RCCHK(dev_alloc(&f.Dinv, (size_t)(ld / ecg::NB) * 2 * ecg::NB * ecg::NB));
   RCCHK(dev_alloc(&f.status, (size_t)4));
   RCCHK(dev_alloc(&f.vec, (size_t)4 * ld));
+  f.ld = ld;
+  RCCHK(ecg::solve_buffers_init(f, x->stream));
   RCCHK(dev_alloc(&f.tvec, (size_t)ld));
   RCCHK(dev_alloc(&f.flags, (size_t)2 * (ld / ecg::NB)));
   HIPCHK(hipMemsetAsync(f.flags, 0, sizeof(int) * 2 * (ld / ecg::NB), x->stream));
@@ -306,6 +308,13 @@ int inverse_iteration(ecg_ctx* x, double sigma, double tol, int maxiter, bool wa
   r = InvitResult();
   for (int it = 0; it < maxiter; ++it) {
     RCCHK(ecg::invit_iteration(c, x->f, v, y, x->stream));
+    if (stop_at_floor && it == 0 && maxiter > 1) {
+      // energy paths: the first measure is never the last one (the second
+      // iteration is queued without a host round trip)
+      r.iters = 1;
+      std::swap(v, y);
+      continue;
+    }
     double sc[2];
     HIPCHK(hipMemcpyAsync(sc, x->f.scal, sizeof(sc), hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));

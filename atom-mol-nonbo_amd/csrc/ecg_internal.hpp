@@ -118,6 +118,7 @@ struct Factor {
 int form_shifted(const Ctx& c, Factor& f, double sigma, hipStream_t s);
 int ldlh_factor(Factor& f, hipStream_t s);
 int ldlh_solve(Factor& f, const zd* b, zd* x, hipStream_t s);
+int solve_buffers_init(Factor& f, hipStream_t s);   // sentinel fill of the forward solve's value buffer
 int invit_iteration(const Ctx& c, Factor& f, const zd* v, zd* x, hipStream_t s);
 int rayleigh(const Ctx& c, Factor& f, const zd* x, double sigma, hipStream_t s);
 int normalise(zd* x, int K, const double* t, hipStream_t s);

@@ -444,7 +444,7 @@ __device__ __attribute__((always_inline)) inline void adjoint_tile(const SysCons
 }
 
 template <int N>
-__global__ void __launch_bounds__(TILE* TILE) k_adjoint(const SysConst sc, int K, int nfru, int tr0, int nt,
+__global__ void __launch_bounds__(TILE* TILE, 2) k_adjoint(const SysConst sc, int K, int nfru, int tr0, int nt,
                                                       double E,
                                                       const double* __restrict__ bra,
                                                       const double* __restrict__ ket,
@@ -461,6 +461,9 @@ __global__ void __launch_bounds__(TILE* TILE) k_adjoint(const SysConst sc, int K
 // the chain rule to vech L, vech B:
 //   dE/dL(i,j) = 4 (RA L)(i,j) + 2 Ra delta_ij / L_jj
 //   dE/dB(i,i) = 2 IB_ii,   dE/dB(i,j) = 4 IB_ij
+// One wave per function: lanes take the slots s = lane, lane + 64, ...
+// (contiguous in memory for one function), then a fixed-order butterfly
+// reduction; lane 0 applies the chain rule.
 template <int N>
 __device__ __attribute__((always_inline)) inline void adjoint_finish_one(int f, int nfru, int nt, int tr0,
                                                                          const double* __restrict__ bra,
@@ -468,11 +471,12 @@ __device__ __attribute__((always_inline)) inline void adjoint_finish_one(int f, 
                                                                          double* __restrict__ grad) {
   constexpr int m = N * (N + 1) / 2, nv = AdjAcc<N>::nvals;
   const int r = f / TILE;
+  const int lane = threadIdx.x & 63;
   double acc[nv];
 #pragma unroll
   for (int e = 0; e < nv; ++e) acc[e] = 0.0;
   // valid slots: bra slots c = 0..r (tile rows >= tr0 only), ket slots r'+1 for r' = max(r,tr0)..nt-1
-  for (int s = 0; s <= nt; ++s) {
+  for (int s = lane; s <= nt; s += 64) {
     bool valid;
     if (s <= r) valid = (r >= tr0);
     else valid = (s - 1 >= tr0);
@@ -481,6 +485,10 @@ __device__ __attribute__((always_inline)) inline void adjoint_finish_one(int f, 
 #pragma unroll
     for (int e = 0; e < nv; ++e) acc[e] += p[e];
   }
+#pragma unroll
+  for (int e = 0; e < nv; ++e)
+    for (int o = 32; o > 0; o >>= 1) acc[e] += __shfl_xor(acc[e], o);
+  if (lane != 0) return;
   const double* L = bra + (size_t)f * BraRec<N>::stride + BraRec<N>::lp;
   double* g = grad + (size_t)(f - nfru) * 2 * m;
 #pragma unroll
@@ -498,9 +506,11 @@ __device__ __attribute__((always_inline)) inline void adjoint_finish_one(int f, 
 }
 
 template <int N>
-__global__ void k_adjoint_finish(int K, int nfru, int nt, int tr0, const double* __restrict__ bra,
-                                 const double* __restrict__ part, double* __restrict__ grad) {
-  const int f = nfru + blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_adjoint_finish(int K, int nfru, int nt, int tr0,
+                                                        const double* __restrict__ bra,
+                                                        const double* __restrict__ part,
+                                                        double* __restrict__ grad) {
+  const int f = nfru + blockIdx.x * 4 + (threadIdx.x >> 6);
   if (f >= K) return;
   adjoint_finish_one<N>(f, nfru, nt, tr0, bra, part, grad);
 }
@@ -526,7 +536,7 @@ __global__ void __launch_bounds__(TILE* TILE) k_offdiag_multi(const ProbDesc* __
 }
 
 template <int N>
-__global__ void __launch_bounds__(TILE* TILE) k_adjoint_multi(const ProbDesc* __restrict__ pd, int np) {
+__global__ void __launch_bounds__(TILE* TILE, 2) k_adjoint_multi(const ProbDesc* __restrict__ pd, int np) {
   extern __shared__ double lds[];
   const int p = find_prob(pd, np, blockIdx.x, true);
   const ProbDesc& P = pd[p];
@@ -537,8 +547,8 @@ __global__ void __launch_bounds__(TILE* TILE) k_adjoint_multi(const ProbDesc* __
 
 // thread -> function f0 + t of the batch's concatenated functions
 template <int N>
-__global__ void k_adjoint_finish_multi(const ProbDesc* __restrict__ pd, int np, int nf) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_adjoint_finish_multi(const ProbDesc* __restrict__ pd, int np, int nf) {
+  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (g >= nf) return;
   int p = 0;
   while (p + 1 < np && pd[p + 1].f0 <= g) ++p;
@@ -650,8 +660,8 @@ static int launch_adjoint(const Ctx& c, int nfru, double E, const zd* cvec, doub
                      c.bra, c.ket, c.diagS, cvec, part);
   if (c.tr) c.tr->end(s);
   if (c.ev_adj1) (void)hipEventRecord(c.ev_adj1, s);
-  hipLaunchKernelGGL(k_adjoint_finish<N>, dim3((K - nfru + 127) / 128), dim3(128), 0, s, K, nfru, nt, tr0,
-                     c.bra, part, grad);
+  hipLaunchKernelGGL(k_adjoint_finish<N>, dim3((K - nfru + 3) / 4), dim3(256), 0, s, K, nfru, nt, tr0, c.bra, part,
+                     grad);
   return (int)hipGetLastError();
 }
 
@@ -724,7 +734,7 @@ int packed_pairs(int n, bool adjoint, const ProbDesc* dpd, int np, long long nti
       const size_t lds = sizeof(double) * TILE *
                          (BraRec<N>::stride + maxT * KetRec<N>::stride + TILE * AdjAcc<N>::nvals);
       hipLaunchKernelGGL(k_adjoint_multi<N>, dim3((unsigned)ntiles), dim3(TILE, TILE), lds, s, dpd, np);
-      hipLaunchKernelGGL(k_adjoint_finish_multi<N>, dim3((nf + 127) / 128), dim3(128), 0, s, dpd, np, nf);
+      hipLaunchKernelGGL(k_adjoint_finish_multi<N>, dim3((nf + 3) / 4), dim3(256), 0, s, dpd, np, nf);
     }
     return (int)hipGetLastError();
   };

@@ -204,8 +204,12 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
         const double det = fma(ap, cq, -fma(bq.x, bq.x, bq.y * bq.y));
         const double prod = ap * det;
         double rd1, rd2, dq;
-        if (prod != 0.0) {
-          const double r = 1.0 / prod;
+        if (prod != 0.0 && fabs(prod) > 1e-300 && fabs(prod) < 1e300) {
+          // 1/prod from the hardware reciprocal refined by two Newton steps
+          // (error below 1 ulp); the exceptional path keeps IEEE division
+          double r = __builtin_amdgcn_rcp(prod);
+          r = fma(fma(-prod, r, 1.0), r, r);
+          r = fma(fma(-prod, r, 1.0), r, r);
           rd1 = det * r;
           rd2 = ap * ap * r;
           dq = det * rd1;
@@ -341,8 +345,8 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
     }
   }
   __syncthreads();
-  // ---- outputs: X, X^H and D first (all the next panel needs), then the
-  // flag (look-ahead use), then L and its mirror in F
+  // ---- outputs: X and D first (all the next panel needs), then the flag
+  // (look-ahead use), then X^H, L and its mirror in F
   zd* Xg = Dinv + (size_t)(j0 / NB) * 2 * NB * NB;
   zd* XHg = Xg + NB * NB;
   {
@@ -359,11 +363,17 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
       const bool in = (i < jb && j < jb);
       const double dg = (in && i == j) ? 1.0 : 0.0;
       Xg[(size_t)j * NB + i] = (in && i > j) ? up[u] : mkz(dg, 0.0);
-      // X^H(i, j) = conj(X(j, i)), X(j, i) (j > i) at A[i][j]
-      XHg[(size_t)j * NB + i] = (in && i < j) ? zconj(lo[u]) : mkz(dg, 0.0);
     }
     for (int p = tid; p < jb; p += blockDim.x) D[j0 + p] = L.d[p];
     if (pub) publish_wg(pub, epoch);
+#pragma unroll
+    for (int u = 0; u < NB * NB / 256; ++u) {
+      const int e = tid + 256 * u, j = e >> 6, i = e & (NB - 1);
+      const bool in = (i < jb && j < jb);
+      const double dg = (in && i == j) ? 1.0 : 0.0;
+      // X^H(i, j) = conj(X(j, i)), X(j, i) (j > i) at A[i][j]
+      XHg[(size_t)j * NB + i] = (in && i < j) ? zconj(lo[u]) : mkz(dg, 0.0);
+    }
 #pragma unroll
     for (int u = 0; u < NB * NB / 256; ++u) {
       const int e = tid + 256 * u, j = e >> 6, i = e & (NB - 1);
@@ -616,65 +626,91 @@ int ldlh_factor(Factor& f, hipStream_t s) {
 
 // ------------------------------------------------------------ solve
 // Sync-free blocked triangular solves.  Workgroup w owns one 64-row block
-// and consumes the solution blocks it depends on as their producers publish
-// them.  Producer (wave 0): stores its 64 values with device-coherent atomic
-// stores, then raises the block's flag to the current epoch with a release
-// store.  Consumer (wave 0): polls flags with relaxed device-coherent loads,
-// one lane per block, takes every leading run of ready blocks at once and
-// reads their values with device-coherent atomic loads issued after the
-// poll returned.  No acquire fence is used: it would invalidate the L2 that
-// holds the factor being streamed.  The next L block is prefetched into
-// registers before polling, and X_b (the inverse of the diagonal block) is
-// staged in LDS at kernel start.  Polling is bounded in time
-// (WAIT_LIMIT_TICKS): past it the wave records an error in status[2] and
-// proceeds, so every wave always exits.  Dependencies only point to lower workgroup indices, which
+// and consumes the solution blocks it depends on as their producers store
+// them.  The solution values are their own ready flags: before a solve the
+// value buffer holds a signalling-NaN sentinel (arithmetic only produces
+// quiet NaNs, so no computed value equals it); a producer (wave 0) stores
+// its 64 values with device-coherent atomic stores followed by a release
+// fence; a consumer (wave 0) loads the values of up to RUN_MAX pending
+// blocks with device-coherent atomic loads, one value per lane and block,
+// and takes the leading run of blocks whose values are all present; the
+// values it loaded are the ones it uses.  Each solve's forward kernel resets
+// the backward kernel's output blocks and the backward kernel resets the
+// forward kernel's, so every reset is ordered before its use by a kernel
+// boundary.  No acquire fence is used: it would invalidate the L2 that holds
+// the factor being streamed.  The next L block is prefetched into registers
+// before polling, and X_b (the inverse of the diagonal block) is staged in
+// LDS at kernel start.  Polling is bounded in time (WAIT_LIMIT_TICKS): past
+// it the wave records an error in status[2] and proceeds, so every wave
+// always exits.  Dependencies only point to lower workgroup indices, which
 // are dispatched no later than the consumer.
 constexpr int RUN_MAX = 8;   // blocks folded per round
+constexpr unsigned long long SOLVE_SENTINEL = 0x7FF4ECB0DEADBEEFull;   // signalling NaN
 
-__device__ inline zd load_published(const zd* p) {
-  const unsigned long long* q = reinterpret_cast<const unsigned long long*>(p);
-  unsigned long long a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  unsigned long long b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return mkz(__longlong_as_double((long long)a), __longlong_as_double((long long)b));
-}
 __device__ inline void store_published(zd* p, zd v) {
   unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
   __hip_atomic_store(q, (unsigned long long)__double_as_longlong(v.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __hip_atomic_store(q + 1, (unsigned long long)__double_as_longlong(v.y), __ATOMIC_RELAXED,
                      __HIP_MEMORY_SCOPE_AGENT);
 }
-// wave 0, after store_published of its values
-__device__ inline void publish_flag(int* flag, int epoch) {
-  if ((threadIdx.x & 63) == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ inline void store_sentinel(zd* p) {
+  unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
+  q[0] = SOLVE_SENTINEL;
+  q[1] = SOLVE_SENTINEL;
 }
 
-// Wave 0: number (>= 1, <= RUN_MAX) of consecutive ready blocks among the
-// `left` blocks whose flags are flags[first + dir * t], t = 0, 1, ...
-// Returns `left` (capped) on a spin timeout.
-__device__ inline int await_run(const int* flags, int first, int dir, int left, int epoch, int* status) {
+// Wave 0: loads value `lane` of the pending blocks base + dir * 64 * t
+// (t < want, each with `valid(t)` present lanes) until a leading run of
+// blocks is complete; the run's values go to vals[t][lane]; returns its
+// length (>= 1).  Returns `want` on a time-out, with status[2] set.
+template <class Valid>
+__device__ inline int await_values(const zd* buf, long long base, int dir, int want, const Valid& valid,
+                                   zd (*vals)[NB], int* status) {
   const int lane = threadIdx.x & 63;
-  const int want = min(left, RUN_MAX);
+  const unsigned long long* q = reinterpret_cast<const unsigned long long*>(buf);
   const unsigned long long t0 = wall_clock64();
   for (;;) {
-    bool ready = false;
-    if (lane < want)
-      ready = __hip_atomic_load(&flags[first + dir * lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
-    const unsigned long long mask = __ballot(ready);
-    const int m = (~mask == 0ull) ? 64 : (__ffsll((long long)~mask) - 1);
-    if (m > 0) return min(m, want);
+    unsigned long long a[RUN_MAX], b[RUN_MAX];
+#pragma unroll
+    for (int t = 0; t < RUN_MAX; ++t) {
+      a[t] = b[t] = SOLVE_SENTINEL;
+      if (t < want && lane < valid(t)) {
+        const long long o = 2 * (base + (long long)dir * NB * t + lane);
+        a[t] = __hip_atomic_load(q + o, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        b[t] = __hip_atomic_load(q + o + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    int m = 0;
+#pragma unroll
+    for (int t = 0; t < RUN_MAX; ++t) {
+      const bool need = (t < want) && (lane < valid(t));
+      const bool here = !need || (a[t] != SOLVE_SENTINEL && b[t] != SOLVE_SENTINEL);
+      const bool all = (__ballot(!here) == 0ull);
+      if (all && m == t && t < want) m = t + 1;
+    }
+    if (m > 0) {
+#pragma unroll
+      for (int t = 0; t < RUN_MAX; ++t)
+        if (t < m)
+          vals[t][lane] = (lane < valid(t)) ? mkz(__longlong_as_double((long long)a[t]), __longlong_as_double((long long)b[t]))
+                                            : mkz(0.0, 0.0);
+      return m;
+    }
     __builtin_amdgcn_s_sleep(1);
     if (wall_clock64() - t0 > WAIT_LIMIT_TICKS) {
       if (lane == 0) atomicExch(&status[2], 1);
+      for (int t = 0; t < want; ++t) vals[t][lane] = mkz(0.0, 0.0);
       return want;
     }
   }
 }
 
-// Forward: L y = b (unit lower), then z = D^{-1} y.
+// Forward: L y = b (unit lower), then z = D^{-1} y; resets block bl of x
+// (the backward solve's output) to the sentinel.
 __global__ void __launch_bounds__(256) k_trsv_fwd(int K, int ld, const zd* __restrict__ F,
                                                   const double* __restrict__ D, const zd* __restrict__ Dinv,
                                                   const zd* __restrict__ bvec, zd* y, zd* __restrict__ z,
-                                                  const int* fin, int* fout, int epoch, int* status) {
+                                                  zd* xreset, int* status) {
   __shared__ zd Xs[NB][NB + 1];
   __shared__ zd ys[RUN_MAX][NB];
   __shared__ zd part[4][NB];
@@ -682,6 +718,7 @@ __global__ void __launch_bounds__(256) k_trsv_fwd(int K, int ld, const zd* __res
   const int bl = blockIdx.x;
   const int b0 = bl * NB, bs = min(NB, K - b0);
   const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
+  if (g == 1 && r < bs) store_sentinel(&xreset[b0 + r]);
   {
     const zd* X = Dinv + (size_t)bl * 2 * NB * NB;
     for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) Xs[e & (NB - 1)][e >> 6] = X[e];   // Xs[i][p] = X(i,p)
@@ -695,8 +732,8 @@ __global__ void __launch_bounds__(256) k_trsv_fwd(int K, int ld, const zd* __res
   }
   while (done < bl) {
     if (g == 0) {
-      const int m = await_run(fin, done, 1, bl - done, epoch, status);
-      for (int t = 0; t < m; ++t) ys[t][r] = load_published(&y[(size_t)(done + t) * NB + r]);
+      const int m = await_values(y, (long long)done * NB, 1, min(bl - done, RUN_MAX), [](int) { return NB; }, ys,
+                                 status);
       if (r == 0) run = m;
     }
     __syncthreads();
@@ -733,15 +770,16 @@ __global__ void __launch_bounds__(256) k_trsv_fwd(int K, int ld, const zd* __res
       store_published(&y[b0 + r], yv);
       z[b0 + r] = yv * (1.0 / D[b0 + r]);
     }
-    publish_flag(&fout[bl], epoch);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   }
 }
 
 // Backward: L^H x = z, L^H read from the strict upper triangle of F.
-// Workgroup w handles block nb-1-w.
+// Workgroup w handles block nb-1-w; resets that block of y (the forward
+// solve's values) to the sentinel for the next solve.
 __global__ void __launch_bounds__(256) k_trsv_bwd(int K, int ld, const zd* __restrict__ F,
                                                   const zd* __restrict__ Dinv, const zd* __restrict__ z,
-                                                  zd* x, const int* fin, int* fout, int epoch, int* status) {
+                                                  zd* x, zd* yreset, int* status) {
   __shared__ zd Xs[NB][NB + 1];
   __shared__ zd xs[RUN_MAX][NB];
   __shared__ zd part[4][NB];
@@ -750,6 +788,7 @@ __global__ void __launch_bounds__(256) k_trsv_bwd(int K, int ld, const zd* __res
   const int bl = nb - 1 - (int)blockIdx.x;
   const int b0 = bl * NB, bs = min(NB, K - b0);
   const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
+  if (g == 1 && r < bs) store_sentinel(&yreset[b0 + r]);
   {
     const zd* XH = Dinv + (size_t)bl * 2 * NB * NB + NB * NB;
     for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) Xs[e & (NB - 1)][e >> 6] = XH[e];   // Xs[i][p] = X^H(i,p)
@@ -766,12 +805,9 @@ __global__ void __launch_bounds__(256) k_trsv_bwd(int K, int ld, const zd* __res
   }
   while (done < need) {
     if (g == 0) {
-      const int m = await_run(fin, nb - 1 - done, -1, need - done, epoch, status);
-      for (int t = 0; t < m; ++t) {
-        const int J = nb - 1 - done - t;
-        const int js = min(NB, K - J * NB);
-        xs[t][r] = (r < js) ? load_published(&x[(size_t)J * NB + r]) : mkz(0.0, 0.0);
-      }
+      const int first = nb - 1 - done;
+      const int m = await_values(x, (long long)first * NB, -1, min(need - done, RUN_MAX),
+                                 [&](int t) { return min(NB, K - (first - t) * NB); }, xs, status);
       if (r == 0) run = m;
     }
     __syncthreads();
@@ -806,24 +842,29 @@ __global__ void __launch_bounds__(256) k_trsv_bwd(int K, int ld, const zd* __res
   __syncthreads();
   if (g == 0) {
     if (r < bs) store_published(&x[b0 + r], part[0][r] + part[1][r] + part[2][r] + part[3][r]);
-    publish_flag(&fout[bl], epoch);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   }
+}
+
+__global__ void k_fill_sentinel(int n, zd* __restrict__ v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) store_sentinel(&v[i]);
+}
+
+int solve_buffers_init(Factor& f, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_sentinel, dim3((f.ld + 255) / 256), dim3(256), 0, s, f.ld, f.vec + 3 * (size_t)f.ld);
+  return (int)hipGetLastError();
 }
 
 int ldlh_solve(Factor& f, const zd* b, zd* x, hipStream_t s) {
   const int nb = (f.K + NB - 1) / NB;
   zd* y = f.vec + 3 * (size_t)f.ld;
   zd* z = f.tvec;
-  int* ff = f.flags;
-  int* fb = f.flags + f.ld / NB;
-  const int e1 = ++f.epoch;
   if (f.tr) f.tr->begin(TR_TRSV_FWD, s);
-  hipLaunchKernelGGL(k_trsv_fwd, dim3(nb), dim3(256), 0, s, f.K, f.ld, f.F, f.D, f.Dinv, b, y, z, ff, ff, e1,
-                     f.status);
+  hipLaunchKernelGGL(k_trsv_fwd, dim3(nb), dim3(256), 0, s, f.K, f.ld, f.F, f.D, f.Dinv, b, y, z, x, f.status);
   if (f.tr) f.tr->end(s);
-  const int e2 = ++f.epoch;
   if (f.tr) f.tr->begin(TR_TRSV_BWD, s);
-  hipLaunchKernelGGL(k_trsv_bwd, dim3(nb), dim3(256), 0, s, f.K, f.ld, f.F, f.Dinv, z, x, fb, fb, e2, f.status);
+  hipLaunchKernelGGL(k_trsv_bwd, dim3(nb), dim3(256), 0, s, f.K, f.ld, f.F, f.Dinv, z, x, y, f.status);
   if (f.tr) f.tr->end(s);
   return (int)hipGetLastError();
 }

@@ -17,7 +17,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_HERE, "libecg_hip.so")
+# ECG_HIP_LIB selects another build of the library (A/B comparisons)
+LIB_PATH = os.environ.get("ECG_HIP_LIB", os.path.join(_HERE, "libecg_hip.so"))
 
 ECG_OK = 0
 ECG_ERR_ARG = -1
