@@ -49,6 +49,15 @@ struct ecg_ctx {
   zd* dD = nullptr;           // stored derivatives (2 npt x nfo x K) of ecg_energy_grad_stored
   size_t dD_cap = 0;
   ecg::Tracer tr;             // per-kernel tracing (off by default)
+  // overlap penalty settings (Glob_OverlapPenaltyAllowed, Glob_MaxOverlapPenalty,
+  // Glob_OverlapPenaltyThreshold2, and Glob_nfru for energy-only calls)
+  bool pen_on = false;
+  double pen_b = 1.0, pen_t2 = 1.0;
+  int pen_nfru = 0;
+  double* pen = nullptr;      // per-function penalty partials, then the total (device)
+  size_t pen_cap = 0;
+  double pen_total = 0.0;     // host copy of the last total
+  bool s_ready = false;       // the device S is assembled from the current parameters
 };
 
 // A packed batch of independent problems (one context each).
@@ -82,6 +91,24 @@ int dev_alloc(T** p, size_t n) {
     *p = nullptr;
     return ECG_ERR_NOMEM;
   }
+  return 0;
+}
+
+// Enqueue the overlap penalty of the device S (pairs j > i, j > nfru) on
+// the context's stream, optionally adding its gradient to dgrad, and the
+// copy of the total to x->pen_total (valid once the stream synchronises).
+int penalty_enqueue(ecg_ctx* x, hipStream_t s, int nfru, double b, double t2, double* dgrad) {
+  const int K = x->c.K;
+  x->pen_total = 0.0;
+  if (nfru >= K) return 0;
+  const size_t need = (size_t)(K - nfru) + 1;
+  if (need > x->pen_cap) {
+    RCCHK(dev_alloc(&x->pen, need));
+    x->pen_cap = need;
+  }
+  double* total = x->pen + (K - nfru);
+  if (ecg::overlap_penalty(x->c, nfru, b / (1.0 - t2), t2, x->pen, total, dgrad, s)) return ECG_ERR_HIP;
+  HIPCHK(hipMemcpyAsync(&x->pen_total, total, sizeof(double), hipMemcpyDeviceToHost, s));
   return 0;
 }
 
@@ -656,6 +683,7 @@ int ecg_ctx_destroy(ecg_ctx* x) {
   if (x->pinned) (void)hipHostFree(x->pinned);
   free_screen(x->scr);
   if (x->dD) (void)hipFree(x->dD);
+  if (x->pen) (void)hipFree(x->pen);
   for (hipEvent_t e : x->tr.pool) (void)hipEventDestroy(e);
   for (int i = 0; i < 8; ++i)
     if (x->e[i]) (void)hipEventDestroy(x->e[i]);
@@ -668,6 +696,7 @@ int ecg_set_params(ecg_ctx* x, int K, const double* params) {
   (void)hipGetLastError();   // start from a clean runtime error state
   if (!x || K < 1 || !params) return ECG_ERR_ARG;
   if (!finite_all(params, (size_t)x->npt * K)) return ECG_ERR_ARG;
+  x->s_ready = false;
   RCCHK(upload_params(x, K, params));
   HIPCHK(hipStreamSynchronize(x->stream));
   return ECG_OK;
@@ -676,6 +705,7 @@ int ecg_set_params(ecg_ctx* x, int K, const double* params) {
 int ecg_update_params(ecg_ctx* x, int first, int count, const double* params) {
   if (!x || !params || first < 1 || count < 1 || first - 1 + count > x->c.K) return ECG_ERR_ARG;
   if (!finite_all(params, (size_t)x->npt * count)) return ECG_ERR_ARG;
+  x->s_ready = false;
   HIPCHK(hipMemcpyAsync(x->c.params + (size_t)x->npt * (first - 1), params, sizeof(double) * x->npt * count,
                         hipMemcpyHostToDevice, x->stream));
   RCCHK(ecg::prep(x->c, x->stream));
@@ -751,6 +781,7 @@ int ecg_compute_mat_elem(ecg_ctx* x, int nmin, int nmax, char method, double app
   if (nmin > 1)
     HIPCHK(hipMemcpyAsync(x->c.diagS, diagS, sizeof(double) * (nmin - 1), hipMemcpyHostToDevice, x->stream));
   RCCHK(ecg::assemble(x->c, nmin - 1, nmax, x->stream));
+  x->s_ready = (nmax == x->c.K);
   return copy_out_hs(x, nmin - 1, nmax, method, approx_energy, H, S, ld, diagH, diagS);
 }
 
@@ -771,6 +802,7 @@ int ecg_compute_mat_elem_and_deriv(ecg_ctx* x, int nfru, char method, double app
       hipMemcpyAsync(x->c.diagS, diagS, sizeof(double) * nfru, hipMemcpyHostToDevice, x->stream) != hipSuccess)
     rc = ECG_ERR_HIP;
   if (!rc) rc = ecg::assemble_deriv(x->c, nfru, dD, x->stream);
+  if (!rc) x->s_ready = true;
   if (!rc && hipMemcpyAsync(D, dD, sizeof(zd) * nD, hipMemcpyDeviceToHost, x->stream) != hipSuccess)
     rc = ECG_ERR_HIP;
   if (!rc && hipStreamSynchronize(x->stream) != hipSuccess) rc = ECG_ERR_HIP;
@@ -809,6 +841,7 @@ int ecg_ghepiis(ecg_ctx* x, int k, int n, double* M, int nM, double* invD, const
       hH[od + 1] = mi + apprlambda * hS[od + 1];
     }
   HIPCHK(hipMemcpyAsync(c.H, hH, sizeof(double) * nd, hipMemcpyHostToDevice, x->stream));
+  x->s_ready = false;
   HIPCHK(hipMemcpyAsync(c.S, hS, sizeof(double) * nd, hipMemcpyHostToDevice, x->stream));
   HIPCHK(hipMemcpyAsync(x->ev[2], v, sizeof(zd) * n, hipMemcpyHostToDevice, x->stream));
   x->have_last = true;
@@ -862,10 +895,15 @@ int ecg_energy(ecg_ctx* x, int which, double sigma, double tol, int maxiter, dou
   RCCHK(ecg::assemble(x->c, 0, K, x->stream));
   HIPCHK(hipEventRecord(x->e[1], x->stream));
   int status = 0;
+  x->s_ready = true;
   RCCHK(solve_root(x, which, sigma, tol, maxiter, E, info, &status));
+  x->pen_total = 0.0;
+  if (x->pen_on) RCCHK(penalty_enqueue(x, x->stream, x->pen_nfru, x->pen_b, x->pen_t2, nullptr));
   HIPCHK(hipEventRecord(x->e[2], x->stream));
   if (c) HIPCHK(hipMemcpyAsync(c, x->ev[2], sizeof(zd) * K, hipMemcpyDeviceToHost, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
+  *E += x->pen_total;   // EnergyGA/EnergyIA return Evalue + Glob_TotalOverlapPenalty
+  if (info) info[6] = x->pen_total;
   x->tms[0] = ev_ms(x->e[0], x->e[1]);
   x->tms[1] = ev_ms(x->e[1], x->e[2]);
   x->tms[2] = 0.0;
@@ -904,9 +942,14 @@ int ecg_energy_grad(ecg_ctx* x, int nfru, int which, double sigma, double tol, i
   c.ev_adj0 = x->e[3];
   c.ev_adj1 = x->e[4];
   RCCHK(ecg::adjoint(c, nfru, *E, x->ev[2], x->part, x->grad, x->stream));
+  x->s_ready = true;
+  x->pen_total = 0.0;
+  if (x->pen_on && K != 1) RCCHK(penalty_enqueue(x, x->stream, nfru, x->pen_b, x->pen_t2, x->grad));
   HIPCHK(hipEventRecord(x->e[7], x->stream));
   HIPCHK(hipMemcpyAsync(grad, x->grad, sizeof(double) * ng, hipMemcpyDeviceToHost, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
+  *E += x->pen_total;   // EnergyGB/EnergyIB: Evalue + Glob_TotalOverlapPenalty
+  if (info) info[6] = x->pen_total;
   c.ev_off0 = c.ev_off1 = c.ev_adj0 = c.ev_adj1 = nullptr;
   x->tms[0] = ev_ms(x->e[0], x->e[1]);
   x->tms[1] = ev_ms(x->e[1], x->e[2]);
@@ -948,9 +991,14 @@ int ecg_energy_grad_stored(ecg_ctx* x, int nfru, int which, double sigma, double
   hipLaunchKernelGGL(k_contract_stored, dim3(nfo), dim3(256), 0, x->stream, x->npt, nfo, K, nfru, x->dD, x->ev[2],
                      *E, x->grad);
   HIPCHK(hipGetLastError());
+  x->s_ready = true;
+  x->pen_total = 0.0;
+  if (x->pen_on && K != 1) RCCHK(penalty_enqueue(x, x->stream, nfru, x->pen_b, x->pen_t2, x->grad));
   HIPCHK(hipEventRecord(x->e[7], x->stream));
   HIPCHK(hipMemcpyAsync(grad, x->grad, sizeof(double) * ng, hipMemcpyDeviceToHost, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
+  *E += x->pen_total;
+  if (info) info[6] = x->pen_total;
   x->tms[0] = ev_ms(x->e[0], x->e[1]);
   x->tms[1] = ev_ms(x->e[1], x->e[2]);
   x->tms[2] = ev_ms(x->e[2], x->e[7]);
@@ -1122,6 +1170,8 @@ int ecg_batch_energy_grad(ecg_batch* b, const double* sigma, double tol, int max
       if (!rc) rc = solve_root(x, 1, sigma[i], tol, maxiter, &E[i], info, &status);
       if (!rc && !b->packed) {
         rc = ecg::adjoint(x->c, 0, E[i], x->ev[2], x->part, x->grad, x->stream);
+        x->pen_total = 0.0;
+        if (!rc && x->pen_on && x->c.K != 1) rc = penalty_enqueue(x, x->stream, 0, x->pen_b, x->pen_t2, x->grad);
         if (!rc && hipMemcpyAsync(grad + goff[i], x->grad, sizeof(double) * (goff[i + 1] - goff[i]),
                                   hipMemcpyDeviceToHost, x->stream) != hipSuccess)
           rc = ECG_ERR_HIP;
@@ -1145,6 +1195,10 @@ int ecg_batch_energy_grad(ecg_batch* b, const double* sigma, double tol, int max
     for (int i = 0; i < np; ++i) HIPCHK(hipStreamWaitEvent(b->lead, b->ev_eig[i], 0));
     for (auto& cl : b->classes)
       RCCHK(ecg::packed_pairs(cl.n, true, cl.d, (int)cl.h.size(), cl.ntiles, cl.nf, cl.maxT, b->lead));
+    for (ecg_ctx* x : b->ctx) {
+      x->pen_total = 0.0;
+      if (x->pen_on && x->c.K != 1) RCCHK(penalty_enqueue(x, b->lead, 0, x->pen_b, x->pen_t2, x->grad));
+    }
     for (int i = 0; i < np; ++i)
       HIPCHK(hipMemcpyAsync(grad + goff[i], b->ctx[i]->grad, sizeof(double) * (goff[i + 1] - goff[i]),
                             hipMemcpyDeviceToHost, b->lead));
@@ -1155,6 +1209,8 @@ int ecg_batch_energy_grad(ecg_batch* b, const double* sigma, double tol, int max
   }
   int worst = 0;
   for (int i = 0; i < np; ++i) {
+    b->ctx[i]->s_ready = true;
+    E[i] += b->ctx[i]->pen_total;   // zero unless the context's penalty is on
     if (!std::isfinite(E[i]) || !finite_all(grad + goff[i], goff[i + 1] - goff[i])) return ECG_ERR_NONFINITE;
     worst = std::max(worst, st[i]);
   }
@@ -1278,6 +1334,7 @@ int ecg_screen_loop(ecg_ctx* x, int nsteps, int ncand, const double* cand, int i
   if (!x->scr.ready || x->scr.ld != x->c.ld || x->scr.Kp != x->c.K - 1) return ECG_ERR_STATE;
   if (!finite_all(cand, (size_t)x->npt * ncand * nsteps)) return ECG_ERR_ARG;
   RCCHK(ensure_screen(x, ncand, nsteps));
+  x->s_ready = false;
   ecg::Screen& sc = x->scr;
   const int npt = x->npt;
   HIPCHK(hipMemcpyAsync(sc.call, cand, sizeof(double) * npt * ncand * nsteps, hipMemcpyHostToDevice, x->stream));
@@ -1323,6 +1380,40 @@ int ecg_screen_loop(ecg_ctx* x, int nsteps, int ncand, const double* cand, int i
     Ebest[i] = rec[2 * i];
     ibest[i] = (int)rec[2 * i + 1];
   }
+  return ECG_OK;
+}
+
+int ecg_set_overlap_penalty(ecg_ctx* x, int allowed, double max_pair_penalty, double threshold2, int nfru) {
+  if (!x || nfru < 0 || !std::isfinite(max_pair_penalty) || !std::isfinite(threshold2)) return ECG_ERR_ARG;
+  x->pen_on = allowed != 0;
+  x->pen_b = max_pair_penalty;
+  x->pen_t2 = threshold2;
+  x->pen_nfru = nfru;
+  return ECG_OK;
+}
+
+int ecg_overlap_penalty(ecg_ctx* x, int nfru, double max_pair_penalty, double threshold2, double* total,
+                        double* grad) {
+  (void)hipGetLastError();   // start from a clean runtime error state
+  if (!x || !total || nfru < 0 || !std::isfinite(max_pair_penalty) || !std::isfinite(threshold2))
+    return ECG_ERR_ARG;
+  const int K = x->c.K;
+  if (K < 1 || !x->s_ready) return ECG_ERR_STATE;
+  if (nfru >= K) return ECG_ERR_ARG;
+  const size_t ng = (size_t)x->npt * (K - nfru);
+  if (grad) {
+    if (!finite_all(grad, ng)) return ECG_ERR_ARG;
+    if (ng > x->grad_cap) {
+      RCCHK(dev_alloc(&x->grad, ng));
+      x->grad_cap = ng;
+    }
+    HIPCHK(hipMemcpyAsync(x->grad, grad, sizeof(double) * ng, hipMemcpyHostToDevice, x->stream));
+  }
+  RCCHK(penalty_enqueue(x, x->stream, nfru, max_pair_penalty, threshold2, grad ? x->grad : nullptr));
+  if (grad) HIPCHK(hipMemcpyAsync(grad, x->grad, sizeof(double) * ng, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  *total = x->pen_total;
+  if (!std::isfinite(*total) || (grad && !finite_all(grad, ng))) return ECG_ERR_NONFINITE;
   return ECG_OK;
 }
 

@@ -97,6 +97,12 @@ int assemble(const Ctx& c, int r0, int r1, hipStream_t s);
 int assemble_deriv(const Ctx& c, int nfru, zd* D, hipStream_t s);
 int adjoint(const Ctx& c, int nfru, double E, const zd* cvec, double* part, double* grad, hipStream_t s);
 size_t adjoint_partial_doubles(int n, int K);
+// Overlap penalty of the device S (pairs j > i, j >= nfru): per-function
+// partials pen[K - nfru], their fixed-order sum to *total (device), and,
+// if grad is not null, the penalty's gradient added to grad (npt per
+// function nfru..K-1).  pc = b / (1 - t2).
+int overlap_penalty(const Ctx& c, int nfru, double pc, double t2, double* pen, double* total, double* grad,
+                    hipStream_t s);
 
 // dense Hermitian linear algebra (ecg_linalg.hip)
 struct Factor {

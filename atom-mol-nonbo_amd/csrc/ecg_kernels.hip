@@ -515,6 +515,95 @@ __global__ void __launch_bounds__(256) k_adjoint_finish(int K, int nfru, int nt,
   adjoint_finish_one<N>(f, nfru, nt, tr0, bra, part, grad);
 }
 
+// ------------------------------------------------------ overlap penalty
+// ComputeOverlapPenalty / ComputeOverlapPenaltyAndAddGradient
+// (src/workproc.f90:1491-1693).  P sums pc (|s_ji|^2 - t2), pc = b/(1-t2),
+// over the pairs j > i with j > nfru whose normalised overlap has
+// |s_ji|^2 > t2.  Gradient terms for function a and partner b of a hit
+// pair (j, i) = (max(a,b), min(a,b)), f = 1/sqrt(S_aa S_bb),
+// D_aa = (dS_aa/dp_a) / S_aa, written as the reference writes them:
+//   a = j:  2 pc f Re(conj(s) dS_ji/dp_a) - pc |s|^2   sqrt(S_bb/S_aa) D_aa
+//   a = i:  2 pc f Re(conj(s) dS_ji/dp_a) - pc Re(s^2) sqrt(S_bb/S_aa) D_aa
+// (the exact derivative of P has |s|^2 and no square-root factor in both;
+// the two agree when S_aa = S_bb and s is real).
+// One wave per function a >= nfru reads column a of the device S, which
+// holds both triangles (s_ji is the element (b, a) for b > a, its
+// conjugate for b < a).  Only hit pairs, rare by the purpose of the
+// penalty, recompute their derivatives.  Pair values of P go to pen[a -
+// nfru] (pairs with j = a); wave sums use a fixed butterfly order.
+template <int N>
+__global__ void __launch_bounds__(256) k_overlap_penalty(const SysConst sc, int K, int nfru, int ld,
+                                                        const double* __restrict__ bra,
+                                                        const double* __restrict__ ket,
+                                                        const double* __restrict__ diagS,
+                                                        const zd* __restrict__ Sm, double pc, double t2,
+                                                        double* __restrict__ pen, double* __restrict__ grad) {
+  constexpr int m = N * (N + 1) / 2, npt = 2 * m, np4 = 4 * m;
+  const int a = nfru + blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (a >= K) return;
+  const int lane = threadIdx.x & 63;
+  const int kstride = sc.T * KetRec<N>::stride;
+  const double Sa = diagS[a];
+  const zd* col = Sm + (size_t)a * ld;
+  double p = 0.0, beta = 0.0, g[npt];
+#pragma unroll
+  for (int e = 0; e < npt; ++e) g[e] = 0.0;
+  for (int b = lane; b < K; b += 64) {
+    if (b == a) continue;
+    zd s = col[b];
+    if (b < a) s = zconj(s);
+    const double s2 = s.x * s.x + s.y * s.y;
+    if (!(s2 > t2)) continue;
+    if (b < a) p += pc * (s2 - t2);
+    if (!grad) continue;
+    const double Sb = diagS[b];
+    const bool ket_side = b > a;
+    const int j = ket_side ? b : a, i = ket_side ? a : b;
+    zd H, S, Dk[np4], Dl[np4];
+    pair_sum_deriv<N>(sc, bra + (size_t)j * BraRec<N>::stride, ket + (size_t)i * kstride,
+                      bra + (size_t)i * BraRec<N>::stride, ket_side, H, S, Dk, Dl);
+    const double c2 = 2.0 * pc / sqrt(Sa * Sb);
+#pragma unroll
+    for (int e = 0; e < npt; ++e) {
+      const zd d = ket_side ? Dl[2 * m + e] : Dk[2 * m + e];
+      g[e] += c2 * (s.x * d.x + s.y * d.y);
+    }
+    const double w = ket_side ? (s.x * s.x - s.y * s.y) : s2;
+    beta += pc * w * sqrt(Sb / Sa);
+  }
+  for (int o = 32; o > 0; o >>= 1) p += __shfl_xor(p, o);
+  if (lane == 0) pen[a - nfru] = p;
+  if (!grad) return;
+  for (int o = 32; o > 0; o >>= 1) beta += __shfl_xor(beta, o);
+  bool any = beta != 0.0;
+#pragma unroll
+  for (int e = 0; e < npt; ++e) {
+    for (int o = 32; o > 0; o >>= 1) g[e] += __shfl_xor(g[e], o);
+    any = any || g[e] != 0.0;
+  }
+  if (lane != 0 || !any) return;
+  zd H, S, Dk[np4], Dl[np4];
+  pair_sum_deriv<N>(sc, bra + (size_t)a * BraRec<N>::stride, ket + (size_t)a * kstride,
+                    bra + (size_t)a * BraRec<N>::stride, false, H, S, Dk, Dl);
+  double* ga = grad + (size_t)(a - nfru) * npt;
+#pragma unroll
+  for (int e = 0; e < npt; ++e) ga[e] += g[e] - beta * (2.0 * Dk[2 * m + e].x / Sa);
+}
+
+// Sum of v[0..n) in a fixed order by one workgroup of 256 threads.
+__global__ void __launch_bounds__(256) k_sum_fixed(const double* __restrict__ v, int n, double* __restrict__ out) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += v[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
 // ------------------------------------------------- packed problem batches
 // One launch covers the pair tiles of every problem of a batch that has
 // the same n: workgroup b belongs to the problem whose tile range
@@ -665,6 +754,16 @@ static int launch_adjoint(const Ctx& c, int nfru, double E, const zd* cvec, doub
   return (int)hipGetLastError();
 }
 
+template <int N>
+static int launch_penalty(const Ctx& c, int nfru, double pc, double t2, double* pen, double* total, double* grad,
+                          hipStream_t s) {
+  const int nf = c.K - nfru;
+  hipLaunchKernelGGL(k_overlap_penalty<N>, dim3((nf + 3) / 4), dim3(256), 0, s, c.sc, c.K, nfru, c.ld, c.bra, c.ket,
+                     c.diagS, c.S, pc, t2, pen, grad);
+  hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(256), 0, s, pen, nf, total);
+  return (int)hipGetLastError();
+}
+
 int prep(const Ctx& c, hipStream_t s) {
   switch (c.n) {
     case 1: return launch_prep<1>(c, s);
@@ -702,6 +801,18 @@ int adjoint(const Ctx& c, int nfru, double E, const zd* cvec, double* part, doub
     case 3: return launch_adjoint<3>(c, nfru, E, cvec, part, grad, s);
     case 4: return launch_adjoint<4>(c, nfru, E, cvec, part, grad, s);
     case 5: return launch_adjoint<5>(c, nfru, E, cvec, part, grad, s);
+  }
+  return -1;
+}
+int overlap_penalty(const Ctx& c, int nfru, double pc, double t2, double* pen, double* total, double* grad,
+                    hipStream_t s) {
+  if (nfru < 0 || nfru >= c.K) return -1;
+  switch (c.n) {
+    case 1: return launch_penalty<1>(c, nfru, pc, t2, pen, total, grad, s);
+    case 2: return launch_penalty<2>(c, nfru, pc, t2, pen, total, grad, s);
+    case 3: return launch_penalty<3>(c, nfru, pc, t2, pen, total, grad, s);
+    case 4: return launch_penalty<4>(c, nfru, pc, t2, pen, total, grad, s);
+    case 5: return launch_penalty<5>(c, nfru, pc, t2, pen, total, grad, s);
   }
   return -1;
 }

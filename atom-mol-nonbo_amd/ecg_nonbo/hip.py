@@ -44,6 +44,7 @@ EXPORTS = [
     "ecg_compute_mat_elem", "ecg_compute_mat_elem_and_deriv", "ecg_ghepiis", "ecg_energy", "ecg_energy_grad",
     "ecg_energy_grad_stored", "ecg_batch_create", "ecg_batch_destroy", "ecg_batch_set_config",
     "ecg_batch_energy_grad", "ecg_trace", "ecg_trace_read", "ecg_get_params", "ecg_screen_prepare", "ecg_screen", "ecg_screen_loop", "ecg_last_timings",
+    "ecg_set_overlap_penalty", "ecg_overlap_penalty",
 ]
 
 
@@ -91,6 +92,8 @@ def load(path: str = LIB_PATH):
     L.ecg_screen.argtypes = [vp, c_int, _d, c_int, _d, _d]
     L.ecg_screen_loop.argtypes = [vp, c_int, c_int, _d, c_int, _d, _i]
     L.ecg_last_timings.argtypes = [vp, _d, c_int]
+    L.ecg_set_overlap_penalty.argtypes = [vp, c_int, c_double, c_double, c_int]
+    L.ecg_overlap_penalty.argtypes = [vp, c_int, c_double, c_double, _d, _d]
     for name in EXPORTS:
         getattr(L, name).restype = c_int
     _lib = L
@@ -249,6 +252,23 @@ class EcgContext:
                                            ctypes.byref(E), _p(g), _p(info))
         _check("ecg_energy_grad_stored", rc, ok=(ECG_OK, ECG_NOT_CONVERGED))
         return rc, E.value, g, info
+
+    def set_overlap_penalty(self, allowed, max_pair_penalty=1.0, threshold2=1.0, nfru=0):
+        """Glob_OverlapPenaltyAllowed / Glob_MaxOverlapPenalty /
+        Glob_OverlapPenaltyThreshold2 (and nfru for energy-only calls)."""
+        _check("ecg_set_overlap_penalty",
+               load().ecg_set_overlap_penalty(self._h, int(bool(allowed)), float(max_pair_penalty),
+                                              float(threshold2), int(nfru)))
+
+    def overlap_penalty(self, nfru, max_pair_penalty, threshold2, grad=None):
+        """(P, grad): the penalty of the device S; grad (if given, npt*(K -
+        nfru)) is returned with the penalty's gradient added."""
+        P = ctypes.c_double()
+        g = None if grad is None else np.array(grad, dtype=np.float64)
+        rc = load().ecg_overlap_penalty(self._h, int(nfru), float(max_pair_penalty), float(threshold2),
+                                        ctypes.byref(P), _p(g) if g is not None else None)
+        _check("ecg_overlap_penalty", rc)
+        return P.value, g
 
     TRACE_LABELS = ("offdiag", "adjoint", "ldl_update", "ldl_diag", "ldl_panel", "trsv_fwd", "trsv_bwd", "gemv",
                     "form_shifted")

@@ -15,6 +15,8 @@
 !                  GHEPIIS                 src/linalg.f90:1781
 !                  EnergyGA / EnergyIA     src/workproc.f90:1697, 1954
 !                  EnergyGB / EnergyIB     src/workproc.f90:1841, 2063
+!                  ComputeOverlapPenalty   src/workproc.f90:1491
+!                  ComputeOverlapPenaltyAndAddGradient  src/workproc.f90:1571
 !                plus EcgHostInit / EcgHostFinish (context set-up, no
 !                counterpart in the reference: ProgramDataInit ends by
 !                calling EcgHostInit).
@@ -187,6 +189,22 @@ module ecg_capi
       real(c_double), intent(out) :: Ebest(*)
       integer(c_int), intent(out) :: ibest(*)
     end function
+    integer(c_int) function ecg_set_overlap_penalty(ctx, allowed, max_pair_penalty, threshold2, nfru) &
+        bind(C, name="ecg_set_overlap_penalty")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: allowed, nfru
+      real(c_double), value :: max_pair_penalty, threshold2
+    end function
+    integer(c_int) function ecg_overlap_penalty(ctx, nfru, max_pair_penalty, threshold2, total, grad) &
+        bind(C, name="ecg_overlap_penalty")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nfru
+      real(c_double), value :: max_pair_penalty, threshold2
+      real(c_double), intent(out) :: total
+      type(c_ptr), value :: grad          ! npt*(K-nfru) doubles or c_null_ptr
+    end function
     integer(c_int) function ecg_last_timings(ctx, ms, nmax) bind(C, name="ecg_last_timings")
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx
@@ -215,6 +233,10 @@ module ecg_globals
   integer :: Glob_WhichEigenvalue = 1
   real(dprec) :: Glob_EigvalTol = 1.0d-12
   integer :: Glob_InvItMaxIter = 20
+  logical :: Glob_OverlapPenaltyAllowed = .false.
+  real(dprec) :: Glob_OverlapPenaltyThreshold2 = 1
+  real(dprec) :: Glob_MaxOverlapPenalty = 1
+  real(dprec) :: Glob_TotalOverlapPenalty = 0
 end module ecg_globals
 
 module ecg_hostpath
@@ -225,7 +247,7 @@ module ecg_hostpath
   implicit none
   private
   public :: EcgHostInit, EcgHostFinish, MatrixElements, ComputeMatElem, ComputeMatElemAndDeriv, GHEPIIS, &
-            EnergyGA, EnergyIA, EnergyGB, EnergyIB
+            EnergyGA, EnergyIA, EnergyGB, EnergyIB, ComputeOverlapPenalty, ComputeOverlapPenaltyAndAddGradient
   type(c_ptr), save :: ctx = c_null_ptr
 
 contains
@@ -245,10 +267,16 @@ contains
     ctx = c_null_ptr
   end subroutine
 
-  ! parameters of functions 1..K to the device
+  ! parameters of functions 1..K to the device, with the overlap penalty
+  ! settings the energy routines apply
   integer function sync_params(K)
     integer, intent(in) :: K
+    integer(c_int) :: allowed
+    allowed = 0
+    if (Glob_OverlapPenaltyAllowed) allowed = 1
     sync_params = ecg_set_params(ctx, K, Glob_NonlinParam)
+    if (sync_params == ECG_OK) sync_params = ecg_set_overlap_penalty(ctx, allowed, Glob_MaxOverlapPenalty, &
+                                                                     Glob_OverlapPenaltyThreshold2, Glob_nfru)
   end function
 
   ! src/matelem.f90:12 — one pair, one symmetry matrix, normalised elements
@@ -314,6 +342,7 @@ contains
     ErrorCode = sync_params(Nmax)
     if (ErrorCode == ECG_OK) ErrorCode = ecg_energy(ctx, Glob_WhichEigenvalue, ieee_value(1.0_dprec, ieee_quiet_nan), &
                                                     Glob_EigvalTol, 60, EnergyGA, Glob_c, info)
+    Glob_TotalOverlapPenalty = info(7)
   end function
 
   ! src/workproc.f90:1954 — as EnergyGA with the shift Glob_ApproxEnergy
@@ -325,6 +354,7 @@ contains
     ErrorCode = sync_params(Nmax)
     if (ErrorCode == ECG_OK) ErrorCode = ecg_energy(ctx, Glob_WhichEigenvalue, Glob_ApproxEnergy, Glob_EigvalTol, &
                                                     Glob_InvItMaxIter, EnergyIA, Glob_c, info)
+    Glob_TotalOverlapPenalty = info(7)
   end function
 
   ! src/workproc.f90:1841 — energy and gradient w.r.t. the parameters of
@@ -340,6 +370,7 @@ contains
     if (ErrorCode == ECG_OK) ErrorCode = ecg_energy_grad(ctx, Glob_nfru, Glob_WhichEigenvalue, &
                                                          ieee_value(1.0_dprec, ieee_quiet_nan), Glob_EigvalTol, 60, &
                                                          Evalue, Gradient, info)
+    Glob_TotalOverlapPenalty = info(7)
   end subroutine
 
   ! src/workproc.f90:2063 — as EnergyGB with the shift Glob_ApproxEnergy
@@ -352,6 +383,32 @@ contains
     ErrorCode = sync_params(Glob_CurrBasisSize)
     if (ErrorCode == ECG_OK) ErrorCode = ecg_energy_grad(ctx, Glob_nfru, Glob_WhichEigenvalue, Glob_ApproxEnergy, &
                                                          Glob_EigvalTol, Glob_InvItMaxIter, Evalue, Gradient, info)
+    Glob_TotalOverlapPenalty = info(7)
+  end subroutine
+
+  ! src/workproc.f90:1491 — penalty of the overlaps assembled on the device
+  ! by the last energy call or ComputeMatElem (pairs j > Glob_nfru)
+  subroutine ComputeOverlapPenalty(MaxPairOverlapPenalty, OverlapThreshold2, TotalPenalty)
+    real(dprec) :: MaxPairOverlapPenalty, OverlapThreshold2, TotalPenalty
+    integer :: rc
+    rc = ecg_overlap_penalty(ctx, Glob_nfru, MaxPairOverlapPenalty, OverlapThreshold2, TotalPenalty, c_null_ptr)
+    if (rc /= ECG_OK) TotalPenalty = ieee_value(1.0_dprec, ieee_quiet_nan)
+  end subroutine
+
+  ! src/workproc.f90:1571 — as ComputeOverlapPenalty, adding the penalty's
+  ! gradient to WkGR (functions Glob_nfru+1.., npt per function)
+  subroutine ComputeOverlapPenaltyAndAddGradient(MaxPairOverlapPenalty, OverlapThreshold2, TotalPenalty, WkGR)
+    real(dprec) :: MaxPairOverlapPenalty, OverlapThreshold2, TotalPenalty, WkGR(:)
+    real(c_double), allocatable, target :: g(:)
+    integer :: rc
+    allocate(g(size(WkGR)))
+    g = WkGR
+    rc = ecg_overlap_penalty(ctx, Glob_nfru, MaxPairOverlapPenalty, OverlapThreshold2, TotalPenalty, c_loc(g))
+    if (rc == ECG_OK) then
+      WkGR = g
+    else
+      TotalPenalty = ieee_value(1.0_dprec, ieee_quiet_nan)
+    end if
   end subroutine
 
 end module ecg_hostpath

@@ -207,6 +207,29 @@ int ecg_screen(ecg_ctx* ctx, int ncand, const double* cand, int iters, double* E
  * -1 - index of the best candidate when it was not accepted. */
 int ecg_screen_loop(ecg_ctx* ctx, int nsteps, int ncand, const double* cand, int iters, double* Ebest, int* ibest);
 
+/* Overlap penalty settings: the reference's Glob_OverlapPenaltyAllowed,
+ * Glob_MaxOverlapPenalty (b) and Glob_OverlapPenaltyThreshold2 (t2)
+ * (src/globvars.f90:309-326), plus the Glob_nfru that energy-only calls
+ * use.  While allowed, ecg_energy returns E + P (EnergyGA/EnergyIA,
+ * src/workproc.f90:1759-1764, 1996-2001), and ecg_energy_grad,
+ * ecg_energy_grad_stored and ecg_batch_energy_grad return E + P and add
+ * the penalty's gradient (EnergyGB/EnergyIB, src/workproc.f90:1942-1945,
+ * 2144-2147; P over the pairs of their own nfru, 0 for batches; nothing
+ * for K = 1).  info[6] of those calls receives P (0 when not allowed).
+ * Screening energies do not include it. */
+int ecg_set_overlap_penalty(ecg_ctx* ctx, int allowed, double max_pair_penalty, double threshold2, int nfru);
+
+/* ComputeOverlapPenalty / ComputeOverlapPenaltyAndAddGradient
+ * (src/workproc.f90:1491-1693) on the device S of the last assembly of the
+ * current parameters (any energy call, or ecg_compute_mat_elem up to row
+ * K): total = sum of b/(1-t2) (|S_ji|^2 - t2) over the pairs j > i,
+ * j > nfru, with |S_ji|^2 > t2.  grad (nullable, npt*(K-nfru) doubles,
+ * ordered as ecg_energy_grad's) has the penalty's gradient terms added,
+ * as the reference adds them to Glob_WkGR.  ECG_ERR_STATE if no such
+ * assembly is on the device. */
+int ecg_overlap_penalty(ecg_ctx* ctx, int nfru, double max_pair_penalty, double threshold2, double* total,
+                        double* grad);
+
 /* Device time (ms, HIP events on the context's stream) of the phases of
  * the last ecg_energy / ecg_energy_grad call: [0] assembly, [1] eigen step
  * (all factorisations and iterations), [2] adjoint pass, [3] whole call,

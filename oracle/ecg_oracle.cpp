@@ -245,6 +245,80 @@ void oracle_gradient(int npt, int nfa, int nfru, const double* D, const double* 
   }
 }
 
+// ComputeOverlapPenalty (src/workproc.f90:1491-1567, single-process
+// branch).  S: the stored normalised overlap, read at (j, i) with j > i
+// (lower triangle), ldS its leading dimension; 1-based loops as written
+// there: first i <= nfru against j > nfru, then i > nfru against j > i.
+double oracle_overlap_penalty(int nfa, int nfru, const double* S, int ldS, double bmax, double t2) {
+  const double pc = bmax / (1.0 - t2);
+  auto Sv = [&](int j, int i) { return ld(S, (long)(i - 1) * ldS + (j - 1)); };
+  double tp = 0.0;
+  for (int i = 1; i <= nfru; ++i)
+    for (int j = nfru + 1; j <= nfa; ++j) {
+      const cd s = Sv(j, i);
+      const double aux = s.re * s.re + s.im * s.im;
+      if (aux > t2) tp += pc * (aux - t2);
+    }
+  for (int i = nfru + 1; i <= nfa; ++i)
+    for (int j = i + 1; j <= nfa; ++j) {
+      const cd s = Sv(j, i);
+      const double aux = s.re * s.re + s.im * s.im;
+      if (aux > t2) tp += pc * (aux - t2);
+    }
+  return tp;
+}
+
+// ComputeOverlapPenaltyAndAddGradient (src/workproc.f90:1571-1693,
+// single-process branch): the penalty, and its gradient terms added to
+// grad (npt * (nfa - nfru)) exactly as written there, from the stored
+// derivatives D (2 npt x nfo x nfa, StoreHSD layout; the S part is rows
+// npt..2npt-1) and diagS.  For a hit pair (j, i), j > i:
+//   grad_j += 2 pc Re(S(j,i) conj(D(S,j,i) - D(S,j,j) S(j,i) tau / 2))
+//   grad_i += 2 pc Re(S(j,i) (D(S,i,j) - D(S,i,i) S(j,i) / (2 tau)))
+// with tau = sqrt(diagS_i / diagS_j).
+double oracle_overlap_penalty_grad(int npt, int nfa, int nfru, const double* S, int ldS, const double* diagS,
+                                   const double* D, double bmax, double t2, double* grad) {
+  const int nfo = nfa - nfru, np4 = 2 * npt;
+  const double pc = bmax / (1.0 - t2), temp1 = 2.0 * pc;
+  auto Sv = [&](int j, int i) { return ld(S, (long)(i - 1) * ldS + (j - 1)); };
+  // Glob_D(npt+m, kf, j) with 1-based m, kf, j
+  auto DS = [&](int m, int kf, int j) {
+    return ld(D, (long)(npt + m - 1) + (long)np4 * ((long)(kf - 1) + (long)nfo * (j - 1)));
+  };
+  auto add_j = [&](int j, int i, cd s) {
+    const double temp2 = std::sqrt(diagS[i - 1] / diagS[j - 1]);
+    for (int m = 1; m <= npt; ++m) {
+      const cd t = DS(m, j - nfru, i) - 0.5 * DS(m, j - nfru, j) * s * temp2;
+      grad[(long)(j - nfru - 1) * npt + m - 1] += temp1 * (s * conj(t)).re;
+    }
+  };
+  double tp = 0.0;
+  for (int i = 1; i <= nfru; ++i)
+    for (int j = nfru + 1; j <= nfa; ++j) {
+      const cd s = Sv(j, i);
+      const double aux = s.re * s.re + s.im * s.im;
+      if (aux > t2) {
+        tp += pc * (aux - t2);
+        add_j(j, i, s);
+      }
+    }
+  for (int i = nfru + 1; i <= nfa; ++i)
+    for (int j = i + 1; j <= nfa; ++j) {
+      const cd s = Sv(j, i);
+      const double aux = s.re * s.re + s.im * s.im;
+      if (aux > t2) {
+        tp += pc * (aux - t2);
+        const double temp2 = std::sqrt(diagS[i - 1] / diagS[j - 1]);
+        for (int m = 1; m <= npt; ++m) {
+          const cd t = DS(m, i - nfru, j) - 0.5 * DS(m, i - nfru, i) * s / temp2;
+          grad[(long)(i - nfru - 1) * npt + m - 1] += temp1 * (s * t).re;
+        }
+        add_j(j, i, s);
+      }
+    }
+  return tp;
+}
+
 // LDLHF (src/linalg.f90:316-455, single-process branch): update the LDL^H
 // factorisation of the Hermitian matrix held in A's lower triangle from
 // row m (1-based) to n.  L^H goes to the strict upper triangle, 1/D to invD.

@@ -57,6 +57,12 @@ def lib():
         L.oracle_ghepiis.argtypes = [ctypes.c_int, ctypes.c_int, _d, ctypes.c_int, _d, _d, ctypes.c_int,
                                      ctypes.c_double, _d, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                      _d, _d]
+        L.oracle_overlap_penalty.restype = ctypes.c_double
+        L.oracle_overlap_penalty.argtypes = [ctypes.c_int, ctypes.c_int, _d, ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_double]
+        L.oracle_overlap_penalty_grad.restype = ctypes.c_double
+        L.oracle_overlap_penalty_grad.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _d, ctypes.c_int, _d,
+                                                  _d, ctypes.c_double, ctypes.c_double, _d]
         _lib = L
     return _lib
 
@@ -195,6 +201,28 @@ def gradient(D, c, E, nfru=0):
     g = np.zeros(npt * nfo)
     lib().oracle_gradient(npt, K, nfru, D.ctypes.data_as(_d), c.ctypes.data_as(_d), float(E), _pd(g))
     return g
+
+
+def overlap_penalty(S, nfru, bmax, t2):
+    """ComputeOverlapPenalty on a stored normalised S (Fortran complex
+    (K, K), lower triangle read)."""
+    assert S.flags.f_contiguous
+    K = S.shape[0]
+    return lib().oracle_overlap_penalty(K, nfru, S.ctypes.data_as(_d), K, float(bmax), float(t2))
+
+
+def overlap_penalty_grad(S, diagS, D, nfru, bmax, t2, grad=None):
+    """ComputeOverlapPenaltyAndAddGradient: returns (P, grad) with the
+    penalty's gradient terms added to grad (zeros if None).  D: (2*npt,
+    nfo, K) complex as assemble_deriv returns it."""
+    assert S.flags.f_contiguous
+    np4, nfo, K = D.shape
+    npt = np4 // 2
+    D = np.asfortranarray(D)
+    g = np.zeros(npt * nfo) if grad is None else np.array(grad, dtype=np.float64)
+    P = lib().oracle_overlap_penalty_grad(npt, K, nfru, S.ctypes.data_as(_d), K, _pd(np.ascontiguousarray(diagS)),
+                                          D.ctypes.data_as(_d), float(bmax), float(t2), _pd(g))
+    return P, g
 
 
 def ghepiis(k, M, invD, B, apprlambda, v, tol, maxiter=20, specif_norm=0):

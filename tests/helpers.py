@@ -52,3 +52,30 @@ def close(a, b, mag, sens, rel=1e-12, factor=16.0):
     d = np.max(np.abs(np.asarray(a) - np.asarray(b)))
     tol = max(rel * mag, factor * sens)
     return d <= tol, d, tol
+
+
+def penalty_basis(seeded, n, K, seed, ndup, eps=0.03):
+    """A seeded basis whose last ndup functions are perturbed copies of
+    functions spread over the rest, so that the overlap penalty has hits
+    between fixed and optimised and between two optimised functions."""
+    p = seeded(n, K, seed)
+    rng = np.random.default_rng(seed + 7)
+    src = np.linspace(0, K - ndup - 1, ndup).astype(int)
+    for t, k in enumerate(src):
+        p[K - ndup + t] = p[k] * (1.0 + eps * rng.standard_normal(p.shape[1]))
+    return p
+
+
+def penalty_threshold(S, lo=0.3, hi=0.97):
+    """t2 in (lo, hi) in the middle of the widest gap between consecutive
+    |s_ji|^2 of the stored lower triangle, so that no pair sits near the
+    threshold; returns (t2, gap, number of pairs above)."""
+    K = S.shape[0]
+    v = np.sort(np.abs(S[np.tril_indices(K, -1)]) ** 2)
+    best = None
+    for a, b in zip(v[:-1], v[1:]):
+        mid = 0.5 * (a + b)
+        if lo < mid < hi and (best is None or b - a > best[1]):
+            best = (mid, b - a)
+    assert best is not None
+    return best[0], best[1], int(np.sum(v > best[0]))

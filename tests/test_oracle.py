@@ -269,3 +269,117 @@ def test_flop_counts_are_structural():
         f3 = {oracle.flops_per_pair(sy, B[i], B[j], 3, term=t) for i, j in [(0, 1), (2, 3)]}
         assert len(f0) == len(f1) == len(f3) == 1
         assert min(f0) < min(f1) < min(f3)
+
+
+# ----------------------------------------------------------- overlap penalty
+def _duplicated_basis(sy, K, seed, eps, real):
+    """Functions K/2.. are perturbed copies of functions 0.., so that the
+    pairs (K/2 + i, i) have large normalised overlaps."""
+    p = seeded_basis(sy.n, K, seed)
+    rng = np.random.default_rng(seed + 1)
+    h = K // 2
+    p[h:] = p[:K - h] * (1.0 + eps * rng.standard_normal((K - h, p.shape[1])))
+    if real:
+        p[:, sy.np_:] = 0.0
+    return p
+
+
+def _threshold_between(S, hits):
+    """t2 halfway between the largest |s|^2 of the other pairs and the
+    smallest of the wanted ones (asserting that there is such a gap)."""
+    K = S.shape[0]
+    a2 = np.abs(S) ** 2
+    want = [a2[j, i] for j, i in hits]
+    other = [a2[j, i] for j in range(K) for i in range(j) if (j, i) not in hits]
+    assert max(other) < min(want), (max(other), min(want))
+    return 0.5 * (max(other) + min(want))
+
+
+def test_overlap_penalty_closed_form():
+    """At t2 = 0 every pair counts, so P = b * (sum of |s_ji|^2 over the
+    strict lower triangle rows j > nfru): a Frobenius norm of a block of
+    the stored S, which pins the pair ranges of both loops."""
+    sy = named_system("hminus")
+    K = 12
+    p = seeded_basis(sy.n, K, SEED_BASE + 61)
+    H, S, dH, dS = oracle.assemble(sy, p, "G")
+    low = np.tril(np.abs(S) ** 2, -1)
+    for nfru in (0, 1, 5, K - 1):
+        P = oracle.overlap_penalty(S, nfru, 1.3, 0.0)
+        ref = 1.3 * low[nfru:, :].sum()
+        assert abs(P - ref) <= 1e-13 * max(1.0, ref), (nfru, P, ref)
+    # nothing above a threshold of one
+    assert oracle.overlap_penalty(S, 0, 1.3, 0.999999) == 0.0
+
+
+def _penalty_fd(sy, p, nfru, b, t2, params_to_check, tol):
+    def pen(q):
+        H, S, dH, dS = oracle.assemble(sy, q, "G")
+        return oracle.overlap_penalty(S, nfru, b, t2)
+
+    H, S, dH, dS, D = oracle.assemble_deriv(sy, p, nfru, "G")
+    P, g = oracle.overlap_penalty_grad(S, dS, D, nfru, b, t2)
+    assert abs(P - pen(p)) <= 1e-14 * max(1.0, P)
+    assert P > 0.0
+    def cd(k, m, h):
+        pp = p.copy()
+        pp[k, m] += h
+        pm = p.copy()
+        pm[k, m] -= h
+        return (pen(pp) - pen(pm)) / (2 * h)
+
+    for k, m in params_to_check:
+        # central differences at h and h/2, Richardson-extrapolated (O(h^4))
+        h = 1e-4 * max(abs(p[k, m]), 1e-2)
+        fd = (4.0 * cd(k, m, 0.5 * h) - cd(k, m, h)) / 3.0
+        an = g[(k - nfru) * sy.npt + m]
+        assert abs(fd - an) <= tol * max(1.0, abs(fd)), (k, m, fd, an)
+
+
+def test_overlap_penalty_gradient_fd_real():
+    """Unit diagonal (one identity term) and real overlaps (B = 0): there
+    the reference's gradient terms are the exact derivative of P.  Hits in
+    both loops (fixed-optimised and optimised-optimised pairs), all vech L
+    parameters of the optimised functions checked."""
+    sy = named_system("hdplus")
+    K, nfru = 10, 2
+    p = _duplicated_basis(sy, K, SEED_BASE + 63, 0.02, real=True)
+    H, S, _, _ = oracle.assemble(sy, p, "G")
+    hits = [(5 + i, i) for i in range(5)]
+    t2 = _threshold_between(S, hits)
+    checks = [(k, m) for k in range(nfru, K) for m in range(sy.np_)]
+    _penalty_fd(sy, p, nfru, 0.7, t2, checks, 1e-7)
+
+
+def test_overlap_penalty_gradient_fd_complex_fixed_partner():
+    """Complex overlaps (B != 0), unit diagonal, hits only between fixed
+    and optimised functions (the reference's first loop): again the exact
+    derivative; every L and B parameter of the optimised functions."""
+    sy = named_system("hdplus")
+    K, nfru = 10, 5
+    p = _duplicated_basis(sy, K, SEED_BASE + 65, 0.02, real=False)
+    H, S, _, _ = oracle.assemble(sy, p, "G")
+    hits = [(5 + i, i) for i in range(5)]
+    t2 = _threshold_between(S, hits)
+    assert np.max(np.abs(np.tril(S, -1).imag)) > 1e-3
+    checks = [(k, m) for k in range(nfru, K) for m in range(sy.npt)]
+    _penalty_fd(sy, p, nfru, 0.7, t2, checks, 1e-7)
+
+
+def test_overlap_penalty_stationary_at_duplicates():
+    """Exact copies (|s| = 1, the maximum of |s|^2) make P stationary, with
+    two terms (T = 2, so S_kk varies with the parameters) and B != 0: the
+    overlap-derivative term and the diagonal normalisation term must
+    cancel, which a dropped or mis-weighted normalisation term breaks."""
+    sy = named_system("hminus")
+    K, nfru = 10, 2
+    p = _duplicated_basis(sy, K, SEED_BASE + 67, 0.0, real=False)
+    H, S, dH, dS, D = oracle.assemble_deriv(sy, p, nfru, "G")
+    assert np.ptp(dS) > 1e-3                      # S_kk is not constant
+    hits = [(5 + i, i) for i in range(5)]
+    t2 = _threshold_between(S, hits)
+    P, g = oracle.overlap_penalty_grad(S, dS, D, nfru, 0.7, t2)
+    assert P > 0.0
+    # size of the individual terms: 2 pc |D_S| summed
+    scale = 2 * 0.7 / (1 - t2) * np.abs(D[sy.npt:]).max()
+    assert np.max(np.abs(g)) <= 1e-12 * scale, (np.max(np.abs(g)), scale)
