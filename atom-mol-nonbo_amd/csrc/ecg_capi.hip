@@ -136,10 +136,7 @@ int ensure_capacity(ecg_ctx* x, int K) {
   RCCHK(dev_alloc(&f.status, (size_t)4));
   RCCHK(dev_alloc(&f.vec, (size_t)4 * ld));
   f.ld = ld;
-  RCCHK(ecg::solve_buffers_init(f, x->stream));
   RCCHK(dev_alloc(&f.tvec, (size_t)ld));
-  RCCHK(dev_alloc(&f.flags, (size_t)2 * (ld / ecg::NB)));
-  HIPCHK(hipMemsetAsync(f.flags, 0, sizeof(int) * 2 * (ld / ecg::NB), x->stream));
   RCCHK(dev_alloc(&f.pflag, (size_t)(ld / ecg::NB) + 2));
   HIPCHK(hipMemsetAsync(f.pflag, 0, sizeof(int) * ((ld / ecg::NB) + 2), x->stream));
   HIPCHK(hipMemsetAsync(f.status, 0, sizeof(int) * 4, x->stream));
@@ -342,10 +339,11 @@ int inverse_iteration(ecg_ctx* x, double sigma, double tol, int maxiter, bool wa
       std::swap(v, y);
       continue;
     }
-    double sc[2];
+    double sc[5];
     HIPCHK(hipMemcpyAsync(sc, x->f.scal, sizeof(sc), hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
     r.nd = sc[0];
+    if (stop_at_floor && it == 1) prev = sc[4];   // the first iteration's measure
     r.iters = it + 1;
     std::swap(v, y);   // v now holds the newest iterate
     if (!std::isfinite(r.nd)) return ECG_ERR_NONFINITE;
@@ -676,7 +674,7 @@ int ecg_ctx_destroy(ecg_ctx* x) {
   if (x->stream) (void)hipStreamSynchronize(x->stream);
   ecg::Ctx& c = x->c;
   void* ptrs[] = {c.params, c.bra, c.ket, c.diagS, c.diagH, c.H, c.S, x->f.F, x->f.D, x->f.W, x->f.Dinv,
-                  x->f.status, x->f.vec, x->f.tvec, x->f.flags, x->f.pflag, x->f.scal, x->part, x->grad, x->ev[0],
+                  x->f.status, x->f.vec, x->f.tvec, x->f.pflag, x->f.scal, x->part, x->grad, x->ev[0],
                   x->ev[1], x->ev[2]};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -871,12 +869,12 @@ int ecg_ghepiis(ecg_ctx* x, int k, int n, double* M, int nM, double* invD, const
     invD[2 * i] = 1.0 / hD[i];
     invD[2 * i + 1] = 0.0;
   }
-  // L^H into the strict upper triangle of M
+  // L^H into the strict upper triangle of M: M(i, j) = conj(L(j, i)), i < j
   for (int j = 1; j < n; ++j)
     for (int i = 0; i < j; ++i) {
-      const size_t od = 2 * ((size_t)j * c.ld + i), om = 2 * ((size_t)j * nM + i);
+      const size_t od = 2 * ((size_t)i * c.ld + j), om = 2 * ((size_t)j * nM + i);
       M[om] = hH[od];
-      M[om + 1] = hH[od + 1];
+      M[om + 1] = -hH[od + 1];
     }
   *lambda = r.lambda;
   if (relacc) *relacc = r.nd;

@@ -107,24 +107,22 @@ int overlap_penalty(const Ctx& c, int nfru, double pc, double t2, double* pen, d
 // dense Hermitian linear algebra (ecg_linalg.hip)
 struct Factor {
   int K = 0, ld = 0, cap = 0;
-  zd* F = nullptr;      // ld x ld: L (unit lower, strict part) after factorisation
+  zd* F = nullptr;      // ld x ld: L (strict lower part) and X^H = L^{-H} (strict upper part)
   double* D = nullptr;  // K pivots
   zd* W = nullptr;      // ld x 2 NB panel scratch (double-buffered by block column)
   zd* Dinv = nullptr;   // nblocks x 2 x NB x NB: inverse X of each diagonal block of L, then X^H
   int* status = nullptr;     // [0] negative pivots, [1] first zero pivot (1-based) or 0,
-                             // [2] triangular-solve wait limit exceeded
+                             // [2] a device-side wait exceeded its time limit
   zd* vec = nullptr;    // 4 x ld work vectors
-  zd* tvec = nullptr;   // ld: D^{-1} L^{-1} b between the two solves
+  zd* tvec = nullptr;   // ld: D^{-1} L^{-1} b between the two solve passes
   double* scal = nullptr;    // 16 scalars
-  int* flags = nullptr;      // 2 x ld/NB block-ready flags of the triangular solves
   int* pflag = nullptr;      // ld/NB + 1 ready flags of the factorisation (diagonal block, column tiles)
-  int epoch = 0;             // flag value of the current solve
+  int epoch = 0;             // flag value of the current factorisation launch
   Tracer* tr = nullptr;
 };
 int form_shifted(const Ctx& c, Factor& f, double sigma, hipStream_t s);
 int ldlh_factor(Factor& f, hipStream_t s);
 int ldlh_solve(Factor& f, const zd* b, zd* x, hipStream_t s);
-int solve_buffers_init(Factor& f, hipStream_t s);   // sentinel fill of the forward solve's value buffer
 int invit_iteration(const Ctx& c, Factor& f, const zd* v, zd* x, hipStream_t s);
 int rayleigh(const Ctx& c, Factor& f, const zd* x, double sigma, hipStream_t s);
 int normalise(zd* x, int K, const double* t, hipStream_t s);

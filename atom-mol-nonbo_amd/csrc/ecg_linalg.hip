@@ -6,8 +6,9 @@
 //   * the solve x = F^{-1} b through L, D, L^H (LDLHS, src/linalg.f90:618-775)
 //   * y = A x and the quadratic forms used by inverse iteration
 //     (MHMV, VMMHMV, src/linalg.f90:1019-1107, 1180-1255)
-// L overwrites the strict lower triangle of F; L^H is mirrored into the
-// strict upper triangle so both solves read memory along columns.
+// L overwrites the strict lower triangle of F.  X = L^{-1} is built by
+// block rows alongside the factorisation and kept as X^H in the strict
+// upper triangle, so that a solve is two GEMV passes over that triangle.
 #include "ecg_internal.hpp"
 
 namespace ecg {
@@ -63,8 +64,8 @@ __device__ inline void wait_wg(const int* f1, const int* f2, int epoch, int* sta
 // One workgroup of 256 threads (4 waves) factors the jb x jb diagonal block
 // at (j0, j0), held in one 64 x 64 LDS array, in 16-wide sub-blocks s:
 //   A. the four waves factor the 16 x 16 diagonal piece with the piece in
-//      registers (one element per lane) and one LDS broadcast of two pivot
-//      rows per pivot pair; the same row operations applied to the
+//      registers (one element per lane) and one LDS broadcast of four pivot
+//      rows per 4-pivot step; the same row operations applied to the
 //      identity give X_ss = L_ss^{-1} (Gauss-Jordan);
 //   B. the sub-panel below: L = A X_ss^H D^{-1}, one 16-row tile per wave
 //      on MFMA;
@@ -75,16 +76,16 @@ __device__ inline void wait_wg(const int* f1, const int* f2, int epoch, int* sta
 // LDS use: lower triangle = the block, then L (the pivots on the diagonal
 // while factoring); strict upper triangle = X transposed (X(i, j), i > j,
 // at A[j][i]; X has a unit diagonal).
-// Outputs: unit-lower L (strict part, and L^H mirrored into the upper
-// triangle of F), D, and X and X^H (column-major 64 x 64 each, used by the
-// panel solve and the triangular solves).
+// Outputs: unit-lower L (strict lower part of F), D, X^H (strict upper part
+// of F: the diagonal block of X^H = L^{-H}), and X and X^H (column-major
+// 64 x 64 each, used by the panel solve and the inverse's block rows).
 constexpr int SB = 16;   // sub-block width
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 struct DiagLds {
   zd A[NB][NB + 1];
-  zd rowA[2][2][SB];   // pivot-pair rows broadcast (double-buffered by pair parity)
-  zd rowX[2][2][SB];
+  zd rowA[2][4][SB];   // pivot-block rows broadcast (double-buffered by step parity)
+  zd rowX[2][4][SB];
   double d[NB];
 };
 
@@ -123,6 +124,44 @@ __device__ __attribute__((always_inline)) inline void mfma16(const FA& fa, const
 }
 __device__ inline int acc16_row(int v) { return 4 * v + ((threadIdx.x & 63) >> 4); }
 __device__ inline int acc16_col() { return threadIdx.x & 15; }
+
+// 2 x 2 Hermitian pivot pair [a, conj(b); b, c] (a, c real): d_0 = a,
+// d_1 = det / a with det = a c - |b|^2.  Both reciprocals come from one,
+// r = 1 / (a det): 1/d_0 = det r, 1/d_1 = a^2 r (hardware reciprocal and
+// two Newton steps, below 1 ulp).  A zero pivot (recorded by the caller)
+// leaves its column unscaled; other exceptional magnitudes use division.
+__device__ inline void pivot_pair(double a, zd b, double c, double& rd0, double& rd1, double& d1) {
+  const double det = fma(a, c, -fma(b.x, b.x, b.y * b.y));
+  const double prod = a * det;
+  if (prod != 0.0 && fabs(prod) > 1e-300 && fabs(prod) < 1e300) {
+    double r = __builtin_amdgcn_rcp(prod);
+    r = fma(fma(-prod, r, 1.0), r, r);
+    r = fma(fma(-prod, r, 1.0), r, r);
+    rd0 = det * r;
+    rd1 = a * a * r;
+    d1 = det * rd0;
+  } else {
+    rd0 = (a != 0.0) ? 1.0 / a : 0.0;
+    d1 = (a != 0.0) ? det * rd0 : c;
+    rd1 = (d1 != 0.0) ? 1.0 / d1 : 0.0;
+  }
+}
+
+// One of four complex values by index, selected per component (a select of
+// the struct itself goes through scratch memory).
+__device__ inline zd zsel4(int k, zd v0, zd v1, zd v2, zd v3) {
+  return mkz(k == 0 ? v0.x : (k == 1 ? v1.x : (k == 2 ? v2.x : v3.x)),
+             k == 0 ? v0.y : (k == 1 ? v1.y : (k == 2 ? v2.y : v3.y)));
+}
+
+// Pivot d of global column col (1-based): stored, counted if negative, the
+// first zero recorded.
+__device__ inline void pivot_record(double d, bool valid, double* dst, int* status, int col) {
+  if (!valid) return;
+  *dst = d;
+  if (d < 0.0) atomicAdd(&status[0], 1);
+  if (d == 0.0) atomicCAS(&status[1], 0, col);
+}
 
 // X(i, j) of the block from the transposed store (unit diagonal, zero above)
 __device__ inline zd x_at(const DiagLds& L, int i, int j) {
@@ -166,11 +205,12 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
     const int c = s * SB;
     const int sb = min(SB, jb - c);
     // ---- A: 16 x 16 LDL^H and X_ss, all four waves (lane = one element:
-    // row 4 wave + lane / 16, column lane % 16), two pivots (p, q = p + 1)
-    // per broadcast.  With a = A(p,p), b = A(q,p), c' = A(q,q) and
-    // det = a c' - |b|^2: d_p = a, L(q,p) = b / a, d_q = det / a, and both
-    // reciprocals come from one division, r = 1 / (a det): 1/a = det r,
-    // 1/d_q = a^2 r.
+    // row 4 wave + lane / 16, column lane % 16), four pivots P = p..p+3 per
+    // broadcast: wave g holds rows P of step g and publishes them (A and X
+    // rows) through LDS; every lane factors the 4 x 4 pivot block itself
+    // (two 2 x 2 pivot pairs) and applies the block step to its element:
+    //   L(i, P) = A(i, P) L_P^{-H} D_P^{-1},  U(P, j) = L_P^{-1} A(P, j),
+    //   A(i, j) -= L(i, P) U(P, j),   X(i, j) -= L(i, P) (L_P^{-1} X(P, j)).
     {
       const int i = 4 * wave + (lane >> 4), j = lane & 15;
       zd a, x;
@@ -182,68 +222,64 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
         x = mkz(id, 0.0);
       }
 #pragma unroll
-      for (int pp = 0; pp < SB / 2; ++pp) {
-        const int p = 2 * pp, q = p + 1;
-        zd* rP = L.rowA[pp & 1][0];
-        zd* rQ = L.rowA[pp & 1][1];
-        zd* xP = L.rowX[pp & 1][0];
-        zd* xQ = L.rowX[pp & 1][1];
-        if (i == p) {
-          rP[j] = a;
-          xP[j] = x;
-        } else if (i == q) {
-          rQ[j] = a;
-          xQ[j] = x;
+      for (int g = 0; g < SB / 4; ++g) {
+        const int p = 4 * g;
+        zd(*rA)[SB] = L.rowA[g & 1];
+        zd(*rX)[SB] = L.rowX[g & 1];
+        if (wave == g) {
+          rA[i - p][j] = a;
+          rX[i - p][j] = x;
         }
         __syncthreads();
-        const double ap = rP[p].x, cq = rQ[q].x;
-        const zd bq = rQ[p];
-        const zd api = rP[i], aqi0 = rQ[i];
-        const zd apj = rP[j], xpj = xP[j];
-        zd aqj = rQ[j], xqj = xQ[j];
-        const double det = fma(ap, cq, -fma(bq.x, bq.x, bq.y * bq.y));
-        const double prod = ap * det;
-        double rd1, rd2, dq;
-        if (prod != 0.0 && fabs(prod) > 1e-300 && fabs(prod) < 1e300) {
-          // 1/prod from the hardware reciprocal refined by two Newton steps
-          // (error below 1 ulp); the exceptional path keeps IEEE division
-          double r = __builtin_amdgcn_rcp(prod);
-          r = fma(fma(-prod, r, 1.0), r, r);
-          r = fma(fma(-prod, r, 1.0), r, r);
-          rd1 = det * r;
-          rd2 = ap * ap * r;
-          dq = det * rd1;
-        } else {   // a zero pivot: recorded below, its column left unscaled
-          rd1 = (ap != 0.0) ? 1.0 / ap : 0.0;
-          dq = (ap != 0.0) ? det * rd1 : cq;
-          rd2 = (dq != 0.0) ? 1.0 / dq : 0.0;
-        }
-        const zd l = bq * rd1;   // L(q, p)
-        // row q after pivot p: A'(q, .) = A(q, .) - L(q,p) A(p, .), same for X
-        aqj = zfms(l, apj, aqj);
-        xqj = zfms(l, xpj, xqj);
-        if (i > q) {
-          const zd lp = zconj(api) * rd1;                   // L(i, p)
-          const zd lq = zconj(zfms(l, api, aqi0)) * rd2;    // L(i, q) = conj(A'(q, i)) / d_q
-          if (j > q) a = zfms(lq, aqj, zfms(lp, apj, a));
-          if (j == p) a = lp;
-          if (j == q) a = lq;
-          x = zfms(lq, xqj, zfms(lp, xpj, x));
-        } else if (i == q) {
-          if (j == p) a = l;
-          x = xqj;
+        // pivot block: pair (0, 1), then the Schur complement pair (2, 3)
+        const double a00 = rA[0][p].x, a11 = rA[1][p + 1].x, a22 = rA[2][p + 2].x, a33 = rA[3][p + 3].x;
+        const zd a10 = rA[1][p], a20 = rA[2][p], a21 = rA[2][p + 1], a30 = rA[3][p], a31 = rA[3][p + 1],
+                 a32 = rA[3][p + 2];
+        double rd0, rd1, d1;
+        pivot_pair(a00, a10, a11, rd0, rd1, d1);
+        const zd l10 = a10 * rd0, l20 = a20 * rd0, l30 = a30 * rd0;
+        const zd a21p = zfms(a20, zconj(l10), a21), a31p = zfms(a30, zconj(l10), a31);
+        const zd l21 = a21p * rd1, l31 = a31p * rd1;
+        const double s22 = a22 - fma(a20.x, l20.x, a20.y * l20.y) - fma(a21p.x, l21.x, a21p.y * l21.y);
+        const double s33 = a33 - fma(a30.x, l30.x, a30.y * l30.y) - fma(a31p.x, l31.x, a31p.y * l31.y);
+        const zd s32 = zfms(a31p, zconj(l21), zfms(a30, zconj(l20), a32));
+        double rd2, rd3, d3;
+        pivot_pair(s22, s32, s33, rd2, rd3, d3);
+        const zd l32 = s32 * rd2;
+        // L_P^{-1} applied to the block's A and X columns j
+        zd u0 = rA[0][j], u1 = rA[1][j], u2 = rA[2][j], u3 = rA[3][j];
+        zd x0 = rX[0][j], x1 = rX[1][j], x2 = rX[2][j], x3 = rX[3][j];
+        u1 = zfms(l10, u0, u1);
+        x1 = zfms(l10, x0, x1);
+        u2 = zfms(l21, u1, zfms(l20, u0, u2));
+        x2 = zfms(l21, x1, zfms(l20, x0, x2));
+        u3 = zfms(l32, u2, zfms(l31, u1, zfms(l30, u0, u3)));
+        x3 = zfms(l32, x2, zfms(l31, x1, zfms(l30, x0, x3)));
+        if (i >= p + 4) {
+          // z = A(i, P) L_P^{-H} with A(i, k) = conj(A(k, i)); L(i, P) = z D_P^{-1}
+          const zd y0 = zconj(rA[0][i]), y1 = zconj(rA[1][i]), y2 = zconj(rA[2][i]), y3 = zconj(rA[3][i]);
+          const zd z1 = zfms(y0, zconj(l10), y1);
+          const zd z2 = zfms(z1, zconj(l21), zfms(y0, zconj(l20), y2));
+          const zd z3 = zfms(z2, zconj(l32), zfms(z1, zconj(l31), zfms(y0, zconj(l30), y3)));
+          const zd L0 = y0 * rd0, L1 = z1 * rd1, L2 = z2 * rd2, L3 = z3 * rd3;
+          if (j >= p + 4) a = zfms(L3, u3, zfms(L2, u2, zfms(L1, u1, zfms(L0, u0, a))));
+          else if (j >= p) a = zsel4(j - p, L0, L1, L2, L3);
+          x = zfms(L3, x3, zfms(L2, x2, zfms(L1, x1, zfms(L0, x0, x))));
+        } else if (i >= p) {
+          const int r = i - p;
+          x = zsel4(r, x0, x1, x2, x3);
+          if (j >= p && j < i) {
+            const int cc = j - p;
+            // L_P(r, cc), r > cc: (1,0) l10; (2,0) l20, (2,1) l21; (3,0..2) l30, l31, l32
+            const zd l2 = (cc == 0) ? l20 : l21;
+            a = zsel4(r, l10, l10, l2, zsel4(cc, l30, l31, l32, l32));
+          }
         }
         if (tid == 0) {
-          if (p < sb) {
-            L.d[c + p] = ap;
-            if (ap < 0.0) atomicAdd(&status[0], 1);
-            if (ap == 0.0) atomicCAS(&status[1], 0, j0 + c + p + 1);
-          }
-          if (q < sb) {
-            L.d[c + q] = dq;
-            if (dq < 0.0) atomicAdd(&status[0], 1);
-            if (dq == 0.0) atomicCAS(&status[1], 0, j0 + c + q + 1);
-          }
+          pivot_record(a00, p + 0 < sb, L.d + c + p + 0, status, j0 + c + p + 1);
+          pivot_record(d1, p + 1 < sb, L.d + c + p + 1, status, j0 + c + p + 2);
+          pivot_record(s22, p + 2 < sb, L.d + c + p + 2, status, j0 + c + p + 3);
+          pivot_record(d3, p + 3 < sb, L.d + c + p + 3, status, j0 + c + p + 4);
         }
       }
       if (i < sb && j < i) {
@@ -346,7 +382,7 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
   }
   __syncthreads();
   // ---- outputs: X and D first (all the next panel needs), then the flag
-  // (look-ahead use), then X^H, L and its mirror in F
+  // (look-ahead use), then X^H (also into F's strict upper triangle) and L
   zd* Xg = Dinv + (size_t)(j0 / NB) * 2 * NB * NB;
   zd* XHg = Xg + NB * NB;
   {
@@ -371,24 +407,16 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
       const int e = tid + 256 * u, j = e >> 6, i = e & (NB - 1);
       const bool in = (i < jb && j < jb);
       const double dg = (in && i == j) ? 1.0 : 0.0;
-      // X^H(i, j) = conj(X(j, i)), X(j, i) (j > i) at A[i][j]
-      XHg[(size_t)j * NB + i] = (in && i < j) ? zconj(lo[u]) : mkz(dg, 0.0);
+      // X^H(i, j) = conj(X(j, i)), X(j, i) (j > i) at A[i][j]; also the
+      // strict upper triangle of the block in F
+      const zd xh = zconj(lo[u]);
+      XHg[(size_t)j * NB + i] = (in && i < j) ? xh : mkz(dg, 0.0);
+      if (in && i < j) F[(size_t)(j0 + j) * ld + j0 + i] = xh;
     }
 #pragma unroll
     for (int u = 0; u < NB * NB / 256; ++u) {
       const int e = tid + 256 * u, j = e >> 6, i = e & (NB - 1);
       if (i < jb && j < jb && i > j) F[(size_t)(j0 + j) * ld + j0 + i] = lo[u];
-    }
-#pragma unroll
-    for (int u = 0; u < NB * NB / 256; ++u) {
-      const int e = tid + 256 * u, i = e >> 6, j = e & (NB - 1);   // row-major walk: coalesced
-      lo[u] = L.A[i][j];
-    }
-#pragma unroll
-    for (int u = 0; u < NB * NB / 256; ++u) {
-      const int e = tid + 256 * u, i = e >> 6, j = e & (NB - 1);
-      // L^H(j, i) = conj(L(i, j)) goes to F(j0 + j, j0 + i) for i > j
-      if (i < jb && j < jb && i > j) F[(size_t)(j0 + i) * ld + j0 + j] = zconj(lo[u]);
     }
   }
 }
@@ -469,8 +497,8 @@ __device__ inline void tile_pos(int a, int q, int v, int& r, int& c) {
 
 // ------------------------------------------------------------ panel solve
 // 16-row strip i0..i0+15 below the diagonal block at j0: W = A X^H (so
-// that W L_JJ^H = A), L = W D^{-1}; L goes to F (and L^H to its upper
-// triangle), W to the panel scratch used by the trailing update.  Wave w
+// that W L_JJ^H = A), L = W D^{-1}; L goes to F, W to the panel scratch
+// used by the trailing update.  Wave w
 // computes columns 16w..16w+15 as one 16 x 16 MFMA tile: all 16 k-steps'
 // operand fragments are loaded first (one memory latency), then the 64
 // MFMAs.
@@ -506,7 +534,6 @@ __device__ __attribute__((always_inline)) inline void panel_strip(int K, int ld,
       const zd l = w * (1.0 / D[j0 + c]);
       W[(size_t)c * ld + r] = w;
       F[(size_t)(j0 + c) * ld + r] = l;
-      F[(size_t)r * ld + j0 + c] = zconj(l);
     }
   }
 }
@@ -519,15 +546,99 @@ __global__ void __launch_bounds__(256) k_ldl_panel(int K, int ld, int j0, zd* __
   panel_strip(K, ld, j0, j0 + NB + blockIdx.x * SB, F, D, Dinv, W);
 }
 
+// ------------------------------------------------- inverse of L by blocks
+// X = L^{-1} (unit lower) is built by block rows alongside the
+// factorisation and stored as X^H in the strict upper triangle of F: block
+// (c, i), c < i, holds X_ic^H; the diagonal blocks' upper parts hold
+// X_ii^H (from the diagonal factor).  Block row i satisfies
+//   X_ic = -X_ii sum_{u=c}^{i-1} L_iu X_uc,   c < i.
+// The launch for block column k (k >= 1) uses panel k - 1, X_kk and block
+// row k - 1 of X, all complete in earlier launches:
+//   row k:      X_kc^H = (R_kc^H - X_{k-1,c}^H L_{k,k-1}^H) X_kk^H,  c < k
+//   rows i > k: R_ic^H -= X_{k-1,c}^H L_{i,k-1}^H,                   c < k
+// where R_ic = -sum_{u=c}^{k-2} L_iu X_uc is kept (as R^H) where X_ic^H
+// will go; its first update (c = k - 1) assigns.  k_ldl_xlast finishes
+// the last block row.  Every product is a 64 x 64 MFMA tile A B^H.
+__device__ __attribute__((always_inline)) inline void xinv_tile(int K, int ld, int k, int i, int c,
+                                                                zd* __restrict__ F, const zd* __restrict__ Dinv,
+                                                                DiagLds& Ld) {
+  const int ib = min(NB, K - i * NB);
+  const bool first = (c == k - 1);
+  // X_{k-1,c}^H: block (c, k-1) of F, or the diagonal block's X^H
+  const zd* T = first ? Dinv + (size_t)(k - 1) * 2 * NB * NB + NB * NB : F + (size_t)(k - 1) * NB * ld + c * NB;
+  const int ldt = first ? NB : ld;
+  const zd* Lb = F + (size_t)(k - 1) * NB * ld + i * NB;   // L_{i,k-1}(s, p) at Lb[p * ld + s]
+  zd* out = F + (size_t)i * NB * ld + c * NB;               // block (c, i): (r, s) at out[s * ld + r]
+  Tile64 t;
+  mfma_tile_abh(T, ldt, NB, Lb, ld, ib, NB, t);
+  if (i > k) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          int r, sc;
+          tile_pos(a, q, v, r, sc);
+          if (sc < ib) {
+            zd* o = out + (size_t)sc * ld + r;
+            const zd old = first ? mkz(0.0, 0.0) : *o;
+            *o = mkz(old.x - t.re[a][q][v], old.y - t.im[a][q][v]);
+          }
+        }
+    return;
+  }
+  // row k: M = R^H - T L^H staged in LDS as M(r, p) at Ld.A[p][r], then
+  // X_kc^H = M X_kk^H
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        int r, sc;
+        tile_pos(a, q, v, r, sc);
+        zd m = mkz(0.0, 0.0);
+        if (sc < ib) {
+          const zd old = first ? mkz(0.0, 0.0) : out[(size_t)sc * ld + r];
+          m = mkz(old.x - t.re[a][q][v], old.y - t.im[a][q][v]);
+        }
+        Ld.A[sc][r] = m;
+      }
+  __syncthreads();
+  const zd* Xkk = Dinv + (size_t)k * 2 * NB * NB;   // X_kk(s, p) at Xkk[p * NB + s]
+  mfma_tile_abh(&Ld.A[0][0], NB + 1, NB, Xkk, NB, ib, ib, t);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        int r, sc;
+        tile_pos(a, q, v, r, sc);
+        if (sc < ib) out[(size_t)sc * ld + r] = mkz(t.re[a][q][v], t.im[a][q][v]);
+      }
+}
+
+// Last block row of X (k = nb - 1), one workgroup per block column c.
+__global__ void __launch_bounds__(256) k_ldl_xlast(int K, int ld, zd* __restrict__ F, const zd* __restrict__ Dinv) {
+  __shared__ DiagLds Ld;
+  const int k = (K + NB - 1) / NB - 1;
+  xinv_tile(K, ld, k, k, blockIdx.x, F, Dinv, Ld);
+}
+
 // -------------------------------------------------------- trailing update
 // One launch per block column j (diagonal block at j0, trailing blocks
 // from t0 = j + 1):
 //   workgroups 0..ntiles-1: lower block pairs (I >= Kb) of the trailing
 //     matrix, C_IK -= W_I L_K^H.  The one that updates the first trailing
 //     diagonal block then factors it (look-ahead).
-//   workgroups ntiles..: 16-row strips of the next block column's panel.
-//     Each waits until the tile holding its rows and the look-ahead factor
-//     are published, then runs the panel solve for its strip.
+//   workgroups ntiles..ntiles+nstrip-1: 16-row strips of the next block
+//     column's panel.  Each waits until the tile holding its rows and the
+//     look-ahead factor are published, then runs the panel solve for its
+//     strip.
+//   workgroups ntiles+nstrip..: block tiles of X = L^{-1} (xinv_tile): the
+//     j (< col) tiles of block row col, then rows col+1.. (no waits).
 // Publication: the producer's threads fence at agent scope, the workgroup
 // synchronises, thread 0 stores the flag (release).  A waiting workgroup's
 // thread 0 polls the flags (relaxed, bounded in time: past the limit it
@@ -541,11 +652,18 @@ __global__ void __launch_bounds__(256) k_ldl_update(int K, int ld, int j0, zd* _
                                                     const zd* __restrict__ Wcur, zd* __restrict__ Wnext,
                                                     double* __restrict__ D, zd* __restrict__ Dinv,
                                                     int* __restrict__ status, int* __restrict__ pflag,
-                                                    int epoch, int ntiles) {
+                                                    int epoch, int ntiles, int nstrip) {
   __shared__ DiagLds Ld;
   const int jb = min(NB, K - j0);
   const int t0 = (j0 + jb) / NB;
   const int b = blockIdx.x;
+  if (b >= ntiles + nstrip) {
+    const int k = j0 / NB, x = b - ntiles - nstrip;
+    const int i = (x < k) ? k : k + 1 + (x - k) / k;
+    const int c = (x < k) ? x : (x - k) % k;
+    xinv_tile(K, ld, k, i, c, F, Dinv, Ld);
+    return;
+  }
   if (b >= ntiles) {
     const int i0 = (t0 + 1) * NB + (b - ntiles) * SB;
     const int bi = (i0 - t0 * NB) / NB;
@@ -615,256 +733,104 @@ int ldlh_factor(Factor& f, hipStream_t s) {
     const int nstrip = rest2 > 0 ? (rest2 + SB - 1) / SB : 0;
     zd* Wcur = f.W + (size_t)(col & 1) * f.ld * NB;
     zd* Wnext = f.W + (size_t)((col + 1) & 1) * f.ld * NB;
+    const int nb = (K + NB - 1) / NB;
+    const int nx = (col >= 1) ? col + (nb - 1 - col) * col : 0;   // tiles of X = L^{-1}
     const int epoch = ++f.epoch;
     if (f.tr) f.tr->begin(TR_LDL_UPDATE, s);
-    hipLaunchKernelGGL(k_ldl_update, dim3(ntiles + nstrip), dim3(256), 0, s, K, f.ld, j0, f.F, Wcur, Wnext, f.D,
-                       f.Dinv, f.status, f.pflag, epoch, ntiles);
+    hipLaunchKernelGGL(k_ldl_update, dim3(ntiles + nstrip + nx), dim3(256), 0, s, K, f.ld, j0, f.F, Wcur, Wnext,
+                       f.D, f.Dinv, f.status, f.pflag, epoch, ntiles, nstrip);
     if (f.tr) f.tr->end(s);
   }
+  if (K > NB) hipLaunchKernelGGL(k_ldl_xlast, dim3((K + NB - 1) / NB - 1), dim3(256), 0, s, K, f.ld, f.F, f.Dinv);
   return (int)hipGetLastError();
 }
 
 // ------------------------------------------------------------ solve
-// Sync-free blocked triangular solves.  Workgroup w owns one 64-row block
-// and consumes the solution blocks it depends on as their producers store
-// them.  The solution values are their own ready flags: before a solve the
-// value buffer holds a signalling-NaN sentinel (arithmetic only produces
-// quiet NaNs, so no computed value equals it); a producer (wave 0) stores
-// its 64 values with device-coherent atomic stores followed by a release
-// fence; a consumer (wave 0) loads the values of up to RUN_MAX pending
-// blocks with device-coherent atomic loads, one value per lane and block,
-// and takes the leading run of blocks whose values are all present; the
-// values it loaded are the ones it uses.  Each solve's forward kernel resets
-// the backward kernel's output blocks and the backward kernel resets the
-// forward kernel's, so every reset is ordered before its use by a kernel
-// boundary.  No acquire fence is used: it would invalidate the L2 that holds
-// the factor being streamed.  The next L block is prefetched into registers
-// before polling, and X_b (the inverse of the diagonal block) is staged in
-// LDS at kernel start.  Polling is bounded in time (WAIT_LIMIT_TICKS): past
-// it the wave records an error in status[2] and proceeds, so every wave
-// always exits.  Dependencies only point to lower workgroup indices, which
-// are dispatched no later than the consumer.
-constexpr int RUN_MAX = 8;   // blocks folded per round
-constexpr unsigned long long SOLVE_SENTINEL = 0x7FF4ECB0DEADBEEFull;   // signalling NaN
+// x = F^{-1} b = X^H D^{-1} X b with X = L^{-1} held as X^H in the strict
+// upper triangle of F (unit diagonal): two passes over that triangle, each
+// an HBM-bound GEMV with no dependency chain.
+//   forward:  z_i = (b_i + sum_{j<i} conj(F(j, i)) b_j) / D_i, one wave per
+//             column i (a contiguous read of the column);
+//   backward: x_j = z_j + sum_{i>j} F(j, i) z_i, workgroups of 64 rows x
+//             XS_CHUNK columns write partial sums that k_xsolve_reduce adds
+//             in a fixed order.
+constexpr int XS_CHUNK = 128;
 
-__device__ inline void store_published(zd* p, zd v) {
-  unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
-  __hip_atomic_store(q, (unsigned long long)__double_as_longlong(v.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(q + 1, (unsigned long long)__double_as_longlong(v.y), __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void store_sentinel(zd* p) {
-  unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
-  q[0] = SOLVE_SENTINEL;
-  q[1] = SOLVE_SENTINEL;
-}
-
-// Wave 0: loads value `lane` of the pending blocks base + dir * 64 * t
-// (t < want, each with `valid(t)` present lanes) until a leading run of
-// blocks is complete; the run's values go to vals[t][lane]; returns its
-// length (>= 1).  Returns `want` on a time-out, with status[2] set.
-template <class Valid>
-__device__ inline int await_values(const zd* buf, long long base, int dir, int want, const Valid& valid,
-                                   zd (*vals)[NB], int* status) {
+__global__ void __launch_bounds__(256) k_xsolve_fwd(int K, int ld, const zd* __restrict__ F,
+                                                    const double* __restrict__ D, const zd* __restrict__ b,
+                                                    zd* __restrict__ z) {
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= K) return;
   const int lane = threadIdx.x & 63;
-  const unsigned long long* q = reinterpret_cast<const unsigned long long*>(buf);
-  const unsigned long long t0 = wall_clock64();
-  for (;;) {
-    unsigned long long a[RUN_MAX], b[RUN_MAX];
-#pragma unroll
-    for (int t = 0; t < RUN_MAX; ++t) {
-      a[t] = b[t] = SOLVE_SENTINEL;
-      if (t < want && lane < valid(t)) {
-        const long long o = 2 * (base + (long long)dir * NB * t + lane);
-        a[t] = __hip_atomic_load(q + o, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        b[t] = __hip_atomic_load(q + o + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
-    int m = 0;
-#pragma unroll
-    for (int t = 0; t < RUN_MAX; ++t) {
-      const bool need = (t < want) && (lane < valid(t));
-      const bool here = !need || (a[t] != SOLVE_SENTINEL && b[t] != SOLVE_SENTINEL);
-      const bool all = (__ballot(!here) == 0ull);
-      if (all && m == t && t < want) m = t + 1;
-    }
-    if (m > 0) {
-#pragma unroll
-      for (int t = 0; t < RUN_MAX; ++t)
-        if (t < m)
-          vals[t][lane] = (lane < valid(t)) ? mkz(__longlong_as_double((long long)a[t]), __longlong_as_double((long long)b[t]))
-                                            : mkz(0.0, 0.0);
-      return m;
-    }
-    __builtin_amdgcn_s_sleep(1);
-    if (wall_clock64() - t0 > WAIT_LIMIT_TICKS) {
-      if (lane == 0) atomicExch(&status[2], 1);
-      for (int t = 0; t < want; ++t) vals[t][lane] = mkz(0.0, 0.0);
-      return want;
-    }
+  const zd* col = F + (size_t)i * ld;
+  double ar = 0.0, ai = 0.0;
+  int j = lane;
+  for (; j + 192 < i; j += 256) {   // four columns' worth of loads in flight
+    const zd f0 = col[j], f1 = col[j + 64], f2 = col[j + 128], f3 = col[j + 192];
+    const zd b0 = b[j], b1 = b[j + 64], b2 = b[j + 128], b3 = b[j + 192];
+    // conj(f) b
+    ar = fma(f0.x, b0.x, fma(f0.y, b0.y, ar));
+    ai = fma(f0.x, b0.y, fma(-f0.y, b0.x, ai));
+    ar = fma(f1.x, b1.x, fma(f1.y, b1.y, ar));
+    ai = fma(f1.x, b1.y, fma(-f1.y, b1.x, ai));
+    ar = fma(f2.x, b2.x, fma(f2.y, b2.y, ar));
+    ai = fma(f2.x, b2.y, fma(-f2.y, b2.x, ai));
+    ar = fma(f3.x, b3.x, fma(f3.y, b3.y, ar));
+    ai = fma(f3.x, b3.y, fma(-f3.y, b3.x, ai));
+  }
+  for (; j < i; j += 64) {
+    const zd f0 = col[j], b0 = b[j];
+    ar = fma(f0.x, b0.x, fma(f0.y, b0.y, ar));
+    ai = fma(f0.x, b0.y, fma(-f0.y, b0.x, ai));
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    ar += __shfl_xor(ar, o);
+    ai += __shfl_xor(ai, o);
+  }
+  if (lane == 0) {
+    const double inv = 1.0 / D[i];
+    z[i] = mkz((b[i].x + ar) * inv, (b[i].y + ai) * inv);
   }
 }
 
-// Forward: L y = b (unit lower), then z = D^{-1} y; resets block bl of x
-// (the backward solve's output) to the sentinel.
-__global__ void __launch_bounds__(256) k_trsv_fwd(int K, int ld, const zd* __restrict__ F,
-                                                  const double* __restrict__ D, const zd* __restrict__ Dinv,
-                                                  const zd* __restrict__ bvec, zd* y, zd* __restrict__ z,
-                                                  zd* xreset, int* status) {
-  __shared__ zd Xs[NB][NB + 1];
-  __shared__ zd ys[RUN_MAX][NB];
-  __shared__ zd part[4][NB];
-  __shared__ int run;
-  const int bl = blockIdx.x;
-  const int b0 = bl * NB, bs = min(NB, K - b0);
+__global__ void __launch_bounds__(256) k_xsolve_bwd(int K, int ld, const zd* __restrict__ F,
+                                                    const zd* __restrict__ z, zd* __restrict__ part) {
+  const int rb = blockIdx.x, ch = blockIdx.y;
+  if (ch < rb / 2) return;   // the chunk's columns all lie left of these rows
+  __shared__ zd red[4][64];
   const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
-  if (g == 1 && r < bs) store_sentinel(&xreset[b0 + r]);
-  {
-    const zd* X = Dinv + (size_t)bl * 2 * NB * NB;
-    for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) Xs[e & (NB - 1)][e >> 6] = X[e];   // Xs[i][p] = X(i,p)
-  }
+  const int j = rb * 64 + r;
+  const int c1 = min(K, (ch + 1) * XS_CHUNK);
   zd acc = mkz(0.0, 0.0);
-  int done = 0;
-  zd pre[NB / 4];   // prefetched column block of L for the next dependency
-  if (bl > 0 && r < bs) {
-#pragma unroll
-    for (int u = 0; u < NB / 4; ++u) pre[u] = F[(size_t)(4 * u + g) * ld + b0 + r];
-  }
-  while (done < bl) {
-    if (g == 0) {
-      const int m = await_values(y, (long long)done * NB, 1, min(bl - done, RUN_MAX), [](int) { return NB; }, ys,
-                                 status);
-      if (r == 0) run = m;
-    }
-    __syncthreads();
-    const int m = run;
-    if (r < bs) {
-#pragma unroll
-      for (int u = 0; u < NB / 4; ++u) acc = zfma(pre[u], ys[0][4 * u + g], acc);
-      for (int t = 1; t < m; ++t) {
-        const zd* col = F + (size_t)((done + t) * NB) * ld + b0 + r;
-        for (int c = g; c < NB; c += 4) acc = zfma(col[(size_t)c * ld], ys[t][c], acc);
-      }
-      if (done + m < bl) {
-        const zd* col = F + (size_t)((done + m) * NB) * ld + b0 + r;
-#pragma unroll
-        for (int u = 0; u < NB / 4; ++u) pre[u] = col[(size_t)(4 * u + g) * ld];
-      }
-    }
-    done += m;
-    __syncthreads();
-  }
-  part[g][r] = acc;
+  if (j < K)
+    for (int i = ch * XS_CHUNK + g; i < c1; i += 4)
+      if (i > j) acc = zfma(F[(size_t)i * ld + j], z[i], acc);
+  red[g][r] = acc;
   __syncthreads();
-  // r_b = b_b - sum; y_b = X_b r_b
-  if (g == 0)
-    ys[0][r] = (r < bs) ? bvec[b0 + r] - (part[0][r] + part[1][r] + part[2][r] + part[3][r]) : mkz(0.0, 0.0);
-  __syncthreads();
-  zd v = mkz(0.0, 0.0);
-  for (int p = g; p <= r; p += 4) v = zfma(Xs[r][p], ys[0][p], v);
-  part[g][r] = v;
-  __syncthreads();
-  if (g == 0) {
-    if (r < bs) {
-      const zd yv = part[0][r] + part[1][r] + part[2][r] + part[3][r];
-      store_published(&y[b0 + r], yv);
-      z[b0 + r] = yv * (1.0 / D[b0 + r]);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  }
+  if (g == 0 && j < K) part[(size_t)ch * ld + j] = red[0][r] + red[1][r] + red[2][r] + red[3][r];
 }
 
-// Backward: L^H x = z, L^H read from the strict upper triangle of F.
-// Workgroup w handles block nb-1-w; resets that block of y (the forward
-// solve's values) to the sentinel for the next solve.
-__global__ void __launch_bounds__(256) k_trsv_bwd(int K, int ld, const zd* __restrict__ F,
-                                                  const zd* __restrict__ Dinv, const zd* __restrict__ z,
-                                                  zd* x, zd* yreset, int* status) {
-  __shared__ zd Xs[NB][NB + 1];
-  __shared__ zd xs[RUN_MAX][NB];
-  __shared__ zd part[4][NB];
-  __shared__ int run;
-  const int nb = (K + NB - 1) / NB;
-  const int bl = nb - 1 - (int)blockIdx.x;
-  const int b0 = bl * NB, bs = min(NB, K - b0);
-  const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
-  if (g == 1 && r < bs) store_sentinel(&yreset[b0 + r]);
-  {
-    const zd* XH = Dinv + (size_t)bl * 2 * NB * NB + NB * NB;
-    for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) Xs[e & (NB - 1)][e >> 6] = XH[e];   // Xs[i][p] = X^H(i,p)
-  }
-  zd acc = mkz(0.0, 0.0);
-  int done = 0;                  // blocks nb-1, nb-2, ... already folded
-  const int need = nb - 1 - bl;
-  zd pre[NB / 4];   // prefetched block of L^H for the next dependency
-  if (need > 0 && r < bs) {
-    const int J = nb - 1, js = min(NB, K - J * NB);
-    const zd* col = F + (size_t)(J * NB) * ld + b0 + r;
-#pragma unroll
-    for (int u = 0; u < NB / 4; ++u) pre[u] = (4 * u + g < js) ? col[(size_t)(4 * u + g) * ld] : mkz(0.0, 0.0);
-  }
-  while (done < need) {
-    if (g == 0) {
-      const int first = nb - 1 - done;
-      const int m = await_values(x, (long long)first * NB, -1, min(need - done, RUN_MAX),
-                                 [&](int t) { return min(NB, K - (first - t) * NB); }, xs, status);
-      if (r == 0) run = m;
-    }
-    __syncthreads();
-    const int m = run;
-    if (r < bs) {
-#pragma unroll
-      for (int u = 0; u < NB / 4; ++u) acc = zfma(pre[u], xs[0][4 * u + g], acc);
-      for (int t = 1; t < m; ++t) {
-        const int J = nb - 1 - done - t;   // a full block: only block nb-1 can be partial
-        const zd* col = F + (size_t)(J * NB) * ld + b0 + r;
-        for (int c = g; c < NB; c += 4) acc = zfma(col[(size_t)c * ld], xs[t][c], acc);
-      }
-      if (done + m < need) {
-        const int J = nb - 1 - done - m;
-        const zd* col = F + (size_t)(J * NB) * ld + b0 + r;
-#pragma unroll
-        for (int u = 0; u < NB / 4; ++u) pre[u] = col[(size_t)(4 * u + g) * ld];
-      }
-    }
-    done += m;
-    __syncthreads();
-  }
-  part[g][r] = acc;
-  __syncthreads();
-  if (g == 0)
-    xs[0][r] = (r < bs) ? z[b0 + r] - (part[0][r] + part[1][r] + part[2][r] + part[3][r]) : mkz(0.0, 0.0);
-  __syncthreads();
-  // x_b = X_b^H rb
-  zd v = mkz(0.0, 0.0);
-  for (int p = r + ((g - r % 4 + 4) % 4); p < bs; p += 4) v = zfma(Xs[r][p], xs[0][p], v);
-  part[g][r] = v;
-  __syncthreads();
-  if (g == 0) {
-    if (r < bs) store_published(&x[b0 + r], part[0][r] + part[1][r] + part[2][r] + part[3][r]);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  }
+__global__ void __launch_bounds__(256) k_xsolve_reduce(int K, int ld, int nch, const zd* __restrict__ part,
+                                                       const zd* __restrict__ z, zd* __restrict__ x) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= K) return;
+  zd s = z[j];
+  for (int ch = (j / 64) / 2; ch < nch; ++ch) s = s + part[(size_t)ch * ld + j];
+  x[j] = s;
 }
 
-__global__ void k_fill_sentinel(int n, zd* __restrict__ v) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) store_sentinel(&v[i]);
-}
-
-int solve_buffers_init(Factor& f, hipStream_t s) {
-  hipLaunchKernelGGL(k_fill_sentinel, dim3((f.ld + 255) / 256), dim3(256), 0, s, f.ld, f.vec + 3 * (size_t)f.ld);
-  return (int)hipGetLastError();
-}
-
+// b and x distinct; f.tvec holds z, f.W the partial sums (ld x nch <= ld x 2 NB).
 int ldlh_solve(Factor& f, const zd* b, zd* x, hipStream_t s) {
-  const int nb = (f.K + NB - 1) / NB;
-  zd* y = f.vec + 3 * (size_t)f.ld;
+  const int K = f.K;
+  const int nrb = (K + 63) / 64, nch = (K + XS_CHUNK - 1) / XS_CHUNK;
+  if (nch > 2 * NB) return -1;
   zd* z = f.tvec;
   if (f.tr) f.tr->begin(TR_TRSV_FWD, s);
-  hipLaunchKernelGGL(k_trsv_fwd, dim3(nb), dim3(256), 0, s, f.K, f.ld, f.F, f.D, f.Dinv, b, y, z, x, f.status);
+  hipLaunchKernelGGL(k_xsolve_fwd, dim3((K + 3) / 4), dim3(256), 0, s, K, f.ld, f.F, f.D, b, z);
   if (f.tr) f.tr->end(s);
   if (f.tr) f.tr->begin(TR_TRSV_BWD, s);
-  hipLaunchKernelGGL(k_trsv_bwd, dim3(nb), dim3(256), 0, s, f.K, f.ld, f.F, f.Dinv, z, x, y, f.status);
+  hipLaunchKernelGGL(k_xsolve_bwd, dim3(nrb, nch), dim3(256), 0, s, K, f.ld, f.F, z, f.W);
+  hipLaunchKernelGGL(k_xsolve_reduce, dim3((K + 255) / 256), dim3(256), 0, s, K, f.ld, nch, f.W, z, x);
   if (f.tr) f.tr->end(s);
   return (int)hipGetLastError();
 }
@@ -938,7 +904,7 @@ __device__ inline double block_max(double v, double* sh) {
 
 // One inverse-iteration bookkeeping step (GHEPIIS, src/linalg.f90:1892-1909):
 // x /= max|Re|,|Im|; tc = x^H v / v^H v; out[0] = sqrt(2K) ||x - tc v||,
-// out[1] = the scale used.
+// out[1] = the scale used; out[4] keeps the previous out[0].
 __global__ void __launch_bounds__(1024) k_invit_finish(int K, zd* __restrict__ x, const zd* __restrict__ v,
                                                        double* __restrict__ out) {
   __shared__ double sh[1024];
@@ -967,6 +933,7 @@ __global__ void __launch_bounds__(1024) k_invit_finish(int K, zd* __restrict__ x
   }
   nd = block_sum(nd, sh);
   if (tid == 0) {
+    out[4] = out[0];
     out[0] = sqrt(2.0 * K) * sqrt(nd);
     out[1] = t1;
   }
