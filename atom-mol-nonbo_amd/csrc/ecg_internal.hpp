@@ -116,7 +116,7 @@ struct Factor {
   zd* vec = nullptr;    // 4 x ld work vectors
   zd* tvec = nullptr;   // ld: D^{-1} L^{-1} b between the two solve passes
   double* scal = nullptr;    // 16 scalars
-  int* pflag = nullptr;      // ld/NB + 1 ready flags of the factorisation (diagonal block, column tiles)
+  int* pflag = nullptr;      // ld/NB + 2 ready flags of the factorisation (diagonal block, column tiles)
   int epoch = 0;             // flag value of the current factorisation launch
   Tracer* tr = nullptr;
 };

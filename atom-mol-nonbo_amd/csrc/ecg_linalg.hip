@@ -801,11 +801,29 @@ __global__ void __launch_bounds__(256) k_xsolve_bwd(int K, int ld, const zd* __r
   const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
   const int j = rb * 64 + r;
   const int c1 = min(K, (ch + 1) * XS_CHUNK);
-  zd acc = mkz(0.0, 0.0);
-  if (j < K)
-    for (int i = ch * XS_CHUNK + g; i < c1; i += 4)
-      if (i > j) acc = zfma(F[(size_t)i * ld + j], z[i], acc);
-  red[g][r] = acc;
+  // eight columns' loads in flight per lane; columns i <= j (lower
+  // triangle, in bounds) are read and masked
+  zd acc0 = mkz(0.0, 0.0), acc1 = mkz(0.0, 0.0);
+  if (j < K) {
+    int i = ch * XS_CHUNK + g;
+    for (; i + 28 < c1; i += 32) {
+      zd fv[8], zv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        fv[u] = F[(size_t)(i + 4 * u) * ld + j];
+        zv[u] = z[i + 4 * u];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const zd t = (i + 4 * u > j) ? zv[u] : mkz(0.0, 0.0);
+        if (u & 1) acc1 = zfma(fv[u], t, acc1);
+        else acc0 = zfma(fv[u], t, acc0);
+      }
+    }
+    for (; i < c1; i += 4)
+      if (i > j) acc0 = zfma(F[(size_t)i * ld + j], z[i], acc0);
+  }
+  red[g][r] = acc0 + acc1;
   __syncthreads();
   if (g == 0 && j < K) part[(size_t)ch * ld + j] = red[0][r] + red[1][r] + red[2][r] + red[3][r];
 }
@@ -939,23 +957,6 @@ __global__ void __launch_bounds__(1024) k_invit_finish(int K, zd* __restrict__ x
   }
 }
 
-// out[2] = Re(x^H y1), out[3] = Re(x^H y2)
-__global__ void __launch_bounds__(1024) k_dots(int K, const zd* __restrict__ x, const zd* __restrict__ y1,
-                                               const zd* __restrict__ y2, double* __restrict__ out) {
-  __shared__ double sh[1024];
-  double a = 0.0, b = 0.0;
-  for (int i = threadIdx.x; i < K; i += blockDim.x) {
-    a += x[i].x * y1[i].x + x[i].y * y1[i].y;
-    b += x[i].x * y2[i].x + x[i].y * y2[i].y;
-  }
-  a = block_sum(a, sh);
-  b = block_sum(b, sh);
-  if (threadIdx.x == 0) {
-    out[2] = a;
-    out[3] = b;
-  }
-}
-
 __global__ void k_scale(int K, zd* __restrict__ x, const double* __restrict__ t) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < K) x[i] = x[i] * (1.0 / sqrt(*t));
@@ -984,14 +985,91 @@ int invit_iteration(const Ctx& c, Factor& f, const zd* v, zd* x, hipStream_t s) 
 }
 
 // Rayleigh quotient pieces: scal[2] = x^H S x, scal[3] = x^H (H - sigma S) x.
+// Quadratic forms of the Rayleigh quotient (VMMHMV, src/linalg.f90:
+// 1180-1255) from the lower triangles of S and M = H - sigma S in one pass:
+// x^H A x = sum_i A_ii |x_i|^2 + 2 Re sum_{i>j} conj(x_i) A_ij x_j.
+// Workgroup = 64 rows x QF_CHUNK columns (below the diagonal); each lane
+// sums A_ij x_j along its row, then Re(conj(x_i) .) is reduced to two
+// doubles per workgroup; k_quad_finish adds those and the diagonal in a
+// fixed order: out[2] = x^H S x, out[3] = x^H M x.
+constexpr int QF_CHUNK = 128;
+
+__global__ void __launch_bounds__(256) k_quad_part(int K, int ld, const zd* __restrict__ H,
+                                                   const zd* __restrict__ S, double sigma,
+                                                   const zd* __restrict__ x, double* __restrict__ part) {
+  __shared__ double red[2][4];
+  const int rb = blockIdx.x, ch = blockIdx.y, nch = gridDim.y;
+  double* out = part + 2 * ((size_t)rb * nch + ch);
+  const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int i = rb * 64 + r;
+  const int c0 = ch * QF_CHUNK;
+  if (c0 > rb * 64 + 63) {   // chunk entirely above the block's rows
+    if (threadIdx.x == 0) out[0] = out[1] = 0.0;
+    return;
+  }
+  const int c1 = min(min(K, c0 + QF_CHUNK), rb * 64 + 64);
+  zd ts = mkz(0.0, 0.0), tm = mkz(0.0, 0.0);
+  if (i < K)
+    for (int j = c0 + g; j < c1; j += 4)
+      if (j < i) {
+        const size_t o = (size_t)j * ld + i;
+        const zd sv = S[o], hv = H[o], xj = x[j];
+        const zd mv = mkz(fma(-sigma, sv.x, hv.x), fma(-sigma, sv.y, hv.y));
+        ts = zfma(sv, xj, ts);
+        tm = zfma(mv, xj, tm);
+      }
+  double qs = 0.0, qm = 0.0;
+  if (i < K) {
+    const zd xi = x[i];
+    qs = 2.0 * (xi.x * ts.x + xi.y * ts.y);
+    qm = 2.0 * (xi.x * tm.x + xi.y * tm.y);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    qs += __shfl_xor(qs, o);
+    qm += __shfl_xor(qm, o);
+  }
+  if (r == 0) {
+    red[0][g] = qs;
+    red[1][g] = qm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[0] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    out[1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_quad_finish(int K, int ld, const zd* __restrict__ H,
+                                                      const zd* __restrict__ S, double sigma,
+                                                      const zd* __restrict__ x, int nparts,
+                                                      const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double sh[1024];
+  double qs = 0.0, qm = 0.0;
+  for (int i = threadIdx.x; i < K; i += blockDim.x) {
+    const size_t o = (size_t)i * ld + i;
+    const double a2 = x[i].x * x[i].x + x[i].y * x[i].y;
+    qs = fma(S[o].x, a2, qs);
+    qm = fma(fma(-sigma, S[o].x, H[o].x), a2, qm);
+  }
+  for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
+    qs += part[2 * p];
+    qm += part[2 * p + 1];
+  }
+  qs = block_sum(qs, sh);
+  qm = block_sum(qm, sh);
+  if (threadIdx.x == 0) {
+    out[2] = qs;
+    out[3] = qm;
+  }
+}
+
+// scal[2] = x^H S x, scal[3] = x^H (H - sigma S) x; partials in f.W.
 int rayleigh(const Ctx& c, Factor& f, const zd* x, double sigma, hipStream_t s) {
-  zd* y1 = f.vec + c.ld;
-  zd* y2 = f.vec + 2 * (size_t)c.ld;
-  int rc = gemv(c.S, nullptr, 0.0, c.ld, c.K, x, y1, f.W, s);
-  if (rc) return rc;
-  rc = gemv(c.H, c.S, sigma, c.ld, c.K, x, y2, f.W, s);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_dots, dim3(1), dim3(1024), 0, s, c.K, x, y1, y2, f.scal);
+  const int nrb = (c.K + 63) / 64, nch = (c.K + QF_CHUNK - 1) / QF_CHUNK;
+  double* part = reinterpret_cast<double*>(f.W);
+  hipLaunchKernelGGL(k_quad_part, dim3(nrb, nch), dim3(256), 0, s, c.K, c.ld, c.H, c.S, sigma, x, part);
+  hipLaunchKernelGGL(k_quad_finish, dim3(1), dim3(1024), 0, s, c.K, c.ld, c.H, c.S, sigma, x, nrb * nch, part,
+                     f.scal);
   return (int)hipGetLastError();
 }
 
