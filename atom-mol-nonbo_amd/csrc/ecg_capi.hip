@@ -319,7 +319,7 @@ __global__ void k_fill_ones(int K, zd* v) {
 // decreasing below 1e-9 (rounding floor), or after maxiter iterations.
 // Result vector (normalised x^H S x = 1) in x->ev[2].
 int inverse_iteration(ecg_ctx* x, double sigma, double tol, int maxiter, bool warm, InvitResult& r,
-                      bool stop_at_floor) {
+                      bool stop_at_floor, int* fst = nullptr) {
   ecg::Ctx& c = x->c;
   const int K = c.K;
   zd* v = x->ev[0];
@@ -340,8 +340,15 @@ int inverse_iteration(ecg_ctx* x, double sigma, double tol, int maxiter, bool wa
       continue;
     }
     double sc[5];
+    int fs[3];
     HIPCHK(hipMemcpyAsync(sc, x->f.scal, sizeof(sc), hipMemcpyDeviceToHost, x->stream));
+    if (fst) HIPCHK(hipMemcpyAsync(fs, x->f.status, sizeof(fs), hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
+    if (fst) {   // the factorisation's status, read here instead of right after it
+      for (int i = 0; i < 3; ++i) fst[i] = fs[i];
+      if (fs[2]) return ECG_ERR_SYNC;
+      if (fs[1]) return 0;   // zero pivot: the caller reports it
+    }
     r.nd = sc[0];
     if (stop_at_floor && it == 1) prev = sc[4];   // the first iteration's measure
     r.iters = it + 1;
@@ -363,6 +370,10 @@ int inverse_iteration(ecg_ctx* x, double sigma, double tol, int maxiter, bool wa
   HIPCHK(hipMemcpyAsync(x->ev[2], v, sizeof(zd) * K, hipMemcpyDeviceToDevice, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   if (st[2]) return ECG_ERR_SYNC;   // a device-side wait (factor or solve) hit its limit
+  if (fst) {
+    for (int i = 0; i < 3; ++i) fst[i] = st[i];
+    if (st[1]) return 0;
+  }
   r.t1 = q[2];
   r.lambda = q[3] / q[2] + sigma;
   x->have_last = true;
@@ -383,13 +394,20 @@ int solve_root(ecg_ctx* x, int which, double sigma, double tol, int maxiter, dou
   InvitResult r;
   *status = 0;
   if (std::isfinite(sigma)) {
-    RCCHK(factor_at(x, sigma, &nneg, &zero));
+    // factorisation queued without a host round trip; its status is read
+    // at the inverse iteration's first synchronisation
+    RCCHK(ecg::form_shifted(x->c, x->f, sigma, x->stream));
+    RCCHK(ecg::ldlh_factor(x->f, x->stream));
     ++nfac;
-    if (zero) {
+    int fst[3] = {0, 0, 0};
+    const int rc = inverse_iteration(x, sigma, tol, maxiter, true, r, true, fst);
+    if (rc == ECG_ERR_SYNC) return rc;
+    if (fst[1]) {
       *status = ECG_SINGULAR;
       return 0;
     }
-    RCCHK(inverse_iteration(x, sigma, tol, maxiter, true, r, true));
+    RCCHK(rc);
+    nneg = fst[0];
   } else {
     std::vector<double> dh(K);
     HIPCHK(hipMemcpyAsync(dh.data(), c.diagH, sizeof(double) * K, hipMemcpyDeviceToHost, x->stream));

@@ -109,7 +109,7 @@ def kernel_table(system, base, K, trace, nsteps, flops_adj):
             w = float(np.mean(upd))
             r.update(bound="mfma", work=w, unit="TFLOP/s", achieved=w / avg / 1e12, peak=FP64_PEAK_TFLOPS)
         elif name in ("trsv_fwd", "trsv_bwd"):
-            w = 16.0 * K * (K + 1) / 2     # the triangle of L streamed once
+            w = 16.0 * K * (K + 1) / 2     # the strict upper triangle (L^-H) streamed once
             r.update(bound="hbm", work=w, unit="GB/s", achieved=w / avg / 1e9, peak=HBM_PEAK_GBS)
         elif name == "offdiag":
             f0 = np.mean([__import__("oracle").oracle.flops_per_pair(system, base[0], base[1], 0, term=t)
@@ -417,11 +417,12 @@ def main():
     ctx.trace(False)
     rows, dominant, _ = kernel_table(system, base, K, trace, ntr, flops_adj)
     dk = rows[dominant]
-    names = {"ldl_update": "k_ldl_update (trailing update + look-ahead diagonal block + next panel)",
-             "adjoint": "k_adjoint<3> (fused E+grad pair pass)", "trsv_fwd": "k_trsv_fwd", "trsv_bwd": "k_trsv_bwd",
+    names = {"ldl_update": "k_ldl_update (trailing update + look-ahead diagonal block + next panel + L^-1 tiles)",
+             "adjoint": "k_adjoint<3> (fused E+grad pair pass)",
+             "trsv_fwd": "k_xsolve_fwd (z = D^-1 L^-1 b)", "trsv_bwd": "k_xsolve_bwd + k_xsolve_reduce (x = L^-H z)",
              "offdiag": "k_offdiag<3>"}
-    rocname = {"ldl_update": "k_ldl_update", "adjoint": "k_adjoint<3>", "trsv_fwd": "k_trsv_fwd",
-               "trsv_bwd": "k_trsv_bwd", "offdiag": "k_offdiag<3>"}
+    rocname = {"ldl_update": "k_ldl_update", "adjoint": "k_adjoint<3>", "trsv_fwd": "k_xsolve_fwd",
+               "trsv_bwd": "k_xsolve_bwd", "offdiag": "k_offdiag<3>"}
     out = {
         "metric": "FP64 pair H,S+grad elements/s (E+grad, K=2000, n=3)",
         "value": npairs / s_per,
