@@ -718,6 +718,21 @@ int ecg_set_params(ecg_ctx* x, int K, const double* params) {
   return ECG_OK;
 }
 
+int ecg_set_params_device(ecg_ctx* x, int K, const double* dparams) {
+  (void)hipGetLastError();   // start from a clean runtime error state
+  if (!x || K < 1 || !dparams) return ECG_ERR_ARG;
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, dparams) != hipSuccess || attr.type != hipMemoryTypeDevice) {
+    (void)hipGetLastError();
+    return ECG_ERR_ARG;
+  }
+  x->s_ready = false;
+  RCCHK(ensure_capacity(x, K));
+  HIPCHK(hipMemcpyAsync(x->c.params, dparams, sizeof(double) * x->npt * K, hipMemcpyDeviceToDevice, x->stream));
+  RCCHK(ecg::prep(x->c, x->stream));
+  return ECG_OK;
+}
+
 int ecg_update_params(ecg_ctx* x, int first, int count, const double* params) {
   if (!x || !params || first < 1 || count < 1 || first - 1 + count > x->c.K) return ECG_ERR_ARG;
   if (!finite_all(params, (size_t)x->npt * count)) return ECG_ERR_ARG;

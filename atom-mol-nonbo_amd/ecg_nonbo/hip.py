@@ -44,7 +44,7 @@ EXPORTS = [
     "ecg_compute_mat_elem", "ecg_compute_mat_elem_and_deriv", "ecg_ghepiis", "ecg_energy", "ecg_energy_grad",
     "ecg_energy_grad_stored", "ecg_batch_create", "ecg_batch_destroy", "ecg_batch_set_config",
     "ecg_batch_energy_grad", "ecg_trace", "ecg_trace_read", "ecg_get_params", "ecg_screen_prepare", "ecg_screen", "ecg_screen_loop", "ecg_last_timings",
-    "ecg_set_overlap_penalty", "ecg_overlap_penalty",
+    "ecg_set_overlap_penalty", "ecg_overlap_penalty", "ecg_set_params_device",
 ]
 
 
@@ -93,6 +93,7 @@ def load(path: str = LIB_PATH):
     L.ecg_screen_loop.argtypes = [vp, c_int, c_int, _d, c_int, _d, _i]
     L.ecg_last_timings.argtypes = [vp, _d, c_int]
     L.ecg_set_overlap_penalty.argtypes = [vp, c_int, c_double, c_double, c_int]
+    L.ecg_set_params_device.argtypes = [vp, c_int, vp]
     L.ecg_overlap_penalty.argtypes = [vp, c_int, c_double, c_double, _d, _d]
     for name in EXPORTS:
         getattr(L, name).restype = c_int
@@ -147,6 +148,12 @@ class EcgContext:
         K = p.shape[0]
         _check("ecg_set_params", load().ecg_set_params(self._h, K, _p(p)))
         self.K = K
+
+    def set_params_device(self, K, dptr):
+        """Parameters of K functions from device memory (an integer device
+        address, e.g. a torch tensor's data_ptr())."""
+        _check("ecg_set_params_device", load().ecg_set_params_device(self._h, int(K), ctypes.c_void_p(int(dptr))))
+        self.K = int(K)
 
     def update_params(self, first, params):
         p = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)

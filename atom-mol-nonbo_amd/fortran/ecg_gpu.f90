@@ -51,6 +51,12 @@ module ecg_capi
       integer(c_int), value :: K
       real(c_double), intent(in) :: params(*)
     end function
+    integer(c_int) function ecg_set_params_device(ctx, K, dparams) bind(C, name="ecg_set_params_device")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: K
+      type(c_ptr), value :: dparams   ! device address
+    end function
     integer(c_int) function ecg_update_params(ctx, first, count, params) bind(C, name="ecg_update_params")
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx

@@ -62,6 +62,15 @@ int ecg_ctx_destroy(ecg_ctx* ctx);
  * src/globvars.f90:141-146).  Functions first..first+count-1 (1-based) are
  * replaced when first > 1 (the rest stay resident). */
 int ecg_set_params(ecg_ctx* ctx, int K, const double* params);
+/* The same from device memory (dparams: npt x K doubles in device memory of
+ * the context's GPU, for callers that keep the parameters resident).  The
+ * copy is queued on the context's stream without waiting: dparams must hold
+ * its values when the call is made (work that produced them completed or
+ * synchronised) and stay unchanged until the next energy or assembly call on
+ * ctx has returned.  Values are not checked for finiteness here; non-finite
+ * parameters surface as ECG_ERR_NONFINITE from that call.  No counterpart in
+ * the reference, whose parameters live in host memory. */
+int ecg_set_params_device(ecg_ctx* ctx, int K, const double* dparams);
 int ecg_update_params(ecg_ctx* ctx, int first, int count, const double* params);
 
 /* One pair and one symmetry matrix P on the device: MatrixElements
