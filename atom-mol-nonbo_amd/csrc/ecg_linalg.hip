@@ -70,9 +70,8 @@ __device__ inline void wait_wg(const int* f1, const int* f2, int epoch, int* sta
 //   B. the sub-panel below: L = A X_ss^H D^{-1}, one 16-row tile per wave
 //      on MFMA;
 //   C. the trailing update A -= (L D) L^H on 16 x 16 lower tiles (MFMA).
-// Then X = L^{-1} is completed by 2 x 2 block recursion on MFMA tiles:
-//   X_10 = -X_11 (L_10 X_00), X_32 = -X_33 (L_32 X_22), then the 32 x 32
-//   block X_21 = -X_22 (L_21 X_11).
+// The off-diagonal blocks of X = L^{-1} are computed in the MFMA slots
+// that B and C leave idle (x_side_job), with two closing rounds.
 // LDS use: lower triangle = the block, then L (the pivots on the diagonal
 // while factoring); strict upper triangle = X transposed (X(i, j), i > j,
 // at A[j][i]; X has a unit diagonal).
@@ -176,6 +175,73 @@ __device__ __attribute__((always_inline)) inline void x_block_finish(const DiagL
   const int ti = t * SB, si = s * SB;
   mfma16([&](int r, int k) { return x_at(L, ti + r, ti + k); },
          [&](int k, int c) { return L.A[si + c][ti + k]; }, acc);
+}
+
+// Side jobs computing the off-diagonal 16 x 16 blocks of X = L^{-1} in the
+// MFMA slots that phases B and C leave idle, then two closing rounds:
+//   X_ts = -X_tt T_ts,  T_ts = sum_{u=s}^{t-1} L_tu X_us,
+// T staged (transposed) where X_ts goes.  Per phase each job writes its
+// own block and reads only blocks completed in earlier phases.
+//   phase C, s=0: wave 2 T10 = L10 X00;  wave 3 T20 = L20 X00
+//   phase B, s=1: wave 2 X10;            wave 3 T30 = L30 X00
+//   phase C, s=1: wave 3 T21 = L21 X11
+//   phase B, s=2: wave 1 T20 += L21 X10; wave 2 T31 = L31 X11; wave 3 T30 += L31 X10
+//   phase C, s=2: wave 1 X21; wave 2 X20; wave 3 T32 = L32 X22
+//   closing 1:    wave 0 X32; wave 1 T31 += L32 X21; wave 2 T30 += L32 X20
+//   closing 2:    wave 1 X31; wave 2 X30
+// Jobs of blocks t >= nsub (a short last diagonal block) are skipped.
+struct XJob {
+  int kind;   // 0 none, 1 T (+)= L_tu X_us, 2 X_ts = -X_tt T_ts
+  int t, s, u;
+  bool acc;
+};
+__device__ inline XJob x_side_job(int phase, int s, int wave) {
+  // phase 0: B, 1: C, 2: closing 1, 3: closing 2
+  switch (phase * 16 + s * 4 + wave) {
+    case 1 * 16 + 0 * 4 + 2: return {1, 1, 0, 0, false};
+    case 1 * 16 + 0 * 4 + 3: return {1, 2, 0, 0, false};
+    case 0 * 16 + 1 * 4 + 2: return {2, 1, 0, 0, false};
+    case 0 * 16 + 1 * 4 + 3: return {1, 3, 0, 0, false};
+    case 1 * 16 + 1 * 4 + 3: return {1, 2, 1, 1, false};
+    case 0 * 16 + 2 * 4 + 1: return {1, 2, 0, 1, true};
+    case 0 * 16 + 2 * 4 + 2: return {1, 3, 1, 1, false};
+    case 0 * 16 + 2 * 4 + 3: return {1, 3, 0, 1, true};
+    case 1 * 16 + 2 * 4 + 1: return {2, 2, 1, 0, false};
+    case 1 * 16 + 2 * 4 + 2: return {2, 2, 0, 0, false};
+    case 1 * 16 + 2 * 4 + 3: return {1, 3, 2, 2, false};
+    case 2 * 16 + 0: return {2, 3, 2, 0, false};
+    case 2 * 16 + 1: return {1, 3, 1, 2, true};
+    case 2 * 16 + 2: return {1, 3, 0, 2, true};
+    case 3 * 16 + 1: return {2, 3, 1, 0, false};
+    case 3 * 16 + 2: return {2, 3, 0, 0, false};
+  }
+  return {0, 0, 0, 0, false};
+}
+__device__ __attribute__((always_inline)) inline void x_side_run(DiagLds& L, int nsub, const XJob j) {
+  if (j.kind == 0 || j.t >= nsub) return;
+  const int ti = j.t * SB, si = j.s * SB;
+  Acc16 acc;
+  if (j.kind == 1) {
+    const int ui = j.u * SB;
+    if (j.acc) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const zd o = L.A[si + acc16_col()][ti + acc16_row(v)];
+        acc.re[v] = o.x;
+        acc.im[v] = o.y;
+      }
+    } else {
+      acc16_zero(acc);
+    }
+    mfma16([&](int r, int k) { return L.A[ti + r][ui + k]; },
+           [&](int k, int c) { return x_at(L, ui + k, si + c); }, acc);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) L.A[si + acc16_col()][ti + acc16_row(v)] = mkz(acc.re[v], acc.im[v]);
+  } else {
+    x_block_finish(L, j.t, j.s, acc);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) L.A[si + acc16_col()][ti + acc16_row(v)] = mkz(-acc.re[v], -acc.im[v]);
+  }
 }
 
 __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int ld, int j0, zd* __restrict__ F,
@@ -291,7 +357,10 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
     const int rows = jb - c - sb;
     const int nt = (rows + SB - 1) / SB;
     // ---- B: sub-panel L(r, c+j) = sum_{k <= j} A(r, c+k) conj(X_ss(j, k)) / d_j
-    if (wave < nt) {
+    // (waves without a row tile take their X side job)
+    if (wave >= nt) {
+      x_side_run(L, nsub, x_side_job(0, s, wave));
+    } else {
       const int r0 = c + SB * (wave + 1);
       Acc16 acc;
       acc16_zero(acc);
@@ -323,63 +392,13 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
         }
       }
     }
+    x_side_run(L, nsub, x_side_job(1, s, wave));
     __syncthreads();
   }
-  // ---- X = L^{-1}: 2 x 2 block recursion.  T is staged in X's own place.
-  // level 1: X_10 (wave 0) and X_32 (wave 1)
-  {
-    const int t = 2 * wave + 1, s = 2 * wave;
-    const bool act = (wave < 2 && t < nsub);
-    Acc16 acc;
-    if (act) {
-      acc16_zero(acc);
-      mfma16([&](int r, int k) { return L.A[t * SB + r][s * SB + k]; },
-             [&](int k, int cc) { return x_at(L, s * SB + k, s * SB + cc); }, acc);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) L.A[s * SB + acc16_col()][t * SB + acc16_row(v)] = mkz(acc.re[v], acc.im[v]);
-    }
-    wave_sync();
-    if (act) x_block_finish(L, t, s, acc);
-    wave_sync();
-    if (act) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v)
-        L.A[s * SB + acc16_col()][t * SB + acc16_row(v)] = mkz(-acc.re[v], -acc.im[v]);
-    }
-  }
+  // ---- X = L^{-1}: the two closing rounds of the side-job schedule
+  x_side_run(L, nsub, x_side_job(2, 0, wave));
   __syncthreads();
-  // level 2: X_ts, t in {2, 3}, s in {0, 1}; wave = 2 (t - 2) + s
-  {
-    const int t = 2 + (wave >> 1), s = wave & 1;
-    const bool act = (t < nsub);
-    Acc16 acc;
-    if (act) {
-      // T_ts = sum_{u = s}^{1} L_tu X_us
-      acc16_zero(acc);
-      for (int u = s; u < 2; ++u)
-        mfma16([&](int r, int k) { return L.A[t * SB + r][u * SB + k]; },
-               [&](int k, int cc) { return x_at(L, u * SB + k, s * SB + cc); }, acc);
-    }
-    __syncthreads();
-    if (act) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) L.A[s * SB + acc16_col()][t * SB + acc16_row(v)] = mkz(acc.re[v], acc.im[v]);
-    }
-    __syncthreads();
-    if (act) {
-      // X_ts = -sum_{u = 2}^{t} X_tu T_us
-      acc16_zero(acc);
-      for (int u = 2; u <= t; ++u)
-        mfma16([&](int r, int k) { return x_at(L, t * SB + r, u * SB + k); },
-               [&](int k, int cc) { return L.A[s * SB + cc][u * SB + k]; }, acc);
-    }
-    __syncthreads();
-    if (act) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v)
-        L.A[s * SB + acc16_col()][t * SB + acc16_row(v)] = mkz(-acc.re[v], -acc.im[v]);
-    }
-  }
+  x_side_run(L, nsub, x_side_job(3, 0, wave));
   __syncthreads();
   // ---- outputs: X and D first (all the next panel needs), then the flag
   // (look-ahead use), then X^H (also into F's strict upper triangle) and L
