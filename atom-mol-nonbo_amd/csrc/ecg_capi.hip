@@ -317,7 +317,10 @@ __global__ void k_fill_ones(int K, zd* v) {
 // src/linalg.f90:1892-1917).  Start vector in x->ev[0] unless warm.
 // Stops when the GHEPIIS measure reaches tol, or when it has stopped
 // decreasing below 1e-9 (rounding floor), or after maxiter iterations.
-// Result vector (normalised x^H S x = 1) in x->ev[2].
+// With stop_at_floor (the energy paths) the first three iterations, the
+// Rayleigh quotient and the normalisation are queued with a single host
+// synchronisation; further iterations only run if the measure asks for
+// them.  Result vector (normalised x^H S x = 1) in x->ev[2].
 int inverse_iteration(ecg_ctx* x, double sigma, double tol, int maxiter, bool warm, InvitResult& r,
                       bool stop_at_floor, int* fst = nullptr) {
   ecg::Ctx& c = x->c;
@@ -330,49 +333,75 @@ int inverse_iteration(ecg_ctx* x, double sigma, double tol, int maxiter, bool wa
     hipLaunchKernelGGL(k_fill_ones, dim3((K + 255) / 256), dim3(256), 0, x->stream, K, v);
   double prev = std::numeric_limits<double>::infinity();
   r = InvitResult();
-  for (int it = 0; it < maxiter; ++it) {
-    RCCHK(ecg::invit_iteration(c, x->f, v, y, x->stream));
-    if (stop_at_floor && it == 0 && maxiter > 1) {
-      // energy paths: the first measure is never the last one (the second
-      // iteration is queued without a host round trip)
-      r.iters = 1;
-      std::swap(v, y);
-      continue;
-    }
-    double sc[5];
-    int fs[3];
-    HIPCHK(hipMemcpyAsync(sc, x->f.scal, sizeof(sc), hipMemcpyDeviceToHost, x->stream));
-    if (fst) HIPCHK(hipMemcpyAsync(fs, x->f.status, sizeof(fs), hipMemcpyDeviceToHost, x->stream));
+  // tail: Rayleigh quotient, normalisation, result copy, host read of the
+  // scalars and the status (one synchronisation)
+  double q[5];
+  int st[4];
+  auto tail = [&]() -> int {
+    RCCHK(ecg::rayleigh(c, x->f, v, sigma, x->stream));
+    RCCHK(ecg::normalise(v, K, x->f.scal + 2, x->stream));
+    HIPCHK(hipMemcpyAsync(q, x->f.scal, sizeof(q), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipMemcpyAsync(st, x->f.status, sizeof(st), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipMemcpyAsync(x->ev[2], v, sizeof(zd) * K, hipMemcpyDeviceToDevice, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
-    if (fst) {   // the factorisation's status, read here instead of right after it
-      for (int i = 0; i < 3; ++i) fst[i] = fs[i];
-      if (fs[2]) return ECG_ERR_SYNC;
-      if (fs[1]) return 0;   // zero pivot: the caller reports it
+    if (st[2]) return ECG_ERR_SYNC;   // a device-side wait (factor or solve) hit its limit
+    if (fst)
+      for (int i = 0; i < 3; ++i) fst[i] = st[i];
+    return 0;
+  };
+  int it = 0;
+  bool done = false;
+  if (stop_at_floor) {
+    // energy paths: the first (up to) three iterations and the tail are
+    // queued without a host round trip; the loop below continues only when
+    // the measure asks for more iterations
+    const int nq = std::min(maxiter, 3);
+    for (; it < nq; ++it) {
+      RCCHK(ecg::invit_iteration(c, x->f, v, y, x->stream));
+      std::swap(v, y);
     }
-    r.nd = sc[0];
-    if (stop_at_floor && it == 1) prev = sc[4];   // the first iteration's measure
-    r.iters = it + 1;
-    std::swap(v, y);   // v now holds the newest iterate
+    RCCHK(tail());
+    if (fst && st[1]) return 0;   // zero pivot: the caller reports it
+    r.iters = nq;
+    r.nd = q[0];
     if (!std::isfinite(r.nd)) return ECG_ERR_NONFINITE;
-    if (tol > 0.0 ? (r.nd <= tol) : false) {
+    const double p1 = (nq >= 2) ? q[4] : std::numeric_limits<double>::infinity();   // previous measure
+    if (tol > 0.0 && r.nd <= tol) {
       r.converged = true;
-      break;
+      done = true;
+    } else if (nq >= 2 && r.nd < 1e-9 && r.nd > 0.5 * p1) {
+      done = true;
+    } else if (nq == maxiter) {
+      done = true;
     }
-    if (stop_at_floor && it >= 1 && r.nd < 1e-9 && r.nd > 0.5 * prev) break;
     prev = r.nd;
   }
-  RCCHK(ecg::rayleigh(c, x->f, v, sigma, x->stream));
-  RCCHK(ecg::normalise(v, K, x->f.scal + 2, x->stream));
-  double q[4];
-  int st[4];
-  HIPCHK(hipMemcpyAsync(q, x->f.scal, sizeof(q), hipMemcpyDeviceToHost, x->stream));
-  HIPCHK(hipMemcpyAsync(st, x->f.status, sizeof(st), hipMemcpyDeviceToHost, x->stream));
-  HIPCHK(hipMemcpyAsync(x->ev[2], v, sizeof(zd) * K, hipMemcpyDeviceToDevice, x->stream));
-  HIPCHK(hipStreamSynchronize(x->stream));
-  if (st[2]) return ECG_ERR_SYNC;   // a device-side wait (factor or solve) hit its limit
-  if (fst) {
-    for (int i = 0; i < 3; ++i) fst[i] = st[i];
-    if (st[1]) return 0;
+  if (!done) {
+    for (; it < maxiter; ++it) {
+      RCCHK(ecg::invit_iteration(c, x->f, v, y, x->stream));
+      double sc[5];
+      int fs[3];
+      HIPCHK(hipMemcpyAsync(sc, x->f.scal, sizeof(sc), hipMemcpyDeviceToHost, x->stream));
+      if (fst) HIPCHK(hipMemcpyAsync(fs, x->f.status, sizeof(fs), hipMemcpyDeviceToHost, x->stream));
+      HIPCHK(hipStreamSynchronize(x->stream));
+      if (fst) {   // the factorisation's status, read here instead of right after it
+        for (int i = 0; i < 3; ++i) fst[i] = fs[i];
+        if (fs[2]) return ECG_ERR_SYNC;
+        if (fs[1]) return 0;   // zero pivot: the caller reports it
+      }
+      r.nd = sc[0];
+      r.iters = it + 1;
+      std::swap(v, y);   // v now holds the newest iterate
+      if (!std::isfinite(r.nd)) return ECG_ERR_NONFINITE;
+      if (tol > 0.0 ? (r.nd <= tol) : false) {
+        r.converged = true;
+        break;
+      }
+      if (stop_at_floor && it >= 1 && r.nd < 1e-9 && r.nd > 0.5 * prev) break;
+      prev = r.nd;
+    }
+    RCCHK(tail());
+    if (fst && st[1]) return 0;
   }
   r.t1 = q[2];
   r.lambda = q[3] / q[2] + sigma;

@@ -430,6 +430,7 @@ def main():
     off_ms = []
     eig_ms = []
     iters = []
+    lines = []
     dsets.sync()
     t_all0 = time.perf_counter()
     for i in range(args.steps):
@@ -441,10 +442,11 @@ def main():
         off_ms.append(tm[5])
         eig_ms.append(tm[1])
         iters.append(inf[0])
-        print("# step %d E=%.15f |g|=%.3e ms=%.3f (assembly %.3f eigen %.3f adjoint %.3f)"
-              % (i, e, np.linalg.norm(g), 1e3 * times[-1], tm[0], tm[1], tm[2]), file=sys.stderr, flush=True)
+        lines.append("# step %d E=%.15f |g|=%.3e ms=%.3f (assembly %.3f eigen %.3f adjoint %.3f)"
+                     % (i, e, np.linalg.norm(g), 1e3 * times[-1], tm[0], tm[1], tm[2]))
     dsets.sync()
     t_all = time.perf_counter() - t_all0
+    print("\n".join(lines), file=sys.stderr, flush=True)
     s_per = t_all / args.steps
     npairs = K * (K + 1) // 2
     f0, f1, f3, flops_adj = algorithmic_flops(system, base, K)
