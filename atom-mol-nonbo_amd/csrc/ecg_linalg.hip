@@ -652,12 +652,14 @@ __global__ void __launch_bounds__(256) k_ldl_xlast(int K, int ld, zd* __restrict
 //   workgroups 0..ntiles-1: lower block pairs (I >= Kb) of the trailing
 //     matrix, C_IK -= W_I L_K^H.  The one that updates the first trailing
 //     diagonal block then factors it (look-ahead).
-//   workgroups ntiles..ntiles+nstrip-1: 16-row strips of the next block
-//     column's panel.  Each waits until the tile holding its rows and the
+//   workgroups ntiles..ntiles+nx-1: block tiles of X = L^{-1} (xinv_tile):
+//     the col tiles of block row col, then rows col+1.. (no waits).  They
+//     come before the strips so that they are dispatched while the strips
+//     would only wait.
+//   workgroups ntiles+nx..: 16-row strips of the next block column's
+//     panel.  Each waits until the tile holding its rows and the
 //     look-ahead factor are published, then runs the panel solve for its
 //     strip.
-//   workgroups ntiles+nstrip..: block tiles of X = L^{-1} (xinv_tile): the
-//     j (< col) tiles of block row col, then rows col+1.. (no waits).
 // Publication: the producer's threads fence at agent scope, the workgroup
 // synchronises, thread 0 stores the flag (release).  A waiting workgroup's
 // thread 0 polls the flags (relaxed, bounded in time: past the limit it
@@ -676,15 +678,16 @@ __global__ void __launch_bounds__(256) k_ldl_update(int K, int ld, int j0, zd* _
   const int jb = min(NB, K - j0);
   const int t0 = (j0 + jb) / NB;
   const int b = blockIdx.x;
-  if (b >= ntiles + nstrip) {
-    const int k = j0 / NB, x = b - ntiles - nstrip;
+  const int nx = (int)gridDim.x - ntiles - nstrip;   // tiles of L^{-1}
+  if (b >= ntiles && b < ntiles + nx) {
+    const int k = j0 / NB, x = b - ntiles;
     const int i = (x < k) ? k : k + 1 + (x - k) / k;
     const int c = (x < k) ? x : (x - k) % k;
     xinv_tile(K, ld, k, i, c, F, Dinv, Ld);
     return;
   }
   if (b >= ntiles) {
-    const int i0 = (t0 + 1) * NB + (b - ntiles) * SB;
+    const int i0 = (t0 + 1) * NB + (b - ntiles - nx) * SB;
     const int bi = (i0 - t0 * NB) / NB;
     wait_wg(&pflag[0], &pflag[1 + bi], epoch, status);
     panel_strip(K, ld, t0 * NB, i0, F, D, Dinv, Wnext);
