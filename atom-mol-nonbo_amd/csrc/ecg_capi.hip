@@ -58,6 +58,16 @@ struct ecg_ctx {
   size_t pen_cap = 0;
   double pen_total = 0.0;     // host copy of the last total
   bool s_ready = false;       // the device S is assembled from the current parameters
+  // Incremental recomputation.  hs_rows leading rows of the device H, S
+  // match the current parameters; fac_rows leading rows of the factor
+  // (L, D, X^H in f) belong to the device H - fac_sigma S.  Any write that
+  // may change rows >= r of H or S lowers fac_rows to r.  hparams is the
+  // host copy of the parameters last uploaded (empty when unknown).
+  std::vector<double> hparams;
+  int hs_rows = 0;
+  int fac_rows = 0;
+  double fac_sigma = 0.0;
+  int last_tail = -1;         // rows of the last tail update, -1 after a full factorisation
 };
 
 // A packed batch of independent problems (one context each).
@@ -112,39 +122,97 @@ int penalty_enqueue(ecg_ctx* x, hipStream_t s, int nfru, double b, double t2, do
   return 0;
 }
 
+// Rows >= row of the device H, S no longer match the parameters.
+void rows_changed(ecg_ctx* x, int row) { x->hs_rows = std::min(x->hs_rows, row); }
+
+// H, S rows >= row are about to be (re)written with values that may differ.
+void hs_rewritten(ecg_ctx* x, int row) { x->fac_rows = std::min(x->fac_rows, row); }
+
+// First function whose parameters differ from the last upload (the
+// shorter length when one list extends the other).
+int first_changed(const ecg_ctx* x, int K, const double* params) {
+  const size_t npt = (size_t)x->npt;
+  const int n = std::min(K, (int)(x->hparams.size() / npt));
+  for (int j = 0; j < n; ++j)
+    if (std::memcmp(x->hparams.data() + (size_t)j * npt, params + (size_t)j * npt, npt * sizeof(double)) != 0)
+      return j;
+  return n;
+}
+
 int ensure_capacity(ecg_ctx* x, int K) {
   ecg::Ctx& c = x->c;
   if (K <= c.cap) {
     c.K = K;
     x->f.K = K;
+    x->hs_rows = std::min(x->hs_rows, K);
+    x->fac_rows = std::min(x->fac_rows, K);
     return 0;
   }
-  const int cap = std::max(K, 64);
-  const int ld = (cap + 63) / 64 * 64;
-  RCCHK(dev_alloc(&c.params, (size_t)x->npt * cap));
-  RCCHK(dev_alloc(&c.bra, (size_t)ecg::bra_stride(c.n) * cap));
-  RCCHK(dev_alloc(&c.ket, (size_t)ecg::ket_stride(c.n) * c.T * cap));
-  RCCHK(dev_alloc(&c.diagS, (size_t)cap));
-  RCCHK(dev_alloc(&c.diagH, (size_t)cap));
-  RCCHK(dev_alloc(&c.H, (size_t)ld * ld));
-  RCCHK(dev_alloc(&c.S, (size_t)ld * ld));
+  // Growing: a quarter of headroom (a basis that grows one function at a
+  // time reallocates rarely), and the leading rows of H, S, diagS, diagH
+  // and of the factor that are still valid move to the new layout, so
+  // that the next call continues from them.
+  const int keep = (c.cap > 0) ? std::min(c.K, std::max(x->hs_rows, x->fac_rows)) : 0;
+  const int want = std::max(std::max(K, 64), c.cap > 0 ? c.cap + c.cap / 4 : 0);
+  const int ld = (want + 63) / 64 * 64;
+  const int cap = ld;
+  const int old_ld = c.ld;
+  zd* oH = c.H;
+  zd* oS = c.S;
+  zd* oF = x->f.F;
+  double* oD = x->f.D;
+  double* odS = c.diagS;
+  double* odH = c.diagH;
+  c.H = c.S = x->f.F = nullptr;
+  x->f.D = c.diagS = c.diagH = nullptr;
+  auto release_old = [&]() {
+    for (void* q : {(void*)oH, (void*)oS, (void*)oF, (void*)oD, (void*)odS, (void*)odH})
+      if (q) (void)hipFree(q);
+  };
+  int rc = 0;
   ecg::Factor& f = x->f;
-  RCCHK(dev_alloc(&f.F, (size_t)ld * ld));
-  RCCHK(dev_alloc(&f.D, (size_t)ld));
-  RCCHK(dev_alloc(&f.W, (size_t)ld * 2 * ecg::NB));
-  RCCHK(dev_alloc(&f.Dinv, (size_t)(ld / ecg::NB) * 2 * ecg::NB * ecg::NB));
-  RCCHK(dev_alloc(&f.status, (size_t)4));
-  RCCHK(dev_alloc(&f.vec, (size_t)4 * ld));
-  f.ld = ld;
-  RCCHK(dev_alloc(&f.tvec, (size_t)ld));
-  RCCHK(dev_alloc(&f.pflag, (size_t)(ld / ecg::NB) + 2));
-  HIPCHK(hipMemsetAsync(f.pflag, 0, sizeof(int) * ((ld / ecg::NB) + 2), x->stream));
-  HIPCHK(hipMemsetAsync(f.status, 0, sizeof(int) * 4, x->stream));
+  if (!rc) rc = dev_alloc(&c.params, (size_t)x->npt * cap);
+  if (!rc) rc = dev_alloc(&c.bra, (size_t)ecg::bra_stride(c.n) * cap);
+  if (!rc) rc = dev_alloc(&c.ket, (size_t)ecg::ket_stride(c.n) * c.T * cap);
+  if (!rc) rc = dev_alloc(&c.diagS, (size_t)cap);
+  if (!rc) rc = dev_alloc(&c.diagH, (size_t)cap);
+  if (!rc) rc = dev_alloc(&c.H, (size_t)ld * ld);
+  if (!rc) rc = dev_alloc(&c.S, (size_t)ld * ld);
+  if (!rc) rc = dev_alloc(&f.F, (size_t)ld * ld);
+  if (!rc) rc = dev_alloc(&f.D, (size_t)ld);
+  if (!rc) rc = dev_alloc(&f.W, (size_t)ld * 2 * ecg::NB);
+  if (!rc) rc = dev_alloc(&f.Dinv, (size_t)(ld / ecg::NB) * 2 * ecg::NB * ecg::NB);
+  if (!rc) rc = dev_alloc(&f.status, (size_t)4);
+  if (!rc) rc = dev_alloc(&f.vec, (size_t)4 * ld);
+  if (!rc) rc = dev_alloc(&f.tvec, (size_t)ld);
+  if (!rc) rc = dev_alloc(&f.pflag, (size_t)(ld / ecg::NB) + 2);
+  if (!rc) rc = dev_alloc(&f.tp, (size_t)0);   // tail-update partials: allocated on first use
+  if (!rc) rc = dev_alloc(&f.scal, (size_t)16);
+  for (int i = 0; i < 3 && !rc; ++i) rc = dev_alloc(&x->ev[i], (size_t)ld);
+  if (rc) {
+    release_old();
+    c.cap = c.K = 0;
+    x->hs_rows = x->fac_rows = 0;
+    x->hparams.clear();
+    return rc;
+  }
+  f.tp_cap = 0;
   f.epoch = 0;
-  RCCHK(dev_alloc(&f.scal, (size_t)16));
-  for (int i = 0; i < 3; ++i) RCCHK(dev_alloc(&x->ev[i], (size_t)ld));
-  HIPCHK(hipMemsetAsync(c.H, 0, sizeof(zd) * ld * ld, x->stream));
-  HIPCHK(hipMemsetAsync(c.S, 0, sizeof(zd) * ld * ld, x->stream));
+  bool ok = hipMemsetAsync(f.pflag, 0, sizeof(int) * ((ld / ecg::NB) + 2), x->stream) == hipSuccess &&
+            hipMemsetAsync(f.status, 0, sizeof(int) * 4, x->stream) == hipSuccess &&
+            hipMemsetAsync(c.H, 0, sizeof(zd) * ld * ld, x->stream) == hipSuccess &&
+            hipMemsetAsync(c.S, 0, sizeof(zd) * ld * ld, x->stream) == hipSuccess;
+  if (ok && keep > 0) {
+    const size_t w = sizeof(zd) * keep, sp = sizeof(zd) * old_ld, dp = sizeof(zd) * ld;
+    ok = hipMemcpy2DAsync(c.H, dp, oH, sp, w, keep, hipMemcpyDeviceToDevice, x->stream) == hipSuccess &&
+         hipMemcpy2DAsync(c.S, dp, oS, sp, w, keep, hipMemcpyDeviceToDevice, x->stream) == hipSuccess &&
+         hipMemcpy2DAsync(f.F, dp, oF, sp, w, keep, hipMemcpyDeviceToDevice, x->stream) == hipSuccess &&
+         hipMemcpyAsync(f.D, oD, sizeof(double) * keep, hipMemcpyDeviceToDevice, x->stream) == hipSuccess &&
+         hipMemcpyAsync(c.diagS, odS, sizeof(double) * keep, hipMemcpyDeviceToDevice, x->stream) == hipSuccess &&
+         hipMemcpyAsync(c.diagH, odH, sizeof(double) * keep, hipMemcpyDeviceToDevice, x->stream) == hipSuccess;
+  }
+  ok = ok && hipStreamSynchronize(x->stream) == hipSuccess;
+  release_old();
   c.cap = cap;
   c.ld = ld;
   c.K = K;
@@ -152,7 +220,10 @@ int ensure_capacity(ecg_ctx* x, int K) {
   f.ld = ld;
   f.K = K;
   x->have_last = false;
-  return 0;
+  x->hs_rows = ok ? std::min(x->hs_rows, keep) : 0;
+  x->fac_rows = ok ? std::min(x->fac_rows, keep) : 0;
+  if (!ok) x->hparams.clear();
+  return ok ? 0 : ECG_ERR_HIP;
 }
 
 int ensure_pinned(ecg_ctx* x, size_t doubles) {
@@ -291,6 +362,7 @@ int copy_out_hs(ecg_ctx* x, int r0, int r1, char method, double sigma, double* H
 // Factor H - sigma S on the device; returns the negative-pivot count and
 // the first zero pivot (0 if none).
 int factor_at(ecg_ctx* x, double sigma, int* nneg, int* zero) {
+  x->fac_rows = 0;
   RCCHK(ecg::form_shifted(x->c, x->f, sigma, x->stream));
   RCCHK(ecg::ldlh_factor(x->f, x->stream));
   int st[3];
@@ -299,6 +371,34 @@ int factor_at(ecg_ctx* x, double sigma, int* nneg, int* zero) {
   if (st[2]) return ECG_ERR_SYNC;   // a device-side wait of the factorisation hit its limit
   *nneg = st[0];
   *zero = st[1];
+  if (!st[1]) {
+    x->fac_rows = x->c.K;
+    x->fac_sigma = sigma;
+  }
+  return 0;
+}
+
+// Queue the factor of H - sigma S: from the first row that differs from
+// the last factorisation at the same sigma when at most NB rows do (the
+// reference's LDLHF from row Nmin), otherwise in full.
+int factor_queue(ecg_ctx* x, double sigma) {
+  const int K = x->c.K, m = x->fac_rows;
+  ecg::Factor& f = x->f;
+  const bool tail = m >= 1 && K - m <= ecg::NB && x->fac_sigma == sigma;
+  x->fac_rows = 0;
+  if (tail) {
+    const size_t need = ecg::tail_partial_entries(f.ld);
+    if (f.tp_cap < need) {
+      RCCHK(dev_alloc(&f.tp, need));
+      f.tp_cap = need;
+    }
+    RCCHK(ecg::ldlh_tail(x->c, f, sigma, m, x->stream));
+  } else {
+    RCCHK(ecg::form_shifted(x->c, f, sigma, x->stream));
+    RCCHK(ecg::ldlh_factor(f, x->stream));
+  }
+  x->fac_sigma = sigma;
+  x->last_tail = tail ? K - m : -1;
   return 0;
 }
 
@@ -417,6 +517,7 @@ int inverse_iteration(ecg_ctx* x, double sigma, double tol, int maxiter, bool wa
 // Rayleigh quotient.
 int solve_root(ecg_ctx* x, int which, double sigma, double tol, int maxiter, double* E, double* info,
                int* status) {
+  const double sigma_in = sigma;
   ecg::Ctx& c = x->c;
   const int K = c.K;
   int nfac = 0, nneg = 0, zero = 0;
@@ -425,8 +526,7 @@ int solve_root(ecg_ctx* x, int which, double sigma, double tol, int maxiter, dou
   if (std::isfinite(sigma)) {
     // factorisation queued without a host round trip; its status is read
     // at the inverse iteration's first synchronisation
-    RCCHK(ecg::form_shifted(x->c, x->f, sigma, x->stream));
-    RCCHK(ecg::ldlh_factor(x->f, x->stream));
+    RCCHK(factor_queue(x, sigma));
     ++nfac;
     int fst[3] = {0, 0, 0};
     const int rc = inverse_iteration(x, sigma, tol, maxiter, true, r, true, fst);
@@ -436,6 +536,7 @@ int solve_root(ecg_ctx* x, int which, double sigma, double tol, int maxiter, dou
       return 0;
     }
     RCCHK(rc);
+    x->fac_rows = K;   // the factor is complete and without zero pivots
     nneg = fst[0];
   } else {
     std::vector<double> dh(K);
@@ -577,6 +678,7 @@ int solve_root(ecg_ctx* x, int which, double sigma, double tol, int maxiter, dou
     info[3] = nfac;
     info[4] = nneg;
     info[5] = r.t1;
+    info[7] = std::isfinite(sigma_in) ? x->last_tail : -1;
   }
   return 0;
 }
@@ -729,6 +831,7 @@ int ecg_ctx_destroy(ecg_ctx* x) {
   free_screen(x->scr);
   if (x->dD) (void)hipFree(x->dD);
   if (x->pen) (void)hipFree(x->pen);
+  if (x->f.tp) (void)hipFree(x->f.tp);
   for (hipEvent_t e : x->tr.pool) (void)hipEventDestroy(e);
   for (int i = 0; i < 8; ++i)
     if (x->e[i]) (void)hipEventDestroy(x->e[i]);
@@ -742,7 +845,10 @@ int ecg_set_params(ecg_ctx* x, int K, const double* params) {
   if (!x || K < 1 || !params) return ECG_ERR_ARG;
   if (!finite_all(params, (size_t)x->npt * K)) return ECG_ERR_ARG;
   x->s_ready = false;
+  const int d = first_changed(x, K, params);
   RCCHK(upload_params(x, K, params));
+  rows_changed(x, d);
+  x->hparams.assign(params, params + (size_t)x->npt * K);
   HIPCHK(hipStreamSynchronize(x->stream));
   return ECG_OK;
 }
@@ -757,6 +863,8 @@ int ecg_set_params_device(ecg_ctx* x, int K, const double* dparams) {
   }
   x->s_ready = false;
   RCCHK(ensure_capacity(x, K));
+  x->hparams.clear();
+  rows_changed(x, 0);
   HIPCHK(hipMemcpyAsync(x->c.params, dparams, sizeof(double) * x->npt * K, hipMemcpyDeviceToDevice, x->stream));
   RCCHK(ecg::prep(x->c, x->stream));
   return ECG_OK;
@@ -766,6 +874,9 @@ int ecg_update_params(ecg_ctx* x, int first, int count, const double* params) {
   if (!x || !params || first < 1 || count < 1 || first - 1 + count > x->c.K) return ECG_ERR_ARG;
   if (!finite_all(params, (size_t)x->npt * count)) return ECG_ERR_ARG;
   x->s_ready = false;
+  rows_changed(x, first - 1);
+  if (x->hparams.size() == (size_t)x->npt * x->c.K)
+    std::memcpy(x->hparams.data() + (size_t)x->npt * (first - 1), params, sizeof(double) * x->npt * count);
   HIPCHK(hipMemcpyAsync(x->c.params + (size_t)x->npt * (first - 1), params, sizeof(double) * x->npt * count,
                         hipMemcpyHostToDevice, x->stream));
   RCCHK(ecg::prep(x->c, x->stream));
@@ -840,6 +951,13 @@ int ecg_compute_mat_elem(ecg_ctx* x, int nmin, int nmax, char method, double app
   // rows < nmin keep their device values; their diagS comes from the caller
   if (nmin > 1)
     HIPCHK(hipMemcpyAsync(x->c.diagS, diagS, sizeof(double) * (nmin - 1), hipMemcpyHostToDevice, x->stream));
+  if (nmin > 1) {   // rows normalised with the caller's diagS: nothing below is trusted afterwards
+    x->hs_rows = 0;
+    hs_rewritten(x, 0);
+  } else {
+    hs_rewritten(x, x->hs_rows);
+    x->hs_rows = std::max(x->hs_rows, nmax);
+  }
   RCCHK(ecg::assemble(x->c, nmin - 1, nmax, x->stream));
   x->s_ready = (nmax == x->c.K);
   return copy_out_hs(x, nmin - 1, nmax, method, approx_energy, H, S, ld, diagH, diagS);
@@ -861,6 +979,8 @@ int ecg_compute_mat_elem_and_deriv(ecg_ctx* x, int nfru, char method, double app
   if (nfru > 0 &&
       hipMemcpyAsync(x->c.diagS, diagS, sizeof(double) * nfru, hipMemcpyHostToDevice, x->stream) != hipSuccess)
     rc = ECG_ERR_HIP;
+  x->hs_rows = 0;
+  hs_rewritten(x, 0);
   if (!rc) rc = ecg::assemble_deriv(x->c, nfru, dD, x->stream);
   if (!rc) x->s_ready = true;
   if (!rc && hipMemcpyAsync(D, dD, sizeof(zd) * nD, hipMemcpyDeviceToHost, x->stream) != hipSuccess)
@@ -900,6 +1020,8 @@ int ecg_ghepiis(ecg_ctx* x, int k, int n, double* M, int nM, double* invD, const
       hH[od] = mr + apprlambda * hS[od];
       hH[od + 1] = mi + apprlambda * hS[od + 1];
     }
+  x->hs_rows = 0;   // the device H, S now hold the caller's matrices
+  hs_rewritten(x, 0);
   HIPCHK(hipMemcpyAsync(c.H, hH, sizeof(double) * nd, hipMemcpyHostToDevice, x->stream));
   x->s_ready = false;
   HIPCHK(hipMemcpyAsync(c.S, hS, sizeof(double) * nd, hipMemcpyHostToDevice, x->stream));
@@ -952,7 +1074,9 @@ int ecg_energy(ecg_ctx* x, int which, double sigma, double tol, int maxiter, dou
   if (K < 1) return ECG_ERR_STATE;
   if (which > K) return ECG_ERR_ARG;
   HIPCHK(hipEventRecord(x->e[0], x->stream));
-  RCCHK(ecg::assemble(x->c, 0, K, x->stream));
+  hs_rewritten(x, x->hs_rows);   // rows below hs_rows are recomputed nowhere
+  RCCHK(ecg::assemble(x->c, x->hs_rows, K, x->stream));
+  x->hs_rows = K;
   HIPCHK(hipEventRecord(x->e[1], x->stream));
   int status = 0;
   x->s_ready = true;
@@ -994,7 +1118,9 @@ int ecg_energy_grad(ecg_ctx* x, int nfru, int which, double sigma, double tol, i
   c.ev_off0 = x->e[5];
   c.ev_off1 = x->e[6];
   HIPCHK(hipEventRecord(x->e[0], x->stream));
-  RCCHK(ecg::assemble(c, 0, K, x->stream));
+  hs_rewritten(x, x->hs_rows);
+  RCCHK(ecg::assemble(c, x->hs_rows, K, x->stream));
+  x->hs_rows = K;
   HIPCHK(hipEventRecord(x->e[1], x->stream));
   int status = 0;
   RCCHK(solve_root(x, which, sigma, tol, maxiter, E, info, &status));
@@ -1042,8 +1168,13 @@ int ecg_energy_grad_stored(ecg_ctx* x, int nfru, int which, double sigma, double
     x->grad_cap = ng;
   }
   HIPCHK(hipEventRecord(x->e[0], x->stream));
-  if (nfru > 0) RCCHK(ecg::assemble(c, 0, nfru, x->stream));
+  // rows >= nfru come from the derivative kernel (its H, S may differ from
+  // the pair kernel's in the last bit)
+  const int r0 = std::min(x->hs_rows, nfru);
+  hs_rewritten(x, r0);
+  if (r0 < nfru) RCCHK(ecg::assemble(c, r0, nfru, x->stream));
   RCCHK(ecg::assemble_deriv(c, nfru, x->dD, x->stream));
+  x->hs_rows = K;
   HIPCHK(hipEventRecord(x->e[1], x->stream));
   int status = 0;
   RCCHK(solve_root(x, which, sigma, tol, maxiter, E, info, &status));
@@ -1200,6 +1331,10 @@ int ecg_batch_energy_grad(ecg_batch* b, const double* sigma, double tol, int max
   };
   // contexts' own streams must have finished earlier work before the lead stream touches their buffers
   for (ecg_ctx* x : b->ctx) HIPCHK(hipStreamSynchronize(x->stream));
+  for (ecg_ctx* x : b->ctx) {   // every matrix and factor is rebuilt in full
+    x->hs_rows = x->c.K;
+    hs_rewritten(x, 0);
+  }
   if (b->packed) {
     RCCHK(batch_descriptors(b, nullptr));
     for (ecg_ctx* x : b->ctx) RCCHK(ecg::assemble_diag(x->c, b->lead));
@@ -1326,7 +1461,9 @@ int ecg_screen_prepare(ecg_ctx* x, double sigma, double tol, int maxiter, double
   ecg::Screen& sc = x->scr;
   sc.ready = false;
   RCCHK(ecg::prep(c, x->stream));
+  hs_rewritten(x, 0);   // the screening factor replaces the energy paths' one
   RCCHK(ecg::assemble(c, 0, K, x->stream));
+  x->hs_rows = K;
   RCCHK(ecg::screen_prepare(c, x->f, sc, sigma, x->stream));
   std::vector<double> D(K);
   int st[4];
@@ -1395,6 +1532,8 @@ int ecg_screen_loop(ecg_ctx* x, int nsteps, int ncand, const double* cand, int i
   if (!finite_all(cand, (size_t)x->npt * ncand * nsteps)) return ECG_ERR_ARG;
   RCCHK(ensure_screen(x, ncand, nsteps));
   x->s_ready = false;
+  x->hparams.clear();      // an accepted candidate replaces function K on the device
+  rows_changed(x, x->c.K - 1);
   ecg::Screen& sc = x->scr;
   const int npt = x->npt;
   HIPCHK(hipMemcpyAsync(sc.call, cand, sizeof(double) * npt * ncand * nsteps, hipMemcpyHostToDevice, x->stream));

@@ -38,7 +38,7 @@ inline int ket_stride(int n) { return n * (n + 1) + 2 * n * n; }
 // after the stream has synchronised.  Off unless enabled on the context.
 enum TraceLabel {
   TR_OFFDIAG = 0, TR_ADJOINT, TR_LDL_UPDATE, TR_LDL_DIAG, TR_LDL_PANEL, TR_TRSV_FWD, TR_TRSV_BWD, TR_GEMV,
-  TR_FORM_SHIFTED, TR_COUNT
+  TR_FORM_SHIFTED, TR_LDL_TAIL, TR_COUNT
 };
 struct Tracer {
   bool on = false;
@@ -118,10 +118,17 @@ struct Factor {
   double* scal = nullptr;    // 16 scalars
   int* pflag = nullptr;      // ld/NB + 2 ready flags of the factorisation (diagonal block, column tiles)
   int epoch = 0;             // flag value of the current factorisation launch
+  zd* tp = nullptr;          // chunk partial sums of the tail update
+  size_t tp_cap = 0;         // entries of tp
   Tracer* tr = nullptr;
 };
 int form_shifted(const Ctx& c, Factor& f, double sigma, hipStream_t s);
 int ldlh_factor(Factor& f, hipStream_t s);
+// Complete the factor of H - sigma S from row m (r = K - m <= NB changed
+// rows; rows < m factored at the same sigma and still in f).  Clears and
+// sets f.status like ldlh_factor.  f.tp must hold tail_partial_entries(ld).
+int ldlh_tail(const Ctx& c, Factor& f, double sigma, int m, hipStream_t s);
+size_t tail_partial_entries(int ld);
 int ldlh_solve(Factor& f, const zd* b, zd* x, hipStream_t s);
 int invit_iteration(const Ctx& c, Factor& f, const zd* v, zd* x, hipStream_t s);
 int rayleigh(const Ctx& c, Factor& f, const zd* x, double sigma, hipStream_t s);

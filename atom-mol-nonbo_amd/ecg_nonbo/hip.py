@@ -278,7 +278,7 @@ class EcgContext:
         return P.value, g
 
     TRACE_LABELS = ("offdiag", "adjoint", "ldl_update", "ldl_diag", "ldl_panel", "trsv_fwd", "trsv_bwd", "gemv",
-                    "form_shifted")
+                    "form_shifted", "ldl_tail")
 
     def trace(self, enable=True):
         _check("ecg_trace", load().ecg_trace(self._h, int(bool(enable))))

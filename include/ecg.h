@@ -120,7 +120,15 @@ int ecg_ghepiis(ecg_ctx* ctx, int k, int n, double* M, int nM, double* invD, con
  * c^H S c = 1 for the normalised S.  info (nullable, 8 doubles):
  *   [0] iterations, [1] convergence measure, [2] final shift,
  *   [3] factorisations, [4] negative pivots at the final shift,
- *   [5] x^H S x before normalisation. */
+ *   [5] x^H S x before normalisation, [7] rows refactored by a tail update
+ *   (-1: full factorisation).
+ * Like EnergyIA(Nmin, ...) (src/workproc.f90:1954-2061), work starts at
+ * the first function whose parameters changed since the previous call
+ * (ecg_set_params compares with the previous upload, ecg_update_params
+ * names it): only rows from there on are reassembled, and with a finite
+ * sigma equal to the previous call's and at most 64 changed rows, the
+ * LDL^H factor and L^{-1} are completed from that row (O(r K^2) for r
+ * rows, the reference's LDLHF from Nmin) instead of recomputed. */
 int ecg_energy(ecg_ctx* ctx, int which, double sigma, double tol, int maxiter, double* E, double* c, double* info);
 
 /* Device-resident energy and analytic gradient (EnergyGB/EnergyIB,
@@ -128,7 +136,9 @@ int ecg_energy(ecg_ctx* ctx, int which, double sigma, double tol, int maxiter, d
  * parameters of functions nfru+1..K, ordered (dE/dvechL_k, dE/dvechB_k)
  * per function (npt*(K-nfru) doubles).  The per-pair derivatives are
  * recomputed inside one fused adjoint pass and contracted with the
- * eigenvector on the device; nothing of size K^2*npt is stored. */
+ * eigenvector on the device; nothing of size K^2*npt is stored.  Rows are
+ * reassembled and refactored from the first changed function as in
+ * ecg_energy. */
 int ecg_energy_grad(ecg_ctx* ctx, int nfru, int which, double sigma, double tol, int maxiter, double* E,
                     double* grad, double* info);
 
