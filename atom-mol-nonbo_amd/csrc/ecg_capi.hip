@@ -683,9 +683,15 @@ int solve_root(ecg_ctx* x, int which, double sigma, double tol, int maxiter, dou
   return 0;
 }
 
+// Milliseconds between two events, or -1 if either was not recorded in
+// this call (e.g. the assembly events when no row needed reassembly); the
+// failed query's error is cleared so that it does not reach a later check.
 float ev_ms(hipEvent_t a, hipEvent_t b) {
   float t = 0.0f;
-  if (hipEventElapsedTime(&t, a, b) != hipSuccess) return -1.0f;
+  if (hipEventElapsedTime(&t, a, b) != hipSuccess) {
+    (void)hipGetLastError();
+    return -1.0f;
+  }
   return t;
 }
 
@@ -871,6 +877,7 @@ int ecg_set_params_device(ecg_ctx* x, int K, const double* dparams) {
 }
 
 int ecg_update_params(ecg_ctx* x, int first, int count, const double* params) {
+  (void)hipGetLastError();   // start from a clean runtime error state
   if (!x || !params || first < 1 || count < 1 || first - 1 + count > x->c.K) return ECG_ERR_ARG;
   if (!finite_all(params, (size_t)x->npt * count)) return ECG_ERR_ARG;
   x->s_ready = false;

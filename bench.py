@@ -257,6 +257,77 @@ def screening_leg(system, base, K, ncand=256, nsteps=100, iters=4, plain_sample=
     }
 
 
+def optimise_leg(system, base, K, nevals=60, warm=3):
+    """One function optimised at a time, as OptCycleI does with
+    NumOfFuncToOpt = 1 (src/workproc.f90:5202-5240): each trial point
+    changes only function K (ecg_update_params), and E with dE/d(params of
+    function K) is evaluated at the same shift, so rows are reassembled and
+    refactored from row K only.  Reported beside the same call after every
+    function has changed (full assembly and factorisation)."""
+    ctx = EcgContext(system)
+    ctx.set_params(base)
+    st, E0, _, _ = ctx.energy(1, float("nan"), 1e-12, 60)
+    sigma = E0 - 1e-6 * abs(E0)
+    ctx.energy_grad(K - 1, 1, sigma, 1e-12, 30)
+    rng = np.random.default_rng(SEED_BASE + 77)
+    trials = base[-1] * (1.0 + 1e-4 * rng.standard_normal((nevals + warm, system.npt)))
+    for j in range(warm):
+        ctx.update_params(K, trials[j])
+        ctx.energy_grad(K - 1, 1, sigma, 1e-12, 30)
+    rows, its = [], []
+    t0 = time.perf_counter()
+    for j in range(warm, warm + nevals):
+        ctx.update_params(K, trials[j])
+        st, e, g, inf = ctx.energy_grad(K - 1, 1, sigma, 1e-12, 30)
+        rows.append(inf[7])
+        its.append(inf[0])
+    t_eval = (time.perf_counter() - t0) / nevals
+    t0 = time.perf_counter()
+    for j in range(warm, warm + nevals):
+        ctx.update_params(K, trials[j])
+        ctx.energy(1, sigma, 1e-12, 30)
+    t_energy = (time.perf_counter() - t0) / nevals
+    ctx.trace(True)
+    for j in range(warm):
+        ctx.update_params(K, trials[j])
+        ctx.energy_grad(K - 1, 1, sigma, 1e-12, 30)
+    trace = ctx.trace_read()
+    ctx.trace(False)
+    # the same call after every function changed
+    nfull = 5
+    t_full = 0.0
+    for j in range(nfull):
+        p = base * (1.0 + 1e-9 * rng.uniform(-1, 1, base.shape))
+        p[-1] = trials[j]
+        ctx.set_params(p)
+        t0 = time.perf_counter()
+        st, e_full, _, inf_full = ctx.energy_grad(K - 1, 1, sigma, 1e-12, 30)
+        t_full += time.perf_counter() - t0
+        assert inf_full[7] == -1
+    t_full /= nfull
+    # check: the last trial point's energy from a fresh context
+    p = base.copy()
+    p[-1] = trials[warm + nevals - 1]
+    fresh = EcgContext(system)
+    fresh.set_params(p)
+    _, e_ref, _, _ = fresh.energy_grad(K - 1, 1, sigma, 1e-12, 30)
+    fresh.close()
+    ctx.close()
+    return {
+        "workload": "H2-like n=3 (T=4), K=%d: %d trial points of function K (parameters x (1 + 1e-4 N(0,1))), "
+                    "E and the gradient of function K at a fixed shift" % (K, nevals),
+        "ms_per_eval": 1e3 * t_eval,
+        "evals_per_s": 1.0 / t_eval,
+        "ms_per_energy_only": 1e3 * t_energy,
+        "ms_full_path_same_call": 1e3 * t_full,
+        "speedup_vs_full_path": t_full / t_eval,
+        "tail_rows": sorted(set(int(r) for r in rows)),
+        "mean_iterations": float(np.mean(its)),
+        "dE_vs_fresh_context": float(e - e_ref),
+        "trace_ms_per_eval": {k: v[1] / warm for k, v in trace.items() if v[0]},
+    }
+
+
 def memory_modes_leg(system, Ks=(1000, 4000), reps=3):
     """E+grad at K=1000 and K=4000 in the two memory modes: recompute (the
     fused adjoint pass; O(K) extra device memory) and stored (per-pair
@@ -378,6 +449,7 @@ def main():
     ap.add_argument("--no-screen", action="store_true", help="skip the candidate-screening leg")
     ap.add_argument("--no-modes", action="store_true", help="skip the stored/recompute memory-mode leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the multi-problem batch leg")
+    ap.add_argument("--no-opt", action="store_true", help="skip the one-function optimisation leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first issue device copies of a known size (reference for HBM counters)")
     args = ap.parse_args()
@@ -506,6 +578,8 @@ def main():
         "kernels": rows,
         "flops_per_pair_term": {"energy": f0, "grad_bra": f1, "grad_both": f3},
     }
+    if not args.no_opt:
+        out["optimise_one_function"] = optimise_leg(system, base, K)
     if not args.no_screen:
         out["screening"] = screening_leg(system, base, K)
     if not args.no_modes:

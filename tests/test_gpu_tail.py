@@ -227,3 +227,25 @@ def test_tail_state_reset_by_other_paths(golden_dir):
     assert info[7] == -1
     _close(E2, f.current_energy, 1e-12)
     ctx.close()
+
+
+def test_tail_no_reassembly_then_update(golden_dir):
+    """A call that reassembles nothing records no assembly events; the
+    calls after it still succeed."""
+    say("no reassembly then ecg_update_params", "inout_1_500")
+    f = _fixture(golden_dir, "inout_1_500")
+    sigma = f.current_energy * f.invit_parameter
+    K = f.basis_size
+    ctx = EcgContext(f.system)
+    ctx.set_params(f.params)
+    ctx.energy(1, sigma, TOL, 20)
+    _, E0, g0, info = ctx.energy_grad(K - 1, 1, sigma, TOL, 20)
+    assert info[7] == 0
+    p = _perturb(f.params, slice(K - 1, K), 701)
+    ctx.update_params(K, p[K - 1])
+    _, E1, g1, info = ctx.energy_grad(K - 1, 1, sigma, TOL, 20)
+    assert info[7] == 1
+    E2, g2 = _fresh_grad(f.system, p, K - 1, 1, sigma)
+    _close(E1, E2)
+    np.testing.assert_allclose(g1, g2, rtol=0, atol=1e-8 * np.max(np.abs(g2)))
+    ctx.close()
