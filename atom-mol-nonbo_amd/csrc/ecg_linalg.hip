@@ -773,122 +773,23 @@ int ldlh_factor(Factor& f, hipStream_t s) {
 // and X11 = L11^{-1} (X^H in the strict upper triangle) are still in F
 // and D.  The factor is completed from row m, as the reference's LDLHF
 // does from row Nmin (src/linalg.f90:316-455), in this storage:
-//   L21 = M21 X11^H D1^{-1}          k_tail_l_part, k_tail_l_fin
-//   S22 = M22 - L21 D1 L21^H         k_tail_schur
-//   L22 D2 L22^H = S22, X22          k_ldl_diag at j0 = m
-//   X21 = -X22 L21 X11               k_tail_y_part, k_tail_x_fin
-// Each of the two products reads the m x m triangle of X11 once (HBM) and
-// does 8 r m^2 / 2 flops.  Scratch: the r rows of M21 (Bt) and L21 (Lt),
-// row-major with stride ld, in f.W; chunk partial sums in f.tp.
-// Partial-sum tasks are waves: column group g (TG columns, one per lane)
-// x chunk q (tc consecutive k) x row group (TR tail rows).  A task whose
-// k range is empty stores nothing, and the finishing kernels skip exactly
-// those chunks; the others' partials are added in chunk order.
-constexpr int TG = 64;   // columns per task (one per lane)
-constexpr int TR = 16;   // tail rows per task
+//   Z = conj(L21) = D1^{-1} X11 conj(M21)^T, row by row: the forward pass
+//       of the solve, z_i(c) = (b_i(c) + sum_{k<c} conj(F(k, c)) b_i(k))
+//       / D_c with b_i = conj(M21(i, :))                 k_tail_fwd<NR>
+//   S22 = M22 - L21 D1 L21^H                             k_tail_schur
+//   L22 D2 L22^H = S22, X22 (one row: the pivot is S22)  k_ldl_diag at j0 = m
+//   X21^H: the backward pass x_i = z_i + (X11^H z_i above the diagonal)
+//       gives x_i = conj(L21 X11)(i, :), then
+//       X^H(c, m + i) = -sum_{j <= i} X22^H(j, i) x_j(c)  k_tail_bwd<NR>, k_tail_xfin
+// Each pass streams the m x m triangle of X11^H once per NR rows (HBM),
+// with 8 r m^2 / 2 flops.  Scratch: the r rows of conj(M21) (Bt) and of
+// Z (Zt), row-major with stride ld, in f.W; backward-pass partial sums in
+// f.tp.
+constexpr int TAIL_CHUNK = 128;   // columns of X11^H per backward-pass workgroup
 
-// M(m + i, c) = H - sigma S for c <= m + i: columns < m to Bt (row i),
-// the rest into the lower triangle of F's tail block.
-__global__ void __launch_bounds__(256) k_tail_form(int ld, int m, const zd* __restrict__ H, const zd* __restrict__ S,
-                                                   double sigma, zd* __restrict__ F, zd* __restrict__ Bt) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
-  if (c > m + i) return;
-  const size_t o = (size_t)c * ld + m + i;
-  const zd h = H[o], s = S[o];
-  const zd v = mkz(fma(-sigma, s.x, h.x), fma(-sigma, s.y, h.y));
-  if (c < m) Bt[(size_t)i * ld + c] = v;
-  else F[o] = v;
-}
-
-__device__ inline void tail_task(int w, int m, int r, int nq, int& g, int& q, int& i0, int& nr) {
-  const int nrg = (r + TR - 1) / TR;
-  const int rg = w % nrg;
-  q = (w / nrg) % nq;
-  g = w / (nrg * nq);
-  i0 = rg * TR;
-  nr = min(TR, r - i0);
-}
-
-// Partial sums over chunk q of (M21 X11^H)(i, c) = sum_{k < c} M21(i, k)
-// X^H(k, c); lane c walks column c of F (X^H(k, c) at F[c ld + k]).
-__global__ void __launch_bounds__(256) k_tail_l_part(int ld, int m, int r, int rp, int tc, int nq,
-                                                     const zd* __restrict__ F, const zd* __restrict__ Bt,
-                                                     zd* __restrict__ part) {
-  const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  const int ng = (m + TG - 1) / TG;
-  if (w >= ng * nq * ((r + TR - 1) / TR)) return;
-  int g, q, i0, nr;
-  tail_task(w, m, r, nq, g, q, i0, nr);
-  const int k0 = q * tc;
-  const int k1 = min(min((q + 1) * tc, m), g * TG + TG - 1);
-  if (k0 >= k1) return;
-  const int c = g * TG + lane;
-  const zd* col = F + (size_t)(c < m ? c : 0) * ld;
-  zd acc[TR];
-#pragma unroll
-  for (int u = 0; u < TR; ++u) acc[u] = mkz(0.0, 0.0);
-  for (int k = k0; k < k1; ++k) {
-    const zd xh = (k < c && c < m) ? col[k] : mkz(0.0, 0.0);
-    const zd* bk = Bt + (size_t)i0 * ld + k;
-#pragma unroll
-    for (int u = 0; u < TR; ++u)
-      if (u < nr) acc[u] = zfma(bk[(size_t)u * ld], xh, acc[u]);
-  }
-  if (c < m)
-#pragma unroll
-    for (int u = 0; u < TR; ++u)
-      if (u < nr) part[((size_t)q * rp + i0 + u) * ld + c] = acc[u];
-}
-
-// L21(i, c) = (M21(i, c) + sum of the chunks) / D_c into F and Lt.
-__global__ void __launch_bounds__(256) k_tail_l_fin(int ld, int m, int rp, int tc, const zd* __restrict__ Bt,
-                                                    const double* __restrict__ D, const zd* __restrict__ part,
-                                                    zd* __restrict__ F, zd* __restrict__ Lt) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
-  if (c >= m) return;
-  const int lim = min(m, (c / TG) * TG + TG - 1);
-  zd s = Bt[(size_t)i * ld + c];
-  for (int q = 0; q * tc < lim; ++q) s = s + part[((size_t)q * rp + i) * ld + c];
-  const zd l = s * (1.0 / D[c]);
-  F[(size_t)c * ld + m + i] = l;
-  Lt[(size_t)i * ld + c] = l;
-}
-
-// S22(i, j) -= sum_{c < m} L21(i, c) D_c conj(L21(j, c)) for i >= j: one
-// wave per (i, j), lanes over c, fixed-order butterfly.
-__global__ void __launch_bounds__(256) k_tail_schur(int ld, int m, int r, const zd* __restrict__ Lt,
-                                                    const double* __restrict__ D, zd* __restrict__ F) {
-  const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (w >= r * (r + 1) / 2) return;
-  int i = (int)((sqrt(8.0 * w + 1.0) - 1.0) * 0.5);
-  while ((i + 1) * (i + 2) / 2 <= w) ++i;
-  while (i * (i + 1) / 2 > w) --i;
-  const int j = w - i * (i + 1) / 2;
-  const zd* li = Lt + (size_t)i * ld;
-  const zd* lj = Lt + (size_t)j * ld;
-  double ar = 0.0, ai = 0.0;
-  for (int c = lane; c < m; c += 64) {
-    const zd a = li[c] * D[c], b = lj[c];
-    ar = fma(a.x, b.x, fma(a.y, b.y, ar));      // a conj(b)
-    ai = fma(a.y, b.x, fma(-a.x, b.y, ai));
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    ar += __shfl_xor(ar, o);
-    ai += __shfl_xor(ai, o);
-  }
-  if (lane == 0) {
-    zd* o = F + (size_t)(m + j) * ld + m + i;
-    *o = mkz(o->x - ar, o->y - ai);
-  }
-}
-
-// Negative pivots among D[0, m) added to status[0] (the tail's own were
-// counted by its diagonal factor).
-__global__ void __launch_bounds__(256) k_count_neg(int m, const double* __restrict__ D, int* __restrict__ status) {
+// status[0] = negative pivots among D[0, m) (the tail's own are added by
+// its diagonal factor), status[1] = status[2] = 0.
+__global__ void __launch_bounds__(256) k_tail_status(int m, const double* __restrict__ D, int* __restrict__ status) {
   __shared__ int cnt[256];
   int n = 0;
   for (int p = threadIdx.x; p < m; p += blockDim.x) n += (D[p] < 0.0) ? 1 : 0;
@@ -898,97 +799,251 @@ __global__ void __launch_bounds__(256) k_count_neg(int m, const double* __restri
     if ((int)threadIdx.x < o) cnt[threadIdx.x] += cnt[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0 && cnt[0]) atomicAdd(&status[0], cnt[0]);
-}
-
-// Partial sums over chunk q of Y(i, c) = sum_{k > c} L21(i, k) X11(k, c);
-// X11(k, c) = conj(F[k ld + c]) (row c of column k: coalesced over lanes).
-__global__ void __launch_bounds__(256) k_tail_y_part(int ld, int m, int r, int rp, int tc, int nq,
-                                                     const zd* __restrict__ F, const zd* __restrict__ Lt,
-                                                     zd* __restrict__ part) {
-  const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  const int ng = (m + TG - 1) / TG;
-  if (w >= ng * nq * ((r + TR - 1) / TR)) return;
-  int g, q, i0, nr;
-  tail_task(w, m, r, nq, g, q, i0, nr);
-  const int k0 = max(q * tc, g * TG + 1);
-  const int k1 = min((q + 1) * tc, m);
-  if (k0 >= k1) return;
-  const int c = g * TG + lane;
-  zd acc[TR];
-#pragma unroll
-  for (int u = 0; u < TR; ++u) acc[u] = mkz(0.0, 0.0);
-  for (int k = k0; k < k1; ++k) {
-    const zd x = (c < k) ? zconj(F[(size_t)k * ld + c]) : mkz(0.0, 0.0);
-    const zd* lk = Lt + (size_t)i0 * ld + k;
-#pragma unroll
-    for (int u = 0; u < TR; ++u)
-      if (u < nr) acc[u] = zfma(lk[(size_t)u * ld], x, acc[u]);
+  if (threadIdx.x == 0) {
+    status[0] = cnt[0];
+    status[1] = 0;
+    status[2] = 0;
   }
-  if (c < m)
-#pragma unroll
-    for (int u = 0; u < TR; ++u)
-      if (u < nr) part[((size_t)q * rp + i0 + u) * ld + c] = acc[u];
 }
 
-// X21(i, c) = -(Y(i, c) + sum_{j < i} X22(i, j) Y(j, c)) with Y = L21 +
-// the chunks, stored as X^H(c, m + i) = F[(m + i) ld + c].  One workgroup
-// per column group; X22(i, j) = conj(F[(m + i) ld + m + j]).
-__global__ void __launch_bounds__(256) k_tail_x_fin(int ld, int m, int r, int rp, int tc, int nq,
-                                                    const zd* __restrict__ Lt, const zd* __restrict__ part,
-                                                    zd* __restrict__ F) {
-  __shared__ zd Y[NB][TG];
-  const int g = blockIdx.x;
+// M(m + i, c) = H - sigma S for c <= m + i: conj of columns < m to Bt
+// (row i), the rest into the lower triangle of F's tail block.
+__global__ void __launch_bounds__(256) k_tail_form(int ld, int m, const zd* __restrict__ H, const zd* __restrict__ S,
+                                                   double sigma, zd* __restrict__ F, zd* __restrict__ Bt) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (c > m + i) return;
+  const size_t o = (size_t)c * ld + m + i;
+  const zd h = H[o], s = S[o];
+  const zd v = mkz(fma(-sigma, s.x, h.x), fma(-sigma, s.y, h.y));
+  if (c < m) Bt[(size_t)i * ld + c] = zconj(v);
+  else F[o] = v;
+}
+
+// Forward pass for NR tail rows (row group blockIdx.y): one wave per
+// column c < m, lanes over k < c (a contiguous read of column c of F),
+// four loads of F in flight per lane.  Writes z_i(c) to Zt and L21(i, c)
+// = conj(z_i(c)) to F.
+template <int NR>
+__global__ void __launch_bounds__(256) k_tail_fwd(int ld, int m, int r, const zd* __restrict__ F,
+                                                  const double* __restrict__ D, const zd* __restrict__ Bt,
+                                                  zd* __restrict__ Zt, zd* __restrict__ Fw) {
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= m) return;
+  const int i0 = blockIdx.y * NR;
+  const int nr = min(NR, r - i0);
+  const int lane = threadIdx.x & 63;
+  const zd* col = F + (size_t)c * ld;
+  const zd* b = Bt + (size_t)i0 * ld;
+  double ar[NR], ai[NR];
+#pragma unroll
+  for (int u = 0; u < NR; ++u) ar[u] = ai[u] = 0.0;
+  int k = lane;
+  for (; k + 192 < c; k += 256) {
+    zd f[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) f[q] = col[k + 64 * q];
+#pragma unroll
+    for (int u = 0; u < NR; ++u) {
+      if (u < nr) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const zd bv = b[(size_t)u * ld + k + 64 * q];
+          ar[u] = fma(f[q].x, bv.x, fma(f[q].y, bv.y, ar[u]));     // conj(f) b
+          ai[u] = fma(f[q].x, bv.y, fma(-f[q].y, bv.x, ai[u]));
+        }
+      }
+    }
+  }
+  for (; k < c; k += 64) {
+    const zd f0 = col[k];
+#pragma unroll
+    for (int u = 0; u < NR; ++u) {
+      if (u < nr) {
+        const zd bv = b[(size_t)u * ld + k];
+        ar[u] = fma(f0.x, bv.x, fma(f0.y, bv.y, ar[u]));
+        ai[u] = fma(f0.x, bv.y, fma(-f0.y, bv.x, ai[u]));
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < NR; ++u) {
+    for (int o = 32; o > 0; o >>= 1) {
+      ar[u] += __shfl_xor(ar[u], o);
+      ai[u] += __shfl_xor(ai[u], o);
+    }
+  }
+  if (lane == 0) {
+    const double inv = 1.0 / D[c];
+#pragma unroll
+    for (int u = 0; u < NR; ++u) {
+      if (u < nr) {
+        const zd bc = b[(size_t)u * ld + c];
+        const zd z = mkz((bc.x + ar[u]) * inv, (bc.y + ai[u]) * inv);
+        Zt[(size_t)(i0 + u) * ld + c] = z;
+        Fw[(size_t)c * ld + m + i0 + u] = zconj(z);
+      }
+    }
+  }
+}
+
+// S22(i, j) -= sum_{c < m} conj(z_i(c)) D_c z_j(c) for i >= j: one
+// workgroup per (i, j), fixed-order reduction.  A single tail row (r = 1)
+// is its own pivot: D[m] and the status are written here.
+__global__ void __launch_bounds__(256) k_tail_schur(int ld, int m, int r, const zd* __restrict__ Zt,
+                                                    double* __restrict__ D, zd* __restrict__ F,
+                                                    int* __restrict__ status) {
+  __shared__ double sr[256], si[256];
+  const int w = blockIdx.x;
+  int i = (int)((sqrt(8.0 * w + 1.0) - 1.0) * 0.5);
+  while ((i + 1) * (i + 2) / 2 <= w) ++i;
+  while (i * (i + 1) / 2 > w) --i;
+  const int j = w - i * (i + 1) / 2;
+  const zd* zi = Zt + (size_t)i * ld;
+  const zd* zj = Zt + (size_t)j * ld;
+  double ar = 0.0, ai = 0.0;
+  for (int c = threadIdx.x; c < m; c += blockDim.x) {
+    const zd a = zi[c] * D[c], b = zj[c];
+    ar = fma(a.x, b.x, fma(a.y, b.y, ar));      // conj(a) b
+    ai = fma(a.x, b.y, fma(-a.y, b.x, ai));
+  }
+  sr[threadIdx.x] = ar;
+  si[threadIdx.x] = ai;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      sr[threadIdx.x] += sr[threadIdx.x + o];
+      si[threadIdx.x] += si[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    zd* o = F + (size_t)(m + j) * ld + m + i;
+    const zd v = mkz(o->x - sr[0], o->y - si[0]);
+    *o = v;
+    if (r == 1) {
+      D[m] = v.x;
+      if (v.x < 0.0) atomicAdd(&status[0], 1);
+      if (v.x == 0.0) atomicCAS(&status[1], 0, m + 1);
+    }
+  }
+}
+
+// Backward pass for NR tail rows: partial sums over chunk ch of
+// x_i(c) = sum_{k > c} F(c, k) z_i(k), F(c, k) = F[k ld + c]; workgroup =
+// 64 rows c x 4 column groups, eight loads of F in flight per lane,
+// columns k <= c read and masked (in bounds: k < m).
+template <int NR>
+__global__ void __launch_bounds__(256) k_tail_bwd(int ld, int m, int r, int rp, const zd* __restrict__ F,
+                                                  const zd* __restrict__ Zt, zd* __restrict__ part) {
+  const int rb = blockIdx.x, ch = blockIdx.y;
+  if ((ch + 1) * TAIL_CHUNK <= rb * 64 + 1) return;   // every column k of the chunk is <= every row c
+  __shared__ zd red[4][64][NR];
+  const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int c = rb * 64 + lane;
+  const int i0 = blockIdx.z * NR;
+  const int nr = min(NR, r - i0);
+  const int k1 = min(m, (ch + 1) * TAIL_CHUNK);
+  zd acc[NR];
+#pragma unroll
+  for (int u = 0; u < NR; ++u) acc[u] = mkz(0.0, 0.0);
+  if (c < m) {
+    int k = ch * TAIL_CHUNK + g;
+    for (; k + 28 < k1; k += 32) {
+      zd fv[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) fv[q] = F[(size_t)(k + 4 * q) * ld + c];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const bool above = k + 4 * q > c;
+#pragma unroll
+        for (int u = 0; u < NR; ++u)
+          if (u < nr) {
+            const zd zv = Zt[(size_t)(i0 + u) * ld + k + 4 * q];
+            if (above) acc[u] = zfma(fv[q], zv, acc[u]);
+          }
+      }
+    }
+    for (; k < k1; k += 4)
+      if (k > c)
+#pragma unroll
+        for (int u = 0; u < NR; ++u)
+          if (u < nr) acc[u] = zfma(F[(size_t)k * ld + c], Zt[(size_t)(i0 + u) * ld + k], acc[u]);
+  }
+#pragma unroll
+  for (int u = 0; u < NR; ++u) red[g][lane][u] = acc[u];
+  __syncthreads();
+  if (g == 0 && c < m)
+#pragma unroll
+    for (int u = 0; u < NR; ++u)
+      if (u < nr)
+        part[((size_t)ch * rp + i0 + u) * ld + c] = red[0][lane][u] + red[1][lane][u] + red[2][lane][u] +
+                                                    red[3][lane][u];
+}
+
+// x_i(c) = z_i(c) + the chunks (those k_tail_bwd wrote for c's row block),
+// then X^H(c, m + i) = -(x_i(c) + sum_{j < i} X22^H(j, i) x_j(c)) with
+// X22^H(j, i) = F[(m + i) ld + m + j].  One workgroup per 64 columns.
+__global__ void __launch_bounds__(256) k_tail_xfin(int ld, int m, int r, int rp, int nch, const zd* __restrict__ Zt,
+                                                   const zd* __restrict__ part, zd* __restrict__ F) {
+  __shared__ zd X[NB][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int c = g * TG + lane;
+  const int c = blockIdx.x * 64 + lane;
   const bool in = c < m;
+  const int rb = blockIdx.x;
   for (int i = wv; i < r; i += 4) {
-    zd s = in ? Lt[(size_t)i * ld + c] : mkz(0.0, 0.0);
-    for (int q = 0; q < nq; ++q)
-      if (in && max(q * tc, g * TG + 1) < min((q + 1) * tc, m)) s = s + part[((size_t)q * rp + i) * ld + c];
-    Y[i][lane] = s;
+    zd s = in ? Zt[(size_t)i * ld + c] : mkz(0.0, 0.0);
+    for (int ch = 0; ch < nch; ++ch)
+      if (in && (ch + 1) * TAIL_CHUNK > rb * 64 + 1) s = s + part[((size_t)ch * rp + i) * ld + c];
+    X[i][lane] = s;
   }
   __syncthreads();
   for (int i = wv; i < r; i += 4) {
-    zd x = Y[i][lane];
+    zd x = X[i][lane];
     const zd* xh = F + (size_t)(m + i) * ld + m;
-    for (int j = 0; j < i; ++j) x = zfma(zconj(xh[j]), Y[j][lane], x);
-    if (in) F[(size_t)(m + i) * ld + c] = mkz(-x.x, x.y);
+    for (int j = 0; j < i; ++j) x = zfma(xh[j], X[j][lane], x);
+    if (in) F[(size_t)(m + i) * ld + c] = mkz(-x.x, -x.y);
   }
 }
 
 size_t tail_partial_entries(int ld) {
-  // max over the two chunkings below of nq x rp x ld
-  const size_t a = (size_t)((ld + 63) / 64) * TR, b = (size_t)((ld + 255) / 256) * NB;
-  return (a > b ? a : b) * (size_t)ld;
+  return (size_t)((ld + TAIL_CHUNK - 1) / TAIL_CHUNK) * NB * (size_t)ld;
+}
+
+template <int NR>
+static void tail_passes(const Ctx& c, Factor& f, int m, int r, int rp, int nch, zd* Bt, zd* Zt, hipStream_t s,
+                        bool first) {
+  const int ld = f.ld, ng = (r + NR - 1) / NR;
+  if (first)
+    hipLaunchKernelGGL(k_tail_fwd<NR>, dim3((m + 3) / 4, ng), dim3(256), 0, s, ld, m, r, f.F, f.D, Bt, Zt, f.F);
+  else
+    hipLaunchKernelGGL(k_tail_bwd<NR>, dim3((m + 63) / 64, nch, ng), dim3(256), 0, s, ld, m, r, rp, f.F, Zt, f.tp);
 }
 
 int ldlh_tail(const Ctx& c, Factor& f, double sigma, int m, hipStream_t s) {
   const int K = c.K, ld = f.ld, r = K - m;
   if (m < 1 || r < 0 || r > NB || !f.tp || ld != c.ld) return -1;
-  const int rp = (r + TR - 1) / TR * TR;
-  const int tc = (rp <= TR) ? 64 : 256;
-  const int nq = (m + tc - 1) / tc, ng = (m + TG - 1) / TG;
-  if ((size_t)nq * rp * ld > f.tp_cap) return -1;
+  const int rp = r;
+  const int nch = (m + TAIL_CHUNK - 1) / TAIL_CHUNK;
+  if ((size_t)nch * NB * ld > f.tp_cap) return -1;
   if (f.tr) f.tr->begin(TR_LDL_TAIL, s);
-  hipLaunchKernelGGL(k_clear_status, dim3(1), dim3(64), 0, s, f.status);
+  hipLaunchKernelGGL(k_tail_status, dim3(1), dim3(256), 0, s, m, f.D, f.status);
   if (r > 0) {
     zd* Bt = f.W;
-    zd* Lt = f.W + (size_t)NB * ld;
-    const long long nw = (long long)ng * nq * (rp / TR);
+    zd* Zt = f.W + (size_t)NB * ld;
     hipLaunchKernelGGL(k_tail_form, dim3((K + 255) / 256, r), dim3(256), 0, s, ld, m, c.H, c.S, sigma, f.F, Bt);
-    hipLaunchKernelGGL(k_tail_l_part, dim3((unsigned)((nw + 3) / 4)), dim3(256), 0, s, ld, m, r, rp, tc, nq, f.F, Bt,
-                       f.tp);
-    hipLaunchKernelGGL(k_tail_l_fin, dim3((m + 255) / 256, r), dim3(256), 0, s, ld, m, rp, tc, Bt, f.D, f.tp, f.F,
-                       Lt);
-    hipLaunchKernelGGL(k_tail_schur, dim3((r * (r + 1) / 2 + 3) / 4), dim3(256), 0, s, ld, m, r, Lt, f.D, f.F);
-    hipLaunchKernelGGL(k_ldl_diag, dim3(1), dim3(256), 0, s, K, ld, m, f.F, f.D, f.Dinv, f.status);
-    hipLaunchKernelGGL(k_tail_y_part, dim3((unsigned)((nw + 3) / 4)), dim3(256), 0, s, ld, m, r, rp, tc, nq, f.F, Lt,
-                       f.tp);
-    hipLaunchKernelGGL(k_tail_x_fin, dim3(ng), dim3(256), 0, s, ld, m, r, rp, tc, nq, Lt, f.tp, f.F);
+    for (int pass = 0; pass < 2; ++pass) {
+      const bool first = pass == 0;
+      if (r == 1) tail_passes<1>(c, f, m, r, rp, nch, Bt, Zt, s, first);
+      else if (r <= 4) tail_passes<4>(c, f, m, r, rp, nch, Bt, Zt, s, first);
+      else tail_passes<8>(c, f, m, r, rp, nch, Bt, Zt, s, first);
+      if (first) {
+        hipLaunchKernelGGL(k_tail_schur, dim3(r * (r + 1) / 2), dim3(256), 0, s, ld, m, r, Zt, f.D, f.F, f.status);
+        if (r > 1) hipLaunchKernelGGL(k_ldl_diag, dim3(1), dim3(256), 0, s, K, ld, m, f.F, f.D, f.Dinv, f.status);
+      }
+    }
+    hipLaunchKernelGGL(k_tail_xfin, dim3((m + 63) / 64), dim3(256), 0, s, ld, m, r, rp, nch, Zt, f.tp, f.F);
   }
-  hipLaunchKernelGGL(k_count_neg, dim3(1), dim3(256), 0, s, m, f.D, f.status);
   if (f.tr) f.tr->end(s);
   return (int)hipGetLastError();
 }
