@@ -185,7 +185,7 @@ int ensure_capacity(ecg_ctx* x, int K) {
   if (!rc) rc = dev_alloc(&f.status, (size_t)4);
   if (!rc) rc = dev_alloc(&f.vec, (size_t)4 * ld);
   if (!rc) rc = dev_alloc(&f.tvec, (size_t)ld);
-  if (!rc) rc = dev_alloc(&f.pflag, (size_t)(ld / ecg::NB) + 2);
+  if (!rc) rc = dev_alloc(&f.pflag, (size_t)(ld / ecg::NB) + 5);
   if (!rc) rc = dev_alloc(&f.tp, (size_t)0);   // tail-update partials: allocated on first use
   if (!rc) rc = dev_alloc(&f.scal, (size_t)16);
   for (int i = 0; i < 3 && !rc; ++i) rc = dev_alloc(&x->ev[i], (size_t)ld);
@@ -198,7 +198,7 @@ int ensure_capacity(ecg_ctx* x, int K) {
   }
   f.tp_cap = 0;
   f.epoch = 0;
-  bool ok = hipMemsetAsync(f.pflag, 0, sizeof(int) * ((ld / ecg::NB) + 2), x->stream) == hipSuccess &&
+  bool ok = hipMemsetAsync(f.pflag, 0, sizeof(int) * ((ld / ecg::NB) + 5), x->stream) == hipSuccess &&
             hipMemsetAsync(f.status, 0, sizeof(int) * 4, x->stream) == hipSuccess &&
             hipMemsetAsync(c.H, 0, sizeof(zd) * ld * ld, x->stream) == hipSuccess &&
             hipMemsetAsync(c.S, 0, sizeof(zd) * ld * ld, x->stream) == hipSuccess;

@@ -458,6 +458,7 @@ struct Tile64 {
   dbl4 re[2][2], im[2][2];
 };
 
+template <int KS = NB / 4>   // k-steps of 4 (kd <= 4 KS)
 __device__ inline void mfma_tile_abh(const zd* __restrict__ Ap, int lda, int ra, const zd* __restrict__ Bp,
                                      int ldb, int rb, int kd, Tile64& t) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -470,11 +471,11 @@ __device__ inline void mfma_tile_abh(const zd* __restrict__ Ap, int lda, int ra,
       t.re[a][q] = dbl4{0.0, 0.0, 0.0, 0.0};
       t.im[a][q] = dbl4{0.0, 0.0, 0.0, 0.0};
     }
-  // all operand fragments are loaded up front (16 k-steps x 4 complex per
+  // all operand fragments are loaded up front (KS k-steps x 4 complex per
   // lane) so the global loads are in flight together
-  zd fa[NB / 4][2], fb[NB / 4][2];
+  zd fa[KS][2], fb[KS][2];
 #pragma unroll
-  for (int s = 0; s < NB / 4; ++s) {
+  for (int s = 0; s < KS; ++s) {
     const int p = 4 * s + lk;
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
@@ -485,7 +486,7 @@ __device__ inline void mfma_tile_abh(const zd* __restrict__ Ap, int lda, int ra,
     }
   }
 #pragma unroll
-  for (int s = 0; s < NB / 4; ++s) {
+  for (int s = 0; s < KS; ++s) {
     double ar[2], ai[2], br[2], bi[2];
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
@@ -649,17 +650,19 @@ __global__ void __launch_bounds__(256) k_ldl_xlast(int K, int ld, zd* __restrict
 // -------------------------------------------------------- trailing update
 // One launch per block column j (diagonal block at j0, trailing blocks
 // from t0 = j + 1):
-//   workgroups 0..ntiles-1: lower block pairs (I >= Kb) of the trailing
-//     matrix, C_IK -= W_I L_K^H.  The one that updates the first trailing
-//     diagonal block then factors it (look-ahead).
-//   workgroups ntiles..ntiles+nx-1: block tiles of X = L^{-1} (xinv_tile):
-//     the col tiles of block row col, then rows col+1.. (no waits).  They
-//     come before the strips so that they are dispatched while the strips
-//     would only wait.
-//   workgroups ntiles+nx..: 16-row strips of the next block column's
-//     panel.  Each waits until the tile holding its rows and the
-//     look-ahead factor are published, then runs the panel solve for its
-//     strip.
+//   workgroups 0..3: the next diagonal tile's update by 16-row quarters
+//     (C_TT -= W_T L_T^H, rows 16q..16q+15, lower part), written to F
+//     and published each on its own flag.
+//   workgroup 4 (look-ahead): waits for the quarters and factors the
+//     tile.
+//   workgroups 5..ntiles+3: the other lower block pairs (I >= Kb) of the
+//     trailing matrix, C_IK -= W_I L_K^H.
+//   then block tiles of X = L^{-1} (xinv_tile): the col tiles of block
+//     row col, then rows col+1.. (no waits).  They come before the strips
+//     so that they are dispatched while the strips would only wait.
+//   last: 16-row strips of the next block column's panel.  Each waits
+//     until the tile holding its rows and the look-ahead factor are
+//     published, then runs the panel solve for its strip.
 // Publication: the producer's threads fence at agent scope, the workgroup
 // synchronises, thread 0 stores the flag (release).  A waiting workgroup's
 // thread 0 polls the flags (relaxed, bounded in time: past the limit it
@@ -669,42 +672,87 @@ __global__ void __launch_bounds__(256) k_ldl_xlast(int K, int ld, zd* __restrict
 // Waits only point to lower workgroup indices, dispatched no later.
 // W is double-buffered by column parity: the tiles read column j's, the
 // strips write column j+1's.
+constexpr int NQ = 4;   // quarters of the look-ahead tile's update
+
 __global__ void __launch_bounds__(256) k_ldl_update(int K, int ld, int j0, zd* __restrict__ F,
                                                     const zd* __restrict__ Wcur, zd* __restrict__ Wnext,
                                                     double* __restrict__ D, zd* __restrict__ Dinv,
                                                     int* __restrict__ status, int* __restrict__ pflag,
-                                                    int epoch, int ntiles, int nstrip) {
+                                                    int qflag, int epoch, int ntiles, int nstrip) {
   __shared__ DiagLds Ld;
   const int jb = min(NB, K - j0);
   const int t0 = (j0 + jb) / NB;
   const int b = blockIdx.x;
-  const int nx = (int)gridDim.x - ntiles - nstrip;   // tiles of L^{-1}
-  if (b >= ntiles && b < ntiles + nx) {
-    const int k = j0 / NB, x = b - ntiles;
+  const int nx = (int)gridDim.x - NQ - ntiles - nstrip;   // tiles of L^{-1}
+  if (b < NQ) {
+    // rows 16b..16b+15 of the next diagonal tile: C(r, c) -= sum_p W(r, p)
+    // conj(L(c, p)) for c <= r, written back to F; wave w takes columns
+    // 16w..16w+15 (w <= b).  All operand fragments are loaded first.
+    const int T0 = t0 * NB, tb = min(NB, K - T0);
+    const int R0 = b * SB, C0 = (threadIdx.x >> 6) * SB;
+    if (C0 <= R0 && R0 < tb) {
+      const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+      zd av[NB / 4], bv[NB / 4], old[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = R0 + acc16_row(v), c = C0 + acc16_col();
+        old[v] = (r < tb && c <= r) ? F[(size_t)(T0 + c) * ld + T0 + r] : mkz(0.0, 0.0);
+      }
+#pragma unroll
+      for (int st = 0; st < NB / 4; ++st) {
+        const int p = 4 * st + lk;
+        av[st] = (R0 + lr < tb && p < jb) ? Wcur[(size_t)p * ld + T0 + R0 + lr] : mkz(0.0, 0.0);
+        bv[st] = (C0 + lr < tb && p < jb) ? zconj(F[(size_t)(j0 + p) * ld + T0 + C0 + lr]) : mkz(0.0, 0.0);
+      }
+      Acc16 acc;
+      acc16_zero(acc);
+#pragma unroll
+      for (int st = 0; st < NB / 4; ++st) {
+        acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(av[st].x, bv[st].x, acc.re, 0, 0, 0);
+        acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[st].y, bv[st].y, acc.re, 0, 0, 0);
+        acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[st].x, bv[st].y, acc.im, 0, 0, 0);
+        acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[st].y, bv[st].x, acc.im, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = R0 + acc16_row(v), c = C0 + acc16_col();
+        if (r < tb && c <= r) F[(size_t)(T0 + c) * ld + T0 + r] = mkz(old[v].x - acc.re[v], old[v].y - acc.im[v]);
+      }
+    }
+    publish_wg(&pflag[qflag + b], epoch);
+    return;
+  }
+  if (b == NQ) {
+    // look-ahead: the next diagonal tile, finished by the four row
+    // quarters, then its factor
+    wait_wg(&pflag[qflag + 0], &pflag[qflag + 1], epoch, status);
+    wait_wg(&pflag[qflag + 2], &pflag[qflag + 3], epoch, status);
+    ldl_diag_block(K, ld, t0 * NB, F, D, Dinv, status, Ld, false, &pflag[0], epoch);
+    return;
+  }
+  if (b >= NQ + ntiles && b < NQ + ntiles + nx) {
+    const int k = j0 / NB, x = b - NQ - ntiles;
     const int i = (x < k) ? k : k + 1 + (x - k) / k;
     const int c = (x < k) ? x : (x - k) % k;
     xinv_tile(K, ld, k, i, c, F, Dinv, Ld);
     return;
   }
-  if (b >= ntiles) {
-    const int i0 = (t0 + 1) * NB + (b - ntiles - nx) * SB;
+  if (b >= NQ + ntiles) {
+    const int i0 = (t0 + 1) * NB + (b - NQ - ntiles - nx) * SB;
     const int bi = (i0 - t0 * NB) / NB;
     wait_wg(&pflag[0], &pflag[1 + bi], epoch, status);
     panel_strip(K, ld, t0 * NB, i0, F, D, Dinv, Wnext);
     return;
   }
-  int bi = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
-  while ((bi + 1) * (bi + 2) / 2 <= b) ++bi;
-  while (bi * (bi + 1) / 2 > b) --bi;
-  const int bk = b - bi * (bi + 1) / 2;
+  const int tb = b - NQ;   // lower-triangle tile index; 0 (the diagonal tile) is the look-ahead's
+  int bi = (int)((sqrt(8.0 * tb + 1.0) - 1.0) * 0.5);
+  while ((bi + 1) * (bi + 2) / 2 <= tb) ++bi;
+  while (bi * (bi + 1) / 2 > tb) --bi;
+  const int bk = tb - bi * (bi + 1) / 2;
   const int I0 = (t0 + bi) * NB, K0 = (t0 + bk) * NB;
   const int ib = min(NB, K - I0), kb = min(NB, K - K0);
   Tile64 t;
   mfma_tile_abh(Wcur + I0, ld, ib, F + (size_t)j0 * ld + K0, ld, kb, jb, t);
-  const bool lookahead = (bi == 0 && bk == 0);
-  if (lookahead)
-    for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) Ld.A[e >> 6][e & (NB - 1)] = mkz(0.0, 0.0);
-  __syncthreads();
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -716,19 +764,10 @@ __global__ void __launch_bounds__(256) k_ldl_update(int K, int ld, int j0, zd* _
         if (r < ib && c < kb && (I0 + r) >= (K0 + c)) {
           zd* o = F + (size_t)(K0 + c) * ld + I0 + r;
           const zd old = *o;
-          const zd nw = mkz(old.x - t.re[a][q][v], old.y - t.im[a][q][v]);
-          *o = nw;
-          if (lookahead) Ld.A[r][c] = nw;
+          *o = mkz(old.x - t.re[a][q][v], old.y - t.im[a][q][v]);
         }
       }
-  if (lookahead) {
-    // this workgroup owns the next diagonal block (its values are in LDS):
-    // factor it now instead of in a separate launch
-    __syncthreads();
-    ldl_diag_block(K, ld, I0, F, D, Dinv, status, Ld, true, &pflag[0], epoch);
-  } else if (bk == 0) {
-    publish_wg(&pflag[1 + bi], epoch);
-  }
+  if (bk == 0) publish_wg(&pflag[1 + bi], epoch);
 }
 
 __global__ void k_clear_status(int* status) {
@@ -759,8 +798,8 @@ int ldlh_factor(Factor& f, hipStream_t s) {
     const int nx = (col >= 1) ? col + (nb - 1 - col) * col : 0;   // tiles of X = L^{-1}
     const int epoch = ++f.epoch;
     if (f.tr) f.tr->begin(TR_LDL_UPDATE, s);
-    hipLaunchKernelGGL(k_ldl_update, dim3(ntiles + nstrip + nx), dim3(256), 0, s, K, f.ld, j0, f.F, Wcur, Wnext,
-                       f.D, f.Dinv, f.status, f.pflag, epoch, ntiles, nstrip);
+    hipLaunchKernelGGL(k_ldl_update, dim3(NQ + ntiles + nstrip + nx), dim3(256), 0, s, K, f.ld, j0, f.F, Wcur,
+                       Wnext, f.D, f.Dinv, f.status, f.pflag, f.ld / NB + 1, epoch, ntiles, nstrip);
     if (f.tr) f.tr->end(s);
   }
   if (K > NB) hipLaunchKernelGGL(k_ldl_xlast, dim3((K + NB - 1) / NB - 1), dim3(256), 0, s, K, f.ld, f.F, f.Dinv);

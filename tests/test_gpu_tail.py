@@ -9,7 +9,7 @@ test compares the tail path (info[7] = rows refactored) with a fresh
 context's full factorisation of the same parameters, whose own parity with
 the oracle is covered elsewhere: energies to 1e-11 relative (a dropped or
 mis-signed term of the update moves E by ~1e-3), the inertia exactly, and
-gradients to 1e-8 relative.
+gradients to 1e-8 of their largest entry (at least 1e-15).
 """
 from __future__ import annotations
 
@@ -62,6 +62,13 @@ def _fresh_grad(sy, p, nfru, which, sigma, stored=False):
 
 def _close(a, b, rel=1e-11):
     assert abs(a - b) <= rel * abs(b), (a, b, a - b)
+
+
+def _grad_close(g1, g2):
+    """1e-8 of the largest entry, floored at 1e-15: at an optimised basis the
+    gradient is ~1e-11 while its terms are O(1), so two correct evaluations
+    differ by their rounding (~1e-16 of the terms), not by 1e-8 of g."""
+    np.testing.assert_allclose(g1, g2, rtol=0, atol=max(1e-8 * np.max(np.abs(g2)), 1e-15))
 
 
 @pytest.mark.parametrize("r", [1, 3, 17, 64, 65])
@@ -123,7 +130,7 @@ def test_tail_optimisation_sequence_no_drift(golden_dir):
             E2, g2 = _fresh_grad(f.system, p, K - 1, 1, sigma)
             print("step", step, "E", E1, E2, "dg", np.max(np.abs(g1 - g2)), np.max(np.abs(g2)))
             _close(E1, E2)
-            np.testing.assert_allclose(g1, g2, rtol=0, atol=1e-8 * np.max(np.abs(g2)))
+            _grad_close(g1, g2)
     ctx.close()
 
 
@@ -199,7 +206,7 @@ def test_tail_stored_path_h2(golden_dir):
     assert info[7] == 2
     E2, g2 = _fresh_grad(sy, q, K - 2, 1, sigma, stored=True)
     _close(E1, E2)
-    np.testing.assert_allclose(g1, g2, rtol=0, atol=1e-8 * np.max(np.abs(g2)))
+    _grad_close(g1, g2)
     ctx.close()
 
 
@@ -247,5 +254,5 @@ def test_tail_no_reassembly_then_update(golden_dir):
     assert info[7] == 1
     E2, g2 = _fresh_grad(f.system, p, K - 1, 1, sigma)
     _close(E1, E2)
-    np.testing.assert_allclose(g1, g2, rtol=0, atol=1e-8 * np.max(np.abs(g2)))
+    _grad_close(g1, g2)
     ctx.close()
