@@ -57,6 +57,8 @@ struct ecg_ctx {
   double* pen = nullptr;      // per-function penalty partials, then the total (device)
   size_t pen_cap = 0;
   double pen_total = 0.0;     // host copy of the last total
+  double* xpart = nullptr;    // expectation-value tile partials, then the 2 NOBS results
+  size_t xpart_cap = 0;
   bool s_ready = false;       // the device S is assembled from the current parameters
   // Incremental recomputation.  hs_rows leading rows of the device H, S
   // match the current parameters; fac_rows leading rows of the factor
@@ -838,6 +840,7 @@ int ecg_ctx_destroy(ecg_ctx* x) {
   if (x->dD) (void)hipFree(x->dD);
   if (x->pen) (void)hipFree(x->pen);
   if (x->f.tp) (void)hipFree(x->f.tp);
+  if (x->xpart) (void)hipFree(x->xpart);
   for (hipEvent_t e : x->tr.pool) (void)hipEventDestroy(e);
   for (int i = 0; i < 8; ++i)
     if (x->e[i]) (void)hipEventDestroy(x->e[i]);
@@ -1621,6 +1624,42 @@ int ecg_overlap_penalty(ecg_ctx* x, int nfru, double max_pair_penalty, double th
   *total = x->pen_total;
   if (!std::isfinite(*total) || (grad && !finite_all(grad, ng))) return ECG_ERR_NONFINITE;
   return ECG_OK;
+}
+
+int ecg_expectation_values(ecg_ctx* x, const double* c, double* values, double* errors) {
+  (void)hipGetLastError();   // start from a clean runtime error state
+  if (!x || !values) return ECG_ERR_ARG;
+  ecg::Ctx& cx = x->c;
+  const int K = cx.K;
+  if (K < 1) return ECG_ERR_STATE;
+  if (!c && !(x->have_last && x->last_K == K)) return ECG_ERR_STATE;
+  if (c && !finite_all(c, 2 * (size_t)K)) return ECG_ERR_ARG;
+  const int m = cx.n * (cx.n + 1) / 2, nobs = 5 * m + 4;
+  const size_t need = ecg::expectation_partial_doubles(cx.n, K) + 2 * (size_t)nobs;
+  if (need > x->xpart_cap) {
+    RCCHK(dev_alloc(&x->xpart, need));
+    x->xpart_cap = need;
+  }
+  if (x->hs_rows < K) {   // diagS (the normalisation) from the current parameters
+    hs_rewritten(x, x->hs_rows);
+    RCCHK(ecg::assemble(cx, x->hs_rows, K, x->stream));
+    x->hs_rows = K;
+  }
+  const zd* cv = x->ev[2];
+  if (c) {
+    HIPCHK(hipMemcpyAsync(x->ev[1], c, sizeof(zd) * K, hipMemcpyHostToDevice, x->stream));
+    cv = x->ev[1];
+  }
+  double* dout = x->xpart + ecg::expectation_partial_doubles(cx.n, K);
+  RCCHK(ecg::expectation_values(cx, cv, x->xpart, dout, x->stream));
+  std::vector<double> h(2 * (size_t)nobs);
+  HIPCHK(hipMemcpyAsync(h.data(), dout, sizeof(double) * 2 * nobs, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  for (int a = 0; a < nobs; ++a) {
+    values[a] = h[a];
+    if (errors) errors[a] = h[nobs + a] * 2.220446049250313e-16;   // epsilon(1.0)
+  }
+  return finite_all(values, nobs) ? ECG_OK : ECG_ERR_NONFINITE;
 }
 
 int ecg_last_timings(ecg_ctx* x, double* ms, int nmax) {

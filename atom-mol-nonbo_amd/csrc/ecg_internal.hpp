@@ -104,6 +104,14 @@ size_t adjoint_partial_doubles(int n, int K);
 int overlap_penalty(const Ctx& c, int nfru, double pc, double t2, double* pen, double* total, double* grad,
                     hipStream_t s);
 
+// Expectation values of MatrixElementsForExpcVals' pair operators in the
+// eigenvector cvec (K complex, normalised for the normalised S): out
+// receives 2 (5 m + 4) doubles, the values then the driver's rounding
+// bounds (before their scaling by epsilon), m = n(n+1)/2.  part: scratch of
+// expectation_partial_doubles(n, K).
+int expectation_values(const Ctx& c, const zd* cvec, double* part, double* out, hipStream_t s);
+size_t expectation_partial_doubles(int n, int K);
+
 // dense Hermitian linear algebra (ecg_linalg.hip)
 struct Factor {
   int K = 0, ld = 0, cap = 0;

@@ -10,6 +10,7 @@
 //   diagS  real S_kk (un-normalised), diagH real H_kk / S_kk.
 // The reference stores the same numbers in the lower triangles only
 // (src/matform.f90:11-51); the host copy-out restores that layout.
+#include <algorithm>
 #include <type_traits>
 
 #include "ecg_internal.hpp"
@@ -513,6 +514,160 @@ __global__ void __launch_bounds__(256) k_adjoint_finish(int K, int nfru, int nt,
   const int f = nfru + blockIdx.x * 4 + (threadIdx.x >> 6);
   if (f >= K) return;
   adjoint_finish_one<N>(f, nfru, nt, tr0, bra, part, grad);
+}
+
+// ---------------------------------------------------- expectation values
+// MatrixElementsForExpcVals (src/matelem.f90:1236-1349) for the operators
+// 1/r^2, 1/r, r, r^2, delta(r) of every pseudo-particle pair (a, b >= a;
+// a = b: r_a, the distance from particle 1), with rs = tr(C^-1 J_ab)^-1/2
+// from the energy's Coulomb part:
+//   1/r^2 = 2 S rs^2, 1/r = (2/sqrt(pi)) S rs, r = (2/sqrt(pi)) S / rs,
+//   r^2 = (3/2) S / rs^2, delta(r) = S rs^3 / pi^(3/2);
+// then H, S, T, V.  Contracted on the fly as the reference's driver does
+// (src/workproc.f90:8158-8230): per pair k >= l and term t,
+// beta = Re(conj(c_k) c_l O), mu = the driver's bound on its rounding,
+// sum += f coef_t beta, err += |f coef_t (mu + 2 |beta|)|, f = 1/diagS_k
+// (k = l) or 2/sqrt(diagS_k diagS_l).  Per tile: fixed-order reduction
+// (wave butterflies, then the four waves in order) into part[tile][2 NOBS];
+// k_expect_finish adds the tiles in order.
+template <int N> struct Expc {
+  static constexpr int m = N * (N + 1) / 2;
+  static constexpr int nobs = 5 * m + 4;
+};
+
+template <int N>
+__global__ void __launch_bounds__(TILE* TILE) k_expect(const SysConst sc, int K, const double* __restrict__ bra,
+                                                     const double* __restrict__ ket,
+                                                     const double* __restrict__ diagS, const zd* __restrict__ cvec,
+                                                     double* __restrict__ part) {
+  constexpr int m = Expc<N>::m, NO = Expc<N>::nobs;
+  extern __shared__ double lds[];
+  int tr, tc;
+  tri_tile(blockIdx.x, 0, tr, tc);
+  const int T = sc.T;
+  const int kb = tr * TILE, lb = tc * TILE;
+  double* sbra = lds;
+  double* sket = lds + TILE * BraRec<N>::stride;
+  const int tid = threadIdx.y * TILE + threadIdx.x;
+  for (int i = tid; i < TILE * BraRec<N>::stride; i += TILE * TILE) {
+    const int f = kb + i / BraRec<N>::stride;
+    sbra[i] = (f < K) ? bra[(size_t)kb * BraRec<N>::stride + i] : 0.0;
+  }
+  const int kstride = T * KetRec<N>::stride;
+  for (int i = tid; i < TILE * kstride; i += TILE * TILE) {
+    const int f = lb + i / kstride;
+    sket[i] = (f < K) ? ket[(size_t)lb * kstride + i] : 0.0;
+  }
+  __syncthreads();
+  double sum[NO], err[NO];
+#pragma unroll
+  for (int a = 0; a < NO; ++a) sum[a] = err[a] = 0.0;
+  const int k = kb + threadIdx.x, l = lb + threadIdx.y;
+  if (k < K && l <= k) {
+    const double* br = sbra + threadIdx.x * BraRec<N>::stride;
+    const double* kr0 = sket + threadIdx.y * kstride;
+    const zd* ck = reinterpret_cast<const zd*>(br + BraRec<N>::ck);
+    const double adet = br[BraRec<N>::adet] * bra[(size_t)l * BraRec<N>::stride + BraRec<N>::adet];
+    const double f = (k == l) ? 1.0 / diagS[k] : 2.0 / sqrt(diagS[k] * diagS[l]);
+    const zd ci = cvec[k], cj = cvec[l];
+    const double wr = ci.x * cj.x + ci.y * cj.y, wi = ci.x * cj.y - ci.y * cj.x;   // conj(c_k) c_l
+    constexpr double two_over_sqrtpi = 1.12837916709551257389615890312154517;
+    constexpr double inv_pi32 = 0.179587122125166564899490489694101733;    // pi^(-3/2)
+    for (int t = 0; t < T; ++t) {
+      const double* kr = kr0 + (size_t)t * KetRec<N>::stride;
+      PairCore<N> pc;
+      pair_energy<N>(sc, ck, reinterpret_cast<const zd*>(kr + KetRec<N>::ct),
+                     reinterpret_cast<const zd*>(kr + KetRec<N>::nl), adet, pc);
+      zd O[NO];
+      int q = 0;
+#pragma unroll
+      for (int a = 0; a < N; ++a)
+#pragma unroll
+        for (int b = a; b < N; ++b, ++q) {
+          const zd rs = pc.rs[(a == b) ? sidx(a, a) : sidx(b, a)];
+          const zd rs2 = rs * rs;
+          const zd Srs = pc.S * rs;
+          O[q] = 2.0 * (pc.S * rs2);
+          O[m + q] = two_over_sqrtpi * Srs;
+          O[2 * m + q] = two_over_sqrtpi * (pc.S * zrcp(rs));
+          O[3 * m + q] = 1.5 * (pc.S * zrcp(rs2));
+          O[4 * m + q] = inv_pi32 * (Srs * rs2);
+        }
+      O[5 * m] = pc.T + pc.V;
+      O[5 * m + 1] = pc.S;
+      O[5 * m + 2] = pc.T;
+      O[5 * m + 3] = pc.V;
+      const double fc = f * sc.coef[t];
+#pragma unroll
+      for (int a = 0; a < NO; ++a) {
+        const double ro = O[a].x, io = O[a].y;
+        const double beta = wr * ro - wi * io;
+        const double mu = fabs(ci.x * (cj.x * ro - cj.y * io)) + fabs(cj.x * (ci.x * ro + ci.y * io)) +
+                          fabs(ci.y * (cj.y * ro + cj.x * io)) + fabs(cj.y * (ci.y * ro - ci.x * io));
+        sum[a] += fc * beta;
+        err[a] += fabs(fc * (mu + 2.0 * fabs(beta)));
+      }
+    }
+  }
+  // fixed-order reduction of the 256 pairs' sums
+  double* red = lds;   // 4 waves x 2 NO (the records are no longer read)
+  __syncthreads();
+  const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+  for (int a = 0; a < NO; ++a) {
+    double v = sum[a], e = err[a];
+    for (int o = 32; o > 0; o >>= 1) {
+      v += __shfl_xor(v, o);
+      e += __shfl_xor(e, o);
+    }
+    if (lane == 0) {
+      red[wave * 2 * NO + a] = v;
+      red[wave * 2 * NO + NO + a] = e;
+    }
+  }
+  __syncthreads();
+  for (int a = tid; a < 2 * NO; a += TILE * TILE)
+    part[(size_t)blockIdx.x * 2 * NO + a] = ((red[a] + red[2 * NO + a]) + red[4 * NO + a]) + red[6 * NO + a];
+}
+
+__global__ void __launch_bounds__(256) k_expect_finish(int nvals, long long ntiles, const double* __restrict__ part,
+                                                       double* __restrict__ out) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nvals) return;
+  double s = 0.0;
+  for (long long t = 0; t < ntiles; ++t) s += part[(size_t)t * nvals + a];
+  out[a] = s;
+}
+
+template <int N>
+static int launch_expect(const Ctx& c, const zd* cvec, double* part, double* out, hipStream_t s) {
+  const int K = c.K;
+  const int nt = (K + TILE - 1) / TILE;
+  const long long ntiles = (long long)nt * (nt + 1) / 2;
+  size_t lds = sizeof(double) * TILE * (BraRec<N>::stride + c.T * KetRec<N>::stride);
+  lds = std::max(lds, sizeof(double) * 8 * Expc<N>::nobs);
+  hipLaunchKernelGGL(k_expect<N>, dim3((unsigned)ntiles), dim3(TILE, TILE), lds, s, c.sc, K, c.bra, c.ket, c.diagS,
+                     cvec, part);
+  const int nv = 2 * Expc<N>::nobs;
+  hipLaunchKernelGGL(k_expect_finish, dim3((nv + 255) / 256), dim3(256), 0, s, nv, ntiles, part, out);
+  return (int)hipGetLastError();
+}
+
+int expectation_values(const Ctx& c, const zd* cvec, double* part, double* out, hipStream_t s) {
+  switch (c.n) {
+    case 1: return launch_expect<1>(c, cvec, part, out, s);
+    case 2: return launch_expect<2>(c, cvec, part, out, s);
+    case 3: return launch_expect<3>(c, cvec, part, out, s);
+    case 4: return launch_expect<4>(c, cvec, part, out, s);
+    case 5: return launch_expect<5>(c, cvec, part, out, s);
+  }
+  return -1;
+}
+
+size_t expectation_partial_doubles(int n, int K) {
+  const int nt = (K + TILE - 1) / TILE;
+  const int m = n * (n + 1) / 2;
+  return (size_t)nt * (nt + 1) / 2 * 2 * (5 * m + 4);
 }
 
 // ------------------------------------------------------ overlap penalty

@@ -44,7 +44,7 @@ EXPORTS = [
     "ecg_compute_mat_elem", "ecg_compute_mat_elem_and_deriv", "ecg_ghepiis", "ecg_energy", "ecg_energy_grad",
     "ecg_energy_grad_stored", "ecg_batch_create", "ecg_batch_destroy", "ecg_batch_set_config",
     "ecg_batch_energy_grad", "ecg_trace", "ecg_trace_read", "ecg_get_params", "ecg_screen_prepare", "ecg_screen", "ecg_screen_loop", "ecg_last_timings",
-    "ecg_set_overlap_penalty", "ecg_overlap_penalty", "ecg_set_params_device",
+    "ecg_set_overlap_penalty", "ecg_overlap_penalty", "ecg_set_params_device", "ecg_expectation_values",
 ]
 
 
@@ -95,6 +95,7 @@ def load(path: str = LIB_PATH):
     L.ecg_set_overlap_penalty.argtypes = [vp, c_int, c_double, c_double, c_int]
     L.ecg_set_params_device.argtypes = [vp, c_int, vp]
     L.ecg_overlap_penalty.argtypes = [vp, c_int, c_double, c_double, _d, _d]
+    L.ecg_expectation_values.argtypes = [vp, _d, _d, _d]
     for name in EXPORTS:
         getattr(L, name).restype = c_int
     _lib = L
@@ -279,6 +280,17 @@ class EcgContext:
 
     TRACE_LABELS = ("offdiag", "adjoint", "ldl_update", "ldl_diag", "ldl_panel", "trsv_fwd", "trsv_bwd", "gemv",
                     "form_shifted", "ldl_tail")
+
+    def expectation_values(self, c=None):
+        """(values, errors) of ecg_expectation_values: 5 m + 4 reals each,
+        m = n(n+1)/2; c None uses the last energy call's eigenvector."""
+        nobs = 5 * self.system.n * (self.system.n + 1) // 2 + 4
+        vals = np.zeros(nobs)
+        errs = np.zeros(nobs)
+        cc = None if c is None else np.ascontiguousarray(c, dtype=np.complex128)
+        _check("ecg_expectation_values", load().ecg_expectation_values(
+            self._h, cc.ctypes.data_as(_d) if cc is not None else None, _p(vals), _p(errs)))
+        return vals, errs
 
     def trace(self, enable=True):
         _check("ecg_trace", load().ecg_trace(self._h, int(bool(enable))))

@@ -17,6 +17,7 @@
 !                  EnergyGB / EnergyIB     src/workproc.f90:1841, 2063
 !                  ComputeOverlapPenalty   src/workproc.f90:1491
 !                  ComputeOverlapPenaltyAndAddGradient  src/workproc.f90:1571
+!                  EcgExpectationValues               src/workproc.f90:8158-8300 (ExpectationValuesG's loop)
 !                plus EcgHostInit / EcgHostFinish (context set-up, no
 !                counterpart in the reference: ProgramDataInit ends by
 !                calling EcgHostInit).
@@ -211,6 +212,14 @@ module ecg_capi
       real(c_double), intent(out) :: total
       type(c_ptr), value :: grad          ! npt*(K-nfru) doubles or c_null_ptr
     end function
+    integer(c_int) function ecg_expectation_values(ctx, c, values, errors) &
+        bind(C, name="ecg_expectation_values")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: c              ! K complex or c_null_ptr (last eigenvector)
+      real(c_double), intent(out) :: values(*)
+      real(c_double), intent(out) :: errors(*)
+    end function
     integer(c_int) function ecg_last_timings(ctx, ms, nmax) bind(C, name="ecg_last_timings")
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx
@@ -253,7 +262,8 @@ module ecg_hostpath
   implicit none
   private
   public :: EcgHostInit, EcgHostFinish, MatrixElements, ComputeMatElem, ComputeMatElemAndDeriv, GHEPIIS, &
-            EnergyGA, EnergyIA, EnergyGB, EnergyIB, ComputeOverlapPenalty, ComputeOverlapPenaltyAndAddGradient
+            EnergyGA, EnergyIA, EnergyGB, EnergyIB, ComputeOverlapPenalty, ComputeOverlapPenaltyAndAddGradient, &
+            EcgExpectationValues
   type(c_ptr), save :: ctx = c_null_ptr
 
 contains
@@ -399,6 +409,17 @@ contains
     integer :: rc
     rc = ecg_overlap_penalty(ctx, Glob_nfru, MaxPairOverlapPenalty, OverlapThreshold2, TotalPenalty, c_null_ptr)
     if (rc /= ECG_OK) TotalPenalty = ieee_value(1.0_dprec, ieee_quiet_nan)
+  end subroutine
+
+  ! src/workproc.f90:8158-8300 — the main loop of ExpectationValuesG on the
+  ! device, with the eigenvector of the last energy call: Values and Errors
+  ! (5 m + 4 each, m = Glob_n (Glob_n + 1) / 2) hold <1/r^2>, <1/r>, <r>,
+  ! <r^2>, <delta(r)> for the pairs (a, b >= a), then <H>, <S>, <T>, <V>,
+  ! and the driver's rounding-error estimates.
+  subroutine EcgExpectationValues(Values, Errors, ErrorCode)
+    real(dprec) :: Values(:), Errors(:)
+    integer :: ErrorCode
+    ErrorCode = ecg_expectation_values(ctx, c_null_ptr, Values, Errors)
   end subroutine
 
   ! src/workproc.f90:1571 — as ComputeOverlapPenalty, adding the penalty's

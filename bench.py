@@ -328,6 +328,31 @@ def optimise_leg(system, base, K, nevals=60, warm=3):
     }
 
 
+def expectation_leg(system, base, K, reps=3):
+    """Expectation values (<1/r^2>, <1/r>, <r>, <r^2>, <delta(r)> of every
+    pseudo-particle pair, <H>, <S>, <T>, <V>) in the lowest eigenvector:
+    one fused pass over the K(K+1)/2 pairs x T terms after an energy call."""
+    ctx = EcgContext(system)
+    ctx.set_params(base)
+    st, E, _, _ = ctx.energy(1, float("nan"), 1e-12, 60)
+    vals, errs = ctx.expectation_values()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        vals, errs = ctx.expectation_values()
+    t = (time.perf_counter() - t0) / reps
+    ctx.close()
+    m = system.n * (system.n + 1) // 2
+    npairs = K * (K + 1) // 2
+    return {
+        "workload": "H2-like n=3 (T=4), K=%d: %d expectation values in the lowest eigenvector" % (K, len(vals)),
+        "ms": 1e3 * t,
+        "pair_terms_per_s": npairs * system.T / t,
+        "E": float(E), "H": float(vals[5 * m]), "S": float(vals[5 * m + 1]),
+        "virial": float(-vals[5 * m + 3] / vals[5 * m + 2]),
+        "max_rel_error_estimate": float(np.max(errs / np.maximum(np.abs(vals), 1e-300))),
+    }
+
+
 def memory_modes_leg(system, Ks=(1000, 4000), reps=3):
     """E+grad at K=1000 and K=4000 in the two memory modes: recompute (the
     fused adjoint pass; O(K) extra device memory) and stored (per-pair
@@ -450,6 +475,7 @@ def main():
     ap.add_argument("--no-modes", action="store_true", help="skip the stored/recompute memory-mode leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the multi-problem batch leg")
     ap.add_argument("--no-opt", action="store_true", help="skip the one-function optimisation leg")
+    ap.add_argument("--no-expect", action="store_true", help="skip the expectation-value leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first issue device copies of a known size (reference for HBM counters)")
     args = ap.parse_args()
@@ -580,6 +606,8 @@ def main():
     }
     if not args.no_opt:
         out["optimise_one_function"] = optimise_leg(system, base, K)
+    if not args.no_expect:
+        out["expectation_values"] = expectation_leg(system, base, K)
     if not args.no_screen:
         out["screening"] = screening_leg(system, base, K)
     if not args.no_modes:

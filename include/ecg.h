@@ -249,6 +249,20 @@ int ecg_set_overlap_penalty(ecg_ctx* ctx, int allowed, double max_pair_penalty, 
 int ecg_overlap_penalty(ecg_ctx* ctx, int nfru, double max_pair_penalty, double threshold2, double* total,
                         double* grad);
 
+/* Expectation values (MatrixElementsForExpcVals, src/matelem.f90:949-1983,
+ * summed as the reference's driver sums them, src/workproc.f90:8158-8300,
+ * with the bra unpermuted and the ket's Y^dagger Y terms) in the
+ * eigenvector c: K complex, normalised c^H S c = 1 for the normalised S;
+ * NULL uses the eigenvector of the last ecg_energy / ecg_energy_grad call.
+ * values (5 m + 4 doubles, m = n(n+1)/2): <1/r^2>, <1/r>, <r>, <r^2>,
+ * <delta(r)>, each for the pseudo-particle pairs (a, b >= a) in the order
+ * (1,1), (1,2), .., (1,n), (2,2), .. (a = b: pseudo-particle a's distance
+ * from particle 1), then <H>, <S>, <T>, <V>.  errors (nullable, same
+ * layout): the driver's estimate of each value's rounding error.  The
+ * reference's Drachmanized delta, mass-velocity, Darwin and orbit-orbit
+ * terms, correlation functions and densities are not computed. */
+int ecg_expectation_values(ecg_ctx* ctx, const double* c, double* values, double* errors);
+
 /* Device time (ms, HIP events on the context's stream) of the phases of
  * the last ecg_energy / ecg_energy_grad call: [0] assembly, [1] eigen step
  * (all factorisations and iterations), [2] adjoint pass, [3] whole call,

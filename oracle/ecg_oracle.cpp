@@ -457,4 +457,101 @@ int oracle_ghepiis(int k, int n, double* M, int nM, double* invD, const double* 
   return err;
 }
 
+
+// ------------------------------------------------- expectation values
+// Pair operators of MatrixElementsForExpcVals (src/matelem.f90:1236-1275,
+// 1306-1349) for symmetry term `term`, bra unpermuted: for every
+// (a, b >= a) (a = b: pseudo-particle a's distance from particle 1), with
+// TrCJ = tr(C^{-1} J_ab) = C^{-1}_aa (+ C^{-1}_bb - 2 C^{-1}_ab):
+//   1/r^2 = 2 S / TrCJ,  1/r = (2/sqrt(pi)) S / sqrt(TrCJ),
+//   r = (2/sqrt(pi)) S sqrt(TrCJ),  r^2 = (3/2) S TrCJ,
+//   delta(r) = S / (pi^(3/2) sqrt(TrCJ) TrCJ);
+// then H, S, T, V.  out: 5 n(n+1)/2 + 4 complex values, each block in the
+// driver's (a, b >= a) order (src/workproc.f90:8183-8212).
+static void expc_pair(const System<double>& sy, const double* pk, const double* pl, const int* Pm, cd* out) {
+  cd H, S;
+  PairParts<double> pp;
+  matrix_elements<double>(sy, pk, pl, Pm, 0, H, S, nullptr, nullptr, &pp);
+  const int n = sy.n, m = n * (n + 1) / 2;
+  const double sqrtpi = 1.77245385090551602729816748334115, pi = 3.14159265358979323846264338327950;
+  const cd cnumb = (2.0 / sqrtpi) * S, caux = 1.5 * S, caux1 = S / (pi * sqrtpi), cmp = 2.0 * S;
+  int c = 0;
+  for (int a = 0; a < n; ++a)
+    for (int b = a; b < n; ++b, ++c) {
+      const cd t = (a == b) ? pp.Ci[a][a] : pp.Ci[a][a] + pp.Ci[b][b] - pp.Ci[b][a] - pp.Ci[b][a];
+      const cd sq = csqrt(t);
+      out[c] = cmp / t;
+      out[m + c] = cnumb / sq;
+      out[2 * m + c] = cnumb * sq;
+      out[3 * m + c] = caux * t;
+      out[4 * m + c] = caux1 / (sq * t);
+    }
+  out[5 * m] = H;
+  out[5 * m + 1] = S;
+  out[5 * m + 2] = pp.T;
+  out[5 * m + 3] = pp.V;
+}
+
+void oracle_expc_pair(int n, int T, const int* P, const int* coef, const double* M, double q0, const double* q,
+                      const double* pk, const double* pl, int term, double* out) {
+  System<double> sy = make_sys<double>(n, T, P, coef, M, q0, q);
+  const int nobs = 5 * n * (n + 1) / 2 + 4;
+  std::vector<cd> o(nobs);
+  expc_pair(sy, pk, pl, P + (size_t)term * n * n, o.data());
+  for (int a = 0; a < nobs; ++a) st(out, a, o[a]);
+}
+
+// The driver's sums (src/workproc.f90:8158-8230 and :8279, symmetry
+// adaptation through the ket's Y^dagger Y terms): over pairs i >= j,
+// factor = 1/diagS_i (i = j) or 2/sqrt(diagS_i diagS_j), per term k and
+// value a, with O = O_ij of term k:
+//   beta = Re(conj(c_i) c_j O),
+//   mu = |Re c_i (Re c_j Re O - Im c_j Im O)| + |Re c_j (Re c_i Re O + Im c_i Im O)|
+//      + |Im c_i (Im c_j Re O + Re c_j Im O)| + |Im c_j (Im c_i Re O - Re c_i Im O)|,
+//   out_a += factor coef_k beta,  err_a += |factor coef_k (mu + 2 |beta|)|,
+// and err is finally scaled by epsilon(1.0).  c: K complex.
+int oracle_expectation_values(int n, int T, const int* P, const int* coef, const double* M, double q0,
+                              const double* q, int K, const double* params, const double* c,
+                              const double* diagS, double* out, double* err, int nthreads) {
+  System<double> sy = make_sys<double>(n, T, P, coef, M, q0, q);
+  const int npt = n * (n + 1), nobs = 5 * n * (n + 1) / 2 + 4;
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  const int nth = omp_get_max_threads();
+  std::vector<double> acc((size_t)nth * 2 * nobs, 0.0);
+#pragma omp parallel
+  {
+    const int tid = omp_get_thread_num();
+    double* s_ = acc.data() + (size_t)tid * 2 * nobs;
+    double* e_ = s_ + nobs;
+    std::vector<cd> o(nobs);
+#pragma omp for schedule(dynamic, 4)
+    for (int i = 0; i < K; ++i)
+      for (int j = 0; j <= i; ++j) {
+        const double rci = c[2 * i], ici = c[2 * i + 1], rcj = c[2 * j], icj = c[2 * j + 1];
+        const double factor = (i == j) ? 1.0 / std::sqrt(diagS[i] * diagS[i]) : 2.0 / std::sqrt(diagS[i] * diagS[j]);
+        for (int k = 0; k < T; ++k) {
+          expc_pair(sy, params + (size_t)npt * i, params + (size_t)npt * j, P + (size_t)k * n * n, o.data());
+          for (int a = 0; a < nobs; ++a) {
+            const double ro = o[a].re, io = o[a].im;
+            const double beta = (rci * rcj + ici * icj) * ro + (ici * rcj - rci * icj) * io;
+            const double mu = std::fabs(rci * (rcj * ro - icj * io)) + std::fabs(rcj * (rci * ro + ici * io)) +
+                              std::fabs(ici * (icj * ro + rcj * io)) + std::fabs(icj * (ici * ro - rci * io));
+            s_[a] += factor * coef[k] * beta;
+            e_[a] += std::fabs(factor * coef[k] * (mu + 2.0 * std::fabs(beta)));
+          }
+        }
+      }
+  }
+  for (int a = 0; a < nobs; ++a) {
+    double sa = 0.0, ea = 0.0;
+    for (int t = 0; t < nth; ++t) {
+      sa += acc[(size_t)t * 2 * nobs + a];
+      ea += acc[(size_t)t * 2 * nobs + nobs + a];
+    }
+    out[a] = sa;
+    if (err) err[a] = ea * 2.220446049250313e-16;
+  }
+  return 0;
+}
+
 }  // extern "C"

@@ -77,12 +77,20 @@ template <class R> struct System {
 };
 
 // ------------------------------------------------------------- one pair
+// Quantities of one pair that MatrixElementsForExpcVals reuses
+// (src/matelem.f90:1180-1235 computes them the same way).
+template <class R> struct PairParts {
+  cx<R> Ci[NN][NN];   // C^{-1}
+  cx<R> T, V;
+};
+
 // Restatement of MatrixElements (src/matelem.f90:12-750) for one symmetry
 // matrix P (column-major n x n).  Dk, Dl receive 4*np complex values each
-// (dH/dvechL, dH/dvechB, dS/dvechL, dS/dvechB) when gradflag asks for them.
+// (dH/dvechL, dH/dvechB, dS/dvechL, dS/dvechB) when gradflag asks for them;
+// parts (optional) receives C^{-1}, T and V.
 template <class R>
 void matrix_elements(const System<R>& sy, const R* pk, const R* pl, const int* Pm, int gradflag,
-                     cx<R>& Hkl, cx<R>& Skl, cx<R>* Dk, cx<R>* Dl) {
+                     cx<R>& Hkl, cx<R>& Skl, cx<R>* Dk, cx<R>* Dl, PairParts<R>* parts = nullptr) {
   const int n = sy.n, np = n * (n + 1) / 2;
   auto P = [&](int i, int j) { return R(Pm[i + n * j]); };
   const bool gk = (gradflag == 1 || gradflag == 3);
@@ -216,6 +224,12 @@ void matrix_elements(const System<R>& sy, const R* pk, const R* pl, const int* P
     Vkl = acc;
   }
   Hkl = Tkl + Vkl;
+  if (parts) {
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) parts->Ci[i][j] = Ci[i][j];
+    parts->T = Tkl;
+    parts->V = Vkl;
+  }
   if (!gk && !gl) return;
 
   cx<R> dHLk[NNP], dHBk[NNP], dSLk[NNP], dSBk[NNP];

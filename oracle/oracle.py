@@ -63,6 +63,12 @@ def lib():
         L.oracle_overlap_penalty_grad.restype = ctypes.c_double
         L.oracle_overlap_penalty_grad.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _d, ctypes.c_int, _d,
                                                   _d, ctypes.c_double, ctypes.c_double, _d]
+        L.oracle_expc_pair.restype = None
+        L.oracle_expc_pair.argtypes = [ctypes.c_int, ctypes.c_int, _i, _i, _d, ctypes.c_double, _d, _d, _d,
+                                       ctypes.c_int, _d]
+        L.oracle_expectation_values.restype = ctypes.c_int
+        L.oracle_expectation_values.argtypes = [ctypes.c_int, ctypes.c_int, _i, _i, _d, ctypes.c_double, _d,
+                                                ctypes.c_int, _d, _d, _d, _d, _d, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -223,6 +229,37 @@ def overlap_penalty_grad(S, diagS, D, nfru, bmax, t2, grad=None):
     P = lib().oracle_overlap_penalty_grad(npt, K, nfru, S.ctypes.data_as(_d), K, _pd(np.ascontiguousarray(diagS)),
                                           D.ctypes.data_as(_d), float(bmax), float(t2), _pd(g))
     return P, g
+
+
+def n_expectation_values(n):
+    """Number of values of expectation_values for n pseudo-particles."""
+    return 5 * n * (n + 1) // 2 + 4
+
+
+def expc_pair(system, pk, pl, term=0):
+    """One pair's operators of MatrixElementsForExpcVals for symmetry term
+    `term` (complex array of n_expectation_values(n) values: 1/r^2, 1/r, r,
+    r^2, delta(r) for (a, b >= a), then H, S, T, V)."""
+    s = _Sys(system)
+    out = np.zeros(n_expectation_values(system.n), np.complex128)
+    lib().oracle_expc_pair(*s.args(), _pd(np.ascontiguousarray(pk, dtype=np.float64)),
+                           _pd(np.ascontiguousarray(pl, dtype=np.float64)), int(term), out.ctypes.data_as(_d))
+    return out
+
+
+def expectation_values(system, params, c, diagS, nthreads=0):
+    """The reference driver's expectation values in the eigenvector c
+    (complex K, c^H S c = 1 for the normalised S) and their rounding-error
+    estimates: (values, errors), each n_expectation_values(n) reals."""
+    s = _Sys(system)
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    K = params.shape[0]
+    nobs = n_expectation_values(system.n)
+    out = np.zeros(nobs)
+    err = np.zeros(nobs)
+    lib().oracle_expectation_values(*s.args(), K, _pd(params), np.ascontiguousarray(c, dtype=np.complex128).ctypes.data_as(_d),
+                                    _pd(np.ascontiguousarray(diagS, dtype=np.float64)), _pd(out), _pd(err), nthreads)
+    return out, err
 
 
 def ghepiis(k, M, invD, B, apprlambda, v, tol, maxiter=20, specif_norm=0):

@@ -109,3 +109,71 @@ def n1_quadrature(m0, m1, q0, q1, pk, pl, dps=30):
         pot = radial(lambda r: 4 * mp.pi * r * mp.exp(-c * r ** 2))
         norm = mp.sqrt(nk * nl)
         return complex(ov / norm), complex(inv_mu2 * kin / norm), complex(mp.mpf(q0) * mp.mpf(q1) * pot / norm)
+
+
+def overlap_shift(system, pk, pl, P, a, b, s, dps=30):
+    """S of the pair with C replaced by C + s J (J = w w^T, w = e_a for
+    a == b, e_a - e_b otherwise): the unnormalised <k| exp(-s r_ab^2) |l>
+    up to the pair's constant, at `dps` digits (mpmath determinant; the
+    working precision grows with log10 s so that C stays resolved beside
+    s J at the quadratures' far nodes)."""
+    s = mp.mpf(s)
+    extra = int(mp.log10(abs(s))) + 5 if abs(s) > 1 else 0
+    with mp.workdps(dps + extra):
+        n = system.n
+        Lk, Bk = _unpack(pk, n)
+        Ll, Bl = _unpack(pl, n)
+        Pm = mp.matrix([[int(P[i][j]) for j in range(n)] for i in range(n)])
+        C = (Lk * Lk.T - mp.mpc(0, 1) * Bk) + Pm.T * (Ll * Ll.T + mp.mpc(0, 1) * Bl) * Pm
+        w = mp.zeros(n, 1)
+        w[a] = 1
+        if b != a:
+            w[b] = -1
+        Cs = C + s * (w * w.T)
+        detLk = mp.fprod([Lk[i, i] for i in range(n)])
+        detLl = mp.fprod([Ll[i, i] for i in range(n)])
+        z = abs(detLk * detLl) / mp.det(Cs)
+        return mp.power(2, mp.mpf(3 * n) / 2) * z * mp.sqrt(z)
+
+
+def expc_by_integrals(system, pk, pl, P, a, b, dps=30):
+    """<1/r^2>, <1/r>, <r>, <r^2>, <delta(r)> of r = |r_a - r_b| (r_a for
+    a == b) between the pair's functions, from the shifted overlap f(s):
+      1/r^2 = int_0^inf exp(-s r^2) ds,   1/r = (2/sqrt(pi)) int_0^inf exp(-t^2 r^2) dt,
+      r = (1/sqrt(pi)) int_0^inf (1 - exp(-t^2 r^2)) / t^2 dt,
+      r^2 = -d/ds exp(-s r^2) at 0,  delta(r) = lim (s/pi)^(3/2) exp(-s r^2)."""
+    with mp.workdps(dps):
+        f = lambda s: overlap_shift(system, pk, pl, P, a, b, s, dps)
+
+        def cquad(g):
+            return mp.mpc(mp.quad(lambda t: mp.re(g(t)), [0, 1, mp.inf]), mp.quad(lambda t: mp.im(g(t)), [0, 1, mp.inf]))
+
+        f0 = f(0)
+        rm2 = cquad(f)
+        rm = 2 / mp.sqrt(mp.pi) * cquad(lambda t: f(t * t))
+        r = 1 / mp.sqrt(mp.pi) * cquad(lambda t: (f0 - f(t * t)) / (t * t))
+        h = mp.mpf(10) ** (-dps // 3)
+        r2 = -(f(h) - f(-h)) / (2 * h)
+        s1 = mp.mpf(10) ** (dps // 2)
+        g = lambda s: (s / mp.pi) ** mp.mpf(1.5) * f(s)
+        delta = 2 * g(2 * s1) - g(s1)
+        return [complex(v) for v in (rm2, rm, r, r2, delta)]
+
+
+def n1_moments(pk, pl, dps=30):
+    """Two-body (n = 1) <k| r^-2, r^-1, r, r^2 |l> and <k| delta(r) |l> by
+    radial quadrature, normalised by sqrt(<k|k><l|l>)."""
+    with mp.workdps(dps):
+        ak = mp.mpf(pk[0]) ** 2 + mp.mpc(0, 1) * mp.mpf(pk[1])
+        al = mp.mpf(pl[0]) ** 2 + mp.mpc(0, 1) * mp.mpf(pl[1])
+        c = mp.conj(ak) + al
+
+        def radial(f):
+            return mp.mpc(mp.quad(lambda r: mp.re(f(r)), [0, mp.inf]), mp.quad(lambda r: mp.im(f(r)), [0, mp.inf]))
+
+        nk = mp.quad(lambda r: 4 * mp.pi * r ** 2 * mp.exp(-2 * mp.re(ak) * r ** 2), [0, mp.inf])
+        nl = mp.quad(lambda r: 4 * mp.pi * r ** 2 * mp.exp(-2 * mp.re(al) * r ** 2), [0, mp.inf])
+        norm = mp.sqrt(nk * nl)
+        vals = [radial(lambda r, p=p: 4 * mp.pi * r ** (2 + p) * mp.exp(-c * r ** 2)) for p in (-2, -1, 1, 2)]
+        vals.append(mp.mpc(1))   # conj(phi_k(0)) phi_l(0)
+        return [complex(v / norm) for v in vals]

@@ -1,0 +1,90 @@
+"""CPU checks that pin the oracle's expectation values (no GPU).
+
+MatrixElementsForExpcVals (src/matelem.f90:949-1983) gives, per pair and
+symmetry term, closed forms of <1/r^2>, <1/r>, <r>, <r^2> and <delta(r)> in
+S and tr(C^-1 J); the driver (src/workproc.f90:8158-8300) contracts them
+with the eigenvector.  Pinned here without retyping those forms:
+  * each operator as an integral, derivative or limit of the overlap with
+    C -> C + s J (exp(-s r^2) inserted between the functions), evaluated
+    at 30-60 digits with mpmath's own determinant (tests/mp_ecg.py), at
+    n = 2 and 3 with B != 0 and non-identity symmetry matrices;
+  * n = 1 against radial quadrature of the actual integrals;
+  * the contraction: <S> = 1 and <H> = E for the eigenvector of the HD+
+    fixture, <H> = <T> + <V>, and the virial ratio -<V>/<T> = 2 of that
+    optimised basis.
+Not built (and not pinned): the Drachmanized delta functions, the
+mass-velocity, Darwin and orbit-orbit terms, correlation functions and
+particle densities on grids.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from ecg_nonbo import make_system, named_system, read_inout, seeded_basis, SEED_BASE
+from oracle import oracle
+import mp_ecg
+
+
+def _pairs_of(n):
+    return [(a, b) for a in range(n) for b in range(a, n)]
+
+
+@pytest.mark.parametrize("name,term", [("hminus", 1), ("h2", 3)])
+def test_expc_pair_against_shifted_overlap_integrals(name, term):
+    sy = named_system(name)
+    p = seeded_basis(sy.n, 6, SEED_BASE + 51)
+    pk, pl = p[2], p[4]
+    o = oracle.expc_pair(sy, pk, pl, term)
+    m = sy.n * (sy.n + 1) // 2
+    P = np.asarray(sy.yhy_mats[term])
+    for c, (a, b) in enumerate(_pairs_of(sy.n)):
+        ref = mp_ecg.expc_by_integrals(sy, pk, pl, P, a, b, dps=24)
+        got = [o[q * m + c] for q in range(5)]
+        for q in range(5):
+            assert abs(got[q] - ref[q]) <= 1e-11 * abs(ref[q]), (name, term, a, b, q, got[q], ref[q])
+    H, S = mp_ecg.pair_hs(sy, pk, pl, P)
+    assert abs(o[5 * m + 1] - complex(S)) <= 1e-12 * abs(complex(S))
+    assert abs(o[5 * m] - complex(H)) <= 1e-11 * abs(complex(H))
+    assert abs(o[5 * m] - o[5 * m + 2] - o[5 * m + 3]) <= 1e-14 * abs(o[5 * m])
+
+
+@pytest.mark.parametrize("pk,pl", [([0.8, 0.3], [1.7, -0.2]), ([0.05, 0.01], [3.0, 1.5]), ([1.2, 0.0], [1.2, 0.0])])
+def test_expc_pair_n1_radial_quadrature(pk, pl):
+    sy = make_system([1.0, 1836.15267261], [1.0, -1.0], "(P11)")
+    o = oracle.expc_pair(sy, np.array(pk), np.array(pl), 0)
+    Skk = oracle.expc_pair(sy, np.array(pk), np.array(pk), 0)[6]
+    Sll = oracle.expc_pair(sy, np.array(pl), np.array(pl), 0)[6]
+    norm = np.sqrt(Skk.real * Sll.real)
+    ref = mp_ecg.n1_moments(pk, pl)
+    for q in range(5):
+        assert abs(o[q] / norm - ref[q]) <= 1e-12 * abs(ref[q]), (q, o[q] / norm, ref[q])
+
+
+def test_expectation_values_hdplus_fixture(golden_dir):
+    """Contraction with the fixture's eigenvector: <S> = 1, <H> = E,
+    <H> = <T> + <V>, virial ratio 2, and the proton-deuteron distance."""
+    f = read_inout(os.path.join(golden_dir, "hdplus", "inout_1_500.txt"))
+    K = f.basis_size
+    sigma = f.current_energy * f.invit_parameter
+    H, S, dH, dS = oracle.assemble(f.system, f.params, "I", sigma)
+    invD = np.zeros(K, np.complex128)
+    err, lam, x, rel, nit = oracle.ghepiis(1, H.copy(order="F"), invD, S.copy(order="F"), sigma,
+                                           np.ones(K, np.complex128), f.eigval_tolerance)
+    Hg, Sg, dHg, dSg = oracle.assemble(f.system, f.params, "G")
+    Sf = oracle.full_hermitian(Hg, Sg, dHg)[1]
+    x = x / np.sqrt((x.conj() @ Sf @ x).real)
+    vals, errs = oracle.expectation_values(f.system, f.params, x, dSg)
+    n = f.system.n
+    m = n * (n + 1) // 2
+    Hx, Sx, Tx, Vx = vals[5 * m:5 * m + 4]
+    assert abs(Sx - 1.0) <= 1e-10
+    assert abs(Hx - lam) <= 1e-10 * abs(lam)
+    assert abs(Hx - Tx - Vx) <= 1e-12
+    assert abs(-Vx / Tx - 2.0) <= 1e-6, -Vx / Tx
+    # r_1 (particle 2 from particle 1 = the deuteron-proton distance of HD+)
+    r12 = vals[2 * m + 0]
+    assert 2.0 < r12 < 2.1, r12
+    assert np.all(errs >= 0.0) and np.all(errs <= 1e-6 * np.maximum(1.0, np.abs(vals)))

@@ -1,0 +1,80 @@
+"""GPU parity of the expectation values (ecg_expectation_values).
+
+Reference behaviour: MatrixElementsForExpcVals (src/matelem.f90:949-1983)
+per pair and symmetry term, summed by the driver over the lower triangle
+with the eigenvector (src/workproc.f90:8158-8300).  The device values are
+compared with the oracle's restatement of both (itself pinned on the CPU
+by tests/test_expect.py): each value to 1e-11 relative plus ten times the
+driver's own rounding bound, the bounds themselves to 1e-6 relative.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from ecg_nonbo import make_system, named_system, read_inout, seeded_basis, SEED_BASE
+from ecg_nonbo.hip import EcgContext
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _compare(vals, errs, ov, oe):
+    tol = 1e-11 * np.abs(ov) + 10.0 * oe + 1e-300
+    bad = np.abs(vals - ov) > tol
+    assert not np.any(bad), (np.nonzero(bad), vals[bad], ov[bad])
+    np.testing.assert_allclose(errs, oe, rtol=1e-6, atol=1e-300)
+
+
+@pytest.mark.parametrize("name,K", [("n1", 24), ("hminus", 40), ("h2", 33), ("li", 30)])
+def test_expectation_values_vs_oracle(name, K):
+    say("ecg_expectation_values", name, K)
+    sy = make_system([1.0, 1836.15267261], [1.0, -1.0], "(P11)") if name == "n1" else named_system(name)
+    p = seeded_basis(sy.n, K, SEED_BASE + 61)
+    rng = np.random.Generator(np.random.PCG64(62))
+    c = rng.normal(size=K) + 1j * rng.normal(size=K)
+    _, _, _, dS = oracle.assemble(sy, p, "G")
+    ov, oe = oracle.expectation_values(sy, p, c, dS)
+    ctx = EcgContext(sy)
+    ctx.set_params(p)
+    vals, errs = ctx.expectation_values(c)
+    ctx.close()
+    print("max rel dev", np.max(np.abs(vals - ov) / (np.abs(ov) + 1e-300)))
+    _compare(vals, errs, ov, oe)
+
+
+def test_expectation_values_hdplus_last_eigenvector(golden_dir):
+    say("ecg_expectation_values", "inout_1_500", "last eigenvector")
+    f = read_inout(os.path.join(golden_dir, "hdplus", "inout_1_500.txt"))
+    sigma = f.current_energy * f.invit_parameter
+    ctx = EcgContext(f.system)
+    ctx.set_params(f.params)
+    _, E, c, _ = ctx.energy(1, sigma, f.eigval_tolerance, 20, want_vector=True)
+    vals, errs = ctx.expectation_values()
+    vals2, _ = ctx.expectation_values(c)
+    ctx.close()
+    np.testing.assert_array_equal(vals, vals2)
+    _, _, _, dS = oracle.assemble(f.system, f.params, "G")
+    ov, oe = oracle.expectation_values(f.system, f.params, c, dS)
+    _compare(vals, errs, ov, oe)
+    m = 3
+    H, S, T, V = vals[5 * m:5 * m + 4]
+    print("HD+ <R>", vals[2 * m], "<H>", H, "E", E, "virial", -V / T)
+    assert abs(S - 1.0) <= 1e-10
+    assert abs(H - E) <= 1e-10 * abs(E)
+    assert abs(-V / T - 2.0) <= 1e-6
+
+
+def test_expectation_values_needs_a_vector():
+    sy = named_system("hminus")
+    ctx = EcgContext(sy)
+    ctx.set_params(seeded_basis(sy.n, 8, SEED_BASE + 63))
+    with pytest.raises(Exception):
+        ctx.expectation_values()
+    ctx.close()
