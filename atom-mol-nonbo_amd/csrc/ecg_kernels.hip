@@ -861,9 +861,120 @@ static int launch_prep(const Ctx& c, hipStream_t s) {
   return (int)hipGetLastError();
 }
 
+// ------------------------------------------------------------ thin rows
+// Rows r0..r1-1 when there are few of them (THIN_ROWS or fewer): the tile
+// kernels would leave most threads of each tile idle while one thread walks
+// all T terms of its pair.  Here a group of G >= T lanes takes one pair, a
+// lane per term; lane 0 adds the terms in term order (the same sums as
+// pair_sum_energy) into raw (H, S) of the pair.  Pairs (k, l), l <= k,
+// numbered row by row from (r0, 0).
+constexpr int THIN_ROWS = 32;
+
+__device__ inline void row_pair(long long p, int r0, int& k, int& l) {
+  const long long x = (long long)r0 * (r0 + 1) / 2 + p;
+  int r = (int)((sqrt(8.0 * (double)x + 1.0) - 1.0) * 0.5);
+  while ((long long)(r + 1) * (r + 2) / 2 <= x) ++r;
+  while ((long long)r * (r + 1) / 2 > x) --r;
+  k = r;
+  l = (int)(x - (long long)r * (r + 1) / 2);
+}
+
+template <int N, int G>
+__global__ void __launch_bounds__(256) k_rows_raw(const SysConst sc, int r0, long long npairs,
+                                                  const double* __restrict__ bra, const double* __restrict__ ket,
+                                                  zd* __restrict__ raw) {
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long p = gid / G;
+  const int t = (int)(gid % G);
+  zd h = mkz(0.0, 0.0), sv = mkz(0.0, 0.0);
+  if (p < npairs && t < sc.T) {
+    int k, l;
+    row_pair(p, r0, k, l);
+    const double* br = bra + (size_t)k * BraRec<N>::stride;
+    const double* kr = ket + ((size_t)l * sc.T + t) * KetRec<N>::stride;
+    const double adet = br[BraRec<N>::adet] * bra[(size_t)l * BraRec<N>::stride + BraRec<N>::adet];
+    PairCore<N> pc;
+    pair_energy<N>(sc, reinterpret_cast<const zd*>(br + BraRec<N>::ck), reinterpret_cast<const zd*>(kr + KetRec<N>::ct),
+                   reinterpret_cast<const zd*>(kr + KetRec<N>::nl), adet, pc);
+    h = sc.coef[t] * (pc.T + pc.V);
+    sv = sc.coef[t] * pc.S;
+  }
+  // lane 0 of the group adds the terms in order: H = ((0 + h_0) + h_1) + ...
+  const int base = (threadIdx.x & 63) - t;
+  zd H = mkz(0.0, 0.0), S = mkz(0.0, 0.0);
+  for (int q = 0; q < G; ++q) {
+    const zd hq = mkz(__shfl(h.x, base + q), __shfl(h.y, base + q));
+    const zd sq = mkz(__shfl(sv.x, base + q), __shfl(sv.y, base + q));
+    if (q < sc.T) {
+      H = H + hq;
+      S = S + sq;
+    }
+  }
+  if (p < npairs && t == 0) {
+    raw[2 * p] = H;
+    raw[2 * p + 1] = S;
+  }
+}
+
+// Normalisation and stores of the thin rows: diagonal pairs as k_diag,
+// the others as offdiag_tile (both triangles).  The rows' own diagS come
+// from raw (this launch), earlier rows' from diagS.
+__global__ void __launch_bounds__(256) k_rows_fin(int r0, long long npairs, int ld, const zd* __restrict__ raw,
+                                                  double* __restrict__ diagS, double* __restrict__ diagH,
+                                                  zd* __restrict__ Hm, zd* __restrict__ Sm) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  int k, l;
+  row_pair(p, r0, k, l);
+  const zd H = raw[2 * p], S = raw[2 * p + 1];
+  const long long base = (long long)r0 * (r0 + 1) / 2;
+  auto dS = [&](int i) {
+    return i >= r0 ? raw[2 * ((long long)i * (i + 1) / 2 + i - base) + 1].x : diagS[i];
+  };
+  if (k == l) {
+    const double sd = S.x, hd = H.x / sd;
+    diagS[k] = sd;
+    diagH[k] = hd;
+    Hm[(size_t)k * ld + k] = mkz(hd, 0.0);
+    Sm[(size_t)k * ld + k] = mkz(1.0, 0.0);
+    return;
+  }
+  const double f = 1.0 / sqrt(dS(k) * dS(l));
+  const zd hn = H * f, sn = S * f;
+  Hm[(size_t)l * ld + k] = hn;
+  Sm[(size_t)l * ld + k] = sn;
+  Hm[(size_t)k * ld + l] = zconj(hn);
+  Sm[(size_t)k * ld + l] = zconj(sn);
+}
+
+template <int N>
+static int launch_rows(const Ctx& c, int r0, int r1, hipStream_t s) {
+  const long long npairs = (long long)r1 * (r1 + 1) / 2 - (long long)r0 * (r0 + 1) / 2;
+  const int T = c.sc.T;
+  const int G = T <= 1 ? 1 : T <= 2 ? 2 : T <= 4 ? 4 : T <= 8 ? 8 : 16;
+  const unsigned nb = (unsigned)((npairs * G + 255) / 256);
+  if (c.ev_off0) (void)hipEventRecord(c.ev_off0, s);
+  if (c.tr) c.tr->begin(TR_OFFDIAG, s);
+  switch (G) {
+    case 1: hipLaunchKernelGGL((k_rows_raw<N, 1>), dim3(nb), dim3(256), 0, s, c.sc, r0, npairs, c.bra, c.ket, c.rowbuf); break;
+    case 2: hipLaunchKernelGGL((k_rows_raw<N, 2>), dim3(nb), dim3(256), 0, s, c.sc, r0, npairs, c.bra, c.ket, c.rowbuf); break;
+    case 4: hipLaunchKernelGGL((k_rows_raw<N, 4>), dim3(nb), dim3(256), 0, s, c.sc, r0, npairs, c.bra, c.ket, c.rowbuf); break;
+    case 8: hipLaunchKernelGGL((k_rows_raw<N, 8>), dim3(nb), dim3(256), 0, s, c.sc, r0, npairs, c.bra, c.ket, c.rowbuf); break;
+    default: hipLaunchKernelGGL((k_rows_raw<N, 16>), dim3(nb), dim3(256), 0, s, c.sc, r0, npairs, c.bra, c.ket, c.rowbuf); break;
+  }
+  hipLaunchKernelGGL(k_rows_fin, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, s, r0, npairs, c.ld, c.rowbuf,
+                     c.diagS, c.diagH, c.H, c.S);
+  if (c.tr) c.tr->end(s);
+  if (c.ev_off1) (void)hipEventRecord(c.ev_off1, s);
+  return (int)hipGetLastError();
+}
+
+size_t thin_row_entries(int cap) { return (size_t)2 * THIN_ROWS * cap; }
+
 template <int N>
 static int launch_assemble(const Ctx& c, int r0, int r1, hipStream_t s) {
   if (r1 <= r0) return 0;
+  if (r1 - r0 <= THIN_ROWS && c.rowbuf) return launch_rows<N>(c, r0, r1, s);
   hipLaunchKernelGGL(k_diag<N>, dim3((r1 - r0 + 127) / 128), dim3(128), 0, s, c.sc, r0, r1, c.ld, c.bra, c.ket,
                      c.diagS, c.diagH, c.H, c.S);
   const int tr0 = r0 / TILE, tr1 = (r1 + TILE - 1) / TILE;

@@ -1236,28 +1236,34 @@ __global__ void k_gemv_reduce(int K, int ld, int nch, const zd* __restrict__ par
 }
 
 // ------------------------------------------------------ vector kernels
-// Block-wide reduction helpers for a 1024-thread workgroup.
+// Block-wide reductions for a workgroup of up to 1024 threads (a multiple
+// of 64): a butterfly within each wave, then the waves' results in index
+// order through LDS (fixed order, two barriers).  sh: >= 16 doubles.
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
 __device__ inline double block_sum(double v, double* sh) {
-  const int tid = threadIdx.x;
-  sh[tid] = v;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  v = wave_sum(v);
+  if (lane == 0) sh[w] = v;
   __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-    if (tid < o) sh[tid] += sh[tid + o];
-    __syncthreads();
-  }
-  const double r = sh[0];
+  double r = 0.0;
+  for (int q = 0; q < nw; ++q) r += sh[q];
   __syncthreads();
   return r;
 }
 __device__ inline double block_max(double v, double* sh) {
-  const int tid = threadIdx.x;
-  sh[tid] = v;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  v = wave_max(v);
+  if (lane == 0) sh[w] = v;
   __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-    if (tid < o) sh[tid] = fmax(sh[tid], sh[tid + o]);
-    __syncthreads();
-  }
-  const double r = sh[0];
+  double r = sh[0];
+  for (int q = 1; q < nw; ++q) r = fmax(r, sh[q]);
   __syncthreads();
   return r;
 }
@@ -1267,7 +1273,7 @@ __device__ inline double block_max(double v, double* sh) {
 // out[1] = the scale used; out[4] keeps the previous out[0].
 __global__ void __launch_bounds__(1024) k_invit_finish(int K, zd* __restrict__ x, const zd* __restrict__ v,
                                                        double* __restrict__ out) {
-  __shared__ double sh[1024];
+  __shared__ double sh[16];
   const int tid = threadIdx.x;
   double mx = 0.0;
   for (int i = tid; i < K; i += blockDim.x) mx = fmax(mx, fmax(fabs(x[i].x), fabs(x[i].y)));
@@ -1385,7 +1391,7 @@ __global__ void __launch_bounds__(1024) k_quad_finish(int K, int ld, const zd* _
                                                       const zd* __restrict__ S, double sigma,
                                                       const zd* __restrict__ x, int nparts,
                                                       const double* __restrict__ part, double* __restrict__ out) {
-  __shared__ double sh[1024];
+  __shared__ double sh[16];
   double qs = 0.0, qm = 0.0;
   for (int i = threadIdx.x; i < K; i += blockDim.x) {
     const size_t o = (size_t)i * ld + i;

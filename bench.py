@@ -357,10 +357,14 @@ def memory_modes_leg(system, Ks=(1000, 4000), reps=3):
     """E+grad at K=1000 and K=4000 in the two memory modes: recompute (the
     fused adjoint pass; O(K) extra device memory) and stored (per-pair
     derivatives in Glob_D's layout, 2 npt x K x K complex, then the
-    contraction pass)."""
+    contraction pass).  Every repetition starts from new parameters (all
+    functions moved by 1e-9 relative, uploaded before its timer starts), so
+    nothing of the previous call's matrices or factor is reused."""
     out = []
+    rng = np.random.default_rng(SEED_BASE + 44)
     for K in Ks:
         base = seeded_basis(system.n, K, SEED_BASE + 3 + K)
+        sets = [base * (1.0 + 1e-9 * rng.uniform(-1, 1, base.shape)) for _ in range(reps + 1)]
         ctx = EcgContext(system)
         ctx.set_params(base)
         st, E0, _, _ = ctx.energy(1, float("nan"), 1e-12, 60)
@@ -368,9 +372,11 @@ def memory_modes_leg(system, Ks=(1000, 4000), reps=3):
         row = {"K": K, "pairs": K * (K + 1) // 2, "D_bytes": 2 * system.npt * K * K * 16}
         for mode in ("recompute", "stored"):
             fn = ctx.energy_grad if mode == "recompute" else ctx.energy_grad_stored
+            ctx.set_params(sets[reps])
             fn(0, 1, sigma, 1e-12, 30)   # warm-up (allocation)
             ts, tm = [], []
             for r in range(reps):
+                ctx.set_params(sets[r])
                 t0 = time.perf_counter()
                 st, E, g, inf = fn(0, 1, sigma, 1e-12, 30)
                 ts.append(time.perf_counter() - t0)
