@@ -420,12 +420,12 @@ __global__ void k_fill_ones(int K, zd* v) {
 // src/linalg.f90:1892-1917).  Start vector in x->ev[0] unless warm.
 // Stops when the GHEPIIS measure reaches tol, or when it has stopped
 // decreasing below 1e-9 (rounding floor), or after maxiter iterations.
-// With stop_at_floor (the energy paths) the first three iterations, the
+// With stop_at_floor (the energy paths) the first nspec iterations, the
 // Rayleigh quotient and the normalisation are queued with a single host
 // synchronisation; further iterations only run if the measure asks for
 // them.  Result vector (normalised x^H S x = 1) in x->ev[2].
 int inverse_iteration(ecg_ctx* x, double sigma, double tol, int maxiter, bool warm, InvitResult& r,
-                      bool stop_at_floor, int* fst = nullptr) {
+                      bool stop_at_floor, int* fst = nullptr, int nspec = 3) {
   ecg::Ctx& c = x->c;
   const int K = c.K;
   zd* v = x->ev[0];
@@ -455,10 +455,10 @@ int inverse_iteration(ecg_ctx* x, double sigma, double tol, int maxiter, bool wa
   int it = 0;
   bool done = false;
   if (stop_at_floor) {
-    // energy paths: the first (up to) three iterations and the tail are
+    // energy paths: the first (up to) nspec iterations and the tail are
     // queued without a host round trip; the loop below continues only when
     // the measure asks for more iterations
-    const int nq = std::min(maxiter, 3);
+    const int nq = std::min(maxiter, nspec);
     for (; it < nq; ++it) {
       RCCHK(ecg::invit_iteration(c, x->f, v, y, x->stream));
       std::swap(v, y);
@@ -532,7 +532,10 @@ int solve_root(ecg_ctx* x, int which, double sigma, double tol, int maxiter, dou
     RCCHK(factor_queue(x, sigma));
     ++nfac;
     int fst[3] = {0, 0, 0};
-    const int rc = inverse_iteration(x, sigma, tol, maxiter, true, r, true, fst);
+    // after a tail update the previous eigenvector is close: two queued
+    // iterations usually meet the criterion (more follow if not)
+    const bool near = x->last_tail >= 0 && x->have_last && x->last_K == K;
+    const int rc = inverse_iteration(x, sigma, tol, maxiter, true, r, true, fst, near ? 2 : 3);
     if (rc == ECG_ERR_SYNC) return rc;
     if (fst[1]) {
       *status = ECG_SINGULAR;
