@@ -181,6 +181,7 @@ int ensure_capacity(ecg_ctx* x, int K) {
   if (!rc) rc = dev_alloc(&c.H, (size_t)ld * ld);
   if (!rc) rc = dev_alloc(&c.S, (size_t)ld * ld);
   if (!rc) rc = dev_alloc(&c.rowbuf, ecg::thin_row_entries(cap));
+  if (!rc) rc = dev_alloc(&c.adjraw, ecg::thin_adjoint_entries(c.n, cap));
   if (!rc) rc = dev_alloc(&f.F, (size_t)ld * ld);
   if (!rc) rc = dev_alloc(&f.D, (size_t)ld);
   if (!rc) rc = dev_alloc(&f.W, (size_t)ld * 2 * ecg::NB);
@@ -834,7 +835,7 @@ int ecg_ctx_destroy(ecg_ctx* x) {
   if (!x) return ECG_ERR_ARG;
   if (x->stream) (void)hipStreamSynchronize(x->stream);
   ecg::Ctx& c = x->c;
-  void* ptrs[] = {c.params, c.bra, c.ket, c.diagS, c.diagH, c.H, c.S, c.rowbuf, x->f.F, x->f.D, x->f.W, x->f.Dinv,
+  void* ptrs[] = {c.params, c.bra, c.ket, c.diagS, c.diagH, c.H, c.S, c.rowbuf, c.adjraw, x->f.F, x->f.D, x->f.W, x->f.Dinv,
                   x->f.status, x->f.vec, x->f.tvec, x->f.pflag, x->f.scal, x->part, x->grad, x->ev[0],
                   x->ev[1], x->ev[2]};
   for (void* p : ptrs)

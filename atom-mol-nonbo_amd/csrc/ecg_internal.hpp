@@ -87,6 +87,7 @@ struct Ctx {
   zd* H = nullptr;            // ld x ld
   zd* S = nullptr;            // ld x ld
   zd* rowbuf = nullptr;       // thin_row_entries(cap): raw (H, S) of the pairs of a few rows
+  double* adjraw = nullptr;   // thin_adjoint_entries(n, cap): per-pair adjoint values of a few rows
   // optional events bracketing the off-diagonal and adjoint kernels
   hipEvent_t ev_off0 = nullptr, ev_off1 = nullptr, ev_adj0 = nullptr, ev_adj1 = nullptr;
   Tracer* tr = nullptr;
@@ -96,6 +97,7 @@ struct Ctx {
 int prep(const Ctx& c, hipStream_t s);
 int assemble(const Ctx& c, int r0, int r1, hipStream_t s);   // rows r0..r1-1 (thin-row kernels for few rows)
 size_t thin_row_entries(int cap);
+size_t thin_adjoint_entries(int n, int cap);
 int assemble_deriv(const Ctx& c, int nfru, zd* D, hipStream_t s);
 int adjoint(const Ctx& c, int nfru, double E, const zd* cvec, double* part, double* grad, hipStream_t s);
 size_t adjoint_partial_doubles(int n, int K);

@@ -1000,6 +1000,158 @@ static int launch_deriv(const Ctx& c, int nfru, zd* D, hipStream_t s) {
   return (int)hipGetLastError();
 }
 
+// Thin rows of the adjoint (functions nfru..K-1, at most THIN_ROWS of
+// them): a group of G >= T lanes per pair (k, l), k >= nfru, l <= k, a
+// lane per term; lane 0 adds the terms in order (the sums of
+// pair_adjoint_sum) and stores the pair's bra and ket values (nv each) in
+// raw.  k_adj_rows_slots then fills the partial slots of each function as
+// adjoint_tile would (same 16-pair sums in the same order), so
+// k_adjoint_finish is unchanged.
+template <int N, int G>
+__global__ void __launch_bounds__(256) k_adj_rows(const SysConst sc, int K, int nfru, long long npairs, double E,
+                                                  const double* __restrict__ bra, const double* __restrict__ ket,
+                                                  const double* __restrict__ diagS, const zd* __restrict__ cvec,
+                                                  double* __restrict__ raw) {
+  constexpr int m = N * (N + 1) / 2, nv = AdjAcc<N>::nvals;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long p = gid / G;
+  const int t = (int)(gid % G);
+  int k = 0, l = 0;
+  if (p < npairs) row_pair(p, nfru, k, l);
+  const bool want_l = (l != k) && (l >= nfru);
+  zd hs = mkz(0.0, 0.0), gb[m], g2[m];
+#pragma unroll
+  for (int e = 0; e < m; ++e) gb[e] = g2[e] = mkz(0.0, 0.0);
+  if (p < npairs && t < sc.T) {
+    const double* br = bra + (size_t)k * BraRec<N>::stride;
+    const double* kr = ket + ((size_t)l * sc.T + t) * KetRec<N>::stride;
+    const zd* ck = reinterpret_cast<const zd*>(br + BraRec<N>::ck);
+    const zd* ct = reinterpret_cast<const zd*>(kr + KetRec<N>::ct);
+    const double adet = br[BraRec<N>::adet] * bra[(size_t)l * BraRec<N>::stride + BraRec<N>::adet];
+    PairCore<N> pc;
+    pair_energy<N>(sc, ck, ct, reinterpret_cast<const zd*>(kr + KetRec<N>::nl), adet, pc);
+    zd gk[m], gbt[m];
+    pair_adjoint<N>(sc, ck, ct, pc, E, gbt, gk);
+    const double c = sc.coef[t];
+    hs = c * (pc.T + pc.V - E * pc.S);
+#pragma unroll
+    for (int e = 0; e < m; ++e) gb[e] = c * gbt[e];
+    if (want_l) {
+      zd gp[m];
+      perm_congruence<N>(sc.P[t], gk, gp);
+#pragma unroll
+      for (int e = 0; e < m; ++e) g2[e] = c * gp[e];
+    }
+  }
+  // lane 0 of the group: the terms in order
+  const int base = (threadIdx.x & 63) - t;
+  zd HS = mkz(0.0, 0.0), Gb[m], Gk[m];
+#pragma unroll
+  for (int e = 0; e < m; ++e) Gb[e] = Gk[e] = mkz(0.0, 0.0);
+  for (int q = 0; q < G; ++q) {
+    const bool use = q < sc.T;
+    const zd hq = mkz(__shfl(hs.x, base + q), __shfl(hs.y, base + q));
+    if (use) HS = HS + hq;
+#pragma unroll
+    for (int e = 0; e < m; ++e) {
+      const zd bq = mkz(__shfl(gb[e].x, base + q), __shfl(gb[e].y, base + q));
+      const zd kq = mkz(__shfl(g2[e].x, base + q), __shfl(g2[e].y, base + q));
+      if (use) {
+        Gb[e] = Gb[e] + bq;
+        if (want_l) Gk[e] = Gk[e] + kq;
+      }
+    }
+  }
+  if (p >= npairs || t != 0) return;
+  const zd w = (zconj(cvec[k]) * cvec[l]) * (1.0 / sqrt(diagS[k] * diagS[l]));
+  const zd a = w * (1.5 * HS);
+  double* vb = raw + (size_t)p * 2 * nv;
+  double* vk = vb + nv;
+#pragma unroll
+  for (int e = 0; e < m; ++e) {
+    const zd g = w * Gb[e];
+    vb[e] = g.x;
+    vb[m + e] = g.y;
+  }
+  vb[2 * m] = a.x;
+  if (want_l) {
+#pragma unroll
+    for (int e = 0; e < m; ++e) {
+      const zd g = w * Gk[e];
+      vk[e] = g.x;
+      vk[m + e] = -g.y;
+    }
+    vk[2 * m] = a.x;
+  } else {
+#pragma unroll
+    for (int e = 0; e < nv; ++e) vk[e] = 0.0;
+  }
+}
+
+// One thread per (function f >= nfru, slot): slot c <= f/16 sums the bra
+// values of pairs (f, 16c..16c+15) with l <= f; slot r' + 1 (r' >= f/16)
+// sums the ket values of pairs (16r'..16r'+15, f) with k > f.
+template <int N>
+__global__ void __launch_bounds__(256) k_adj_rows_slots(int K, int nfru, int nt, const double* __restrict__ raw,
+                                                        double* __restrict__ part) {
+  constexpr int nv = AdjAcc<N>::nvals;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = nfru + (int)(gid / (nt + 1));
+  const int slot = (int)(gid % (nt + 1));
+  if (f >= K) return;
+  const int r = f / TILE;
+  const long long base = (long long)nfru * (nfru + 1) / 2;
+  double sum[nv];
+#pragma unroll
+  for (int e = 0; e < nv; ++e) sum[e] = 0.0;
+  if (slot <= r) {
+    for (int y = 0; y < TILE; ++y) {
+      const int l = slot * TILE + y;
+      if (l > f) break;
+      const double* v = raw + ((long long)f * (f + 1) / 2 + l - base) * 2 * nv;
+#pragma unroll
+      for (int e = 0; e < nv; ++e) sum[e] += v[e];
+    }
+  } else {
+    const int rp = slot - 1;
+    for (int y = 0; y < TILE; ++y) {
+      const int k = rp * TILE + y;
+      if (k >= K) break;
+      if (k <= f) continue;
+      const double* v = raw + ((long long)k * (k + 1) / 2 + f - base) * 2 * nv + nv;
+#pragma unroll
+      for (int e = 0; e < nv; ++e) sum[e] += v[e];
+    }
+  }
+  double* out = part + ((size_t)f * (nt + 1) + slot) * nv;
+#pragma unroll
+  for (int e = 0; e < nv; ++e) out[e] = sum[e];
+}
+
+template <int N>
+static void launch_adj_rows(const Ctx& c, int nfru, double E, const zd* cvec, double* part, hipStream_t s) {
+  const int K = c.K, nt = (K + TILE - 1) / TILE;
+  const long long npairs = (long long)K * (K + 1) / 2 - (long long)nfru * (nfru + 1) / 2;
+  const int T = c.sc.T;
+  const int G = T <= 1 ? 1 : T <= 2 ? 2 : T <= 4 ? 4 : T <= 8 ? 8 : 16;
+  const unsigned nb = (unsigned)((npairs * G + 255) / 256);
+  switch (G) {
+    case 1: hipLaunchKernelGGL((k_adj_rows<N, 1>), dim3(nb), dim3(256), 0, s, c.sc, K, nfru, npairs, E, c.bra, c.ket, c.diagS, cvec, c.adjraw); break;
+    case 2: hipLaunchKernelGGL((k_adj_rows<N, 2>), dim3(nb), dim3(256), 0, s, c.sc, K, nfru, npairs, E, c.bra, c.ket, c.diagS, cvec, c.adjraw); break;
+    case 4: hipLaunchKernelGGL((k_adj_rows<N, 4>), dim3(nb), dim3(256), 0, s, c.sc, K, nfru, npairs, E, c.bra, c.ket, c.diagS, cvec, c.adjraw); break;
+    case 8: hipLaunchKernelGGL((k_adj_rows<N, 8>), dim3(nb), dim3(256), 0, s, c.sc, K, nfru, npairs, E, c.bra, c.ket, c.diagS, cvec, c.adjraw); break;
+    default: hipLaunchKernelGGL((k_adj_rows<N, 16>), dim3(nb), dim3(256), 0, s, c.sc, K, nfru, npairs, E, c.bra, c.ket, c.diagS, cvec, c.adjraw); break;
+  }
+  const long long nslots = (long long)(K - nfru) * (nt + 1);
+  hipLaunchKernelGGL(k_adj_rows_slots<N>, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s, K, nfru, nt,
+                     c.adjraw, part);
+}
+
+size_t thin_adjoint_entries(int n, int cap) {
+  const int m = n * (n + 1) / 2;
+  return (size_t)THIN_ROWS * cap * 2 * (2 * m + 1);
+}
+
 template <int N>
 static int launch_adjoint(const Ctx& c, int nfru, double E, const zd* cvec, double* part, double* grad,
                           hipStream_t s) {
@@ -1011,8 +1163,11 @@ static int launch_adjoint(const Ctx& c, int nfru, double E, const zd* cvec, doub
                      (BraRec<N>::stride + c.T * KetRec<N>::stride + TILE * AdjAcc<N>::nvals);
   if (c.ev_adj0) (void)hipEventRecord(c.ev_adj0, s);
   if (c.tr) c.tr->begin(TR_ADJOINT, s);
-  hipLaunchKernelGGL(k_adjoint<N>, dim3((unsigned)ntiles), dim3(TILE, TILE), lds, s, c.sc, K, nfru, tr0, nt, E,
-                     c.bra, c.ket, c.diagS, cvec, part);
+  if (K - nfru <= THIN_ROWS && c.adjraw)
+    launch_adj_rows<N>(c, nfru, E, cvec, part, s);
+  else
+    hipLaunchKernelGGL(k_adjoint<N>, dim3((unsigned)ntiles), dim3(TILE, TILE), lds, s, c.sc, K, nfru, tr0, nt, E,
+                       c.bra, c.ket, c.diagS, cvec, part);
   if (c.tr) c.tr->end(s);
   if (c.ev_adj1) (void)hipEventRecord(c.ev_adj1, s);
   hipLaunchKernelGGL(k_adjoint_finish<N>, dim3((K - nfru + 3) / 4), dim3(256), 0, s, K, nfru, nt, tr0, c.bra, part,

@@ -250,7 +250,8 @@ def _oracle_energy_grad(sy, params, sigma, nfru=0):
     return lam, g
 
 
-@pytest.mark.parametrize("name,K,nfru", [("hminus", 40, 0), ("h2", 36, 0), ("h2", 30, 12), ("li", 24, 0)])
+@pytest.mark.parametrize("name,K,nfru", [("hminus", 40, 0), ("h2", 36, 0), ("h2", 30, 12), ("li", 24, 0),
+                                         ("h2", 70, 45), ("li", 50, 18), ("hminus", 67, 66)])
 def test_energy_grad_vs_oracle(name, K, nfru):
     sy = named_system(name)
     params = seeded_basis(sy.n, K, SEED_BASE + 500 + K)
