@@ -328,6 +328,34 @@ def optimise_leg(system, base, K, nevals=60, warm=3):
     }
 
 
+def opt_cycle_leg(system, base, K, nfunc=16, max_evals=12):
+    """One cycle of the cyclic optimiser (OptCycleI's loop) over the last
+    nfunc functions, at most max_evals energy evaluations each, at a fixed
+    shift; every evaluation runs on the incremental path."""
+    from ecg_nonbo.optimise import BasisOrder, opt_cycle
+    ctx = EcgContext(system)
+    ctx.set_params(base)
+    st, E0, _, _ = ctx.energy(1, float("nan"), 1e-12, 60)
+    sigma = E0 - 1e-6 * abs(E0)
+    order = BasisOrder.identity(base)
+    t0 = time.perf_counter()
+    recs = opt_cycle(ctx, order, K - nfunc + 1, K, 1, max_evals, sigma, max_step=0.05)
+    t = time.perf_counter() - t0
+    ctx.close()
+    nev = sum(r.evaluations for r in recs) + 2 * len(recs)
+    tails = [x for r in recs for x in r.tail_rows]
+    return {
+        "workload": "H2-like n=3 (T=4), K=%d: one optimisation cycle over functions %d..%d, <= %d energy "
+                    "evaluations each (BFGS stand-in for DRMNG)" % (K, K - nfunc + 1, K, max_evals),
+        "s_per_function": t / len(recs),
+        "ms_per_energy_call": 1e3 * t / nev,
+        "energy_calls": nev,
+        "E_start": float(E0), "E_end": float(recs[-1].energy_after),
+        "accepted": int(sum(r.accepted for r in recs)),
+        "max_tail_rows": int(max(tails)), "full_factorisations": int(sum(1 for x in tails if x < 0)),
+    }
+
+
 def expectation_leg(system, base, K, reps=3):
     """Expectation values (<1/r^2>, <1/r>, <r>, <r^2>, <delta(r)> of every
     pseudo-particle pair, <H>, <S>, <T>, <V>) in the lowest eigenvector:
@@ -482,6 +510,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the multi-problem batch leg")
     ap.add_argument("--no-opt", action="store_true", help="skip the one-function optimisation leg")
     ap.add_argument("--no-expect", action="store_true", help="skip the expectation-value leg")
+    ap.add_argument("--no-cycle", action="store_true", help="skip the optimisation-cycle leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first issue device copies of a known size (reference for HBM counters)")
     args = ap.parse_args()
@@ -614,6 +643,8 @@ def main():
         out["optimise_one_function"] = optimise_leg(system, base, K)
     if not args.no_expect:
         out["expectation_values"] = expectation_leg(system, base, K)
+    if not args.no_cycle:
+        out["opt_cycle"] = opt_cycle_leg(system, base, K)
     if not args.no_screen:
         out["screening"] = screening_leg(system, base, K)
     if not args.no_modes:
