@@ -677,7 +677,9 @@ int solve_root(ecg_ctx* x, int which, double sigma, double tol, int maxiter, dou
     sigma = s;
   }
   *E = r.lambda;
-  if (!r.converged && tol > 0.0 && r.nd > tol) *status = ECG_NOT_CONVERGED;
+  // tol <= 0 is GHEPIIS's floor mode: success when the measure at the
+  // rounding floor is <= |tol| (src/linalg.f90:1902-1906)
+  if (!r.converged && r.nd > std::fabs(tol)) *status = ECG_NOT_CONVERGED;
   if (info) {
     info[0] = r.iters;
     info[1] = r.nd;
@@ -1078,7 +1080,7 @@ int ecg_ghepiis(ecg_ctx* x, int k, int n, double* M, int nM, double* invD, const
   *lambda = r.lambda;
   if (relacc) *relacc = r.nd;
   if (numiter) *numiter = r.iters;
-  if (!r.converged && tol > 0.0) return ECG_NOT_CONVERGED;
+  if (!r.converged && r.nd > std::fabs(tol)) return ECG_NOT_CONVERGED;
   return ECG_OK;
 }
 

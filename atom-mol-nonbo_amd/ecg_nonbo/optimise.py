@@ -27,7 +27,57 @@ from typing import Callable, List, Optional
 
 import numpy as np
 
-__all__ = ["BasisOrder", "num_rows_to_perm_for_unit_shift", "bfgs_minimise", "opt_cycle", "OptStepRecord"]
+__all__ = ["BasisOrder", "num_rows_to_perm_for_unit_shift", "bfgs_minimise", "lbfgs_minimise", "opt_cycle",
+           "OptStepRecord", "EnergyCalls", "full_opt1", "FullOptResult", "FAILED_STATUSES"]
+
+# Statuses with which an energy call counts as a failure (the reference's
+# ErrorCode /= 0): singular H - sigma S, inverse iteration not converged,
+# root not isolated, non-finite pair element.  Other negative statuses are
+# library errors and raise.
+FAILED_STATUSES = (1, 2, 3, -4)
+
+
+class EnergyCalls:
+    """The energy routines an optimiser loop calls, bound to a context and a
+    shift: EnergyIA / EnergyIAM (`energy`) and EnergyIB (`energy_grad`),
+    src/workproc.f90:1954-2154 for method 'I' (sigma finite, the fixed
+    Glob_ApproxEnergy) and EnergyGA / EnergyGB (:1697-1950) for method 'G'
+    (sigma NaN: the device isolates root `which` by inertia counts).  Each
+    call returns its status instead of raising for the statuses in
+    FAILED_STATUSES, as the reference's routines return ErrorCode."""
+
+    def __init__(self, ctx, sigma: float, which: int = 1, tol: float = 1e-12, maxiter: int = 30):
+        self.ctx = ctx
+        self.sigma = float(sigma)
+        self.which = int(which)
+        self.tol = float(tol)
+        self.maxiter = int(maxiter)
+        self.tails: List[int] = []
+
+    def _call(self, fn, *args, **kw):
+        try:
+            return fn(*args, **kw)
+        except RuntimeError as e:
+            code = getattr(e, "code", None)
+            if code in FAILED_STATUSES:
+                return code, np.inf, None, np.full(8, np.nan)
+            raise
+
+    def energy(self, params, want_vector: bool = False):
+        """(status, E, c or None, info) of the basis `params` (K x npt)."""
+        self.ctx.set_params(params)
+        rc, E, c, info = self._call(self.ctx.energy, self.which, self.sigma, self.tol, self.maxiter,
+                                    want_vector=want_vector)
+        self.tails.append(int(info[7]) if np.isfinite(info[7]) else -1)
+        return rc, E, c, info
+
+    def energy_grad(self, params, nfru: int):
+        """(status, E, gradient of functions nfru+1..K, info)."""
+        self.ctx.set_params(params)
+        rc, E, g, info = self._call(self.ctx.energy_grad, nfru, self.which, self.sigma, self.tol, self.maxiter)
+        self.tails.append(int(info[7]) if np.isfinite(info[7]) else -1)
+        return rc, E, g, info
+
 
 
 def num_rows_to_perm_for_unit_shift(j: int) -> int:
@@ -82,23 +132,38 @@ class BasisOrder:
         self.func_num[seg] = self.func_num[seg][order].copy()
 
 
-def bfgs_minimise(fg: Callable[[np.ndarray], tuple], f_only: Callable[[np.ndarray], float], x0: np.ndarray,
-                  max_evals: int, max_step: float, scale: np.ndarray):
-    """Minimise with at most max_evals energy evaluations (gradient calls
-    count as evaluations, as the reference counts EnergyIA and EnergyIB
-    calls against its limit).  Variables are scaled by `scale` (the role of
-    DRMNG's D vector); the first step's scaled length is at most max_step.
-    Returns (x_best, f_best, evaluations)."""
+def _quasi_newton(fg, f_only, x0, max_evals, max_step, scale, memory):
     x = np.array(x0, dtype=np.float64)
     f, g = fg(x)
     nev = 1
     best = (x.copy(), f)
+    if not np.isfinite(f):
+        return best[0], best[1], nev
     n = x.size
-    Hinv = np.eye(n)
+    Hinv = np.eye(n) if memory is None else None
+    pairs: list = []                       # (s, y, rho) of the limited-memory form
     gs = g / scale
+
+    def direction(q):
+        if memory is None:
+            return -(Hinv @ q)
+        # two-loop recursion (Nocedal & Wright, Algorithm 7.4)
+        q = q.copy()
+        al = []
+        for s_, y_, r_ in reversed(pairs):
+            a_ = r_ * (s_ @ q)
+            al.append(a_)
+            q -= a_ * y_
+        if pairs:
+            s_, y_, _ = pairs[-1]
+            q *= (s_ @ y_) / (y_ @ y_)
+        for (s_, y_, r_), a_ in zip(pairs, reversed(al)):
+            q += (a_ - r_ * (y_ @ q)) * s_
+        return -q
+
     first = True
     while nev < max_evals:
-        p = -(Hinv @ gs)
+        p = direction(gs)
         if first:
             nrm = np.linalg.norm(p)
             if nrm > max_step:
@@ -106,7 +171,9 @@ def bfgs_minimise(fg: Callable[[np.ndarray], tuple], f_only: Callable[[np.ndarra
             first = False
         slope = gs @ p
         if slope >= 0.0:   # not a descent direction: restart from steepest descent
-            Hinv = np.eye(n)
+            if memory is None:
+                Hinv = np.eye(n)
+            pairs.clear()
             p = -gs * min(1.0, max_step / max(np.linalg.norm(gs), 1e-300))
             slope = gs @ p
             if slope >= 0.0:
@@ -127,8 +194,9 @@ def bfgs_minimise(fg: Callable[[np.ndarray], tuple], f_only: Callable[[np.ndarra
                 break
         if not accepted or nev >= max_evals:
             break
-        fn, gn = fg(xn)
-        nev += 1
+        fn, gn = fg(xn)             # gradient at an accepted point: not counted
+        if not np.isfinite(fn):
+            break
         if fn < best[1]:
             best = (xn.copy(), fn)
         gsn = gn / scale
@@ -137,12 +205,39 @@ def bfgs_minimise(fg: Callable[[np.ndarray], tuple], f_only: Callable[[np.ndarra
         sy = s @ yv
         if sy > 1e-300:
             rho = 1.0 / sy
-            V = np.eye(n) - rho * np.outer(s, yv)
-            Hinv = V @ Hinv @ V.T + rho * np.outer(s, s)
+            if memory is None:
+                V = np.eye(n) - rho * np.outer(s, yv)
+                Hinv = V @ Hinv @ V.T + rho * np.outer(s, s)
+            else:
+                pairs.append((s, yv, rho))
+                if len(pairs) > memory:
+                    pairs.pop(0)
         x, f, gs = xn, fn, gsn
         if np.linalg.norm(gs) * np.linalg.norm(s) < 1e-15 * max(1.0, abs(f)):
             break
     return best[0], best[1], nev
+
+
+def bfgs_minimise(fg: Callable[[np.ndarray], tuple], f_only: Callable[[np.ndarray], float], x0: np.ndarray,
+                  max_evals: int, max_step: float, scale: np.ndarray):
+    """Minimise with at most max_evals energy evaluations: the starting
+    point and every line-search point count, the gradient calls at accepted
+    points do not, as the reference counts only DRMNG's function requests
+    (IV(1)=1) against MaxEnergyEval (src/workproc.f90:4141-4147, 4219).
+    Variables are scaled by `scale` (the role of DRMNG's D vector); the
+    first step's scaled length is at most max_step.
+    Dense inverse-Hessian update, for the few variables of one or a few
+    functions.  Returns (x_best, f_best, evaluations)."""
+    return _quasi_newton(fg, f_only, x0, max_evals, max_step, scale, None)
+
+
+def lbfgs_minimise(fg: Callable[[np.ndarray], tuple], f_only: Callable[[np.ndarray], float], x0: np.ndarray,
+                   max_evals: int, max_step: float, scale: np.ndarray, memory: int = 12):
+    """bfgs_minimise with the limited-memory inverse Hessian (the last
+    `memory` steps), for simultaneous optimisation of a whole basis, where
+    a dense matrix of (K npt)^2 entries would cost more per step than the
+    energy and gradient themselves."""
+    return _quasi_newton(fg, f_only, x0, max_evals, max_step, scale, int(memory))
 
 
 @dataclasses.dataclass
@@ -158,7 +253,7 @@ class OptStepRecord:
 def opt_cycle(ctx, order: BasisOrder, func_begin: int, func_end: int, num_cycles: int, max_energy_eval: int,
               sigma: float, overlap_threshold: float = -1.0, lin_coeff_threshold: float = -1.0,
               num_func_to_opt: int = 1, num_func_to_shift: int = 1, history: Optional[dict] = None,
-              tol: float = 1e-12, max_step: float = 0.1,
+              tol: float = 1e-12, max_step: float = 0.1, which: int = 1, maxiter: int = 30,
               on_step: Optional[Callable[[OptStepRecord, BasisOrder], None]] = None) -> List[OptStepRecord]:
     """OptCycleI (src/workproc.f90:4902-5423) on a device context: cycles of
     optimising functions func_begin..func_end (their numbering in `order`'s
@@ -166,8 +261,13 @@ def opt_cycle(ctx, order: BasisOrder, func_begin: int, func_end: int, num_cycles
     ones by num_func_to_shift.  `history` holds the reference's
     Glob_History(K) fields 'cycles_done' and 'init_func_at_last_step' (and
     receives 'energy'); a run can resume from them.  sigma: the shift
-    (Glob_ApproxEnergy).  Returns one record per step; `order` ends in the
-    input order of the functions (sorted back, as the reference leaves it)."""
+    (Glob_ApproxEnergy) for OptCycleI; NaN gives OptCycleG
+    (src/workproc.f90:4383-4898, the same loop over EnergyGA / EnergyGB:
+    root `which` isolated on every call).  Returns one record per step;
+    `order` ends in the input order of the functions (sorted back, as the
+    reference leaves it).  An energy call that fails inside the minimiser
+    counts as an infinite energy; failures of the calls outside it raise,
+    as the reference stops there."""
     K = len(order.func_num)
     npt = order.params.shape[1]
     if history is None:
@@ -188,15 +288,15 @@ def opt_cycle(ctx, order: BasisOrder, func_begin: int, func_end: int, num_cycles
     if ip > 0:
         order.permute_functions(fbn, K - ip)
 
+    ev = EnergyCalls(ctx, sigma, which, tol, maxiter)
+
     def energy(which_grad: Optional[int]):
-        ctx.set_params(order.params)
         if which_grad is None:
-            rc, E, _, info = ctx.energy(1, sigma, tol, 30)
-            return rc, E, None, info
-        return ctx.energy_grad(which_grad, 1, sigma, tol, 30)
+            return ev.energy(order.params)
+        return ev.energy_grad(order.params, which_grad)
 
     rc, E_cur, _, info = energy(None)
-    if rc not in (0, 2):
+    if rc != 0:
         raise RuntimeError("OptCycle: initial energy failed with status %d" % rc)
     for cycle in range(history["cycles_done"] + 1, num_cycles + 1):
         cur_begin = history["init_func_at_last_step"] + nfs
@@ -206,7 +306,7 @@ def opt_cycle(ctx, order: BasisOrder, func_begin: int, func_end: int, num_cycles
             nfo = min(func_end - cur + 1, num_func_to_opt)
             nfru = K - nfo
             opt_iter += 1
-            tails = []
+            t0 = len(ev.tails)
             if opt_iter != 1:
                 ns = min(nfs, nfo)
                 order.permute_functions(K - nfo - ns + 1, K - ns)
@@ -216,8 +316,7 @@ def opt_cycle(ctx, order: BasisOrder, func_begin: int, func_end: int, num_cycles
                 i = (j - nr + 1) if q - nfo >= nr * 2 else (K - q + 1)
                 order.permute_functions2(i, j, m)
                 rc, E_cur, _, info = energy(None)
-                tails.append(int(info[7]))
-                if rc not in (0, 2):
+                if rc != 0:
                     raise RuntimeError("OptCycle: energy after the permutation failed with status %d" % rc)
             x_init = order.params[nfru:].reshape(-1).copy()
             e_before = E_cur
@@ -228,22 +327,18 @@ def opt_cycle(ctx, order: BasisOrder, func_begin: int, func_end: int, num_cycles
 
             def f_only(x):
                 set_x(x)
-                rc_, e_, _, inf_ = energy(None)
-                tails.append(int(inf_[7]))
-                return e_ if rc_ in (0, 2) else np.inf
+                rc_, e_, _, _ = energy(None)
+                return e_ if rc_ == 0 else np.inf
 
             def fg(x):
                 set_x(x)
-                rc_, e_, g_, inf_ = energy(nfru)
-                tails.append(int(inf_[7]))
-                return (e_, g_) if rc_ in (0, 2) else (np.inf, np.zeros_like(x))
+                rc_, e_, g_, _ = energy(nfru)
+                return (e_, g_) if rc_ == 0 else (np.inf, np.zeros_like(x))
 
             x_best, e_best, nev = bfgs_minimise(fg, f_only, x_init, max_energy_eval, max_step, scale)
             set_x(x_best)
-            ctx.set_params(order.params)
-            rc, E_new, c, info = ctx.energy(1, sigma, tol, 30, want_vector=True)
-            tails.append(int(info[7]))
-            ok = rc in (0, 2) and E_new <= e_before
+            rc, E_new, c, info = ev.energy(order.params, want_vector=True)
+            ok = rc == 0 and E_new <= e_before
             if ok and overlap_threshold > 0.0:
                 pen, _ = ctx.overlap_penalty(nfru, 1.0, overlap_threshold ** 2)
                 ok = pen <= 0.0
@@ -252,7 +347,8 @@ def opt_cycle(ctx, order: BasisOrder, func_begin: int, func_end: int, num_cycles
             if not ok:
                 set_x(x_init)
                 rc, E_new, _, info = energy(None)
-                tails.append(int(info[7]))
+                if rc != 0:
+                    raise RuntimeError("OptCycle: energy at the step's starting point failed with status %d" % rc)
             E_cur = E_new
             last = cur > func_end - nfs
             history["energy"] = E_cur
@@ -262,7 +358,8 @@ def opt_cycle(ctx, order: BasisOrder, func_begin: int, func_end: int, num_cycles
             else:
                 history["init_func_at_last_step"] = cur
             rec = OptStepRecord(func=int(order.func_num[K - 1]), energy_before=float(e_before),
-                                energy_after=float(E_cur), evaluations=nev, accepted=bool(ok), tail_rows=tails)
+                                energy_after=float(E_cur), evaluations=nev, accepted=bool(ok),
+                                tail_rows=list(ev.tails[t0:]))
             records.append(rec)
             if on_step:
                 on_step(rec, order)
@@ -273,3 +370,88 @@ def opt_cycle(ctx, order: BasisOrder, func_begin: int, func_end: int, num_cycles
     order.sort_decreasing(func_begin, K)
     order.reverse(func_begin, K)
     return records
+
+
+@dataclasses.dataclass
+class FullOptResult:
+    energy_before: float      # E + P at the starting point (P = 0 without the penalty)
+    energy_after: float       # energy without the penalty at the end (Glob_CurrEnergy)
+    penalty: float            # Glob_TotalOverlapPenalty at the end
+    evaluations: int          # energy evaluations (the reference's NumOfEnergyEval)
+    minimiser: str            # "bfgs" (dense) or "lbfgs"
+
+
+def full_opt1(ctx, order: BasisOrder, init_func: int, final_func: int, max_energy_eval: int, sigma: float,
+              overlap_threshold: float = 1.0, max_overlap_penalty: float = 1.0, history: Optional[dict] = None,
+              which: int = 1, tol: float = 1e-12, maxiter: int = 30, max_step: float = 0.1, memory: int = 12,
+              dense_limit: int = 600) -> FullOptResult:
+    """FullOpt1I (src/workproc.f90:5962-6422; FullOpt1G, :5428-5958, with
+    sigma NaN): simultaneous optimisation of functions init_func..final_func
+    (positions in `order`), E + P and its gradient from the device.
+
+    Steps as the reference: the overlap penalty is on when
+    overlap_threshold < 1 (t2 = threshold^2, b = max_overlap_penalty,
+    :6026-6032); functions init_func..final_func move to the end of the
+    basis (PermuteFunctions, :6113-6127); the minimiser runs with at most
+    max_energy_eval energy evaluations over their npt (final - init + 1)
+    parameters with the gradient of the last K - nfru functions (EnergyIB);
+    the energy without the penalty becomes the current energy (:6328-6332)
+    and history['energy'], history['num_energy_evals_full_opt'] grow as
+    Glob_History(K) does (:6356-6360); the functions that followed
+    final_func move back behind them (:6366-6378), restoring the input
+    order.  The penalty is switched off on return.
+
+    Not as the reference: DRMNG (with its Hessian file) is replaced by the
+    dense BFGS stand-in up to `dense_limit` variables and by limited-memory
+    BFGS above (a dense matrix at K = 2000 would be 4.6 GB on the host), so
+    the optimisation path differs; the best point found is the one kept."""
+    K = len(order.func_num)
+    if not (1 <= init_func <= final_func <= K):
+        raise ValueError("FullOpt1: need 1 <= InitFunc <= FinalFunc <= K")
+    npt = order.params.shape[1]
+    nfo = final_func - init_func + 1
+    nfru = K - nfo
+    allowed = overlap_threshold < 1.0
+    ctx.set_overlap_penalty(allowed, max_overlap_penalty, overlap_threshold ** 2 if allowed else 1.0, nfru)
+    try:
+        if final_func != K:
+            order.permute_functions(init_func, final_func)
+        ev = EnergyCalls(ctx, sigma, which, tol, maxiter)
+        rc, E0, _, _ = ev.energy(order.params)
+        if rc != 0:
+            raise RuntimeError("FullOpt1: initial energy failed with status %d" % rc)
+        x0 = order.params[nfru:].reshape(-1).copy()
+
+        def set_x(x):
+            order.params[nfru:] = x.reshape(nfo, npt)
+
+        def f_only(x):
+            set_x(x)
+            rc_, e_, _, _ = ev.energy(order.params)
+            return e_ if rc_ == 0 else np.inf
+
+        def fg(x):
+            set_x(x)
+            rc_, e_, g_, _ = ev.energy_grad(order.params, nfru)
+            return (e_, g_) if rc_ == 0 else (np.inf, np.zeros_like(x))
+
+        dense = x0.size <= dense_limit
+        scale = np.ones(x0.size)
+        if dense:
+            x_best, _, nev = bfgs_minimise(fg, f_only, x0, max_energy_eval, max_step, scale)
+        else:
+            x_best, _, nev = lbfgs_minimise(fg, f_only, x0, max_energy_eval, max_step, scale, memory)
+        set_x(x_best)
+        rc, E, _, info = ev.energy(order.params)
+        if rc != 0:
+            raise RuntimeError("FullOpt1: energy after the optimisation failed with status %d" % rc)
+        P = float(info[6]) if allowed else 0.0
+        if history is not None:
+            history["energy"] = E - P
+            history["num_energy_evals_full_opt"] = history.get("num_energy_evals_full_opt", 0) + nev
+        if final_func != K:
+            order.permute_functions(init_func, init_func + (K - final_func) - 1)
+        return FullOptResult(energy_before=float(E0), energy_after=float(E - P), penalty=P, evaluations=nev,
+                             minimiser="bfgs" if dense else "lbfgs")
+    finally:
+        ctx.set_overlap_penalty(False, 1.0, 1.0, 0)

@@ -15,8 +15,8 @@ method 'I' with its Glob_ApproxEnergy); nothing of a previous step's H, S,
 factor or gradient is reused.
 
 Metric: pair H,S+grad elements per second = K(K+1)/2 / (seconds per E+grad).
-Prints one JSON line (see the driver contract) with a `roofline` object for
-the dominant kernel (the fused adjoint pass, timed with HIP events on the
+Prints one JSON line with a `roofline` object for the dominant kernel (the
+one with the largest share of the step, timed with HIP events on the
 library's stream) and a `cpu_baseline` object (the oracle's pair algebra on
 the host cores over a bounded random sample of the same pairs, plus the
 host eigen step at full size).
@@ -339,11 +339,12 @@ def opt_cycle_leg(system, base, K, nfunc=16, max_evals=12):
     sigma = E0 - 1e-6 * abs(E0)
     order = BasisOrder.identity(base)
     t0 = time.perf_counter()
-    recs = opt_cycle(ctx, order, K - nfunc + 1, K, 1, max_evals, sigma, max_step=0.05)
+    # GHEPIIS's floor mode (tol < 0): iterate to the rounding floor, accept a measure <= 1e-9
+    recs = opt_cycle(ctx, order, K - nfunc + 1, K, 1, max_evals, sigma, max_step=0.05, tol=-1e-9)
     t = time.perf_counter() - t0
     ctx.close()
-    nev = sum(r.evaluations for r in recs) + 2 * len(recs)
     tails = [x for r in recs for x in r.tail_rows]
+    nev = len(tails)                       # device energy calls of the steps (one tail entry each)
     return {
         "workload": "H2-like n=3 (T=4), K=%d: one optimisation cycle over functions %d..%d, <= %d energy "
                     "evaluations each (BFGS stand-in for DRMNG)" % (K, K - nfunc + 1, K, max_evals),

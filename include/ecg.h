@@ -106,7 +106,10 @@ int ecg_compute_mat_elem_and_deriv(ecg_ctx* ctx, int nfru, char method, double a
  * v: start vector; x: eigenvector (normalised x^H B x = 1 when
  * specif_norm = 0, x^H x = 1 when 1, max |Re|,|Im| = 1 otherwise).
  * Returns 0, ECG_SINGULAR or ECG_NOT_CONVERGED like the reference's
- * ErrorCode. */
+ * ErrorCode.  tol > 0: converged when the measure is <= tol; tol <= 0
+ * (the reference's other mode): the iteration runs to its rounding floor
+ * and succeeds when the measure there is <= |tol|.  The energy paths below
+ * take tol the same way. */
 int ecg_ghepiis(ecg_ctx* ctx, int k, int n, double* M, int nM, double* invD, const double* B, int nB,
                 double apprlambda, const double* v, double tol, int maxiter, int specif_norm, double* lambda,
                 double* x, double* relacc, int* numiter);

@@ -34,8 +34,10 @@ def test_opt_cycle_hdplus_last_functions(golden_dir):
     sigma = E0 - 1e-4 * abs(E0)
     order = BasisOrder.identity(p0)
     hist = {"cycles_done": 0, "init_func_at_last_step": 0}
+    # GHEPIIS's floor mode: on this truncated HD+ basis the reference's own
+    # iteration levels off above the file's tolerance of 1e-12
     recs = opt_cycle(ctx, order, 185, 200, 1, max_energy_eval=10, sigma=sigma, overlap_threshold=0.99,
-                     lin_coeff_threshold=1e6, history=hist, max_step=0.05)
+                     lin_coeff_threshold=1e6, history=hist, max_step=0.05, tol=-1e-9)
     energies = [r.energy_after for r in recs]
     print("E0", E0, "after", energies[-1], "steps", len(recs), "accepted", sum(r.accepted for r in recs))
     assert [r.func for r in recs] == list(range(185, 201))
