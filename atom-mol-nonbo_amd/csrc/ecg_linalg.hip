@@ -1179,8 +1179,10 @@ __global__ void __launch_bounds__(256) k_xsolve_reduce(int K, int ld, int nch, c
   x[j] = s;
 }
 
-// b and x distinct; f.tvec holds z, f.W the partial sums (ld x nch <= ld x 2 NB).
-int ldlh_solve(Factor& f, const zd* b, zd* x, hipStream_t s) {
+// The two passes: z (f.tvec) and the backward pass's partial sums (f.W,
+// ld x nch <= ld x 2 NB); x = z + the partials' fixed-order sum is left to
+// the caller (k_xsolve_reduce, or fused into the next kernel).
+static int ldlh_passes(Factor& f, const zd* b, hipStream_t s, bool close_trace) {
   const int K = f.K;
   const int nrb = (K + 63) / 64, nch = (K + XS_CHUNK - 1) / XS_CHUNK;
   if (nch > 2 * NB) return -1;
@@ -1190,49 +1192,19 @@ int ldlh_solve(Factor& f, const zd* b, zd* x, hipStream_t s) {
   if (f.tr) f.tr->end(s);
   if (f.tr) f.tr->begin(TR_TRSV_BWD, s);
   hipLaunchKernelGGL(k_xsolve_bwd, dim3(nrb, nch), dim3(256), 0, s, K, f.ld, f.F, z, f.W);
-  hipLaunchKernelGGL(k_xsolve_reduce, dim3((K + 255) / 256), dim3(256), 0, s, K, f.ld, nch, f.W, z, x);
-  if (f.tr) f.tr->end(s);
+  if (f.tr && close_trace) f.tr->end(s);
   return (int)hipGetLastError();
 }
 
-// -------------------------------------------------------------- GEMV
-// y = (A - shift A2) x for full column-major ld x K matrices (A2 may be
-// null).  Workgroup = 64 rows x 4 column groups over a chunk of columns;
-// partial sums per chunk are reduced by k_gemv_reduce in a fixed order.
-constexpr int GEMV_CHUNK = 128;
-
-__global__ void __launch_bounds__(256) k_gemv_part(int K, int ld, const zd* __restrict__ A,
-                                                   const zd* __restrict__ A2, double shift,
-                                                   const zd* __restrict__ x, zd* __restrict__ part) {
-  __shared__ zd red[4][64];
-  const int r = threadIdx.x % 64, g = threadIdx.x / 64;
-  const int i = blockIdx.x * 64 + r;
-  const int c0 = blockIdx.y * GEMV_CHUNK;
-  const int c1 = min(K, c0 + GEMV_CHUNK);
-  zd acc = mkz(0.0, 0.0);
-  if (i < K) {
-    if (A2) {
-      for (int c = c0 + g; c < c1; c += 4) {
-        const size_t o = (size_t)c * ld + i;
-        const zd a = A[o], a2 = A2[o];
-        const zd m = mkz(fma(-shift, a2.x, a.x), fma(-shift, a2.y, a.y));
-        acc = zfma(m, x[c], acc);
-      }
-    } else {
-      for (int c = c0 + g; c < c1; c += 4) acc = zfma(A[(size_t)c * ld + i], x[c], acc);
-    }
-  }
-  red[g][r] = acc;
-  __syncthreads();
-  if (g == 0 && i < K) part[(size_t)blockIdx.y * ld + i] = red[0][r] + red[1][r] + red[2][r] + red[3][r];
-}
-
-__global__ void k_gemv_reduce(int K, int ld, int nch, const zd* __restrict__ part, zd* __restrict__ y) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= K) return;
-  zd s = mkz(0.0, 0.0);
-  for (int q = 0; q < nch; ++q) s = s + part[(size_t)q * ld + i];
-  y[i] = s;
+// b and x distinct; f.tvec holds z, f.W the partial sums (ld x nch <= ld x 2 NB).
+int ldlh_solve(Factor& f, const zd* b, zd* x, hipStream_t s) {
+  const int K = f.K;
+  const int nch = (K + XS_CHUNK - 1) / XS_CHUNK;
+  int rc = ldlh_passes(f, b, s, false);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_xsolve_reduce, dim3((K + 255) / 256), dim3(256), 0, s, K, f.ld, nch, f.W, f.tvec, x);
+  if (f.tr) f.tr->end(s);
+  return (int)hipGetLastError();
 }
 
 // ------------------------------------------------------ vector kernels
@@ -1305,30 +1277,130 @@ __global__ void __launch_bounds__(1024) k_invit_finish(int K, zd* __restrict__ x
   }
 }
 
+// k_xsolve_reduce and k_invit_finish in one workgroup, for K <= 1024
+// IF_REG: x_j = z_j + the backward pass's partial sums of the chunks right
+// of j (k_xsolve_reduce's order), kept in registers through the same
+// passes and block reductions as k_invit_finish (bitwise the same results).
+constexpr int IF_REG = 8;
+
+__global__ void __launch_bounds__(1024) k_invit_finish_red(int K, int ld, int nch, const zd* __restrict__ part,
+                                                           const zd* __restrict__ z, const zd* __restrict__ v,
+                                                           zd* __restrict__ x, double* __restrict__ out) {
+  __shared__ double sh[16];
+  const int tid = threadIdx.x;
+  zd xr[IF_REG], vr[IF_REG];
+  double mx = 0.0;
+#pragma unroll
+  for (int u = 0; u < IF_REG; ++u) {
+    const int j = tid + 1024 * u;
+    xr[u] = vr[u] = mkz(0.0, 0.0);
+    if (j < K) {
+      zd a = z[j];
+      for (int ch = (j / 64) / 2; ch < nch; ++ch) a = a + part[(size_t)ch * ld + j];
+      xr[u] = a;
+      vr[u] = v[j];
+      mx = fmax(mx, fmax(fabs(a.x), fabs(a.y)));
+    }
+  }
+  const double t1 = block_max(mx, sh);
+  const double inv = 1.0 / t1;
+  double ar = 0.0, ai = 0.0, rr = 0.0;
+#pragma unroll
+  for (int u = 0; u < IF_REG; ++u) {
+    const int j = tid + 1024 * u;
+    if (j < K) {
+      const zd xi = xr[u] * inv;
+      xr[u] = xi;
+      x[j] = xi;
+      const zd vi = vr[u];
+      ar += xi.x * vi.x + xi.y * vi.y;     // conj(x) v
+      ai += xi.x * vi.y - xi.y * vi.x;
+      rr += vi.x * vi.x + vi.y * vi.y;
+    }
+  }
+  ar = block_sum(ar, sh);
+  ai = block_sum(ai, sh);
+  rr = block_sum(rr, sh);
+  const zd tc = mkz(ar / rr, ai / rr);
+  double nd = 0.0;
+#pragma unroll
+  for (int u = 0; u < IF_REG; ++u) {
+    const int j = tid + 1024 * u;
+    if (j < K) {
+      const zd d = xr[u] - tc * vr[u];
+      nd += d.x * d.x + d.y * d.y;
+    }
+  }
+  nd = block_sum(nd, sh);
+  if (tid == 0) {
+    out[4] = out[0];
+    out[0] = sqrt(2.0 * K) * sqrt(nd);
+    out[1] = t1;
+  }
+}
+
+// w = S^H v with MHMV's formula (src/linalg.f90:1037-1044):
+// w_i = sum_j conj(S(j, i)) v_j over the whole stored column i (contiguous),
+// one wave per column, no partial sums.
+__global__ void __launch_bounds__(256) k_mhmv(int K, int ld, const zd* __restrict__ A, const zd* __restrict__ v,
+                                              zd* __restrict__ w) {
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= K) return;
+  const int lane = threadIdx.x & 63;
+  const zd* col = A + (size_t)i * ld;
+  double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+  int j = lane;
+  for (; j + 192 < K; j += 256) {   // four loads in flight, two accumulator pairs
+    const zd f0 = col[j], f1 = col[j + 64], f2 = col[j + 128], f3 = col[j + 192];
+    const zd b0 = v[j], b1 = v[j + 64], b2 = v[j + 128], b3 = v[j + 192];
+    ar = fma(f0.x, b0.x, fma(f0.y, b0.y, ar));
+    ai = fma(f0.x, b0.y, fma(-f0.y, b0.x, ai));
+    br = fma(f1.x, b1.x, fma(f1.y, b1.y, br));
+    bi = fma(f1.x, b1.y, fma(-f1.y, b1.x, bi));
+    ar = fma(f2.x, b2.x, fma(f2.y, b2.y, ar));
+    ai = fma(f2.x, b2.y, fma(-f2.y, b2.x, ai));
+    br = fma(f3.x, b3.x, fma(f3.y, b3.y, br));
+    bi = fma(f3.x, b3.y, fma(-f3.y, b3.x, bi));
+  }
+  for (; j < K; j += 64) {
+    const zd f0 = col[j], b0 = v[j];
+    ar = fma(f0.x, b0.x, fma(f0.y, b0.y, ar));
+    ai = fma(f0.x, b0.y, fma(-f0.y, b0.x, ai));
+  }
+  ar += br;
+  ai += bi;
+  for (int o = 32; o > 0; o >>= 1) {
+    ar += __shfl_xor(ar, o);
+    ai += __shfl_xor(ai, o);
+  }
+  if (lane == 0) w[i] = mkz(ar, ai);
+}
+
 __global__ void k_scale(int K, zd* __restrict__ x, const double* __restrict__ t) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < K) x[i] = x[i] * (1.0 / sqrt(*t));
 }
 
-static int gemv(const zd* A, const zd* A2, double shift, int ld, int K, const zd* x, zd* y, zd* part,
-                hipStream_t s) {
-  const int nch = (K + GEMV_CHUNK - 1) / GEMV_CHUNK;
-  hipLaunchKernelGGL(k_gemv_part, dim3((K + 63) / 64, nch), dim3(256), 0, s, K, ld, A, A2, shift, x, part);
-  hipLaunchKernelGGL(k_gemv_reduce, dim3((K + 255) / 256), dim3(256), 0, s, K, ld, nch, part, y);
-  return (int)hipGetLastError();
-}
-
 // One inverse-iteration step: x = F^{-1} (S v), then the GHEPIIS
 // normalisation and convergence measure (written to f.scal[0..1]).
-// Work: f.vec holds 4 vectors of ld entries; the GEMV partials use
-// f.W (ld x NB >= ld x ceil(K/128)).
+// S v is MHMV's column form (k_mhmv, into f.vec); the solve's partial
+// sums use f.W, and for K <= 1024 IF_REG their reduction is fused into the
+// bookkeeping kernel.
 int invit_iteration(const Ctx& c, Factor& f, const zd* v, zd* x, hipStream_t s) {
+  const int K = c.K;
   zd* w = f.vec;
-  int rc = gemv(c.S, nullptr, 0.0, c.ld, c.K, v, w, f.W, s);
-  if (rc) return rc;
+  hipLaunchKernelGGL(k_mhmv, dim3((K + 3) / 4), dim3(256), 0, s, K, c.ld, c.S, v, w);
+  int rc;
+  if (K <= 1024 * IF_REG) {
+    rc = ldlh_passes(f, w, s, true);
+    if (rc) return rc;
+    const int nch = (K + XS_CHUNK - 1) / XS_CHUNK;
+    hipLaunchKernelGGL(k_invit_finish_red, dim3(1), dim3(1024), 0, s, K, f.ld, nch, f.W, f.tvec, v, x, f.scal);
+    return (int)hipGetLastError();
+  }
   rc = ldlh_solve(f, w, x, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_invit_finish, dim3(1), dim3(1024), 0, s, c.K, x, v, f.scal);
+  hipLaunchKernelGGL(k_invit_finish, dim3(1), dim3(1024), 0, s, K, x, v, f.scal);
   return (int)hipGetLastError();
 }
 
