@@ -410,25 +410,32 @@ def test_energy_grad_stored_mode(name, K, nfru, seed):
 def test_set_params_device_matches_host():
     """ecg_set_params_device (parameters already in HBM) gives the same
     energy and gradient, bit for bit, as ecg_set_params; host memory is
-    refused with ECG_ERR_ARG."""
-    import torch
+    refused with ECG_ERR_ARG.  The device copy is made with the HIP runtime
+    the library itself loaded (hipMalloc / hipMemcpy through ctypes)."""
+    import ctypes
     from ecg_nonbo.hip import EcgError
     sy = named_system("h2")
     K = 70
     print("[case] set_params_device", K, flush=True)
-    p = seeded_basis(sy.n, K, SEED_BASE + 940)
+    p = np.ascontiguousarray(seeded_basis(sy.n, K, SEED_BASE + 940))
     a = EcgContext(sy)
     a.set_params(p)
     _, Ea, ga, _ = a.energy_grad(0, 1, float("nan"), 1e-12, 60)
     b = EcgContext(sy)
-    d = torch.from_numpy(p).to("cuda")
-    torch.cuda.synchronize()
-    b.set_params_device(K, d.data_ptr())
-    _, Eb, gb, _ = b.energy_grad(0, 1, float("nan"), 1e-12, 60)
-    assert Ea == Eb
-    assert np.array_equal(ga, gb)
-    with pytest.raises(EcgError) as e:
-        b.set_params_device(K, p.ctypes.data)
-    assert e.value.code == -1
-    a.close()
-    b.close()
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    dptr = ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(dptr), ctypes.c_size_t(p.nbytes)) == 0
+    try:
+        assert hip.hipMemcpy(dptr, p.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(p.nbytes), 1) == 0
+        assert hip.hipDeviceSynchronize() == 0
+        b.set_params_device(K, dptr.value)
+        _, Eb, gb, _ = b.energy_grad(0, 1, float("nan"), 1e-12, 60)
+        assert Ea == Eb
+        assert np.array_equal(ga, gb)
+        with pytest.raises(EcgError) as e:
+            b.set_params_device(K, p.ctypes.data)
+        assert e.value.code == -1
+    finally:
+        a.close()
+        b.close()
+        hip.hipFree(dptr)
