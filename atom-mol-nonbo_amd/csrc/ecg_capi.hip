@@ -1644,6 +1644,7 @@ int ecg_expectation_values(ecg_ctx* x, const double* c, double* values, double* 
   const int m = cx.n * (cx.n + 1) / 2, nobs = 5 * m + 4;
   const size_t need = ecg::expectation_partial_doubles(cx.n, K) + 2 * (size_t)nobs;
   if (need > x->xpart_cap) {
+    x->xpart_cap = 0;   // stays 0 if the allocation fails
     RCCHK(dev_alloc(&x->xpart, need));
     x->xpart_cap = need;
   }
@@ -1667,6 +1668,40 @@ int ecg_expectation_values(ecg_ctx* x, const double* c, double* values, double* 
     if (errors) errors[a] = h[nobs + a] * 2.220446049250313e-16;   // epsilon(1.0)
   }
   return finite_all(values, nobs) ? ECG_OK : ECG_ERR_NONFINITE;
+}
+
+int ecg_correlation_functions(ecg_ctx* x, const double* c, int ngrid, const double* grid, double* values) {
+  (void)hipGetLastError();   // start from a clean runtime error state
+  if (!x || !values || ngrid < 1 || !grid) return ECG_ERR_ARG;
+  if (!finite_all(grid, (size_t)ngrid)) return ECG_ERR_ARG;
+  ecg::Ctx& cx = x->c;
+  const int K = cx.K;
+  if (K < 1) return ECG_ERR_STATE;
+  if (!c && !(x->have_last && x->last_K == K)) return ECG_ERR_STATE;
+  if (c && !finite_all(c, 2 * (size_t)K)) return ECG_ERR_ARG;
+  const int m = cx.n * (cx.n + 1) / 2;
+  const size_t nout = (size_t)m * ngrid;
+  const size_t need = ecg::correlation_partial_doubles(cx.n, K) + nout;
+  if (need > x->xpart_cap) {
+    x->xpart_cap = 0;   // stays 0 if the allocation fails
+    RCCHK(dev_alloc(&x->xpart, need));
+    x->xpart_cap = need;
+  }
+  if (x->hs_rows < K) {   // diagS (the normalisation) from the current parameters
+    hs_rewritten(x, x->hs_rows);
+    RCCHK(ecg::assemble(cx, x->hs_rows, K, x->stream));
+    x->hs_rows = K;
+  }
+  const zd* cv = x->ev[2];
+  if (c) {
+    HIPCHK(hipMemcpyAsync(x->ev[1], c, sizeof(zd) * K, hipMemcpyHostToDevice, x->stream));
+    cv = x->ev[1];
+  }
+  double* dout = x->xpart + ecg::correlation_partial_doubles(cx.n, K);
+  RCCHK(ecg::correlation_functions(cx, cv, ngrid, grid, x->xpart, dout, x->stream));
+  HIPCHK(hipMemcpyAsync(values, dout, sizeof(double) * nout, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return finite_all(values, nout) ? ECG_OK : ECG_ERR_NONFINITE;
 }
 
 int ecg_last_timings(ecg_ctx* x, double* ms, int nmax) {

@@ -115,6 +115,11 @@ int overlap_penalty(const Ctx& c, int nfru, double pc, double t2, double* pen, d
 // expectation_partial_doubles(n, K).
 int expectation_values(const Ctx& c, const zd* cvec, double* part, double* out, hipStream_t s);
 size_t expectation_partial_doubles(int n, int K);
+// Pair correlation functions at ngrid radii (host array grid): out (device,
+// n(n+1)/2 x ngrid, pair index fastest); part: correlation_partial_doubles.
+int correlation_functions(const Ctx& c, const zd* cvec, int ngrid, const double* grid, double* part, double* out,
+                          hipStream_t s);
+size_t correlation_partial_doubles(int n, int K);
 
 // dense Hermitian linear algebra (ecg_linalg.hip)
 struct Factor {

@@ -670,6 +670,148 @@ size_t expectation_partial_doubles(int n, int K) {
   return (size_t)nt * (nt + 1) / 2 * 2 * (5 * m + 4);
 }
 
+// ------------------------------------------------ correlation functions
+// Pair correlation functions on a radial grid, the AreCorrFuncNeeded part
+// of MatrixElementsForExpcVals (src/matelem.f90:1948-1959):
+//   CF_kl(q, r) = S_kl exp(-r^2 / TrCJ_q) / (sqrt(TrCJ_q) TrCJ_q) / pi^(3/2)
+//              = S_kl rs_q^3 exp(-r^2 rs_q^2) / pi^(3/2),
+// q over the pseudo-particle pairs (a, b >= a) as the other pair values,
+// contracted with the vector like the driver's CFDMEkl_s
+// (src/workproc.f90:8234-8243; no rounding estimate there).  CF_q(0) is
+// <delta(r_q)>.  One launch per CF_CHUNK grid points (r^2 in r2), values
+// (point, pair) with the pair index fastest; fixed-order reduction.
+constexpr int CF_CHUNK = 4;
+
+template <int N>
+__global__ void __launch_bounds__(TILE* TILE) k_corr(const SysConst sc, int K, const double* __restrict__ bra,
+                                                   const double* __restrict__ ket,
+                                                   const double* __restrict__ diagS, const zd* __restrict__ cvec,
+                                                   double r2_0, double r2_1, double r2_2, double r2_3, int ng,
+                                                   double* __restrict__ part) {
+  constexpr int m = N * (N + 1) / 2, NV = CF_CHUNK * m;
+  extern __shared__ double lds[];
+  int tr, tc;
+  tri_tile(blockIdx.x, 0, tr, tc);
+  const int T = sc.T;
+  const int kb = tr * TILE, lb = tc * TILE;
+  double* sbra = lds;
+  double* sket = lds + TILE * BraRec<N>::stride;
+  const int tid = threadIdx.y * TILE + threadIdx.x;
+  for (int i = tid; i < TILE * BraRec<N>::stride; i += TILE * TILE) {
+    const int f = kb + i / BraRec<N>::stride;
+    sbra[i] = (f < K) ? bra[(size_t)kb * BraRec<N>::stride + i] : 0.0;
+  }
+  const int kstride = T * KetRec<N>::stride;
+  for (int i = tid; i < TILE * kstride; i += TILE * TILE) {
+    const int f = lb + i / kstride;
+    sket[i] = (f < K) ? ket[(size_t)lb * kstride + i] : 0.0;
+  }
+  __syncthreads();
+  const double r2[CF_CHUNK] = {r2_0, r2_1, r2_2, r2_3};
+  double sum[NV];
+#pragma unroll
+  for (int a = 0; a < NV; ++a) sum[a] = 0.0;
+  const int k = kb + threadIdx.x, l = lb + threadIdx.y;
+  if (k < K && l <= k) {
+    const double* br = sbra + threadIdx.x * BraRec<N>::stride;
+    const double* kr0 = sket + threadIdx.y * kstride;
+    const zd* ck = reinterpret_cast<const zd*>(br + BraRec<N>::ck);
+    const double adet = br[BraRec<N>::adet] * bra[(size_t)l * BraRec<N>::stride + BraRec<N>::adet];
+    const double f = (k == l) ? 1.0 / diagS[k] : 2.0 / sqrt(diagS[k] * diagS[l]);
+    const zd ci = cvec[k], cj = cvec[l];
+    const double wr = ci.x * cj.x + ci.y * cj.y, wi = ci.x * cj.y - ci.y * cj.x;   // conj(c_k) c_l
+    constexpr double inv_pi32 = 0.179587122125166564899490489694101733;    // pi^(-3/2)
+    for (int t = 0; t < T; ++t) {
+      const double* kr = kr0 + (size_t)t * KetRec<N>::stride;
+      PairCore<N> pc;
+      pair_energy<N>(sc, ck, reinterpret_cast<const zd*>(kr + KetRec<N>::ct),
+                     reinterpret_cast<const zd*>(kr + KetRec<N>::nl), adet, pc);
+      const double fc = f * sc.coef[t];
+      int q = 0;
+#pragma unroll
+      for (int a = 0; a < N; ++a)
+#pragma unroll
+        for (int b = a; b < N; ++b, ++q) {
+          const zd rs = pc.rs[(a == b) ? sidx(a, a) : sidx(b, a)];
+          const zd rs2 = rs * rs;
+          const zd A = inv_pi32 * ((pc.S * rs) * rs2);
+#pragma unroll
+          for (int g = 0; g < CF_CHUNK; ++g) {
+            if (g >= ng) break;
+            // exp(-r^2 rs^2) for complex rs^2
+            const double er = exp(-r2[g] * rs2.x);
+            double sn, cs;
+            sincos(-r2[g] * rs2.y, &sn, &cs);
+            const zd O = A * mkz(er * cs, er * sn);
+            sum[g * m + q] += fc * (wr * O.x - wi * O.y);
+          }
+        }
+    }
+  }
+  double* red = lds;   // 4 waves x NV
+  __syncthreads();
+  const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+  for (int a = 0; a < NV; ++a) {
+    double v = sum[a];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0) red[wave * NV + a] = v;
+  }
+  __syncthreads();
+  for (int a = tid; a < NV; a += TILE * TILE)
+    part[(size_t)blockIdx.x * NV + a] = ((red[a] + red[NV + a]) + red[2 * NV + a]) + red[3 * NV + a];
+}
+
+__global__ void __launch_bounds__(256) k_corr_finish(int nv, int ncopy, long long ntiles, const double* __restrict__ part,
+                                                     double* __restrict__ out) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= ncopy) return;
+  double s = 0.0;
+  for (long long t = 0; t < ntiles; ++t) s += part[(size_t)t * nv + a];
+  out[a] = s;
+}
+
+template <int N>
+static int launch_corr(const Ctx& c, const zd* cvec, int ngrid, const double* grid, double* part, double* out,
+                       hipStream_t s) {
+  constexpr int m = N * (N + 1) / 2, NV = CF_CHUNK * m;
+  const int K = c.K;
+  const int nt = (K + TILE - 1) / TILE;
+  const long long ntiles = (long long)nt * (nt + 1) / 2;
+  size_t lds = sizeof(double) * TILE * (BraRec<N>::stride + c.T * KetRec<N>::stride);
+  lds = std::max(lds, sizeof(double) * 4 * NV);
+  for (int g0 = 0; g0 < ngrid; g0 += CF_CHUNK) {
+    double r2[CF_CHUNK];
+    const int ng = std::min(CF_CHUNK, ngrid - g0);
+    for (int g = 0; g < CF_CHUNK; ++g) r2[g] = (g < ng) ? grid[g0 + g] * grid[g0 + g] : 0.0;
+    hipLaunchKernelGGL(k_corr<N>, dim3((unsigned)ntiles), dim3(TILE, TILE), lds, s, c.sc, K, c.bra, c.ket,
+                       c.diagS, cvec, r2[0], r2[1], r2[2], r2[3], ng, part);
+    // the chunk's values (ng of its CF_CHUNK points) go to their place in out
+    hipLaunchKernelGGL(k_corr_finish, dim3((ng * m + 255) / 256), dim3(256), 0, s, NV, ng * m, ntiles, part,
+                       out + (size_t)g0 * m);
+    if (ng < CF_CHUNK) break;
+  }
+  return (int)hipGetLastError();
+}
+
+int correlation_functions(const Ctx& c, const zd* cvec, int ngrid, const double* grid, double* part, double* out,
+                          hipStream_t s) {
+  switch (c.n) {
+    case 1: return launch_corr<1>(c, cvec, ngrid, grid, part, out, s);
+    case 2: return launch_corr<2>(c, cvec, ngrid, grid, part, out, s);
+    case 3: return launch_corr<3>(c, cvec, ngrid, grid, part, out, s);
+    case 4: return launch_corr<4>(c, cvec, ngrid, grid, part, out, s);
+    case 5: return launch_corr<5>(c, cvec, ngrid, grid, part, out, s);
+  }
+  return -1;
+}
+
+size_t correlation_partial_doubles(int n, int K) {
+  const int nt = (K + TILE - 1) / TILE;
+  const int m = n * (n + 1) / 2;
+  return (size_t)nt * (nt + 1) / 2 * CF_CHUNK * m;
+}
+
 // ------------------------------------------------------ overlap penalty
 // ComputeOverlapPenalty / ComputeOverlapPenaltyAndAddGradient
 // (src/workproc.f90:1491-1693).  P sums pc (|s_ji|^2 - t2), pc = b/(1-t2),

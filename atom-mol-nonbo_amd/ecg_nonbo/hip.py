@@ -45,6 +45,7 @@ EXPORTS = [
     "ecg_energy_grad_stored", "ecg_batch_create", "ecg_batch_destroy", "ecg_batch_set_config",
     "ecg_batch_energy_grad", "ecg_trace", "ecg_trace_read", "ecg_get_params", "ecg_screen_prepare", "ecg_screen", "ecg_screen_loop", "ecg_last_timings",
     "ecg_set_overlap_penalty", "ecg_overlap_penalty", "ecg_set_params_device", "ecg_expectation_values",
+    "ecg_correlation_functions",
 ]
 
 
@@ -96,6 +97,7 @@ def load(path: str = LIB_PATH):
     L.ecg_set_params_device.argtypes = [vp, c_int, vp]
     L.ecg_overlap_penalty.argtypes = [vp, c_int, c_double, c_double, _d, _d]
     L.ecg_expectation_values.argtypes = [vp, _d, _d, _d]
+    L.ecg_correlation_functions.argtypes = [vp, _d, c_int, _d, _d]
     for name in EXPORTS:
         getattr(L, name).restype = c_int
     _lib = L
@@ -291,6 +293,17 @@ class EcgContext:
         _check("ecg_expectation_values", load().ecg_expectation_values(
             self._h, cc.ctypes.data_as(_d) if cc is not None else None, _p(vals), _p(errs)))
         return vals, errs
+
+    def correlation_functions(self, grid, c=None):
+        """ecg_correlation_functions: (ngrid, m) array, row g = the
+        pseudo-particle pair correlation functions at radius grid[g]."""
+        m = self.system.n * (self.system.n + 1) // 2
+        g = np.ascontiguousarray(grid, dtype=np.float64).ravel()
+        out = np.zeros((g.size, m))
+        cc = None if c is None else np.ascontiguousarray(c, dtype=np.complex128)
+        _check("ecg_correlation_functions", load().ecg_correlation_functions(
+            self._h, cc.ctypes.data_as(_d) if cc is not None else None, g.size, _p(g), _p(out)))
+        return out
 
     def trace(self, enable=True):
         _check("ecg_trace", load().ecg_trace(self._h, int(bool(enable))))

@@ -18,6 +18,7 @@
 !                  ComputeOverlapPenalty   src/workproc.f90:1491
 !                  ComputeOverlapPenaltyAndAddGradient  src/workproc.f90:1571
 !                  EcgExpectationValues               src/workproc.f90:8158-8300 (ExpectationValuesG's loop)
+!                  EcgCorrelationFunctions            src/workproc.f90:8234-8243 (its correlation functions)
 !                plus EcgHostInit / EcgHostFinish (context set-up, no
 !                counterpart in the reference: ProgramDataInit ends by
 !                calling EcgHostInit).
@@ -220,6 +221,15 @@ module ecg_capi
       real(c_double), intent(out) :: values(*)
       real(c_double), intent(out) :: errors(*)
     end function
+    integer(c_int) function ecg_correlation_functions(ctx, c, ngrid, grid, values) &
+        bind(C, name="ecg_correlation_functions")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: c              ! K complex or c_null_ptr (last eigenvector)
+      integer(c_int), value :: ngrid
+      real(c_double), intent(in) :: grid(*)
+      real(c_double), intent(out) :: values(*)
+    end function
     integer(c_int) function ecg_last_timings(ctx, ms, nmax) bind(C, name="ecg_last_timings")
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx
@@ -263,7 +273,7 @@ module ecg_hostpath
   private
   public :: EcgHostInit, EcgHostFinish, MatrixElements, ComputeMatElem, ComputeMatElemAndDeriv, GHEPIIS, &
             EnergyGA, EnergyIA, EnergyGB, EnergyIB, ComputeOverlapPenalty, ComputeOverlapPenaltyAndAddGradient, &
-            EcgExpectationValues
+            EcgExpectationValues, EcgCorrelationFunctions
   type(c_ptr), save :: ctx = c_null_ptr
 
 contains
@@ -420,6 +430,15 @@ contains
     real(dprec) :: Values(:), Errors(:)
     integer :: ErrorCode
     ErrorCode = ecg_expectation_values(ctx, c_null_ptr, Values, Errors)
+  end subroutine
+
+  ! src/workproc.f90:8234-8243 with src/matelem.f90:1948-1959 — the pair
+  ! correlation functions CF(b, a) at the radii CFGrid(a), b over the
+  ! pairs (i, j >= i), in the eigenvector of the last energy call
+  subroutine EcgCorrelationFunctions(NumCFGridPoints, CFGrid, CF, ErrorCode)
+    integer :: NumCFGridPoints, ErrorCode
+    real(dprec) :: CFGrid(:), CF(:, :)
+    ErrorCode = ecg_correlation_functions(ctx, c_null_ptr, NumCFGridPoints, CFGrid, CF)
   end subroutine
 
   ! src/workproc.f90:1571 — as ComputeOverlapPenalty, adding the penalty's

@@ -266,6 +266,16 @@ int ecg_overlap_penalty(ecg_ctx* ctx, int nfru, double max_pair_penalty, double 
  * terms, correlation functions and densities are not computed. */
 int ecg_expectation_values(ecg_ctx* ctx, const double* c, double* values, double* errors);
 
+/* Pair correlation functions on a radial grid: the AreCorrFuncNeeded part
+ * of MatrixElementsForExpcVals (src/matelem.f90:1948-1959),
+ *   CF_kl(q, r) = S_kl exp(-r^2 / TrCJ_q) / (sqrt(TrCJ_q) TrCJ_q) / pi^(3/2),
+ * contracted with the eigenvector of the last energy call (c == NULL) or
+ * with c (K complex) as the driver's CFDMEkl_s (src/workproc.f90:
+ * 8234-8243).  grid: ngrid radii; values: n(n+1)/2 x ngrid doubles, pair
+ * index q fastest (pairs (a, b >= a) in the order of the other pair
+ * values).  CF_q(0) equals <delta(r_q)>; 4 pi int r^2 CF_q(r) dr = <S>. */
+int ecg_correlation_functions(ecg_ctx* ctx, const double* c, int ngrid, const double* grid, double* values);
+
 /* Device time (ms, HIP events on the context's stream) of the phases of
  * the last ecg_energy / ecg_energy_grad call: [0] assembly, [1] eigen step
  * (all factorisations and iterations), [2] adjoint pass, [3] whole call,

@@ -554,4 +554,57 @@ int oracle_expectation_values(int n, int T, const int* P, const int* coef, const
   return 0;
 }
 
+// Correlation functions (src/matelem.f90:1948-1959): per pair (a, b >= a)
+// and radius r, CF = S exp(-r^2 / TrCJ) / (sqrt(TrCJ) TrCJ) / (pi sqrt(pi)),
+// TrCJ = w^T C^-1 w; contracted as the driver's CFDMEkl_s
+// (src/workproc.f90:8234-8243): out[g m + q] += factor coef_k beta.
+int oracle_correlation_functions(int n, int T, const int* P, const int* coef, const double* M, double q0,
+                                 const double* q, int K, const double* params, const double* c,
+                                 const double* diagS, int ngrid, const double* grid, double* out, int nthreads) {
+  System<double> sy = make_sys<double>(n, T, P, coef, M, q0, q);
+  const int npt = n * (n + 1), m = n * (n + 1) / 2, nv = m * ngrid;
+  const double sqrtpi = 1.77245385090551602729816748334115, pi = 3.14159265358979323846264338327950;
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  const int nth = omp_get_max_threads();
+  std::vector<double> acc((size_t)nth * nv, 0.0);
+#pragma omp parallel
+  {
+    const int tid = omp_get_thread_num();
+    double* s_ = acc.data() + (size_t)tid * nv;
+#pragma omp for schedule(dynamic, 4)
+    for (int i = 0; i < K; ++i)
+      for (int j = 0; j <= i; ++j) {
+        const double rci = c[2 * i], ici = c[2 * i + 1], rcj = c[2 * j], icj = c[2 * j + 1];
+        const double factor = (i == j) ? 1.0 / std::sqrt(diagS[i] * diagS[i]) : 2.0 / std::sqrt(diagS[i] * diagS[j]);
+        for (int k = 0; k < T; ++k) {
+          cd H, S;
+          PairParts<double> pp;
+          matrix_elements<double>(sy, params + (size_t)npt * i, params + (size_t)npt * j, P + (size_t)k * n * n, 0,
+                                  H, S, nullptr, nullptr, &pp);
+          const cd ctemp = S / (pi * sqrtpi);
+          int qq = 0;
+          for (int a = 0; a < n; ++a)
+            for (int b = a; b < n; ++b, ++qq) {
+              const cd t = (a == b) ? pp.Ci[a][a] : pp.Ci[a][a] + pp.Ci[b][b] - pp.Ci[b][a] - pp.Ci[b][a];
+              const cd den = csqrt(t) * t;
+              for (int g = 0; g < ngrid; ++g) {
+                const cd e = cd{-grid[g] * grid[g], 0.0} / t;
+                const double mag = std::exp(e.re);
+                const cd ex = {mag * std::cos(e.im), mag * std::sin(e.im)};
+                const cd o = (ctemp * ex) / den;
+                const double beta = (rci * rcj + ici * icj) * o.re + (ici * rcj - rci * icj) * o.im;
+                s_[(size_t)g * m + qq] += factor * coef[k] * beta;
+              }
+            }
+        }
+      }
+  }
+  for (int a = 0; a < nv; ++a) {
+    double sa = 0.0;
+    for (int t = 0; t < nth; ++t) sa += acc[(size_t)t * nv + a];
+    out[a] = sa;
+  }
+  return 0;
+}
+
 }  // extern "C"

@@ -69,6 +69,9 @@ def lib():
         L.oracle_expectation_values.restype = ctypes.c_int
         L.oracle_expectation_values.argtypes = [ctypes.c_int, ctypes.c_int, _i, _i, _d, ctypes.c_double, _d,
                                                 ctypes.c_int, _d, _d, _d, _d, _d, ctypes.c_int]
+        L.oracle_correlation_functions.restype = ctypes.c_int
+        L.oracle_correlation_functions.argtypes = [ctypes.c_int, ctypes.c_int, _i, _i, _d, ctypes.c_double, _d,
+                                                   ctypes.c_int, _d, _d, _d, ctypes.c_int, _d, _d, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -260,6 +263,21 @@ def expectation_values(system, params, c, diagS, nthreads=0):
     lib().oracle_expectation_values(*s.args(), K, _pd(params), np.ascontiguousarray(c, dtype=np.complex128).ctypes.data_as(_d),
                                     _pd(np.ascontiguousarray(diagS, dtype=np.float64)), _pd(out), _pd(err), nthreads)
     return out, err
+
+
+def correlation_functions(system, params, c, diagS, grid, nthreads=0):
+    """The reference driver's pair correlation functions at the radii in
+    grid, in the vector c: (ngrid, m) reals, row g for grid[g]."""
+    s = _Sys(system)
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    K = params.shape[0]
+    m = system.n * (system.n + 1) // 2
+    g = np.ascontiguousarray(grid, dtype=np.float64).ravel()
+    out = np.zeros((g.size, m))
+    lib().oracle_correlation_functions(*s.args(), K, _pd(params), np.ascontiguousarray(c, dtype=np.complex128).ctypes.data_as(_d),
+                                       _pd(np.ascontiguousarray(diagS, dtype=np.float64)), g.size, _pd(g), _pd(out),
+                                       nthreads)
+    return out
 
 
 def ghepiis(k, M, invD, B, apprlambda, v, tol, maxiter=20, specif_norm=0):

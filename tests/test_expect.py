@@ -12,9 +12,14 @@ with the eigenvector.  Pinned here without retyping those forms:
   * the contraction: <S> = 1 and <H> = E for the eigenvector of the HD+
     fixture, <H> = <T> + <V>, and the virial ratio -<V>/<T> = 2 of that
     optimised basis.
+  * the pair correlation functions CF_q(r) (src/matelem.f90:1948-1959)
+    through what fixes them independently of their formula: CF_q(0) is
+    <delta(r_q)>, 4 pi int r^2 CF_q dr is <S> and 4 pi int r^4 CF_q dr is
+    <r_q^2> (adaptive quadrature of the oracle's CF over r), with random
+    complex vectors.
 Not built (and not pinned): the Drachmanized delta functions, the
-mass-velocity, Darwin and orbit-orbit terms, correlation functions and
-particle densities on grids.
+mass-velocity, Darwin and orbit-orbit terms and particle densities on
+grids.
 """
 from __future__ import annotations
 
@@ -88,3 +93,27 @@ def test_expectation_values_hdplus_fixture(golden_dir):
     r12 = vals[2 * m + 0]
     assert 2.0 < r12 < 2.1, r12
     assert np.all(errs >= 0.0) and np.all(errs <= 1e-6 * np.maximum(1.0, np.abs(vals)))
+
+
+@pytest.mark.parametrize("name,K", [("hminus", 12), ("h2", 10)])
+def test_correlation_functions_moments(name, K):
+    from scipy.integrate import quad_vec
+    sy = named_system(name)
+    p = seeded_basis(sy.n, K, SEED_BASE + 71)
+    rng = np.random.Generator(np.random.PCG64(72))
+    c = rng.normal(size=K) + 1j * rng.normal(size=K)
+    _, _, _, dS = oracle.assemble(sy, p, "G")
+    vals, _ = oracle.expectation_values(sy, p, c, dS)
+    m = sy.n * (sy.n + 1) // 2
+    delta, r2, S = vals[4 * m:5 * m], vals[3 * m:4 * m], vals[5 * m + 1]
+    cf0 = oracle.correlation_functions(sy, p, c, dS, [0.0])[0]
+    np.testing.assert_allclose(cf0, delta, rtol=1e-13, atol=0)
+    # moments over r (the map r = x / (1 - x) takes [0, 1) to [0, inf))
+    def integrand(x):
+        r = x / (1.0 - x)
+        jac = 1.0 / (1.0 - x) ** 2
+        cf = oracle.correlation_functions(sy, p, c, dS, [r])[0]
+        return np.concatenate([4 * np.pi * r ** 2 * cf, 4 * np.pi * r ** 4 * cf]) * jac
+    mom, err = quad_vec(integrand, 0.0, 1.0 - 1e-12, epsabs=1e-13, epsrel=1e-11, limit=2000)
+    np.testing.assert_allclose(mom[:m], S, rtol=2e-9)
+    np.testing.assert_allclose(mom[m:], r2, rtol=2e-9)

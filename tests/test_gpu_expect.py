@@ -78,3 +78,47 @@ def test_expectation_values_needs_a_vector():
     with pytest.raises(Exception):
         ctx.expectation_values()
     ctx.close()
+
+
+@pytest.mark.parametrize("name,K,ngrid", [("n1", 24, 7), ("hminus", 40, 9), ("h2", 33, 4), ("li", 30, 5)])
+def test_correlation_functions_vs_oracle(name, K, ngrid):
+    """ecg_correlation_functions against the oracle (itself pinned on the
+    CPU by its r = 0 value and radial moments): 1e-12 relative to the
+    largest value of each pair's function, over grids whose length is and
+    is not a multiple of the device's four-point chunks."""
+    say("ecg_correlation_functions", name, K, ngrid)
+    sy = make_system([1.0, 1836.15267261], [1.0, -1.0], "(P11)") if name == "n1" else named_system(name)
+    p = seeded_basis(sy.n, K, SEED_BASE + 61)
+    rng = np.random.Generator(np.random.PCG64(63))
+    c = rng.normal(size=K) + 1j * rng.normal(size=K)
+    grid = np.linspace(0.0, 3.0, ngrid)
+    _, _, _, dS = oracle.assemble(sy, p, "G")
+    ref = oracle.correlation_functions(sy, p, c, dS, grid)
+    ctx = EcgContext(sy)
+    ctx.set_params(p)
+    got = ctx.correlation_functions(grid, c)
+    vals, _ = ctx.expectation_values(c)
+    ctx.close()
+    m = sy.n * (sy.n + 1) // 2
+    scale = np.max(np.abs(ref), axis=0)
+    print("max rel dev", np.max(np.abs(got - ref) / scale))
+    assert np.all(np.abs(got - ref) <= 1e-12 * scale + 1e-300)
+    np.testing.assert_allclose(got[0], vals[4 * m:5 * m], rtol=1e-12)   # CF(0) = <delta>
+
+
+def test_correlation_functions_hdplus_last_eigenvector(golden_dir):
+    say("ecg_correlation_functions", "inout_1_500", "last eigenvector")
+    f = read_inout(os.path.join(golden_dir, "hdplus", "inout_1_500.txt"))
+    sigma = f.current_energy * f.invit_parameter
+    ctx = EcgContext(f.system)
+    ctx.set_params(f.params)
+    _, E, c, _ = ctx.energy(1, sigma, -1e-9, 20, want_vector=True)
+    grid = np.linspace(0.0, 4.0, 6)
+    got = ctx.correlation_functions(grid)
+    got2 = ctx.correlation_functions(grid, c)
+    ctx.close()
+    np.testing.assert_array_equal(got, got2)
+    _, _, _, dS = oracle.assemble(f.system, f.params, "G")
+    ref = oracle.correlation_functions(f.system, f.params, c, dS, grid)
+    scale = np.max(np.abs(ref), axis=0)
+    assert np.all(np.abs(got - ref) <= 1e-11 * scale)
