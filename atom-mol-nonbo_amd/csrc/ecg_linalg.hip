@@ -1296,7 +1296,15 @@ __global__ void __launch_bounds__(1024) k_invit_finish_red(int K, int ld, int nc
     xr[u] = vr[u] = mkz(0.0, 0.0);
     if (j < K) {
       zd a = z[j];
-      for (int ch = (j / 64) / 2; ch < nch; ++ch) a = a + part[(size_t)ch * ld + j];
+      // the partials in k_xsolve_reduce's order, eight loads in flight
+      for (int ch0 = (j / 64) / 2; ch0 < nch; ch0 += 8) {
+        zd t[8];
+#pragma unroll
+        for (int w = 0; w < 8; ++w) t[w] = (ch0 + w < nch) ? part[(size_t)(ch0 + w) * ld + j] : mkz(0.0, 0.0);
+#pragma unroll
+        for (int w = 0; w < 8; ++w)
+          if (ch0 + w < nch) a = a + t[w];
+      }
       xr[u] = a;
       vr[u] = v[j];
       mx = fmax(mx, fmax(fabs(a.x), fabs(a.y)));
