@@ -1670,17 +1670,17 @@ int ecg_expectation_values(ecg_ctx* x, const double* c, double* values, double* 
   return finite_all(values, nobs) ? ECG_OK : ECG_ERR_NONFINITE;
 }
 
-int ecg_correlation_functions(ecg_ctx* x, const double* c, int ngrid, const double* grid, double* values) {
-  (void)hipGetLastError();   // start from a clean runtime error state
-  if (!x || !values || ngrid < 1 || !grid) return ECG_ERR_ARG;
+// Shared set-up of the radial-grid values: argument checks, the partial-
+// sum buffer, diagS and the vector on the device.
+static int grid_values_setup(ecg_ctx* x, const double* c, int ngrid, const double* grid, size_t nout,
+                             const zd** cv, double** dout) {
+  if (!x || ngrid < 1 || !grid) return ECG_ERR_ARG;
   if (!finite_all(grid, (size_t)ngrid)) return ECG_ERR_ARG;
   ecg::Ctx& cx = x->c;
   const int K = cx.K;
   if (K < 1) return ECG_ERR_STATE;
   if (!c && !(x->have_last && x->last_K == K)) return ECG_ERR_STATE;
   if (c && !finite_all(c, 2 * (size_t)K)) return ECG_ERR_ARG;
-  const int m = cx.n * (cx.n + 1) / 2;
-  const size_t nout = (size_t)m * ngrid;
   const size_t need = ecg::correlation_partial_doubles(cx.n, K) + nout;
   if (need > x->xpart_cap) {
     x->xpart_cap = 0;   // stays 0 if the allocation fails
@@ -1692,13 +1692,40 @@ int ecg_correlation_functions(ecg_ctx* x, const double* c, int ngrid, const doub
     RCCHK(ecg::assemble(cx, x->hs_rows, K, x->stream));
     x->hs_rows = K;
   }
-  const zd* cv = x->ev[2];
+  *cv = x->ev[2];
   if (c) {
     HIPCHK(hipMemcpyAsync(x->ev[1], c, sizeof(zd) * K, hipMemcpyHostToDevice, x->stream));
-    cv = x->ev[1];
+    *cv = x->ev[1];
   }
-  double* dout = x->xpart + ecg::correlation_partial_doubles(cx.n, K);
-  RCCHK(ecg::correlation_functions(cx, cv, ngrid, grid, x->xpart, dout, x->stream));
+  *dout = x->xpart + ecg::correlation_partial_doubles(cx.n, K);
+  return ECG_OK;
+}
+
+int ecg_correlation_functions(ecg_ctx* x, const double* c, int ngrid, const double* grid, double* values) {
+  (void)hipGetLastError();   // start from a clean runtime error state
+  if (!x || !values) return ECG_ERR_ARG;
+  const size_t nout = (size_t)(x->c.n * (x->c.n + 1) / 2) * (ngrid > 0 ? ngrid : 0);
+  const zd* cv = nullptr;
+  double* dout = nullptr;
+  RCCHK(grid_values_setup(x, c, ngrid, grid, nout, &cv, &dout));
+  RCCHK(ecg::correlation_functions(x->c, cv, ngrid, grid, x->xpart, dout, x->stream));
+  HIPCHK(hipMemcpyAsync(values, dout, sizeof(double) * nout, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return finite_all(values, nout) ? ECG_OK : ECG_ERR_NONFINITE;
+}
+
+int ecg_particle_densities(ecg_ctx* x, const double* c, const double* masses, int ngrid, const double* grid,
+                           double* values) {
+  (void)hipGetLastError();   // start from a clean runtime error state
+  if (!x || !values || !masses) return ECG_ERR_ARG;
+  const int npart = x->c.n + 1;
+  for (int i = 0; i < npart; ++i)
+    if (!(masses[i] > 0.0) || !std::isfinite(masses[i])) return ECG_ERR_ARG;
+  const size_t nout = (size_t)npart * (ngrid > 0 ? ngrid : 0);
+  const zd* cv = nullptr;
+  double* dout = nullptr;
+  RCCHK(grid_values_setup(x, c, ngrid, grid, nout, &cv, &dout));
+  RCCHK(ecg::particle_densities(x->c, cv, masses, ngrid, grid, x->xpart, dout, x->stream));
   HIPCHK(hipMemcpyAsync(values, dout, sizeof(double) * nout, hipMemcpyDeviceToHost, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   return finite_all(values, nout) ? ECG_OK : ECG_ERR_NONFINITE;

@@ -119,6 +119,10 @@ size_t expectation_partial_doubles(int n, int K);
 // n(n+1)/2 x ngrid, pair index fastest); part: correlation_partial_doubles.
 int correlation_functions(const Ctx& c, const zd* cvec, int ngrid, const double* grid, double* part, double* out,
                           hipStream_t s);
+// Particle densities at ngrid radii from the n + 1 particle masses: out
+// (device, (n + 1) x ngrid, particle index fastest).
+int particle_densities(const Ctx& c, const zd* cvec, const double* masses, int ngrid, const double* grid,
+                       double* part, double* out, hipStream_t s);
 size_t correlation_partial_doubles(int n, int K);
 
 // dense Hermitian linear algebra (ecg_linalg.hip)

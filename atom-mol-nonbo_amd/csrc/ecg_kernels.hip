@@ -672,7 +672,7 @@ size_t expectation_partial_doubles(int n, int K) {
 
 // ------------------------------------------------ correlation functions
 // Pair correlation functions on a radial grid, the AreCorrFuncNeeded part
-// of MatrixElementsForExpcVals (src/matelem.f90:1948-1959):
+// of MatrixElementsForExpcVals (src/matelem.f90:1934-1946):
 //   CF_kl(q, r) = S_kl exp(-r^2 / TrCJ_q) / (sqrt(TrCJ_q) TrCJ_q) / pi^(3/2)
 //              = S_kl rs_q^3 exp(-r^2 rs_q^2) / pi^(3/2),
 // q over the pseudo-particle pairs (a, b >= a) as the other pair values,
@@ -680,15 +680,22 @@ size_t expectation_partial_doubles(int n, int K) {
 // (src/workproc.f90:8234-8243; no rounding estimate there).  CF_q(0) is
 // <delta(r_q)>.  One launch per CF_CHUNK grid points (r^2 in r2), values
 // (point, pair) with the pair index fastest; fixed-order reduction.
+// Particle densities (ArePartDensNeeded, src/matelem.f90:1948-1963) are
+// the same form with TrCJ replaced by b_i^T C^-1 b_i for the vectors b_i of
+// Glob_bvc (src/workproc.f90:612-621), i = 1..n+1 (DENS = true).
 constexpr int CF_CHUNK = 4;
 
-template <int N>
+struct DensVec {
+  double b[6][5];   // b[i][p]: component p of particle i's vector
+};
+
+template <int N, bool DENS>
 __global__ void __launch_bounds__(TILE* TILE) k_corr(const SysConst sc, int K, const double* __restrict__ bra,
                                                    const double* __restrict__ ket,
                                                    const double* __restrict__ diagS, const zd* __restrict__ cvec,
                                                    double r2_0, double r2_1, double r2_2, double r2_3, int ng,
-                                                   double* __restrict__ part) {
-  constexpr int m = N * (N + 1) / 2, NV = CF_CHUNK * m;
+                                                   const DensVec dv, double* __restrict__ part) {
+  constexpr int m = DENS ? N + 1 : N * (N + 1) / 2, NV = CF_CHUNK * m;
   extern __shared__ double lds[];
   int tr, tc;
   tri_tile(blockIdx.x, 0, tr, tc);
@@ -727,25 +734,40 @@ __global__ void __launch_bounds__(TILE* TILE) k_corr(const SysConst sc, int K, c
       pair_energy<N>(sc, ck, reinterpret_cast<const zd*>(kr + KetRec<N>::ct),
                      reinterpret_cast<const zd*>(kr + KetRec<N>::nl), adet, pc);
       const double fc = f * sc.coef[t];
-      int q = 0;
+      auto accumulate = [&](int q, zd rs) {
+        const zd rs2 = rs * rs;
+        const zd A = inv_pi32 * ((pc.S * rs) * rs2);
 #pragma unroll
-      for (int a = 0; a < N; ++a)
-#pragma unroll
-        for (int b = a; b < N; ++b, ++q) {
-          const zd rs = pc.rs[(a == b) ? sidx(a, a) : sidx(b, a)];
-          const zd rs2 = rs * rs;
-          const zd A = inv_pi32 * ((pc.S * rs) * rs2);
-#pragma unroll
-          for (int g = 0; g < CF_CHUNK; ++g) {
-            if (g >= ng) break;
-            // exp(-r^2 rs^2) for complex rs^2
-            const double er = exp(-r2[g] * rs2.x);
-            double sn, cs;
-            sincos(-r2[g] * rs2.y, &sn, &cs);
-            const zd O = A * mkz(er * cs, er * sn);
-            sum[g * m + q] += fc * (wr * O.x - wi * O.y);
-          }
+        for (int g = 0; g < CF_CHUNK; ++g) {
+          if (g >= ng) break;
+          // exp(-r^2 rs^2) for complex rs^2
+          const double er = exp(-r2[g] * rs2.x);
+          double sn, cs;
+          sincos(-r2[g] * rs2.y, &sn, &cs);
+          const zd O = A * mkz(er * cs, er * sn);
+          sum[g * m + q] += fc * (wr * O.x - wi * O.y);
         }
+      };
+      if constexpr (DENS) {
+#pragma unroll
+        for (int i = 0; i <= N; ++i) {
+          // b_i^T C^-1 b_i in the reference's order (diagonal, then twice the lower part)
+          zd tq = mkz(0.0, 0.0);
+#pragma unroll
+          for (int p = 0; p < N; ++p) {
+            tq = tq + (dv.b[i][p] * dv.b[i][p]) * pc.Ci[sidx(p, p)];
+#pragma unroll
+            for (int q = p + 1; q < N; ++q) tq = tq + (2.0 * dv.b[i][q] * dv.b[i][p]) * pc.Ci[sidx(q, p)];
+          }
+          accumulate(i, zrcp(zsqrt(tq)));
+        }
+      } else {
+        int q = 0;
+#pragma unroll
+        for (int a = 0; a < N; ++a)
+#pragma unroll
+          for (int b = a; b < N; ++b, ++q) accumulate(q, pc.rs[(a == b) ? sidx(a, a) : sidx(b, a)]);
+      }
     }
   }
   double* red = lds;   // 4 waves x NV
@@ -771,10 +793,10 @@ __global__ void __launch_bounds__(256) k_corr_finish(int nv, int ncopy, long lon
   out[a] = s;
 }
 
-template <int N>
-static int launch_corr(const Ctx& c, const zd* cvec, int ngrid, const double* grid, double* part, double* out,
-                       hipStream_t s) {
-  constexpr int m = N * (N + 1) / 2, NV = CF_CHUNK * m;
+template <int N, bool DENS>
+static int launch_corr(const Ctx& c, const zd* cvec, int ngrid, const double* grid, const DensVec& dv, double* part,
+                       double* out, hipStream_t s) {
+  constexpr int m = DENS ? N + 1 : N * (N + 1) / 2, NV = CF_CHUNK * m;
   const int K = c.K;
   const int nt = (K + TILE - 1) / TILE;
   const long long ntiles = (long long)nt * (nt + 1) / 2;
@@ -784,8 +806,8 @@ static int launch_corr(const Ctx& c, const zd* cvec, int ngrid, const double* gr
     double r2[CF_CHUNK];
     const int ng = std::min(CF_CHUNK, ngrid - g0);
     for (int g = 0; g < CF_CHUNK; ++g) r2[g] = (g < ng) ? grid[g0 + g] * grid[g0 + g] : 0.0;
-    hipLaunchKernelGGL(k_corr<N>, dim3((unsigned)ntiles), dim3(TILE, TILE), lds, s, c.sc, K, c.bra, c.ket,
-                       c.diagS, cvec, r2[0], r2[1], r2[2], r2[3], ng, part);
+    hipLaunchKernelGGL((k_corr<N, DENS>), dim3((unsigned)ntiles), dim3(TILE, TILE), lds, s, c.sc, K, c.bra, c.ket,
+                       c.diagS, cvec, r2[0], r2[1], r2[2], r2[3], ng, dv, part);
     // the chunk's values (ng of its CF_CHUNK points) go to their place in out
     hipLaunchKernelGGL(k_corr_finish, dim3((ng * m + 255) / 256), dim3(256), 0, s, NV, ng * m, ntiles, part,
                        out + (size_t)g0 * m);
@@ -796,19 +818,40 @@ static int launch_corr(const Ctx& c, const zd* cvec, int ngrid, const double* gr
 
 int correlation_functions(const Ctx& c, const zd* cvec, int ngrid, const double* grid, double* part, double* out,
                           hipStream_t s) {
+  const DensVec dv{};
   switch (c.n) {
-    case 1: return launch_corr<1>(c, cvec, ngrid, grid, part, out, s);
-    case 2: return launch_corr<2>(c, cvec, ngrid, grid, part, out, s);
-    case 3: return launch_corr<3>(c, cvec, ngrid, grid, part, out, s);
-    case 4: return launch_corr<4>(c, cvec, ngrid, grid, part, out, s);
-    case 5: return launch_corr<5>(c, cvec, ngrid, grid, part, out, s);
+    case 1: return launch_corr<1, false>(c, cvec, ngrid, grid, dv, part, out, s);
+    case 2: return launch_corr<2, false>(c, cvec, ngrid, grid, dv, part, out, s);
+    case 3: return launch_corr<3, false>(c, cvec, ngrid, grid, dv, part, out, s);
+    case 4: return launch_corr<4, false>(c, cvec, ngrid, grid, dv, part, out, s);
+    case 5: return launch_corr<5, false>(c, cvec, ngrid, grid, dv, part, out, s);
+  }
+  return -1;
+}
+
+int particle_densities(const Ctx& c, const zd* cvec, const double* masses, int ngrid, const double* grid,
+                       double* part, double* out, hipStream_t s) {
+  // Glob_bvc (src/workproc.f90:612-621): b_i = -m_(2..n+1) / Mtot for every
+  // particle i, plus 1 in component i-1 for i >= 2
+  const int n = c.n, npart = n + 1;
+  DensVec dv{};
+  double mtot = 0.0;
+  for (int i = 0; i < npart; ++i) mtot += masses[i];
+  for (int i = 0; i < npart; ++i)
+    for (int p = 0; p < n; ++p) dv.b[i][p] = -masses[p + 1] / mtot + ((i >= 1 && p == i - 1) ? 1.0 : 0.0);
+  switch (n) {
+    case 1: return launch_corr<1, true>(c, cvec, ngrid, grid, dv, part, out, s);
+    case 2: return launch_corr<2, true>(c, cvec, ngrid, grid, dv, part, out, s);
+    case 3: return launch_corr<3, true>(c, cvec, ngrid, grid, dv, part, out, s);
+    case 4: return launch_corr<4, true>(c, cvec, ngrid, grid, dv, part, out, s);
+    case 5: return launch_corr<5, true>(c, cvec, ngrid, grid, dv, part, out, s);
   }
   return -1;
 }
 
 size_t correlation_partial_doubles(int n, int K) {
   const int nt = (K + TILE - 1) / TILE;
-  const int m = n * (n + 1) / 2;
+  const int m = std::max(n * (n + 1) / 2, n + 1);   // pair functions or particle densities
   return (size_t)nt * (nt + 1) / 2 * CF_CHUNK * m;
 }
 

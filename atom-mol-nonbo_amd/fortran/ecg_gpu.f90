@@ -19,6 +19,7 @@
 !                  ComputeOverlapPenaltyAndAddGradient  src/workproc.f90:1571
 !                  EcgExpectationValues               src/workproc.f90:8158-8300 (ExpectationValuesG's loop)
 !                  EcgCorrelationFunctions            src/workproc.f90:8234-8243 (its correlation functions)
+!                  EcgParticleDensities               src/workproc.f90:8245-8254 (its particle densities)
 !                plus EcgHostInit / EcgHostFinish (context set-up, no
 !                counterpart in the reference: ProgramDataInit ends by
 !                calling EcgHostInit).
@@ -230,6 +231,16 @@ module ecg_capi
       real(c_double), intent(in) :: grid(*)
       real(c_double), intent(out) :: values(*)
     end function
+    integer(c_int) function ecg_particle_densities(ctx, c, masses, ngrid, grid, values) &
+        bind(C, name="ecg_particle_densities")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: c              ! K complex or c_null_ptr (last eigenvector)
+      real(c_double), intent(in) :: masses(*)
+      integer(c_int), value :: ngrid
+      real(c_double), intent(in) :: grid(*)
+      real(c_double), intent(out) :: values(*)
+    end function
     integer(c_int) function ecg_last_timings(ctx, ms, nmax) bind(C, name="ecg_last_timings")
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx
@@ -273,7 +284,7 @@ module ecg_hostpath
   private
   public :: EcgHostInit, EcgHostFinish, MatrixElements, ComputeMatElem, ComputeMatElemAndDeriv, GHEPIIS, &
             EnergyGA, EnergyIA, EnergyGB, EnergyIB, ComputeOverlapPenalty, ComputeOverlapPenaltyAndAddGradient, &
-            EcgExpectationValues, EcgCorrelationFunctions
+            EcgExpectationValues, EcgCorrelationFunctions, EcgParticleDensities
   type(c_ptr), save :: ctx = c_null_ptr
 
 contains
@@ -432,13 +443,22 @@ contains
     ErrorCode = ecg_expectation_values(ctx, c_null_ptr, Values, Errors)
   end subroutine
 
-  ! src/workproc.f90:8234-8243 with src/matelem.f90:1948-1959 — the pair
+  ! src/workproc.f90:8234-8243 with src/matelem.f90:1934-1946 — the pair
   ! correlation functions CF(b, a) at the radii CFGrid(a), b over the
   ! pairs (i, j >= i), in the eigenvector of the last energy call
   subroutine EcgCorrelationFunctions(NumCFGridPoints, CFGrid, CF, ErrorCode)
     integer :: NumCFGridPoints, ErrorCode
     real(dprec) :: CFGrid(:), CF(:, :)
     ErrorCode = ecg_correlation_functions(ctx, c_null_ptr, NumCFGridPoints, CFGrid, CF)
+  end subroutine
+
+  ! src/workproc.f90:8245-8254 with src/matelem.f90:1948-1963 — the particle
+  ! densities Dens(i, a) at the radii DensGrid(a), i = 1..n+1, from the
+  ! particle masses Mass(1:n+1) (Glob_Mass), in the last eigenvector
+  subroutine EcgParticleDensities(Mass, NumDensGridPoints, DensGrid, Dens, ErrorCode)
+    integer :: NumDensGridPoints, ErrorCode
+    real(dprec) :: Mass(:), DensGrid(:), Dens(:, :)
+    ErrorCode = ecg_particle_densities(ctx, c_null_ptr, Mass, NumDensGridPoints, DensGrid, Dens)
   end subroutine
 
   ! src/workproc.f90:1571 — as ComputeOverlapPenalty, adding the penalty's

@@ -267,7 +267,7 @@ int ecg_overlap_penalty(ecg_ctx* ctx, int nfru, double max_pair_penalty, double 
 int ecg_expectation_values(ecg_ctx* ctx, const double* c, double* values, double* errors);
 
 /* Pair correlation functions on a radial grid: the AreCorrFuncNeeded part
- * of MatrixElementsForExpcVals (src/matelem.f90:1948-1959),
+ * of MatrixElementsForExpcVals (src/matelem.f90:1934-1946),
  *   CF_kl(q, r) = S_kl exp(-r^2 / TrCJ_q) / (sqrt(TrCJ_q) TrCJ_q) / pi^(3/2),
  * contracted with the eigenvector of the last energy call (c == NULL) or
  * with c (K complex) as the driver's CFDMEkl_s (src/workproc.f90:
@@ -275,6 +275,15 @@ int ecg_expectation_values(ecg_ctx* ctx, const double* c, double* values, double
  * index q fastest (pairs (a, b >= a) in the order of the other pair
  * values).  CF_q(0) equals <delta(r_q)>; 4 pi int r^2 CF_q(r) dr = <S>. */
 int ecg_correlation_functions(ecg_ctx* ctx, const double* c, int ngrid, const double* grid, double* values);
+
+/* Particle densities on a radial grid: the ArePartDensNeeded part of
+ * MatrixElementsForExpcVals (src/matelem.f90:1948-1963), the form above with
+ * TrCJ replaced by b_i^T C^-1 b_i, where b_i (Glob_bvc, src/workproc.f90:
+ * 612-621) comes from the n + 1 particle masses (Glob_Mass: masses[0] is
+ * particle 1, the reference particle).  values: (n + 1) x ngrid doubles,
+ * particle index fastest, as the driver's Dens(i, a). */
+int ecg_particle_densities(ecg_ctx* ctx, const double* c, const double* masses, int ngrid, const double* grid,
+                           double* values);
 
 /* Device time (ms, HIP events on the context's stream) of the phases of
  * the last ecg_energy / ecg_energy_grad call: [0] assembly, [1] eigen step

@@ -554,7 +554,7 @@ int oracle_expectation_values(int n, int T, const int* P, const int* coef, const
   return 0;
 }
 
-// Correlation functions (src/matelem.f90:1948-1959): per pair (a, b >= a)
+// Correlation functions (src/matelem.f90:1934-1946): per pair (a, b >= a)
 // and radius r, CF = S exp(-r^2 / TrCJ) / (sqrt(TrCJ) TrCJ) / (pi sqrt(pi)),
 // TrCJ = w^T C^-1 w; contracted as the driver's CFDMEkl_s
 // (src/workproc.f90:8234-8243): out[g m + q] += factor coef_k beta.
@@ -596,6 +596,69 @@ int oracle_correlation_functions(int n, int T, const int* P, const int* coef, co
                 s_[(size_t)g * m + qq] += factor * coef[k] * beta;
               }
             }
+        }
+      }
+  }
+  for (int a = 0; a < nv; ++a) {
+    double sa = 0.0;
+    for (int t = 0; t < nth; ++t) sa += acc[(size_t)t * nv + a];
+    out[a] = sa;
+  }
+  return 0;
+}
+
+// Particle densities (src/matelem.f90:1948-1963) with Glob_bvc
+// (src/workproc.f90:612-621) built from the n + 1 masses: per particle i and
+// radius r, Dens = S exp(-r^2 / caux) / (sqrt(caux) caux) / (pi sqrt(pi)),
+// caux = sum_p [bvc(p,i)^2 Ci(p,p) + sum_{q>p} 2 bvc(q,i) bvc(p,i) Ci(q,p)];
+// contracted as the driver's CFDMEkl_s (src/workproc.f90:8245-8254).
+int oracle_particle_densities(int n, int T, const int* P, const int* coef, const double* M, double q0,
+                              const double* q, int K, const double* params, const double* c, const double* diagS,
+                              const double* masses, int ngrid, const double* grid, double* out, int nthreads) {
+  System<double> sy = make_sys<double>(n, T, P, coef, M, q0, q);
+  const int npt = n * (n + 1), npart = n + 1, nv = npart * ngrid;
+  const double sqrtpi = 1.77245385090551602729816748334115, pi = 3.14159265358979323846264338327950;
+  double mtot = 0.0;
+  for (int i = 0; i < npart; ++i) mtot += masses[i];
+  std::vector<double> bvc((size_t)n * npart);   // bvc[p + n i]
+  for (int i = 0; i < npart; ++i)
+    for (int p = 0; p < n; ++p) bvc[p + (size_t)n * i] = -masses[p + 1] / mtot;
+  for (int i = 1; i < npart; ++i) bvc[(i - 1) + (size_t)n * i] += 1.0;
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  const int nth = omp_get_max_threads();
+  std::vector<double> acc((size_t)nth * nv, 0.0);
+#pragma omp parallel
+  {
+    const int tid = omp_get_thread_num();
+    double* s_ = acc.data() + (size_t)tid * nv;
+#pragma omp for schedule(dynamic, 4)
+    for (int i = 0; i < K; ++i)
+      for (int j = 0; j <= i; ++j) {
+        const double rci = c[2 * i], ici = c[2 * i + 1], rcj = c[2 * j], icj = c[2 * j + 1];
+        const double factor = (i == j) ? 1.0 / std::sqrt(diagS[i] * diagS[i]) : 2.0 / std::sqrt(diagS[i] * diagS[j]);
+        for (int k = 0; k < T; ++k) {
+          cd H, S;
+          PairParts<double> pp;
+          matrix_elements<double>(sy, params + (size_t)npt * i, params + (size_t)npt * j, P + (size_t)k * n * n, 0,
+                                  H, S, nullptr, nullptr, &pp);
+          const cd ctemp = S / (pi * sqrtpi);
+          for (int pa = 0; pa < npart; ++pa) {
+            const double* b = bvc.data() + (size_t)n * pa;
+            cd caux = {0.0, 0.0};
+            for (int p = 0; p < n; ++p) {
+              caux = caux + (b[p] * b[p]) * pp.Ci[p][p];
+              for (int qq = p + 1; qq < n; ++qq) caux = caux + (2.0 * b[qq] * b[p]) * pp.Ci[qq][p];
+            }
+            const cd den = csqrt(caux) * caux;
+            for (int g = 0; g < ngrid; ++g) {
+              const cd e = cd{-grid[g] * grid[g], 0.0} / caux;
+              const double mag = std::exp(e.re);
+              const cd ex = {mag * std::cos(e.im), mag * std::sin(e.im)};
+              const cd o = (ctemp * ex) / den;
+              const double beta = (rci * rcj + ici * icj) * o.re + (ici * rcj - rci * icj) * o.im;
+              s_[(size_t)g * npart + pa] += factor * coef[k] * beta;
+            }
+          }
         }
       }
   }

@@ -72,6 +72,9 @@ def lib():
         L.oracle_correlation_functions.restype = ctypes.c_int
         L.oracle_correlation_functions.argtypes = [ctypes.c_int, ctypes.c_int, _i, _i, _d, ctypes.c_double, _d,
                                                    ctypes.c_int, _d, _d, _d, ctypes.c_int, _d, _d, ctypes.c_int]
+        L.oracle_particle_densities.restype = ctypes.c_int
+        L.oracle_particle_densities.argtypes = [ctypes.c_int, ctypes.c_int, _i, _i, _d, ctypes.c_double, _d,
+                                                ctypes.c_int, _d, _d, _d, _d, ctypes.c_int, _d, _d, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -277,6 +280,22 @@ def correlation_functions(system, params, c, diagS, grid, nthreads=0):
     lib().oracle_correlation_functions(*s.args(), K, _pd(params), np.ascontiguousarray(c, dtype=np.complex128).ctypes.data_as(_d),
                                        _pd(np.ascontiguousarray(diagS, dtype=np.float64)), g.size, _pd(g), _pd(out),
                                        nthreads)
+    return out
+
+
+def particle_densities(system, params, c, diagS, masses, grid, nthreads=0):
+    """The reference driver's particle densities (particles 1..n+1, masses
+    in that order) at the radii in grid, in the vector c: (ngrid, n + 1)."""
+    s = _Sys(system)
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    K = params.shape[0]
+    g = np.ascontiguousarray(grid, dtype=np.float64).ravel()
+    ms = np.ascontiguousarray(masses, dtype=np.float64).ravel()
+    assert ms.size == system.n + 1
+    out = np.zeros((g.size, system.n + 1))
+    lib().oracle_particle_densities(*s.args(), K, _pd(params), np.ascontiguousarray(c, dtype=np.complex128).ctypes.data_as(_d),
+                                    _pd(np.ascontiguousarray(diagS, dtype=np.float64)), _pd(ms), g.size, _pd(g),
+                                    _pd(out), nthreads)
     return out
 
 

@@ -12,14 +12,19 @@ with the eigenvector.  Pinned here without retyping those forms:
   * the contraction: <S> = 1 and <H> = E for the eigenvector of the HD+
     fixture, <H> = <T> + <V>, and the virial ratio -<V>/<T> = 2 of that
     optimised basis.
-  * the pair correlation functions CF_q(r) (src/matelem.f90:1948-1959)
+  * the pair correlation functions CF_q(r) (src/matelem.f90:1934-1946)
     through what fixes them independently of their formula: CF_q(0) is
     <delta(r_q)>, 4 pi int r^2 CF_q dr is <S> and 4 pi int r^4 CF_q dr is
     <r_q^2> (adaptive quadrature of the oracle's CF over r), with random
     complex vectors.
+  * the particle densities (src/matelem.f90:1948-1963, vectors Glob_bvc
+    from the masses): at n = 1 they are the pair function at a scaled
+    radius, Dens_i(r) = CF(r / |b_i|) / |b_i|^3; at n = 2, 3 their
+    moments 4 pi int r^2 Dens_i dr = <S> and 4 pi int r^4 Dens_i dr =
+    <|b_i . r|^2>, the latter built from the pinned <r_p^2> and
+    <|r_p - r_q|^2>.
 Not built (and not pinned): the Drachmanized delta functions, the
-mass-velocity, Darwin and orbit-orbit terms and particle densities on
-grids.
+mass-velocity, Darwin and orbit-orbit terms.
 """
 from __future__ import annotations
 
@@ -117,3 +122,61 @@ def test_correlation_functions_moments(name, K):
     mom, err = quad_vec(integrand, 0.0, 1.0 - 1e-12, epsabs=1e-13, epsrel=1e-11, limit=2000)
     np.testing.assert_allclose(mom[:m], S, rtol=2e-9)
     np.testing.assert_allclose(mom[m:], r2, rtol=2e-9)
+
+
+def _bvc(masses):
+    """Glob_bvc (src/workproc.f90:612-621), columns = particles."""
+    ms = np.asarray(masses, dtype=float)
+    n = ms.size - 1
+    b = np.tile(-ms[1:, None] / ms.sum(), (1, n + 1))
+    for i in range(1, n + 1):
+        b[i - 1, i] += 1.0
+    return b
+
+
+def test_particle_densities_n1_scaled_pair_function():
+    sy = make_system([2.0, 1.0], [1.0, -1.0], "(P11)")   # b = (-1/3, 2/3): both densities resolved
+    K = 10
+    p = seeded_basis(sy.n, K, SEED_BASE + 73)
+    rng = np.random.Generator(np.random.PCG64(74))
+    c = rng.normal(size=K) + 1j * rng.normal(size=K)
+    _, _, _, dS = oracle.assemble(sy, p, "G")
+    grid = np.linspace(0.0, 2.5, 11)
+    dens = oracle.particle_densities(sy, p, c, dS, sy.masses, grid)
+    b = _bvc(sy.masses)[0]
+    for i in range(2):
+        cf = oracle.correlation_functions(sy, p, c, dS, grid / abs(b[i]))[:, 0] / abs(b[i]) ** 3
+        np.testing.assert_allclose(dens[:, i], cf, rtol=1e-12, atol=1e-14 * np.max(np.abs(cf)))
+
+
+@pytest.mark.parametrize("name,K", [("hminus", 10), ("h2", 8)])
+def test_particle_densities_moments(name, K):
+    from scipy.integrate import quad_vec
+    sy = named_system(name)
+    n = sy.n
+    p = seeded_basis(n, K, SEED_BASE + 75)
+    rng = np.random.Generator(np.random.PCG64(76))
+    c = rng.normal(size=K) + 1j * rng.normal(size=K)
+    _, _, _, dS = oracle.assemble(sy, p, "G")
+    vals, _ = oracle.expectation_values(sy, p, c, dS)
+    m = n * (n + 1) // 2
+    r2v, S = vals[3 * m:4 * m], vals[5 * m + 1]
+    pairs = [(a, b) for a in range(n) for b in range(a, n)]
+    R = np.zeros((n, n))                        # <r_p . r_q>
+    for q, (a, b) in enumerate(pairs):
+        if a == b:
+            R[a, a] = r2v[q]
+    for q, (a, b) in enumerate(pairs):
+        if a != b:
+            R[a, b] = R[b, a] = 0.5 * (R[a, a] + R[b, b] - r2v[q])
+    B = _bvc(sy.masses)
+    want2 = np.array([B[:, i] @ R @ B[:, i] for i in range(n + 1)])
+
+    def integrand(x):
+        r = x / (1.0 - x)
+        jac = 1.0 / (1.0 - x) ** 2
+        d = oracle.particle_densities(sy, p, c, dS, sy.masses, [r])[0]
+        return np.concatenate([4 * np.pi * r ** 2 * d, 4 * np.pi * r ** 4 * d]) * jac
+    mom, _ = quad_vec(integrand, 0.0, 1.0 - 1e-12, epsabs=1e-13, epsrel=1e-11, limit=2000)
+    np.testing.assert_allclose(mom[:n + 1], S, rtol=2e-9)
+    np.testing.assert_allclose(mom[n + 1:], want2, rtol=2e-9)

@@ -122,3 +122,26 @@ def test_correlation_functions_hdplus_last_eigenvector(golden_dir):
     ref = oracle.correlation_functions(f.system, f.params, c, dS, grid)
     scale = np.max(np.abs(ref), axis=0)
     assert np.all(np.abs(got - ref) <= 1e-11 * scale)
+
+
+@pytest.mark.parametrize("name,K,ngrid", [("n1", 24, 5), ("hminus", 40, 6), ("h2", 33, 4), ("li", 30, 3)])
+def test_particle_densities_vs_oracle(name, K, ngrid):
+    """ecg_particle_densities against the oracle (pinned on the CPU by the
+    n = 1 scaling identity and the radial moments)."""
+    say("ecg_particle_densities", name, K, ngrid)
+    sy = make_system([2.0, 1.0], [1.0, -1.0], "(P11)") if name == "n1" else named_system(name)
+    p = seeded_basis(sy.n, K, SEED_BASE + 64)
+    rng = np.random.Generator(np.random.PCG64(65))
+    c = rng.normal(size=K) + 1j * rng.normal(size=K)
+    grid = np.linspace(0.0, 2.0, ngrid)
+    _, _, _, dS = oracle.assemble(sy, p, "G")
+    ref = oracle.particle_densities(sy, p, c, dS, sy.masses, grid)
+    ctx = EcgContext(sy)
+    ctx.set_params(p)
+    got = ctx.particle_densities(grid, c)
+    with pytest.raises(Exception):
+        ctx.particle_densities(grid, c, masses=np.r_[sy.masses[:-1], -1.0])
+    ctx.close()
+    scale = np.max(np.abs(ref), axis=0)
+    print("max rel dev", np.max(np.abs(got - ref) / scale))
+    assert np.all(np.abs(got - ref) <= 1e-12 * scale + 1e-300)
