@@ -1670,12 +1670,10 @@ int ecg_expectation_values(ecg_ctx* x, const double* c, double* values, double* 
   return finite_all(values, nobs) ? ECG_OK : ECG_ERR_NONFINITE;
 }
 
-// Shared set-up of the radial-grid values: argument checks, the partial-
-// sum buffer, diagS and the vector on the device.
-static int grid_values_setup(ecg_ctx* x, const double* c, int ngrid, const double* grid, size_t nout,
-                             const zd** cv, double** dout) {
-  if (!x || ngrid < 1 || !grid) return ECG_ERR_ARG;
-  if (!finite_all(grid, (size_t)ngrid)) return ECG_ERR_ARG;
+// Shared set-up of the post-processing values (correlation functions,
+// densities, relativistic terms): the partial-sum buffer, diagS and the
+// vector on the device.
+static int post_values_setup(ecg_ctx* x, const double* c, size_t nout, const zd** cv, double** dout) {
   ecg::Ctx& cx = x->c;
   const int K = cx.K;
   if (K < 1) return ECG_ERR_STATE;
@@ -1701,6 +1699,38 @@ static int grid_values_setup(ecg_ctx* x, const double* c, int ngrid, const doubl
   return ECG_OK;
 }
 
+static int grid_values_setup(ecg_ctx* x, const double* c, int ngrid, const double* grid, size_t nout,
+                             const zd** cv, double** dout) {
+  if (!x || ngrid < 1 || !grid) return ECG_ERR_ARG;
+  if (!finite_all(grid, (size_t)ngrid)) return ECG_ERR_ARG;
+  return post_values_setup(x, c, nout, cv, dout);
+}
+
+static bool masses_ok(const ecg_ctx* x, const double* masses) {
+  for (int i = 0; i <= x->c.n; ++i)
+    if (!(masses[i] > 0.0) || !std::isfinite(masses[i])) return false;
+  return true;
+}
+
+int ecg_relativistic_corrections(ecg_ctx* x, const double* c, const double* masses, double* values, double* errors) {
+  (void)hipGetLastError();   // start from a clean runtime error state
+  if (!x || !values || !masses || !masses_ok(x, masses)) return ECG_ERR_ARG;
+  const zd* cv = nullptr;
+  double* dout = nullptr;
+  RCCHK(post_values_setup(x, c, 4, &cv, &dout));
+  RCCHK(ecg::relativistic_values(x->c, cv, masses, x->xpart, dout, x->stream));
+  double h[4];
+  HIPCHK(hipMemcpyAsync(h, dout, sizeof(h), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  values[0] = h[0];
+  values[1] = h[1];
+  if (errors) {
+    errors[0] = h[2] * 2.220446049250313e-16;   // epsilon(1.0)
+    errors[1] = h[3] * 2.220446049250313e-16;
+  }
+  return finite_all(values, 2) ? ECG_OK : ECG_ERR_NONFINITE;
+}
+
 int ecg_correlation_functions(ecg_ctx* x, const double* c, int ngrid, const double* grid, double* values) {
   (void)hipGetLastError();   // start from a clean runtime error state
   if (!x || !values) return ECG_ERR_ARG;
@@ -1717,10 +1747,8 @@ int ecg_correlation_functions(ecg_ctx* x, const double* c, int ngrid, const doub
 int ecg_particle_densities(ecg_ctx* x, const double* c, const double* masses, int ngrid, const double* grid,
                            double* values) {
   (void)hipGetLastError();   // start from a clean runtime error state
-  if (!x || !values || !masses) return ECG_ERR_ARG;
+  if (!x || !values || !masses || !masses_ok(x, masses)) return ECG_ERR_ARG;
   const int npart = x->c.n + 1;
-  for (int i = 0; i < npart; ++i)
-    if (!(masses[i] > 0.0) || !std::isfinite(masses[i])) return ECG_ERR_ARG;
   const size_t nout = (size_t)npart * (ngrid > 0 ? ngrid : 0);
   const zd* cv = nullptr;
   double* dout = nullptr;

@@ -123,6 +123,10 @@ int correlation_functions(const Ctx& c, const zd* cvec, int ngrid, const double*
 // (device, (n + 1) x ngrid, particle index fastest).
 int particle_densities(const Ctx& c, const zd* cvec, const double* masses, int ngrid, const double* grid,
                        double* part, double* out, hipStream_t s);
+// Mass-velocity and Darwin values (out: 2 values, 2 bound sums) from the
+// n + 1 particle masses; part: correlation_partial_doubles.
+int relativistic_values(const Ctx& c, const zd* cvec, const double* masses, double* part, double* out,
+                        hipStream_t s);
 size_t correlation_partial_doubles(int n, int K);
 
 // dense Hermitian linear algebra (ecg_linalg.hip)

@@ -849,6 +849,195 @@ int particle_densities(const Ctx& c, const zd* cvec, const double* masses, int n
   return -1;
 }
 
+// ------------------------------------------ mass-velocity and Darwin
+// The relativistic pair values of MatrixElementsForExpcVals without the
+// Drachman forms (src/matelem.f90:1479-1667, 1866-1884), contracted with
+// the driver's bound as the other expectation values.  For particle 1
+// the direction vector is the column sums, for particle i + 1 column i:
+// u = Ck^* j, v = tCl j, a = j^T Ck^* j, b = j^T tCl j, and with
+// Wc1 = u u^T, Wc2 = v v^T the reference's traces become
+//   tr4 = u^T C^-1 u, tr5 = v^T C^-1 v, cnumb = (u^T C^-1 v)^2,
+//   caux = a tr5 + b tr4,
+//   MV_i = 36 S (tr4 tr5 + 2/3 cnumb - caux + a b) / m_i^3,  MV = -sum MV_i / 8;
+// Darwin = sum_q dq_q delta_q with the host's coefficients dq (charges,
+// masses and -pi/2).
+struct RelConst {
+  double inv_m3[6];   // 1 / m^3 of particles 1..n+1
+  double dq[15];      // Darwin coefficient of each pair's delta
+};
+
+template <int N>
+__global__ void __launch_bounds__(TILE* TILE) k_relat(const SysConst sc, int K, const double* __restrict__ bra,
+                                                    const double* __restrict__ ket,
+                                                    const double* __restrict__ diagS, const zd* __restrict__ cvec,
+                                                    const RelConst rc, double* __restrict__ part) {
+  constexpr int NO = 2;
+  extern __shared__ double lds[];
+  int tr, tc;
+  tri_tile(blockIdx.x, 0, tr, tc);
+  const int T = sc.T;
+  const int kb = tr * TILE, lb = tc * TILE;
+  double* sbra = lds;
+  double* sket = lds + TILE * BraRec<N>::stride;
+  const int tid = threadIdx.y * TILE + threadIdx.x;
+  for (int i = tid; i < TILE * BraRec<N>::stride; i += TILE * TILE) {
+    const int f = kb + i / BraRec<N>::stride;
+    sbra[i] = (f < K) ? bra[(size_t)kb * BraRec<N>::stride + i] : 0.0;
+  }
+  const int kstride = T * KetRec<N>::stride;
+  for (int i = tid; i < TILE * kstride; i += TILE * TILE) {
+    const int f = lb + i / kstride;
+    sket[i] = (f < K) ? ket[(size_t)lb * kstride + i] : 0.0;
+  }
+  __syncthreads();
+  double sum[NO], err[NO];
+#pragma unroll
+  for (int a = 0; a < NO; ++a) sum[a] = err[a] = 0.0;
+  const int k = kb + threadIdx.x, l = lb + threadIdx.y;
+  if (k < K && l <= k) {
+    const double* br = sbra + threadIdx.x * BraRec<N>::stride;
+    const double* kr0 = sket + threadIdx.y * kstride;
+    const zd* ck = reinterpret_cast<const zd*>(br + BraRec<N>::ck);
+    const double adet = br[BraRec<N>::adet] * bra[(size_t)l * BraRec<N>::stride + BraRec<N>::adet];
+    const double f = (k == l) ? 1.0 / diagS[k] : 2.0 / sqrt(diagS[k] * diagS[l]);
+    const zd ci = cvec[k], cj = cvec[l];
+    const double wr = ci.x * cj.x + ci.y * cj.y, wi = ci.x * cj.y - ci.y * cj.x;   // conj(c_k) c_l
+    constexpr double inv_pi32 = 0.179587122125166564899490489694101733;    // pi^(-3/2)
+    for (int t = 0; t < T; ++t) {
+      const double* kr = kr0 + (size_t)t * KetRec<N>::stride;
+      const zd* ct = reinterpret_cast<const zd*>(kr + KetRec<N>::ct);
+      PairCore<N> pc;
+      pair_energy<N>(sc, ck, ct, reinterpret_cast<const zd*>(kr + KetRec<N>::nl), adet, pc);
+      zd mv = mkz(0.0, 0.0);
+#pragma unroll
+      for (int part_i = 0; part_i <= N; ++part_i) {
+        zd u[N], v[N], a, b;
+        if (part_i == 0) {
+#pragma unroll
+          for (int p = 0; p < N; ++p) {
+            u[p] = v[p] = mkz(0.0, 0.0);
+#pragma unroll
+            for (int q = 0; q < N; ++q) {
+              u[p] = u[p] + ck[p >= q ? sidx(p, q) : sidx(q, p)];
+              v[p] = v[p] + ct[p >= q ? sidx(p, q) : sidx(q, p)];
+            }
+          }
+          a = b = mkz(0.0, 0.0);
+#pragma unroll
+          for (int p = 0; p < N; ++p) {
+            a = a + u[p];
+            b = b + v[p];
+          }
+        } else {
+          const int c0 = part_i - 1;
+#pragma unroll
+          for (int p = 0; p < N; ++p) {
+            u[p] = ck[p >= c0 ? sidx(p, c0) : sidx(c0, p)];
+            v[p] = ct[p >= c0 ? sidx(p, c0) : sidx(c0, p)];
+          }
+          a = u[c0];
+          b = v[c0];
+        }
+        // C^-1 u and C^-1 v, then the three quadratic forms
+        zd tr4 = mkz(0.0, 0.0), tr5 = mkz(0.0, 0.0), x = mkz(0.0, 0.0);
+#pragma unroll
+        for (int p = 0; p < N; ++p) {
+          zd cu = mkz(0.0, 0.0), cv = mkz(0.0, 0.0);
+#pragma unroll
+          for (int q = 0; q < N; ++q) {
+            const zd cpq = pc.Ci[p >= q ? sidx(p, q) : sidx(q, p)];
+            cu = zfma(cpq, u[q], cu);
+            cv = zfma(cpq, v[q], cv);
+          }
+          tr4 = zfma(u[p], cu, tr4);
+          tr5 = zfma(v[p], cv, tr5);
+          x = zfma(u[p], cv, x);
+        }
+        const zd br_ = tr4 * tr5 + (2.0 / 3.0) * (x * x) - (a * tr5 + b * tr4) + a * b;
+        mv = zfma(mkz(36.0 * rc.inv_m3[part_i], 0.0), pc.S * br_, mv);
+      }
+      zd O[NO];
+      O[0] = -0.125 * mv;
+      zd dw = mkz(0.0, 0.0);
+      int q = 0;
+#pragma unroll
+      for (int a = 0; a < N; ++a)
+#pragma unroll
+        for (int b = a; b < N; ++b, ++q) {
+          const zd rs = pc.rs[(a == b) ? sidx(a, a) : sidx(b, a)];
+          dw = zfma(mkz(rc.dq[q], 0.0), inv_pi32 * ((pc.S * rs) * (rs * rs)), dw);
+        }
+      O[1] = dw;
+      const double fc = f * sc.coef[t];
+#pragma unroll
+      for (int a = 0; a < NO; ++a) {
+        const double ro = O[a].x, io = O[a].y;
+        const double beta = wr * ro - wi * io;
+        const double mu = fabs(ci.x * (cj.x * ro - cj.y * io)) + fabs(cj.x * (ci.x * ro + ci.y * io)) +
+                          fabs(ci.y * (cj.y * ro + cj.x * io)) + fabs(cj.y * (ci.y * ro - ci.x * io));
+        sum[a] += fc * beta;
+        err[a] += fabs(fc * (mu + 2.0 * fabs(beta)));
+      }
+    }
+  }
+  double* red = lds;
+  __syncthreads();
+  const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+  for (int a = 0; a < NO; ++a) {
+    double v = sum[a], e = err[a];
+    for (int o = 32; o > 0; o >>= 1) {
+      v += __shfl_xor(v, o);
+      e += __shfl_xor(e, o);
+    }
+    if (lane == 0) {
+      red[wave * 2 * NO + a] = v;
+      red[wave * 2 * NO + NO + a] = e;
+    }
+  }
+  __syncthreads();
+  for (int a = tid; a < 2 * NO; a += TILE * TILE)
+    part[(size_t)blockIdx.x * 2 * NO + a] = ((red[a] + red[2 * NO + a]) + red[4 * NO + a]) + red[6 * NO + a];
+}
+
+template <int N>
+static int launch_relat(const Ctx& c, const zd* cvec, const RelConst& rc, double* part, double* out, hipStream_t s) {
+  const int K = c.K;
+  const int nt = (K + TILE - 1) / TILE;
+  const long long ntiles = (long long)nt * (nt + 1) / 2;
+  size_t lds = sizeof(double) * TILE * (BraRec<N>::stride + c.T * KetRec<N>::stride);
+  lds = std::max(lds, sizeof(double) * 16);
+  hipLaunchKernelGGL(k_relat<N>, dim3((unsigned)ntiles), dim3(TILE, TILE), lds, s, c.sc, K, c.bra, c.ket, c.diagS,
+                     cvec, rc, part);
+  hipLaunchKernelGGL(k_expect_finish, dim3(1), dim3(256), 0, s, 4, ntiles, part, out);
+  return (int)hipGetLastError();
+}
+
+int relativistic_values(const Ctx& c, const zd* cvec, const double* masses, double* part, double* out,
+                        hipStream_t s) {
+  // Darwin coefficients (src/matelem.f90:1866-1884): -pi/2 (1/m_0^2 + 1/m_a^2)
+  // q0 q_a for the pair (a, a), -pi/2 (1/m_a^2 + 1/m_b^2) q_a q_b for (a, b)
+  const int n = c.n;
+  RelConst rc{};
+  for (int i = 0; i <= n; ++i) rc.inv_m3[i] = 1.0 / (masses[i] * masses[i] * masses[i]);
+  const double hp = -1.57079632679489661923132169163975144;
+  int q = 0;
+  for (int a = 0; a < n; ++a)
+    for (int b = a; b < n; ++b, ++q) {
+      const double ia = 1.0 / (masses[a + 1] * masses[a + 1]);
+      rc.dq[q] = (a == b) ? hp * (1.0 / (masses[0] * masses[0]) + ia) * c.sc.q0 * c.sc.q[a]
+                          : hp * (ia + 1.0 / (masses[b + 1] * masses[b + 1])) * c.sc.q[a] * c.sc.q[b];
+    }
+  switch (n) {
+    case 1: return launch_relat<1>(c, cvec, rc, part, out, s);
+    case 2: return launch_relat<2>(c, cvec, rc, part, out, s);
+    case 3: return launch_relat<3>(c, cvec, rc, part, out, s);
+    case 4: return launch_relat<4>(c, cvec, rc, part, out, s);
+    case 5: return launch_relat<5>(c, cvec, rc, part, out, s);
+  }
+  return -1;
+}
+
 size_t correlation_partial_doubles(int n, int K) {
   const int nt = (K + TILE - 1) / TILE;
   const int m = std::max(n * (n + 1) / 2, n + 1);   // pair functions or particle densities

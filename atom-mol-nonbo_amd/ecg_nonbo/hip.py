@@ -45,7 +45,7 @@ EXPORTS = [
     "ecg_energy_grad_stored", "ecg_batch_create", "ecg_batch_destroy", "ecg_batch_set_config",
     "ecg_batch_energy_grad", "ecg_trace", "ecg_trace_read", "ecg_get_params", "ecg_screen_prepare", "ecg_screen", "ecg_screen_loop", "ecg_last_timings",
     "ecg_set_overlap_penalty", "ecg_overlap_penalty", "ecg_set_params_device", "ecg_expectation_values",
-    "ecg_correlation_functions", "ecg_particle_densities",
+    "ecg_correlation_functions", "ecg_particle_densities", "ecg_relativistic_corrections",
 ]
 
 
@@ -99,6 +99,7 @@ def load(path: str = LIB_PATH):
     L.ecg_expectation_values.argtypes = [vp, _d, _d, _d]
     L.ecg_correlation_functions.argtypes = [vp, _d, c_int, _d, _d]
     L.ecg_particle_densities.argtypes = [vp, _d, _d, c_int, _d, _d]
+    L.ecg_relativistic_corrections.argtypes = [vp, _d, _d, _d, _d]
     for name in EXPORTS:
         getattr(L, name).restype = c_int
     _lib = L
@@ -320,6 +321,20 @@ class EcgContext:
         _check("ecg_particle_densities", load().ecg_particle_densities(
             self._h, cc.ctypes.data_as(_d) if cc is not None else None, _p(ms), g.size, _p(g), _p(out)))
         return out
+
+    def relativistic_corrections(self, c=None, masses=None):
+        """ecg_relativistic_corrections: (values, errors) = ([MV, Darwin],
+        their rounding estimates); masses default to the system's."""
+        npart = self.system.n + 1
+        ms = np.ascontiguousarray(self.system.masses if masses is None else masses, dtype=np.float64).ravel()
+        if ms.size != npart:
+            raise ValueError("need %d masses" % npart)
+        vals = np.zeros(2)
+        errs = np.zeros(2)
+        cc = None if c is None else np.ascontiguousarray(c, dtype=np.complex128)
+        _check("ecg_relativistic_corrections", load().ecg_relativistic_corrections(
+            self._h, cc.ctypes.data_as(_d) if cc is not None else None, _p(ms), _p(vals), _p(errs)))
+        return vals, errs
 
     def trace(self, enable=True):
         _check("ecg_trace", load().ecg_trace(self._h, int(bool(enable))))

@@ -20,6 +20,7 @@
 !                  EcgExpectationValues               src/workproc.f90:8158-8300 (ExpectationValuesG's loop)
 !                  EcgCorrelationFunctions            src/workproc.f90:8234-8243 (its correlation functions)
 !                  EcgParticleDensities               src/workproc.f90:8245-8254 (its particle densities)
+!                  EcgRelativisticCorrections         src/matelem.f90:1479-1667, 1866-1884 (MV, Darwin)
 !                plus EcgHostInit / EcgHostFinish (context set-up, no
 !                counterpart in the reference: ProgramDataInit ends by
 !                calling EcgHostInit).
@@ -231,6 +232,15 @@ module ecg_capi
       real(c_double), intent(in) :: grid(*)
       real(c_double), intent(out) :: values(*)
     end function
+    integer(c_int) function ecg_relativistic_corrections(ctx, c, masses, values, errors) &
+        bind(C, name="ecg_relativistic_corrections")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: c              ! K complex or c_null_ptr (last eigenvector)
+      real(c_double), intent(in) :: masses(*)
+      real(c_double), intent(out) :: values(*)
+      real(c_double), intent(out) :: errors(*)
+    end function
     integer(c_int) function ecg_particle_densities(ctx, c, masses, ngrid, grid, values) &
         bind(C, name="ecg_particle_densities")
       import :: c_int, c_double, c_ptr
@@ -284,7 +294,8 @@ module ecg_hostpath
   private
   public :: EcgHostInit, EcgHostFinish, MatrixElements, ComputeMatElem, ComputeMatElemAndDeriv, GHEPIIS, &
             EnergyGA, EnergyIA, EnergyGB, EnergyIB, ComputeOverlapPenalty, ComputeOverlapPenaltyAndAddGradient, &
-            EcgExpectationValues, EcgCorrelationFunctions, EcgParticleDensities
+            EcgExpectationValues, EcgCorrelationFunctions, EcgParticleDensities, &
+            EcgRelativisticCorrections
   type(c_ptr), save :: ctx = c_null_ptr
 
 contains
@@ -455,6 +466,15 @@ contains
   ! src/workproc.f90:8245-8254 with src/matelem.f90:1948-1963 — the particle
   ! densities Dens(i, a) at the radii DensGrid(a), i = 1..n+1, from the
   ! particle masses Mass(1:n+1) (Glob_Mass), in the last eigenvector
+  ! src/workproc.f90:8158-8300 with src/matelem.f90:1479-1667, 1866-1884 —
+  ! MV and Darwin (Values(1:2)) with their rounding estimates, from the
+  ! particle masses Mass(1:n+1), in the last eigenvector
+  subroutine EcgRelativisticCorrections(Mass, Values, Errors, ErrorCode)
+    real(dprec) :: Mass(:), Values(:), Errors(:)
+    integer :: ErrorCode
+    ErrorCode = ecg_relativistic_corrections(ctx, c_null_ptr, Mass, Values, Errors)
+  end subroutine
+
   subroutine EcgParticleDensities(Mass, NumDensGridPoints, DensGrid, Dens, ErrorCode)
     integer :: NumDensGridPoints, ErrorCode
     real(dprec) :: Mass(:), DensGrid(:), Dens(:, :)

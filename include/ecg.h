@@ -285,6 +285,16 @@ int ecg_correlation_functions(ecg_ctx* ctx, const double* c, int ngrid, const do
 int ecg_particle_densities(ecg_ctx* ctx, const double* c, const double* masses, int ngrid, const double* grid,
                            double* values);
 
+/* Mass-velocity and Darwin corrections (without the alpha^2 factor and
+ * without the Drachman forms): the MVkl and Darwinkl pair values of
+ * MatrixElementsForExpcVals (src/matelem.f90:1479-1667, 1866-1884)
+ * contracted with the vector (c == NULL: the last energy call's) as the
+ * driver does, with its rounding estimates (src/workproc.f90:8158-8300).
+ * masses: the n + 1 particle masses (Glob_Mass).  values, errors: 2 each
+ * (MV, Darwin); errors nullable. */
+int ecg_relativistic_corrections(ecg_ctx* ctx, const double* c, const double* masses, double* values,
+                                 double* errors);
+
 /* Device time (ms, HIP events on the context's stream) of the phases of
  * the last ecg_energy / ecg_energy_grad call: [0] assembly, [1] eigen step
  * (all factorisations and iterations), [2] adjoint pass, [3] whole call,

@@ -670,4 +670,153 @@ int oracle_particle_densities(int n, int T, const int* P, const int* coef, const
   return 0;
 }
 
+// Mass-velocity and Darwin pair values (src/matelem.f90:1479-1667 and
+// 1866-1884), restated with the reference's loops: for particle 1 the
+// column sums of tCk (and tCl) stand for J = 1 1^T, for particle i + 1 the
+// column i (J = e_i e_i^T);
+//   Wc1 = tCk J tCk, tr1 = tr(J tCk), Wc2 = tCl J tCl, tr2 = tr(J tCl),
+//   MV_i = 36 S (tr(C^-1 Wc1) tr(C^-1 Wc2) + 2/3 tr(C^-1 Wc1 C^-1 Wc2)
+//                - tr(C^-1 (tr1 Wc2 + tr2 Wc1)) + tr1 tr2) / m_i^3,
+//   MV = -sum_i MV_i / 8;
+//   Darwin = -pi/2 [sum_i (1/m_0^2 + 1/m_i^2) q0 q_i delta(i, i)
+//                   + sum_{i != j} q_i q_j delta(i, j) / m_i^2].
+static void relat_pair(const System<double>& sy, const double* pk, const double* pl, const int* Pm,
+                       const double* masses, cd* out) {
+  cd H, S;
+  PairParts<double> pp;
+  matrix_elements<double>(sy, pk, pl, Pm, 0, H, S, nullptr, nullptr, &pp);
+  const int n = sy.n;
+  const double sqrtpi = 1.77245385090551602729816748334115, pi = 3.14159265358979323846264338327950;
+  cd MV = {0.0, 0.0};
+  for (int part = 0; part <= n; ++part) {
+    cd Wc1[NN][NN], Wc2[NN][NN], tr1, tr2;
+    if (part == 0) {
+      cd uk[NN], ul[NN];
+      for (int p = 0; p < n; ++p) {
+        uk[p] = {0.0, 0.0};
+        ul[p] = {0.0, 0.0};
+        for (int q = 0; q < n; ++q) {
+          uk[p] = uk[p] + pp.Ck[q][p];
+          ul[p] = ul[p] + pp.tCl[q][p];
+        }
+      }
+      tr1 = {0.0, 0.0};
+      tr2 = {0.0, 0.0};
+      for (int p = 0; p < n; ++p) {
+        tr1 = tr1 + uk[p];
+        tr2 = tr2 + ul[p];
+      }
+      for (int p = 0; p < n; ++p)
+        for (int q = 0; q < n; ++q) {
+          cd a = {0.0, 0.0}, b = {0.0, 0.0};
+          for (int i = 0; i < n; ++i) {
+            a = a + pp.Ck[p][i] * uk[q];
+            b = b + pp.tCl[p][i] * ul[q];
+          }
+          Wc1[p][q] = a;
+          Wc2[p][q] = b;
+        }
+    } else {
+      const int i = part - 1;
+      for (int p = 0; p < n; ++p)
+        for (int q = 0; q < n; ++q) {
+          Wc1[q][p] = pp.Ck[q][i] * pp.Ck[i][p];
+          Wc2[q][p] = pp.tCl[q][i] * pp.tCl[i][p];
+        }
+      tr1 = pp.Ck[i][i];
+      tr2 = pp.tCl[i][i];
+    }
+    cd Wc4[NN][NN], Wc5[NN][NN];
+    for (int q = 0; q < n; ++q)
+      for (int p = 0; p < n; ++p) {
+        cd a = {0.0, 0.0}, b = {0.0, 0.0};
+        for (int j = 0; j < n; ++j) {
+          a = a + pp.Ci[q][j] * Wc1[j][p];
+          b = b + pp.Ci[q][j] * Wc2[j][p];
+        }
+        Wc4[q][p] = a;
+        Wc5[q][p] = b;
+      }
+    cd tr4 = {0.0, 0.0}, tr5 = {0.0, 0.0}, cnumb = {0.0, 0.0}, caux = {0.0, 0.0};
+    for (int p = 0; p < n; ++p) {
+      tr4 = tr4 + Wc4[p][p];
+      tr5 = tr5 + Wc5[p][p];
+      for (int q = 0; q < n; ++q) {
+        cnumb = cnumb + Wc4[p][q] * Wc5[q][p];
+        caux = caux + pp.Ci[p][q] * (tr1 * Wc2[q][p] + tr2 * Wc1[q][p]);
+      }
+    }
+    const double m3 = masses[part] * masses[part] * masses[part];
+    MV = MV + (36.0 * S * (tr4 * tr5 + (2.0 / 3.0) * cnumb - caux + tr1 * tr2)) / m3;
+  }
+  out[0] = -0.125 * MV;
+  // Darwin from the pair delta functions
+  cd darwin = {0.0, 0.0};
+  auto delta = [&](int a, int b) {
+    const cd t = (a == b) ? pp.Ci[a][a] : pp.Ci[a][a] + pp.Ci[b][b] - pp.Ci[b][a] - pp.Ci[b][a];
+    return (S / (pi * sqrtpi)) / (csqrt(t) * t);
+  };
+  for (int i = 0; i < n; ++i)
+    darwin = darwin + ((1.0 / (masses[0] * masses[0]) + 1.0 / (masses[i + 1] * masses[i + 1])) * sy.q0 * sy.q[i]) *
+                          delta(i, i);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      if (j != i) darwin = darwin + (sy.q[i] * sy.q[j] / (masses[i + 1] * masses[i + 1])) * delta(i, j);
+  out[1] = (-pi / 2.0) * darwin;
+}
+
+void oracle_relat_pair(int n, int T, const int* P, const int* coef, const double* M, double q0, const double* q,
+                       const double* pk, const double* pl, int term, const double* masses, double* out) {
+  System<double> sy = make_sys<double>(n, T, P, coef, M, q0, q);
+  cd o[2];
+  relat_pair(sy, pk, pl, P + (size_t)term * n * n, masses, o);
+  st(out, 0, o[0]);
+  st(out, 1, o[1]);
+}
+
+// MV and Darwin in the vector c with the driver's contraction and rounding
+// estimate (as oracle_expectation_values): out[2], err[2].
+int oracle_relativistic(int n, int T, const int* P, const int* coef, const double* M, double q0, const double* q,
+                        int K, const double* params, const double* c, const double* diagS, const double* masses,
+                        double* out, double* err, int nthreads) {
+  System<double> sy = make_sys<double>(n, T, P, coef, M, q0, q);
+  const int npt = n * (n + 1);
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  const int nth = omp_get_max_threads();
+  std::vector<double> acc((size_t)nth * 4, 0.0);
+#pragma omp parallel
+  {
+    const int tid = omp_get_thread_num();
+    double* s_ = acc.data() + (size_t)tid * 4;
+#pragma omp for schedule(dynamic, 4)
+    for (int i = 0; i < K; ++i)
+      for (int j = 0; j <= i; ++j) {
+        const double rci = c[2 * i], ici = c[2 * i + 1], rcj = c[2 * j], icj = c[2 * j + 1];
+        const double factor = (i == j) ? 1.0 / std::sqrt(diagS[i] * diagS[i]) : 2.0 / std::sqrt(diagS[i] * diagS[j]);
+        for (int k = 0; k < T; ++k) {
+          cd o[2];
+          relat_pair(sy, params + (size_t)npt * i, params + (size_t)npt * j, P + (size_t)k * n * n, masses, o);
+          for (int a = 0; a < 2; ++a) {
+            const double ro = o[a].re, io = o[a].im;
+            const double beta = (rci * rcj + ici * icj) * ro + (ici * rcj - rci * icj) * io;
+            const double mu = std::fabs(rci * (rcj * ro - icj * io)) + std::fabs(rcj * (rci * ro + ici * io)) +
+                              std::fabs(ici * (icj * ro + rcj * io)) + std::fabs(icj * (ici * ro - rci * io));
+            s_[a] += factor * coef[k] * beta;
+            s_[2 + a] += std::fabs(factor * coef[k] * (mu + 2.0 * std::fabs(beta)));
+          }
+        }
+      }
+  }
+  for (int a = 0; a < 2; ++a) {
+    double sa = 0.0, ea = 0.0;
+    for (int t = 0; t < nth; ++t) {
+      sa += acc[(size_t)t * 4 + a];
+      ea += acc[(size_t)t * 4 + 2 + a];
+    }
+    out[a] = sa;
+    if (err) err[a] = ea * 2.220446049250313e-16;
+  }
+  return 0;
+}
+
 }  // extern "C"

@@ -82,6 +82,7 @@ template <class R> struct System {
 template <class R> struct PairParts {
   cx<R> Ci[NN][NN];   // C^{-1}
   cx<R> T, V;
+  cx<R> Ck[NN][NN], tCl[NN][NN];   // the bra's Ck^* and the permuted ket's tCl
 };
 
 // Restatement of MatrixElements (src/matelem.f90:12-750) for one symmetry
@@ -226,7 +227,11 @@ void matrix_elements(const System<R>& sy, const R* pk, const R* pl, const int* P
   Hkl = Tkl + Vkl;
   if (parts) {
     for (int i = 0; i < n; ++i)
-      for (int j = 0; j < n; ++j) parts->Ci[i][j] = Ci[i][j];
+      for (int j = 0; j < n; ++j) {
+        parts->Ci[i][j] = Ci[i][j];
+        parts->Ck[i][j] = Ck[i][j];
+        parts->tCl[i][j] = tCl[i][j];
+      }
     parts->T = Tkl;
     parts->V = Vkl;
   }

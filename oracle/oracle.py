@@ -75,6 +75,12 @@ def lib():
         L.oracle_particle_densities.restype = ctypes.c_int
         L.oracle_particle_densities.argtypes = [ctypes.c_int, ctypes.c_int, _i, _i, _d, ctypes.c_double, _d,
                                                 ctypes.c_int, _d, _d, _d, _d, ctypes.c_int, _d, _d, ctypes.c_int]
+        L.oracle_relat_pair.restype = None
+        L.oracle_relat_pair.argtypes = [ctypes.c_int, ctypes.c_int, _i, _i, _d, ctypes.c_double, _d, _d, _d,
+                                        ctypes.c_int, _d, _d]
+        L.oracle_relativistic.restype = ctypes.c_int
+        L.oracle_relativistic.argtypes = [ctypes.c_int, ctypes.c_int, _i, _i, _d, ctypes.c_double, _d,
+                                          ctypes.c_int, _d, _d, _d, _d, _d, _d, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -297,6 +303,30 @@ def particle_densities(system, params, c, diagS, masses, grid, nthreads=0):
                                     _pd(np.ascontiguousarray(diagS, dtype=np.float64)), _pd(ms), g.size, _pd(g),
                                     _pd(out), nthreads)
     return out
+
+
+def relat_pair(system, pk, pl, term, masses):
+    """(MV_kl, Darwin_kl) of one pair and symmetry term (complex)."""
+    s = _Sys(system)
+    out = np.zeros(4)
+    lib().oracle_relat_pair(*s.args(), _pd(np.ascontiguousarray(pk, dtype=np.float64)),
+                            _pd(np.ascontiguousarray(pl, dtype=np.float64)), term,
+                            _pd(np.ascontiguousarray(masses, dtype=np.float64)), _pd(out))
+    return complex(out[0], out[1]), complex(out[2], out[3])
+
+
+def relativistic(system, params, c, diagS, masses, nthreads=0):
+    """The driver's mass-velocity and Darwin values in the vector c and
+    their rounding estimates: ([MV, Darwin], [MV_e, Darwin_e])."""
+    s = _Sys(system)
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    out = np.zeros(2)
+    err = np.zeros(2)
+    lib().oracle_relativistic(*s.args(), params.shape[0], _pd(params),
+                              np.ascontiguousarray(c, dtype=np.complex128).ctypes.data_as(_d),
+                              _pd(np.ascontiguousarray(diagS, dtype=np.float64)),
+                              _pd(np.ascontiguousarray(masses, dtype=np.float64)), _pd(out), _pd(err), nthreads)
+    return out, err
 
 
 def ghepiis(k, M, invD, B, apprlambda, v, tol, maxiter=20, specif_norm=0):

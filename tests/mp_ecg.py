@@ -177,3 +177,40 @@ def n1_moments(pk, pl, dps=30):
         vals = [radial(lambda r, p=p: 4 * mp.pi * r ** (2 + p) * mp.exp(-c * r ** 2)) for p in (-2, -1, 1, 2)]
         vals.append(mp.mpc(1))   # conj(phi_k(0)) phi_l(0)
         return [complex(v / norm) for v in vals]
+
+
+def mv_ratio_by_derivatives(system, pk, pl, P, masses, dps=30):
+    """MV_kl / S_kl of one pair and symmetry term from Gaussian-moment
+    derivatives: for particle i with direction matrix J_i (1 1^T for
+    particle 1, e_i e_i^T for particle i + 1), Laplacians of the two
+    Gaussians give <D phi_k | D phi_l> / <phi_k | phi_l> =
+    16 f_st / f + 24 b f_s / f + 24 a f_t / f + 36 a b with
+    f(s, t) = det(C + s P + t Q)^(-3/2), P = Ck J Ck, Q = tCl J tCl,
+    a = tr(J Ck), b = tr(J tCl); then MV = -(1/8) sum_i (.) / m_i^3."""
+    with mp.workdps(dps):
+        n = system.n
+        Lk, Bk = _unpack(pk, n)
+        Ll, Bl = _unpack(pl, n)
+        Pm = mp.matrix([[int(P[i][j]) for j in range(n)] for i in range(n)])
+        Ck = Lk * Lk.T - mp.mpc(0, 1) * Bk
+        tCl = Pm.T * (Ll * Ll.T + mp.mpc(0, 1) * Bl) * Pm
+        C = Ck + tCl
+        total = mp.mpc(0)
+        for part in range(n + 1):
+            if part == 0:
+                J = mp.matrix([[1] * n for _ in range(n)])
+            else:
+                J = mp.zeros(n, n)
+                J[part - 1, part - 1] = 1
+            Pq = Ck * J * Ck
+            Qq = tCl * J * tCl
+            a = sum(J[i, j] * Ck[j, i] for i in range(n) for j in range(n))
+            b = sum(J[i, j] * tCl[j, i] for i in range(n) for j in range(n))
+            f = lambda s, t: mp.det(C + s * Pq + t * Qq) ** mp.mpf(-1.5)
+            f0 = f(0, 0)
+            fs = mp.diff(lambda s: f(s, 0), 0)
+            ft = mp.diff(lambda t: f(0, t), 0)
+            fst = mp.diff(f, (0, 0), (1, 1))
+            ratio = 16 * fst / f0 + 24 * b * fs / f0 + 24 * a * ft / f0 + 36 * a * b
+            total += ratio / mp.mpf(masses[part]) ** 3
+        return complex(-total / 8)

@@ -23,8 +23,14 @@ with the eigenvector.  Pinned here without retyping those forms:
     moments 4 pi int r^2 Dens_i dr = <S> and 4 pi int r^4 Dens_i dr =
     <|b_i . r|^2>, the latter built from the pinned <r_p^2> and
     <|r_p - r_q|^2>.
+  * the mass-velocity pair value (src/matelem.f90:1479-1667) against the
+    Laplacians of the two Gaussians: at n = 1 the closed form
+    <D phi_k|D phi_l> / <phi_k|phi_l> = 60 a^2 b^2 / (a + b)^2, at n = 2, 3
+    Gaussian-moment derivatives of det(C + s P + t Q)^(-3/2) at 30 digits
+    (mpmath) for every particle; the Darwin pair value against its
+    definition through the pinned delta functions.
 Not built (and not pinned): the Drachmanized delta functions, the
-mass-velocity, Darwin and orbit-orbit terms.
+Drachmanized mass-velocity and Darwin terms and the orbit-orbit term.
 """
 from __future__ import annotations
 
@@ -180,3 +186,39 @@ def test_particle_densities_moments(name, K):
     mom, _ = quad_vec(integrand, 0.0, 1.0 - 1e-12, epsabs=1e-13, epsrel=1e-11, limit=2000)
     np.testing.assert_allclose(mom[:n + 1], S, rtol=2e-9)
     np.testing.assert_allclose(mom[n + 1:], want2, rtol=2e-9)
+
+
+def test_mass_velocity_n1_closed_form():
+    sy = make_system([3.0, 1.0], [1.0, -1.0], "(P11)")
+    p = seeded_basis(1, 6, SEED_BASE + 77)
+    for k, l in [(0, 0), (1, 3), (4, 2)]:
+        mv, _ = oracle.relat_pair(sy, p[k], p[l], 0, sy.masses)
+        _, S = oracle.relat_pair(sy, p[k], p[l], 0, sy.masses)[0], oracle.expc_pair(sy, p[k], p[l], 0)[6]
+        a = p[k][0] ** 2 - 1j * p[k][1]
+        b = p[l][0] ** 2 + 1j * p[l][1]
+        want = -(1 / 8) * (1 / 27.0 + 1.0) * 60 * a * a * b * b / (a + b) ** 2
+        assert abs(mv / S - want) <= 1e-12 * abs(want), (k, l, mv / S, want)
+
+
+@pytest.mark.parametrize("name,term", [("hminus", 1), ("h2", 3), ("h2", 0)])
+def test_mass_velocity_against_moment_derivatives(name, term):
+    sy = named_system(name)
+    p = seeded_basis(sy.n, 6, SEED_BASE + 78)
+    m = sy.n * (sy.n + 1) // 2
+    for k, l in [(2, 4), (5, 1)]:
+        mv, dw = oracle.relat_pair(sy, p[k], p[l], term, sy.masses)
+        o = oracle.expc_pair(sy, p[k], p[l], term)
+        S = o[5 * m + 1]
+        want = mp_ecg.mv_ratio_by_derivatives(sy, p[k], p[l], sy.yhy_mats[term], sy.masses)
+        assert abs(mv / S - want) <= 1e-10 * abs(want), (name, term, k, l, mv / S, want)
+        # Darwin from its definition through the pair delta functions
+        delta = o[4 * m:5 * m]
+        ms, q0, q = sy.masses, sy.q0, sy.q
+        pairs = [(a, b) for a in range(sy.n) for b in range(a, sy.n)]
+        d = 0
+        for c, (a, b) in enumerate(pairs):
+            if a == b:
+                d += (1 / ms[0] ** 2 + 1 / ms[a + 1] ** 2) * q0 * q[a] * delta[c]
+            else:
+                d += (1 / ms[a + 1] ** 2 + 1 / ms[b + 1] ** 2) * q[a] * q[b] * delta[c]
+        assert abs(dw - (-np.pi / 2) * d) <= 1e-13 * abs(dw)

@@ -145,3 +145,25 @@ def test_particle_densities_vs_oracle(name, K, ngrid):
     scale = np.max(np.abs(ref), axis=0)
     print("max rel dev", np.max(np.abs(got - ref) / scale))
     assert np.all(np.abs(got - ref) <= 1e-12 * scale + 1e-300)
+
+
+@pytest.mark.parametrize("name,K", [("n1", 24), ("hminus", 40), ("h2", 33), ("li", 30)])
+def test_relativistic_corrections_vs_oracle(name, K):
+    """ecg_relativistic_corrections (MV, Darwin) against the oracle, which
+    restates the reference's loops (pinned on the CPU by the Laplacians of
+    the Gaussians and the delta functions); the device uses the same
+    traces in rank-one form, so values agree to rounding and the bounds to
+    1e-6 relative."""
+    say("ecg_relativistic_corrections", name, K)
+    sy = make_system([2.0, 1.0], [1.0, -1.0], "(P11)") if name == "n1" else named_system(name)
+    p = seeded_basis(sy.n, K, SEED_BASE + 66)
+    rng = np.random.Generator(np.random.PCG64(67))
+    c = rng.normal(size=K) + 1j * rng.normal(size=K)
+    _, _, _, dS = oracle.assemble(sy, p, "G")
+    ov, oe = oracle.relativistic(sy, p, c, dS, sy.masses)
+    ctx = EcgContext(sy)
+    ctx.set_params(p)
+    vals, errs = ctx.relativistic_corrections(c)
+    ctx.close()
+    print("MV", vals[0], ov[0], "Darwin", vals[1], ov[1])
+    _compare(vals, errs, ov, oe)
