@@ -154,3 +154,23 @@ def test_run_bbop_hdplus(golden_dir, tmp_path):
     vals, _ = reps[-1].detail
     m = 3
     assert abs(vals[5 * m] - f.current_energy) <= 1e-10 * abs(Ef)       # <H> = E
+
+
+def test_basis_enl_from_an_empty_basis(golden_dir):
+    """BASIS_ENL from Kstart = 1, as a fresh run begins: the first function
+    from the uniform generator (a one-function context), then growth to
+    six functions with root search (method 'G')."""
+    say("basis_enl", "G", "K 0 -> 6")
+    f = _part(golden_dir, 1)
+    ctx = EcgContext(f.system)
+    order = BasisOrder.identity(np.zeros((0, f.system.npt)))
+    hist = []
+    recs = basis_enl(ctx, order, 6, 1, 40, 8, float("nan"), _rng(9), generator_param=f.generator_param,
+                     history=hist, tol=TOL)
+    E = [r.energy_after for r in recs]
+    print("energies", E)
+    assert len(order.func_num) == 6 and len(hist) == 6
+    assert all(b < a for a, b in zip(E, E[1:]))
+    Ef = _fresh_energy(f.system, order.params, float("nan"), TOL)
+    ctx.close()
+    assert abs(Ef - E[-1]) <= 1e-11 * abs(Ef)
