@@ -357,6 +357,38 @@ def opt_cycle_leg(system, base, K, nfunc=16, max_evals=12):
     }
 
 
+def enlarge_leg(system, base, K, n_trials=64, max_evals=6):
+    """One step of the basis-enlargement loop (BasisEnlI's) for function K:
+    n_trials candidates from the reference's generator around functions
+    1..K-1, scored one energy call each (assembly of row K, tail update,
+    inverse iterations) or all in one batched screening call, the best one
+    kept and optimised with at most max_evals energy evaluations."""
+    from ecg_nonbo.enlarge import basis_enl
+    from ecg_nonbo.optimise import BasisOrder
+    out = {"workload": "H2-like n=3 (T=4): function %d added to %d, %d trials (generator p1=0.7, s1=0.3, "
+                       "s2=3.0), <= %d energy evaluations of optimisation, fixed shift" % (K, K - 1, n_trials, max_evals)}
+    for mode in ("exact", "screen"):
+        ctx = EcgContext(system)
+        try:
+            ctx.set_params(base[:K - 1])
+            _, E0, _, _ = ctx.energy(1, float("nan"), 1e-12, 60)
+            sigma = E0 - 1e-6 * abs(E0)
+            order = BasisOrder.identity(base[:K - 1])
+            rng = np.random.Generator(np.random.PCG64(SEED_BASE + 9))
+            t0 = time.perf_counter()
+            rec = basis_enl(ctx, order, K, 1, n_trials, max_evals, sigma, rng, generator_param=(0.7, 0.3, 3.0),
+                            tol=-1e-9, trials=mode, max_step=0.05)[0]
+            t = time.perf_counter() - t0
+            out[mode] = {"s_per_step": t, "ms_per_trial_incl_optimisation": 1e3 * t / n_trials,
+                         "E_before": rec.energy_before, "E_best_trial": rec.energy_trials,
+                         "E_after": rec.energy_after, "prototype": rec.prototype, "trial_failures": rec.trial_failures}
+        except RuntimeError as e:   # the loop's own stop conditions (as the reference's)
+            out[mode] = {"error": str(e)}
+        finally:
+            ctx.close()
+    return out
+
+
 def expectation_leg(system, base, K, reps=3):
     """Expectation values (<1/r^2>, <1/r>, <r>, <r^2>, <delta(r)> of every
     pseudo-particle pair, <H>, <S>, <T>, <V>) in the lowest eigenvector:
@@ -512,12 +544,13 @@ def main():
     ap.add_argument("--no-opt", action="store_true", help="skip the one-function optimisation leg")
     ap.add_argument("--no-expect", action="store_true", help="skip the expectation-value leg")
     ap.add_argument("--no-cycle", action="store_true", help="skip the optimisation-cycle leg")
+    ap.add_argument("--no-enlarge", action="store_true", help="skip the basis-enlargement leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first issue device copies of a known size (reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the path on a single MI355X "
-              "(see DESIGN.md for how it would shard); --gpus %d is not supported" % args.gpus,
+              "(multi-GPU sharding is a design note only); --gpus %d is not supported" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
@@ -646,6 +679,8 @@ def main():
         out["expectation_values"] = expectation_leg(system, base, K)
     if not args.no_cycle:
         out["opt_cycle"] = opt_cycle_leg(system, base, K)
+    if not args.no_enlarge:
+        out["basis_enlargement"] = enlarge_leg(system, base, K)
     if not args.no_screen:
         out["screening"] = screening_leg(system, base, K)
     if not args.no_modes:
