@@ -16,11 +16,15 @@ guards as the reference:
 * FULL_OPT1 m K InitFunc FinalFunc MaxEnergyEval Q R ... HessFile: when the
   first number is the basis size (FullOpt1I / FullOpt1G,
   `optimise.full_opt1`; no Hessian file is read or written);
-* EXPC_VALS G K: expectation values of the current eigenvector
-  (the device part of ExpectationValuesG);
+* EXPC_VALS m K: expectation values of the current eigenvector (the
+  device part of ExpectationValuesG; run for either method, where the
+  reference runs it for 'G' only);
 * SAVE_FILE K file: SaveResults into that file;
-* the elimination and separation steps and DENSITIES are not part of this
-  library: they are reported as skipped.
+* the elimination and separation steps are not part of this library, and
+  DENSITIES (grids and output files with the equivalent-pair averaging of
+  the reference's writer) is left to the caller, who has
+  EcgContext.correlation_functions / particle_densities: they are
+  reported as skipped.
 
 Method 'I' uses the fixed shift, method 'G' isolates root
 WHICH_EIGENVALUE on every energy call.  After every step that changes the

@@ -169,6 +169,8 @@ def basis_enl(ctx, order: BasisOrder, k_stop: int, k_step: int, n_trials: int, m
     p1, s1, s2 = (float(v) for v in generator_param)
     if trials not in ("exact", "screen"):
         raise ValueError("trials must be 'exact' or 'screen'")
+    if n_trials < 1 or k_step < 1:
+        raise ValueError("BasisEnl needs NTrials >= 1 and Kstep >= 1")
     use_screen = trials == "screen" and k_step == 1 and np.isfinite(sigma)
     npt = order.params.shape[1]
     K = len(order.func_num)
