@@ -350,12 +350,14 @@ def test_fortran_host_layer_fixture(golden_dir, tmp_path):
     assert abs(g1 - g[0]) <= 1e-12 + 1e-7 * abs(g[0]) and abs(glast - g[-1]) <= 1e-12 + 1e-7 * abs(g[-1])
 
 
-@pytest.mark.parametrize("name,K,seed,grad", [("hminus", 500, 0, False), ("h2", 1000, 2, True)])
+@pytest.mark.parametrize("name,K,seed,grad", [("hminus", 500, 0, False), ("h2", 1000, 2, True),
+                                              ("h2", 2000, 3, True)])
 def test_seeded_configuration_vs_oracle(name, K, seed, grad):
     """Full-size seeded configurations: H^- (n=2, T=2) K=500 assembly +
-    lowest root; H2 (n=3, T=4) K=1000 energy + gradient (12 parameters per
-    function).  The oracle runs the reference algorithm on the host (stored
-    derivatives, GHEPIIS restatement, EnergyGB contraction)."""
+    lowest root; H2 (n=3, T=4) K=1000 and the benchmark's own K=2000 basis
+    (seed 20261206): energy + gradient (12 parameters per function).  The
+    oracle runs the reference algorithm on the host (stored derivatives,
+    GHEPIIS restatement, EnergyGB contraction)."""
     sy = named_system(name)
     params = seeded_basis(sy.n, K, SEED_BASE + seed)
     say("seeded configuration", name, K, "seed", SEED_BASE + seed, "grad", grad)
