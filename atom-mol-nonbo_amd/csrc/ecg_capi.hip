@@ -39,8 +39,11 @@ struct ecg_ctx {
   double* grad = nullptr;
   size_t grad_cap = 0;
   zd* ev[3] = {nullptr, nullptr, nullptr};   // iteration vectors; ev[2] = last eigenvector
-  bool have_last = false;
+  bool have_last = false;     // ev[2] holds a vector of length last_K (warm start of inverse iteration)
   int last_K = 0;
+  bool vec_current = false;   // ev[2] is the eigenvector of the current parameters (cleared by every
+                              // call that changes them; the post-processing entries need it)
+  bool params_set = false;    // the device parameters are the caller's current ones
   hipEvent_t e[8] = {};
   double tms[8] = {};
   double* pinned = nullptr;   // pinned host staging
@@ -126,6 +129,14 @@ int penalty_enqueue(ecg_ctx* x, hipStream_t s, int nfru, double b, double t2, do
 
 // Rows >= row of the device H, S no longer match the parameters.
 void rows_changed(ecg_ctx* x, int row) { x->hs_rows = std::min(x->hs_rows, row); }
+
+// The parameters changed: the last eigenvector no longer belongs to them
+// (it stays as a warm start of the next inverse iteration), and a screening
+// session built on the old ones is closed.
+void params_changed(ecg_ctx* x) {
+  x->vec_current = false;
+  x->scr.ready = false;
+}
 
 // H, S rows >= row are about to be (re)written with values that may differ.
 void hs_rewritten(ecg_ctx* x, int row) { x->fac_rows = std::min(x->fac_rows, row); }
@@ -224,6 +235,8 @@ int ensure_capacity(ecg_ctx* x, int K) {
   f.ld = ld;
   f.K = K;
   x->have_last = false;
+  x->vec_current = false;
+  x->params_set = false;   // params/bra/ket were reallocated: the caller uploads them next
   x->hs_rows = ok ? std::min(x->hs_rows, keep) : 0;
   x->fac_rows = ok ? std::min(x->fac_rows, keep) : 0;
   if (!ok) x->hparams.clear();
@@ -419,8 +432,14 @@ __global__ void k_fill_ones(int K, zd* v) {
 
 // Inverse iteration with the current factorisation (GHEPIIS loop,
 // src/linalg.f90:1892-1917).  Start vector in x->ev[0] unless warm.
-// Stops when the GHEPIIS measure reaches tol, or when it has stopped
-// decreasing below 1e-9 (rounding floor), or after maxiter iterations.
+// Stops when the GHEPIIS measure reaches tol (tol > 0), after maxiter
+// iterations, and:
+//   * without stop_at_floor and tol <= 0 (GHEPIIS's own floor rule,
+//     src/linalg.f90:1902-1906): converged at the first iteration whose
+//     measure is larger than the previous one and <= |tol|;
+//   * with stop_at_floor (the energy paths): when the measure has stopped
+//     halving below 1e-9 (the rounding floor; success is then decided by
+//     the caller against |tol|).
 // With stop_at_floor (the energy paths) the first nspec iterations, the
 // Rayleigh quotient and the normalisation are queued with a single host
 // synchronisation; further iterations only run if the measure asks for
@@ -497,7 +516,7 @@ int inverse_iteration(ecg_ctx* x, double sigma, double tol, int maxiter, bool wa
       r.iters = it + 1;
       std::swap(v, y);   // v now holds the newest iterate
       if (!std::isfinite(r.nd)) return ECG_ERR_NONFINITE;
-      if (tol > 0.0 ? (r.nd <= tol) : false) {
+      if (tol > 0.0 ? (r.nd <= tol) : (!stop_at_floor && r.nd > prev && r.nd <= std::fabs(tol))) {
         r.converged = true;
         break;
       }
@@ -862,7 +881,10 @@ int ecg_set_params(ecg_ctx* x, int K, const double* params) {
   if (!finite_all(params, (size_t)x->npt * K)) return ECG_ERR_ARG;
   x->s_ready = false;
   const int d = first_changed(x, K, params);
+  if (d < K || K != x->c.K || !x->params_set) params_changed(x);
+  x->params_set = false;
   RCCHK(upload_params(x, K, params));
+  x->params_set = true;
   rows_changed(x, d);
   x->hparams.assign(params, params + (size_t)x->npt * K);
   HIPCHK(hipStreamSynchronize(x->stream));
@@ -878,11 +900,14 @@ int ecg_set_params_device(ecg_ctx* x, int K, const double* dparams) {
     return ECG_ERR_ARG;
   }
   x->s_ready = false;
+  params_changed(x);
+  x->params_set = false;
   RCCHK(ensure_capacity(x, K));
   x->hparams.clear();
   rows_changed(x, 0);
   HIPCHK(hipMemcpyAsync(x->c.params, dparams, sizeof(double) * x->npt * K, hipMemcpyDeviceToDevice, x->stream));
   RCCHK(ecg::prep(x->c, x->stream));
+  x->params_set = true;
   return ECG_OK;
 }
 
@@ -890,7 +915,9 @@ int ecg_update_params(ecg_ctx* x, int first, int count, const double* params) {
   (void)hipGetLastError();   // start from a clean runtime error state
   if (!x || !params || first < 1 || count < 1 || first - 1 + count > x->c.K) return ECG_ERR_ARG;
   if (!finite_all(params, (size_t)x->npt * count)) return ECG_ERR_ARG;
+  if (!x->params_set) return ECG_ERR_STATE;
   x->s_ready = false;
+  params_changed(x);
   rows_changed(x, first - 1);
   if (x->hparams.size() == (size_t)x->npt * x->c.K)
     std::memcpy(x->hparams.data() + (size_t)x->npt * (first - 1), params, sizeof(double) * x->npt * count);
@@ -963,12 +990,13 @@ int ecg_compute_mat_elem(ecg_ctx* x, int nmin, int nmax, char method, double app
   (void)hipGetLastError();   // start from a clean runtime error state
   if (!x || !H || !S || !diagH || !diagS) return ECG_ERR_ARG;
   if (method != 'G' && method != 'I') return ECG_ERR_ARG;
-  if (x->c.K < 1) return ECG_ERR_STATE;
+  if (x->c.K < 1 || !x->params_set) return ECG_ERR_STATE;
   if (nmin < 1 || nmax > x->c.K || nmin > nmax || ld < nmax) return ECG_ERR_ARG;
   // rows < nmin keep their device values; their diagS comes from the caller
   if (nmin > 1)
     HIPCHK(hipMemcpyAsync(x->c.diagS, diagS, sizeof(double) * (nmin - 1), hipMemcpyHostToDevice, x->stream));
   if (nmin > 1) {   // rows normalised with the caller's diagS: nothing below is trusted afterwards
+    x->scr.ready = false;
     x->hs_rows = 0;
     hs_rewritten(x, 0);
   } else {
@@ -981,31 +1009,36 @@ int ecg_compute_mat_elem(ecg_ctx* x, int nmin, int nmax, char method, double app
 }
 
 int ecg_compute_mat_elem_and_deriv(ecg_ctx* x, int nfru, char method, double approx_energy, double* H,
-                                   double* S, int ld, double* diagH, double* diagS, double* D) {
+                                   double* S, int ld, double* diagH, double* diagS, double* D, int ldD) {
   (void)hipGetLastError();   // start from a clean runtime error state
   if (!x || !H || !S || !diagH || !diagS || !D) return ECG_ERR_ARG;
   if (method != 'G' && method != 'I') return ECG_ERR_ARG;
   const int K = x->c.K;
-  if (K < 1) return ECG_ERR_STATE;
-  if (nfru < 0 || nfru >= K || ld < K) return ECG_ERR_ARG;
+  if (K < 1 || !x->params_set) return ECG_ERR_STATE;
+  if (nfru < 0 || nfru >= K || ld < K || ldD < K - nfru) return ECG_ERR_ARG;
   const int nfo = K - nfru;
-  const size_t nD = (size_t)2 * x->npt * nfo * K;   // complex entries
+  const size_t nD = (size_t)2 * x->npt * nfo * K;   // complex entries on the device
   zd* dD = nullptr;
   if (hipMalloc((void**)&dD, sizeof(zd) * nD) != hipSuccess) return ECG_ERR_NOMEM;
   int rc = 0;
   if (nfru > 0 &&
       hipMemcpyAsync(x->c.diagS, diagS, sizeof(double) * nfru, hipMemcpyHostToDevice, x->stream) != hipSuccess)
     rc = ECG_ERR_HIP;
+  if (nfru > 0) x->scr.ready = false;   // rows normalised with the caller's diagS
   x->hs_rows = 0;
   hs_rewritten(x, 0);
   if (!rc) rc = ecg::assemble_deriv(x->c, nfru, dD, x->stream);
   if (!rc) x->s_ready = true;
-  if (!rc && hipMemcpyAsync(D, dD, sizeof(zd) * nD, hipMemcpyDeviceToHost, x->stream) != hipSuccess)
+  // D(:, k, l) of the caller's Glob_D(2 npt, ldD, *): K column blocks of
+  // 2 npt x nfo complex, placed with the pitch of its allocated extent
+  const size_t wbytes = sizeof(zd) * 2 * x->npt * nfo, pitch = sizeof(zd) * 2 * x->npt * (size_t)ldD;
+  if (!rc && hipMemcpy2DAsync(D, pitch, dD, wbytes, wbytes, K, hipMemcpyDeviceToHost, x->stream) != hipSuccess)
     rc = ECG_ERR_HIP;
   if (!rc && hipStreamSynchronize(x->stream) != hipSuccess) rc = ECG_ERR_HIP;
   (void)hipFree(dD);
   if (rc) return rc < 0 ? rc : ECG_ERR_HIP;
-  if (!finite_all(D, 2 * nD)) return ECG_ERR_NONFINITE;
+  for (int l = 0; l < K; ++l)
+    if (!finite_all(D + (size_t)l * pitch / sizeof(double), wbytes / sizeof(double))) return ECG_ERR_NONFINITE;
   return copy_out_hs(x, nfru, K, method, approx_energy, H, S, ld, diagH, diagS);
 }
 
@@ -1016,10 +1049,21 @@ int ecg_ghepiis(ecg_ctx* x, int k, int n, double* M, int nM, double* invD, const
   if (!x || !M || !invD || !B || !v || !lambda || !xout || n < 1 || k < 1 || k > n || nM < n || nB < n ||
       maxiter < 1)
     return ECG_ERR_ARG;
+  // The device H, S and the eigenvector slot take the caller's problem: the
+  // session state of the context's own basis is closed, and when the size
+  // differs from the context's basis its parameters are no longer usable
+  // (an energy call then returns ECG_ERR_STATE until ecg_set_params).
+  params_changed(x);
+  if (n != x->c.K) {
+    x->params_set = false;
+    x->hparams.clear();
+  }
   RCCHK(ensure_capacity(x, n));
   ecg::Ctx& c = x->c;
-  // Device H := lower(M) + sigma S mirrored, S := B, so that the shared
-  // kernels see H - sigma S = M.
+  // Device H := M (Hermitian from its lower triangle), S := B; the factor
+  // and the iteration run at shift 0, so that the matrix factored is M
+  // itself (no rounding from rebuilding it as (M + lambda B) - lambda B),
+  // and apprlambda is added to the Rayleigh quotient at the end.
   const size_t nd = (size_t)2 * c.ld * n;
   RCCHK(ensure_pinned(x, 2 * nd));
   double* hH = x->pinned;
@@ -1034,8 +1078,8 @@ int ecg_ghepiis(ecg_ctx* x, int k, int n, double* M, int nM, double* invD, const
       double mr = M[om], mi = M[om + 1];
       if (i < j) mi = -mi;                          // upper = conj(lower)
       if (i == j) mi = 0.0;
-      hH[od] = mr + apprlambda * hS[od];
-      hH[od + 1] = mi + apprlambda * hS[od + 1];
+      hH[od] = mr;
+      hH[od + 1] = mi;
     }
   x->hs_rows = 0;   // the device H, S now hold the caller's matrices
   hs_rewritten(x, 0);
@@ -1046,10 +1090,11 @@ int ecg_ghepiis(ecg_ctx* x, int k, int n, double* M, int nM, double* invD, const
   x->have_last = true;
   x->last_K = n;
   int nneg = 0, zero = 0;
-  RCCHK(factor_at(x, apprlambda, &nneg, &zero));
+  RCCHK(factor_at(x, 0.0, &nneg, &zero));
   if (zero) return ECG_SINGULAR;
   InvitResult r;
-  RCCHK(inverse_iteration(x, apprlambda, tol, maxiter, true, r, false));
+  RCCHK(inverse_iteration(x, 0.0, tol, maxiter, true, r, false));
+  r.lambda += apprlambda;
   // x->ev[2] holds x normalised to x^H B x = 1
   std::vector<double> hx(2 * (size_t)n), hD(n);
   HIPCHK(hipMemcpyAsync(hx.data(), x->ev[2], sizeof(zd) * n, hipMemcpyDeviceToHost, x->stream));
@@ -1088,7 +1133,7 @@ int ecg_energy(ecg_ctx* x, int which, double sigma, double tol, int maxiter, dou
   (void)hipGetLastError();   // start from a clean runtime error state
   if (!x || !E || which < 1 || maxiter < 1) return ECG_ERR_ARG;
   const int K = x->c.K;
-  if (K < 1) return ECG_ERR_STATE;
+  if (K < 1 || !x->params_set) return ECG_ERR_STATE;
   if (which > K) return ECG_ERR_ARG;
   HIPCHK(hipEventRecord(x->e[0], x->stream));
   hs_rewritten(x, x->hs_rows);   // rows below hs_rows are recomputed nowhere
@@ -1097,7 +1142,9 @@ int ecg_energy(ecg_ctx* x, int which, double sigma, double tol, int maxiter, dou
   HIPCHK(hipEventRecord(x->e[1], x->stream));
   int status = 0;
   x->s_ready = true;
+  x->vec_current = false;
   RCCHK(solve_root(x, which, sigma, tol, maxiter, E, info, &status));
+  x->vec_current = (status == 0 || status == ECG_NOT_CONVERGED);
   x->pen_total = 0.0;
   if (x->pen_on) RCCHK(penalty_enqueue(x, x->stream, x->pen_nfru, x->pen_b, x->pen_t2, nullptr));
   HIPCHK(hipEventRecord(x->e[2], x->stream));
@@ -1120,7 +1167,7 @@ int ecg_energy_grad(ecg_ctx* x, int nfru, int which, double sigma, double tol, i
   if (!x || !E || !grad || which < 1 || maxiter < 1) return ECG_ERR_ARG;
   ecg::Ctx& c = x->c;
   const int K = c.K;
-  if (K < 1) return ECG_ERR_STATE;
+  if (K < 1 || !x->params_set) return ECG_ERR_STATE;
   if (nfru < 0 || nfru >= K || which > K) return ECG_ERR_ARG;
   const size_t np = ecg::adjoint_partial_doubles(c.n, K);
   if (np > x->part_cap) {
@@ -1140,7 +1187,9 @@ int ecg_energy_grad(ecg_ctx* x, int nfru, int which, double sigma, double tol, i
   x->hs_rows = K;
   HIPCHK(hipEventRecord(x->e[1], x->stream));
   int status = 0;
+  x->vec_current = false;
   RCCHK(solve_root(x, which, sigma, tol, maxiter, E, info, &status));
+  x->vec_current = (status == 0 || status == ECG_NOT_CONVERGED);
   HIPCHK(hipEventRecord(x->e[2], x->stream));
   c.ev_adj0 = x->e[3];
   c.ev_adj1 = x->e[4];
@@ -1171,7 +1220,7 @@ int ecg_energy_grad_stored(ecg_ctx* x, int nfru, int which, double sigma, double
   if (!x || !E || !grad || which < 1 || maxiter < 1) return ECG_ERR_ARG;
   ecg::Ctx& c = x->c;
   const int K = c.K;
-  if (K < 1) return ECG_ERR_STATE;
+  if (K < 1 || !x->params_set) return ECG_ERR_STATE;
   if (nfru < 0 || nfru >= K || which > K) return ECG_ERR_ARG;
   const int nfo = K - nfru;
   const size_t nD = (size_t)2 * x->npt * nfo * K;
@@ -1194,7 +1243,9 @@ int ecg_energy_grad_stored(ecg_ctx* x, int nfru, int which, double sigma, double
   x->hs_rows = K;
   HIPCHK(hipEventRecord(x->e[1], x->stream));
   int status = 0;
+  x->vec_current = false;
   RCCHK(solve_root(x, which, sigma, tol, maxiter, E, info, &status));
+  x->vec_current = (status == 0 || status == ECG_NOT_CONVERGED);
   HIPCHK(hipEventRecord(x->e[2], x->stream));
   hipLaunchKernelGGL(k_contract_stored, dim3(nfo), dim3(256), 0, x->stream, x->npt, nfo, K, nfru, x->dD, x->ev[2],
                      *E, x->grad);
@@ -1222,7 +1273,7 @@ int ecg_batch_create(int nprob, ecg_ctx** ctxs, ecg_batch** out) {
   if (!out || nprob < 1 || !ctxs) return ECG_ERR_ARG;
   *out = nullptr;
   for (int i = 0; i < nprob; ++i)
-    if (!ctxs[i] || ctxs[i]->c.K < 1) return ECG_ERR_STATE;
+    if (!ctxs[i] || ctxs[i]->c.K < 1 || !ctxs[i]->params_set) return ECG_ERR_STATE;
   ecg_batch* b = new ecg_batch();
   b->ctx.assign(ctxs, ctxs + nprob);
   if (hipStreamCreateWithFlags(&b->lead, hipStreamNonBlocking) != hipSuccess ||
@@ -1379,7 +1430,9 @@ int ecg_batch_energy_grad(ecg_batch* b, const double* sigma, double tol, int max
       } else {
         rc = ecg::assemble(x->c, 0, x->c.K, x->stream);
       }
+      x->vec_current = false;
       if (!rc) rc = solve_root(x, 1, sigma[i], tol, maxiter, &E[i], info, &status);
+      if (!rc) x->vec_current = (status == 0 || status == ECG_NOT_CONVERGED);
       if (!rc && !b->packed) {
         rc = ecg::adjoint(x->c, 0, E[i], x->ev[2], x->part, x->grad, x->stream);
         x->pen_total = 0.0;
@@ -1462,7 +1515,7 @@ int ecg_trace_read(ecg_ctx* x, double* count, double* total_ms, int nmax) {
 
 int ecg_get_params(ecg_ctx* x, double* params) {
   if (!x || !params) return ECG_ERR_ARG;
-  if (x->c.K < 1) return ECG_ERR_STATE;
+  if (x->c.K < 1 || !x->params_set) return ECG_ERR_STATE;
   HIPCHK(hipMemcpyAsync(params, x->c.params, sizeof(double) * x->npt * x->c.K, hipMemcpyDeviceToHost, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   return ECG_OK;
@@ -1473,10 +1526,11 @@ int ecg_screen_prepare(ecg_ctx* x, double sigma, double tol, int maxiter, double
   if (!x || !std::isfinite(sigma) || maxiter < 1) return ECG_ERR_ARG;
   ecg::Ctx& c = x->c;
   const int K = c.K;
-  if (K < 2) return ECG_ERR_STATE;
+  if (K < 2 || !x->params_set) return ECG_ERR_STATE;
   RCCHK(ensure_screen(x, 1, 0));
   ecg::Screen& sc = x->scr;
   sc.ready = false;
+  x->vec_current = false;
   RCCHK(ecg::prep(c, x->stream));
   hs_rewritten(x, 0);   // the screening factor replaces the energy paths' one
   RCCHK(ecg::assemble(c, 0, K, x->stream));
@@ -1510,6 +1564,7 @@ int ecg_screen_prepare(ecg_ctx* x, double sigma, double tol, int maxiter, double
     info[3] = r.nd;
   }
   sc.ready = true;
+  x->vec_current = true;
   return (r.converged || r.nd < 1e-9) ? ECG_OK : ECG_NOT_CONVERGED;
 }
 
@@ -1592,6 +1647,7 @@ int ecg_screen_loop(ecg_ctx* x, int nsteps, int ncand, const double* cand, int i
   (void)hipGraphDestroy(g);
   x->tms[0] = ev_ms(x->e[0], x->e[1]);
   x->have_last = false;
+  x->vec_current = false;
   for (int i = 0; i < nsteps; ++i) {
     Ebest[i] = rec[2 * i];
     ibest[i] = (int)rec[2 * i + 1];
@@ -1638,8 +1694,8 @@ int ecg_expectation_values(ecg_ctx* x, const double* c, double* values, double* 
   if (!x || !values) return ECG_ERR_ARG;
   ecg::Ctx& cx = x->c;
   const int K = cx.K;
-  if (K < 1) return ECG_ERR_STATE;
-  if (!c && !(x->have_last && x->last_K == K)) return ECG_ERR_STATE;
+  if (K < 1 || !x->params_set) return ECG_ERR_STATE;
+  if (!c && !(x->vec_current && x->last_K == K)) return ECG_ERR_STATE;
   if (c && !finite_all(c, 2 * (size_t)K)) return ECG_ERR_ARG;
   const int m = cx.n * (cx.n + 1) / 2, nobs = 5 * m + 4;
   const size_t need = ecg::expectation_partial_doubles(cx.n, K) + 2 * (size_t)nobs;
@@ -1676,8 +1732,8 @@ int ecg_expectation_values(ecg_ctx* x, const double* c, double* values, double* 
 static int post_values_setup(ecg_ctx* x, const double* c, size_t nout, const zd** cv, double** dout) {
   ecg::Ctx& cx = x->c;
   const int K = cx.K;
-  if (K < 1) return ECG_ERR_STATE;
-  if (!c && !(x->have_last && x->last_K == K)) return ECG_ERR_STATE;
+  if (K < 1 || !x->params_set) return ECG_ERR_STATE;
+  if (!c && !(x->vec_current && x->last_K == K)) return ECG_ERR_STATE;
   if (c && !finite_all(c, 2 * (size_t)K)) return ECG_ERR_ARG;
   const size_t need = ecg::correlation_partial_doubles(cx.n, K) + nout;
   if (need > x->xpart_cap) {
@@ -1757,6 +1813,15 @@ int ecg_particle_densities(ecg_ctx* x, const double* c, const double* masses, in
   HIPCHK(hipMemcpyAsync(values, dout, sizeof(double) * nout, hipMemcpyDeviceToHost, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   return finite_all(values, nout) ? ECG_OK : ECG_ERR_NONFINITE;
+}
+
+int ecg_eigenvector(ecg_ctx* x, double* c) {
+  if (!x || !c) return ECG_ERR_ARG;
+  const int K = x->c.K;
+  if (K < 1 || !x->vec_current || x->last_K != K) return ECG_ERR_STATE;
+  HIPCHK(hipMemcpyAsync(c, x->ev[2], sizeof(zd) * K, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return ECG_OK;
 }
 
 int ecg_last_timings(ecg_ctx* x, double* ms, int nmax) {

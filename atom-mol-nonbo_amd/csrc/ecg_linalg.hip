@@ -45,6 +45,13 @@ __device__ inline void publish_wg(int* flag, int epoch) {
   if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The poll itself is relaxed; once both flags are seen, an acquire fence at
+// agent scope pairs with the producer's release.  The eight XCDs of the
+// chip each have their own L2, which is not coherent with the others':
+// the fence invalidates this CU's vector cache and the non-coherent lines
+// of its XCD's L2, so the loads after the barrier read what the producer
+// (possibly on another XCD) wrote back before publishing.  The workgroup
+// barrier then carries the acquire to the other threads.
 __device__ inline void wait_wg(const int* f1, const int* f2, int epoch, int* status) {
   if (threadIdx.x == 0) {
     const unsigned long long t0 = wall_clock64();
@@ -56,6 +63,7 @@ __device__ inline void wait_wg(const int* f1, const int* f2, int epoch, int* sta
         break;
       }
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
 }
@@ -666,10 +674,18 @@ __global__ void __launch_bounds__(256) k_ldl_xlast(int K, int ld, zd* __restrict
 // Publication: the producer's threads fence at agent scope, the workgroup
 // synchronises, thread 0 stores the flag (release).  A waiting workgroup's
 // thread 0 polls the flags (relaxed, bounded in time: past the limit it
-// records status[2] and proceeds, so every workgroup exits); the data it
-// then reads was written in this launch by the producer only, and caches
-// hold no older copy of it (they are invalidated at kernel start).
-// Waits only point to lower workgroup indices, dispatched no later.
+// records status[2] and proceeds, so every workgroup exits), then fences
+// with acquire at agent scope before the workgroup barrier, so that the
+// data written by a producer on another XCD is read from memory, not from
+// a stale line of this XCD's L2 or this CU's cache.
+// Waits only point to lower workgroup indices, dispatched no later: a
+// workgroup can only wait on one that the dispatcher placed before it,
+// which already holds its CU slot and never waits on a higher index, so
+// the chain of waits ends.  A wait on a higher index has no such
+// guarantee: the waiting workgroups can fill every slot the awaited one
+// needs (a variant that let panel strips wait on sibling strips of higher
+// index ran past its time limit for this reason).  A CPU test decodes the
+// task maps of every launch and checks this rule.
 // W is double-buffered by column parity: the tiles read column j's, the
 // strips write column j+1's.
 constexpr int NQ = 4;   // quarters of the look-ahead tile's update

@@ -45,7 +45,7 @@ EXPORTS = [
     "ecg_energy_grad_stored", "ecg_batch_create", "ecg_batch_destroy", "ecg_batch_set_config",
     "ecg_batch_energy_grad", "ecg_trace", "ecg_trace_read", "ecg_get_params", "ecg_screen_prepare", "ecg_screen", "ecg_screen_loop", "ecg_last_timings",
     "ecg_set_overlap_penalty", "ecg_overlap_penalty", "ecg_set_params_device", "ecg_expectation_values",
-    "ecg_correlation_functions", "ecg_particle_densities", "ecg_relativistic_corrections",
+    "ecg_correlation_functions", "ecg_particle_densities", "ecg_relativistic_corrections", "ecg_eigenvector",
 ]
 
 
@@ -76,7 +76,7 @@ def load(path: str = LIB_PATH):
     L.ecg_update_params.argtypes = [vp, c_int, c_int, _d]
     L.ecg_matrix_elements.argtypes = [vp, _d, _d, _i, c_int, _d, _d, _d, _d]
     L.ecg_compute_mat_elem.argtypes = [vp, c_int, c_int, c_char, c_double, _d, _d, c_int, _d, _d]
-    L.ecg_compute_mat_elem_and_deriv.argtypes = [vp, c_int, c_char, c_double, _d, _d, c_int, _d, _d, _d]
+    L.ecg_compute_mat_elem_and_deriv.argtypes = [vp, c_int, c_char, c_double, _d, _d, c_int, _d, _d, _d, c_int]
     L.ecg_ghepiis.argtypes = [vp, c_int, c_int, _d, c_int, _d, _d, c_int, c_double, _d, c_double, c_int, c_int,
                               _d, _d, _d, _i]
     L.ecg_energy.argtypes = [vp, c_int, c_double, c_double, c_int, _d, _d, _d]
@@ -100,6 +100,7 @@ def load(path: str = LIB_PATH):
     L.ecg_correlation_functions.argtypes = [vp, _d, c_int, _d, _d]
     L.ecg_particle_densities.argtypes = [vp, _d, _d, c_int, _d, _d]
     L.ecg_relativistic_corrections.argtypes = [vp, _d, _d, _d, _d]
+    L.ecg_eigenvector.argtypes = [vp, _d]
     for name in EXPORTS:
         getattr(L, name).restype = c_int
     _lib = L
@@ -196,10 +197,12 @@ class EcgContext:
         return H, S, diagH, diagS
 
     def ComputeMatElemAndDeriv(self, nfru=0, method="G", approx_energy=0.0, H=None, S=None, diagH=None,
-                               diagS=None):
+                               diagS=None, ldD=None):
         """Rows nfru+1..K with derivatives.  Rows <= nfru are not recomputed:
         pass the arrays of an earlier ComputeMatElem (their diagS normalises
-        the new rows), as the reference's global state provides."""
+        the new rows), as the reference's global state provides.  ldD: the
+        second extent of the returned D (Glob_D's NumOfFuncToOpt, at least
+        K - nfru); entries beyond K - nfru stay zero."""
         K = self.K
         if H is None:
             if nfru > 0:
@@ -209,10 +212,11 @@ class EcgContext:
                 S = np.zeros((K, K), np.complex128, order="F")
                 diagH = np.zeros(K)
                 diagS = np.zeros(K)
-        D = np.zeros((2 * self.npt, K - nfru, K), np.complex128, order="F")
+        ldD = K - nfru if ldD is None else int(ldD)
+        D = np.zeros((2 * self.npt, ldD, K), np.complex128, order="F")
         _check("ecg_compute_mat_elem_and_deriv", load().ecg_compute_mat_elem_and_deriv(
             self._h, nfru, method.encode(), float(approx_energy), H.ctypes.data_as(_d), S.ctypes.data_as(_d),
-            K, _p(diagH), _p(diagS), D.ctypes.data_as(_d)))
+            K, _p(diagH), _p(diagS), D.ctypes.data_as(_d), ldD))
         return H, S, diagH, diagS, D
 
     def GHEPIIS(self, k, M, invD, B, apprlambda, v, tol, maxiter=20, specif_norm=0):
@@ -382,6 +386,14 @@ class EcgContext:
         _check("ecg_screen_loop", load().ecg_screen_loop(self._h, ns, nc, _p(cand), iters, _p(Eb),
                                                          ib.ctypes.data_as(_i)))
         return Eb, ib
+
+    def eigenvector(self):
+        """The last energy call's eigenvector (c^H S c = 1), as EnergyGB
+        leaves it in Glob_c; raises EcgError (ECG_ERR_STATE) after a
+        parameter change."""
+        c = np.zeros(self.K, np.complex128)
+        _check("ecg_eigenvector", load().ecg_eigenvector(self._h, c.ctypes.data_as(_d)))
+        return c
 
     def timings(self):
         t = np.zeros(8)

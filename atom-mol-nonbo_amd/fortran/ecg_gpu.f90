@@ -14,6 +14,7 @@
 !                  ComputeMatElemAndDeriv  src/matform.f90:234
 !                  GHEPIIS                 src/linalg.f90:1781
 !                  EnergyGA / EnergyIA     src/workproc.f90:1697, 1954
+!                  EnergyGAM / EnergyIAM   src/workproc.f90:1775, 2014
 !                  EnergyGB / EnergyIB     src/workproc.f90:1841, 2063
 !                  ComputeOverlapPenalty   src/workproc.f90:1491
 !                  ComputeOverlapPenaltyAndAddGradient  src/workproc.f90:1571
@@ -88,10 +89,10 @@ module ecg_capi
       real(c_double), intent(inout) :: diagH(*), diagS(*)
     end function
     integer(c_int) function ecg_compute_mat_elem_and_deriv(ctx, nfru, method, approx_energy, H, S, ld, &
-        diagH, diagS, D) bind(C, name="ecg_compute_mat_elem_and_deriv")
+        diagH, diagS, D, ldD) bind(C, name="ecg_compute_mat_elem_and_deriv")
       import :: c_int, c_double, c_double_complex, c_ptr, c_char
       type(c_ptr), value :: ctx
-      integer(c_int), value :: nfru, ld
+      integer(c_int), value :: nfru, ld, ldD
       character(kind=c_char), value :: method
       real(c_double), value :: approx_energy
       complex(c_double_complex), intent(inout) :: H(*), S(*), D(*)
@@ -251,6 +252,11 @@ module ecg_capi
       real(c_double), intent(in) :: grid(*)
       real(c_double), intent(out) :: values(*)
     end function
+    integer(c_int) function ecg_eigenvector(ctx, c) bind(C, name="ecg_eigenvector")
+      import :: c_int, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(out) :: c(*)
+    end function
     integer(c_int) function ecg_last_timings(ctx, ms, nmax) bind(C, name="ecg_last_timings")
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx
@@ -279,6 +285,16 @@ module ecg_globals
   integer :: Glob_WhichEigenvalue = 1
   real(dprec) :: Glob_EigvalTol = 1.0d-12
   integer :: Glob_InvItMaxIter = 20
+  complex(dprec), allocatable :: Glob_LastEigvector(:)
+  ! Method 'G' (ZHEGVX in the reference) has no tolerance or iteration
+  ! limit of its own.  On the device its root is refined by inverse
+  ! iteration run to its rounding floor: the call succeeds when GHEPIIS's
+  ! measure there is <= Ecg_GFloorBound (an eigenvector error of 1e-7 gives
+  ! an energy error of order 1e-14 relative), within Ecg_GIterLimit
+  ! iterations.  These two are this layer's own settings, not reference
+  ! globals.
+  real(dprec) :: Ecg_GFloorBound = 1.0d-7
+  integer :: Ecg_GIterLimit = 60
   logical :: Glob_OverlapPenaltyAllowed = .false.
   real(dprec) :: Glob_OverlapPenaltyThreshold2 = 1
   real(dprec) :: Glob_MaxOverlapPenalty = 1
@@ -293,7 +309,7 @@ module ecg_hostpath
   implicit none
   private
   public :: EcgHostInit, EcgHostFinish, MatrixElements, ComputeMatElem, ComputeMatElemAndDeriv, GHEPIIS, &
-            EnergyGA, EnergyIA, EnergyGB, EnergyIB, ComputeOverlapPenalty, ComputeOverlapPenaltyAndAddGradient, &
+            EnergyGA, EnergyIA, EnergyGAM, EnergyIAM, EnergyGB, EnergyIB, ComputeOverlapPenalty, ComputeOverlapPenaltyAndAddGradient, &
             EcgExpectationValues, EcgCorrelationFunctions, EcgParticleDensities, &
             EcgRelativisticCorrections
   type(c_ptr), save :: ctx = c_null_ptr
@@ -361,7 +377,7 @@ contains
     rc = sync_params(Nmax)
     if (rc == ECG_OK) rc = ecg_compute_mat_elem_and_deriv(ctx, Nmin - 1, Glob_GSEPSolutionMethod, &
                                                           Glob_ApproxEnergy, Glob_H, Glob_S, Glob_HSLeadDim, &
-                                                          Glob_diagH, Glob_diagS, Glob_D)
+                                                          Glob_diagH, Glob_diagS, Glob_D, size(Glob_D, 2))
     if (rc /= ECG_OK) Glob_diagS(Nmin) = ieee_value(1.0_dprec, ieee_quiet_nan)
   end subroutine
 
@@ -387,11 +403,50 @@ contains
     integer :: Nmin, Nmax, ErrorCode
     logical :: AreMatElemNeeded
     real(dprec) :: info(8)
+    complex(dprec), allocatable :: cv(:)
+    info = 0
+    EnergyGA = ieee_value(1.0_dprec, ieee_quiet_nan)
+    allocate(cv(max(Nmax, 1)))   ! ZHEGVX(JOBZ='N'): Glob_c is not written
     ErrorCode = sync_params(Nmax)
     if (ErrorCode == ECG_OK) ErrorCode = ecg_energy(ctx, Glob_WhichEigenvalue, ieee_value(1.0_dprec, ieee_quiet_nan), &
-                                                    Glob_EigvalTol, 60, EnergyGA, Glob_c, info)
-    Glob_TotalOverlapPenalty = info(7)
+                                                    -Ecg_GFloorBound, Ecg_GIterLimit, EnergyGA, cv, info)
+    if (ErrorCode == ECG_OK) Glob_TotalOverlapPenalty = info(7)
   end function
+
+  ! src/workproc.f90:1775 — as EnergyGA, with the eigenvector (c^H S c = 1,
+  ! ZHEGVX's normalisation for ITYPE=1) in Glob_c
+  real(dprec) function EnergyGAM(Nmin, Nmax, AreMatElemNeeded, ErrorCode)
+    integer :: Nmin, Nmax, ErrorCode
+    logical :: AreMatElemNeeded
+    real(dprec) :: info(8)
+    info = 0
+    EnergyGAM = ieee_value(1.0_dprec, ieee_quiet_nan)
+    ErrorCode = sync_params(Nmax)
+    if (ErrorCode == ECG_OK) ErrorCode = ecg_energy(ctx, Glob_WhichEigenvalue, ieee_value(1.0_dprec, ieee_quiet_nan), &
+                                                    -Ecg_GFloorBound, Ecg_GIterLimit, EnergyGAM, Glob_c, info)
+    if (ErrorCode == ECG_OK) Glob_TotalOverlapPenalty = info(7)
+  end function
+
+  ! The vectors GHEPIIS leaves behind in EnergyIA/EnergyIAM
+  ! (src/workproc.f90:1989-1991, 2038-2040): x, normalised as SpecifNorm
+  ! asks, goes to Glob_LastEigvector; Glob_c is GHEPIIS's work vector v,
+  ! i.e. the iterate before the last one scaled to max |Re|,|Im| = 1.  The
+  ! device returns the final iterate (c^H S c = 1); its max-abs scaling
+  ! stands in for v, from which it differs by less than the convergence
+  ! measure.
+  subroutine iteration_vectors(n, c, specif_norm)
+    integer, intent(in) :: n, specif_norm
+    complex(dprec), intent(in) :: c(:)
+    real(dprec) :: t
+    if (.not. allocated(Glob_LastEigvector)) allocate(Glob_LastEigvector(size(Glob_c)))
+    t = maxval(max(abs(real(c(1:n))), abs(aimag(c(1:n)))))
+    Glob_c(1:n) = c(1:n) / t
+    if (specif_norm == 0) then
+      Glob_LastEigvector(1:n) = c(1:n)
+    else
+      Glob_LastEigvector(1:n) = Glob_c(1:n)
+    end if
+  end subroutine
 
   ! src/workproc.f90:1954 — as EnergyGA with the shift Glob_ApproxEnergy
   ! (method 'I')
@@ -399,10 +454,32 @@ contains
     integer :: Nmin, Nmax, ErrorCode
     logical :: AreMatElemNeeded
     real(dprec) :: info(8)
+    complex(dprec), allocatable :: cv(:)
+    info = 0
+    EnergyIA = ieee_value(1.0_dprec, ieee_quiet_nan)
+    allocate(cv(max(Nmax, 1)))
     ErrorCode = sync_params(Nmax)
     if (ErrorCode == ECG_OK) ErrorCode = ecg_energy(ctx, Glob_WhichEigenvalue, Glob_ApproxEnergy, Glob_EigvalTol, &
-                                                    Glob_InvItMaxIter, EnergyIA, Glob_c, info)
-    Glob_TotalOverlapPenalty = info(7)
+                                                    Glob_InvItMaxIter, EnergyIA, cv, info)
+    if (ErrorCode == ECG_OK .or. ErrorCode == ECG_NOT_CONVERGED) call iteration_vectors(Nmax, cv, -1)
+    if (ErrorCode == ECG_OK) Glob_TotalOverlapPenalty = info(7)
+  end function
+
+  ! src/workproc.f90:2014 — as EnergyIA with SpecifNorm = 0: the
+  ! eigenvector normalised to x^H S x = 1 in Glob_LastEigvector
+  real(dprec) function EnergyIAM(Nmin, Nmax, AreMatElemNeeded, ErrorCode)
+    integer :: Nmin, Nmax, ErrorCode
+    logical :: AreMatElemNeeded
+    real(dprec) :: info(8)
+    complex(dprec), allocatable :: cv(:)
+    info = 0
+    EnergyIAM = ieee_value(1.0_dprec, ieee_quiet_nan)
+    allocate(cv(max(Nmax, 1)))
+    ErrorCode = sync_params(Nmax)
+    if (ErrorCode == ECG_OK) ErrorCode = ecg_energy(ctx, Glob_WhichEigenvalue, Glob_ApproxEnergy, Glob_EigvalTol, &
+                                                    Glob_InvItMaxIter, EnergyIAM, cv, info)
+    if (ErrorCode == ECG_OK .or. ErrorCode == ECG_NOT_CONVERGED) call iteration_vectors(Nmax, cv, 0)
+    if (ErrorCode == ECG_OK) Glob_TotalOverlapPenalty = info(7)
   end function
 
   ! src/workproc.f90:1841 — energy and gradient w.r.t. the parameters of
@@ -414,11 +491,14 @@ contains
     logical :: AreMatElemNeeded
     integer :: ErrorCode
     real(dprec) :: info(8)
+    info = 0
     ErrorCode = sync_params(Glob_CurrBasisSize)
     if (ErrorCode == ECG_OK) ErrorCode = ecg_energy_grad(ctx, Glob_nfru, Glob_WhichEigenvalue, &
-                                                         ieee_value(1.0_dprec, ieee_quiet_nan), Glob_EigvalTol, 60, &
-                                                         Evalue, Gradient, info)
-    Glob_TotalOverlapPenalty = info(7)
+                                                         ieee_value(1.0_dprec, ieee_quiet_nan), -Ecg_GFloorBound, &
+                                                         Ecg_GIterLimit, Evalue, Gradient, info)
+    if (ErrorCode == ECG_OK) Glob_TotalOverlapPenalty = info(7)
+    ! ZHEGVX(JOBZ='V') leaves the eigenvector in Glob_c
+    if (ErrorCode == ECG_OK) ErrorCode = ecg_eigenvector(ctx, Glob_c)
   end subroutine
 
   ! src/workproc.f90:2063 — as EnergyGB with the shift Glob_ApproxEnergy
@@ -428,10 +508,16 @@ contains
     logical :: AreMatElemNeeded
     integer :: ErrorCode
     real(dprec) :: info(8)
+    info = 0
     ErrorCode = sync_params(Glob_CurrBasisSize)
     if (ErrorCode == ECG_OK) ErrorCode = ecg_energy_grad(ctx, Glob_nfru, Glob_WhichEigenvalue, Glob_ApproxEnergy, &
                                                          Glob_EigvalTol, Glob_InvItMaxIter, Evalue, Gradient, info)
-    Glob_TotalOverlapPenalty = info(7)
+    if (ErrorCode == ECG_OK) Glob_TotalOverlapPenalty = info(7)
+    ! GHEPIIS(SpecifNorm = 0): x^H S x = 1 in Glob_LastEigvector, copied to Glob_c
+    if (ErrorCode == ECG_OK .or. ErrorCode == ECG_NOT_CONVERGED) then
+      if (.not. allocated(Glob_LastEigvector)) allocate(Glob_LastEigvector(size(Glob_c)))
+      if (ecg_eigenvector(ctx, Glob_c) == ECG_OK) Glob_LastEigvector(1:Glob_CurrBasisSize) = Glob_c(1:Glob_CurrBasisSize)
+    end if
   end subroutine
 
   ! src/workproc.f90:1491 — penalty of the overlaps assembled on the device

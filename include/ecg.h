@@ -93,10 +93,14 @@ int ecg_compute_mat_elem(ecg_ctx* ctx, int nmin, int nmax, char method, double a
                          double* S, int ld, double* diagH, double* diagS);
 
 /* ComputeMatElemAndDeriv(nfru+1, K) + StoreHSD (src/matform.f90:55-115,
- * 234-360).  D: (2*npt) x (K - nfru) x ld complex (Glob_D).  H, S, diagH,
- * diagS as ecg_compute_mat_elem for rows nfru+1..K. */
+ * 234-360).  D: Glob_D(2*npt, ldD, K) complex (src/workproc.f90:4992
+ * allocates Glob_D(2*npt, NumOfFuncToOpt, cbs)); ldD >= K - nfru is its
+ * allocated second extent, so that the last chunk of a cycle, with fewer
+ * functions than NumOfFuncToOpt, lands where the reference stores it.
+ * Only D(:, 1:K-nfru, 1:K) is written.  H, S, diagH, diagS as
+ * ecg_compute_mat_elem for rows nfru+1..K. */
 int ecg_compute_mat_elem_and_deriv(ecg_ctx* ctx, int nfru, char method, double approx_energy, double* H,
-                                   double* S, int ld, double* diagH, double* diagS, double* D);
+                                   double* S, int ld, double* diagH, double* diagS, double* D, int ldD);
 
 /* GHEPIIS (src/linalg.f90:1781-1926) on host arrays: inverse iteration for
  * the eigenvalue of (A, B) nearest apprlambda.  M: lower triangle holds
@@ -107,9 +111,18 @@ int ecg_compute_mat_elem_and_deriv(ecg_ctx* ctx, int nfru, char method, double a
  * specif_norm = 0, x^H x = 1 when 1, max |Re|,|Im| = 1 otherwise).
  * Returns 0, ECG_SINGULAR or ECG_NOT_CONVERGED like the reference's
  * ErrorCode.  tol > 0: converged when the measure is <= tol; tol <= 0
- * (the reference's other mode): the iteration runs to its rounding floor
- * and succeeds when the measure there is <= |tol|.  The energy paths below
- * take tol the same way. */
+ * (the reference's other mode, src/linalg.f90:1902-1906): converged at the
+ * first iteration whose measure rises above the previous one and is
+ * <= |tol|; numiter counts that iteration.  k is accepted for the
+ * reference's argument list only: the factor is always recomputed from
+ * row 1, so M's upper triangle need not hold a factor.  M is factored as
+ * given (the shift is added to the Rayleigh quotient afterwards).
+ * The device H, S of ctx then hold M and B: the context's basis must be
+ * reassembled, and when n differs from its K the parameters must be set
+ * again (energy calls return ECG_ERR_STATE until ecg_set_params).
+ * The energy paths below take tol > 0 the same way; with tol <= 0 they
+ * run the iteration to its rounding floor (the measure stops halving below
+ * 1e-9) and succeed when the measure there is <= |tol|. */
 int ecg_ghepiis(ecg_ctx* ctx, int k, int n, double* M, int nM, double* invD, const double* B, int nB,
                 double apprlambda, const double* v, double tol, int maxiter, int specif_norm, double* lambda,
                 double* x, double* relacc, int* numiter);
@@ -256,7 +269,9 @@ int ecg_overlap_penalty(ecg_ctx* ctx, int nfru, double max_pair_penalty, double 
  * summed as the reference's driver sums them, src/workproc.f90:8158-8300,
  * with the bra unpermuted and the ket's Y^dagger Y terms) in the
  * eigenvector c: K complex, normalised c^H S c = 1 for the normalised S;
- * NULL uses the eigenvector of the last ecg_energy / ecg_energy_grad call.
+ * NULL uses the eigenvector of the last ecg_energy / ecg_energy_grad call,
+ * which must belong to the current parameters (ECG_ERR_STATE after any
+ * call that changed them, or after a failed energy call).
  * values (5 m + 4 doubles, m = n(n+1)/2): <1/r^2>, <1/r>, <r>, <r^2>,
  * <delta(r)>, each for the pseudo-particle pairs (a, b >= a) in the order
  * (1,1), (1,2), .., (1,n), (2,2), .. (a = b: pseudo-particle a's distance
@@ -294,6 +309,12 @@ int ecg_particle_densities(ecg_ctx* ctx, const double* c, const double* masses, 
  * (MV, Darwin); errors nullable. */
 int ecg_relativistic_corrections(ecg_ctx* ctx, const double* c, const double* masses, double* values,
                                  double* errors);
+
+/* The eigenvector (K complex, c^H S c = 1 for the normalised S) of the last
+ * energy call on the current parameters: what EnergyGB / EnergyIB leave
+ * in Glob_c (src/workproc.f90:1885-1890, 2114-2119).  ECG_ERR_STATE when
+ * the parameters changed since, or the last energy call failed. */
+int ecg_eigenvector(ecg_ctx* ctx, double* c);
 
 /* Device time (ms, HIP events on the context's stream) of the phases of
  * the last ecg_energy / ecg_energy_grad call: [0] assembly, [1] eigen step

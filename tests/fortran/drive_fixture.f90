@@ -6,14 +6,17 @@
 !   mass matrix (n*n, column-major) / q0 / q(1..n) / K which approx_energy /
 !   K columns of 2*n(n+1)/2 parameters
 ! Prints "GHEPIIS <E> <err> <iters>", "ENERGYIA <E> <err>",
-! "ENERGYIB <E> <err> <|grad|> <grad(1)> <grad(npt*K)>".
+! "ENERGYIB <E> <err> <|grad|> <grad(1)> <grad(npt*K)>", then method 'G'
+! (Glob_EigvalTol as read, which ZHEGVX does not use): "ENERGYGA <E> <err>",
+! "ENERGYGAM <E> <err> <c^H S c>", "ENERGYGB <E> <err> <|grad|> <c^H S c>",
+! and "ENERGYIAM <E> <err> <x^H S x> <max |Re|,|Im| of Glob_c>".
 program drive_fixture
   use ecg_globals
   use ecg_hostpath
   implicit none
   character(len=512) :: path
   integer :: u, T, K, i, err, iters, ld
-  real(dprec) :: approx, lambda, relacc, E
+  real(dprec) :: approx, lambda, relacc, E, tol
   complex(dprec), allocatable :: v(:), w(:), x(:)
   real(dprec), allocatable :: grad(:)
 
@@ -31,6 +34,8 @@ program drive_fixture
   read(u, *) Glob_PseudoCharge0
   read(u, *) Glob_PseudoCharge
   read(u, *) K, Glob_WhichEigenvalue, approx
+  read(u, *) tol
+  Glob_EigvalTol = tol
   allocate(Glob_NonlinParam(Glob_npt, K))
   read(u, *) Glob_NonlinParam
   close(u)
@@ -66,5 +71,28 @@ program drive_fixture
   call EnergyIB(E, grad, .true., err)
   print '(a,es25.16e3,1x,i0,3(1x,es25.16e3))', 'ENERGYIB ', E, err, sqrt(sum(grad**2)), grad(1), grad(Glob_npt * K)
 
+  ! method 'G': the reference's ZHEGVX has no tolerance; Glob_EigvalTol
+  ! keeps the value read above and must not make these calls fail
+  Glob_GSEPSolutionMethod = 'G'
+  E = EnergyGA(1, K, .true., err)
+  print '(a,es25.16e3,1x,i0)', 'ENERGYGA ', E, err
+  Glob_c = (0.0_dprec, 0.0_dprec)
+  E = EnergyGAM(1, K, .true., err)
+  print '(a,es25.16e3,1x,i0,1x,es25.16e3)', 'ENERGYGAM ', E, err, snorm(Glob_c)
+  Glob_c = (0.0_dprec, 0.0_dprec)
+  call EnergyGB(E, grad, .true., err)
+  print '(a,es25.16e3,1x,i0,2(1x,es25.16e3))', 'ENERGYGB ', E, err, sqrt(sum(grad**2)), snorm(Glob_c)
+  E = EnergyIAM(1, K, .true., err)
+  print '(a,es25.16e3,1x,i0,2(1x,es25.16e3))', 'ENERGYIAM ', E, err, snorm(Glob_LastEigvector), &
+        maxval(max(abs(real(Glob_c)), abs(aimag(Glob_c))))
+
   call EcgHostFinish()
+
+contains
+
+  ! c^H S c with the normalised S of the 'I' assembly above (both triangles)
+  real(dprec) function snorm(c)
+    complex(dprec), intent(in) :: c(:)
+    snorm = real(dot_product(c(1:K), matmul(Glob_S(1:K, 1:K), c(1:K))))
+  end function
 end program drive_fixture

@@ -47,7 +47,7 @@ def test_fortran_interfaces_cover_every_entry_point():
     bound = sorted(set(re.findall(r'bind\(C, name="(ecg_\w+)"\)', src)))
     assert bound == _declared()
     for routine in ("MatrixElements", "ComputeMatElem", "ComputeMatElemAndDeriv", "GHEPIIS", "EnergyGA",
-                    "EnergyIA", "EnergyGB", "EnergyIB"):
+                    "EnergyIA", "EnergyGAM", "EnergyIAM", "EnergyGB", "EnergyIB"):
         assert re.search(r"^\s*(subroutine|real\(dprec\) function) %s\(" % routine, src, flags=re.M), routine
 
 
