@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -33,6 +34,7 @@ struct ecg_ctx {
   ecg::Ctx c;
   ecg::Factor f;
   hipStream_t stream = nullptr;
+  int dev = 0;                // the HIP device current when the context was created
   int npt = 0;
   double* part = nullptr;
   size_t part_cap = 0;
@@ -766,6 +768,85 @@ int launch_one_pair(const ecg::Ctx& c, const ecg::SysConst& sc, const double* br
   return (int)hipGetLastError();
 }
 
+// One pair on the device (MatrixElements): the two parameter columns, their
+// bra/ket records and the kernel's output are staged in dev (at least
+// ME_STAGE_DOUBLES doubles, 256-byte aligned), which the caller provides.
+constexpr int ME_OFF_PARAMS = 0, ME_OFF_BRA = 64, ME_OFF_KET = 192, ME_OFF_OUT = 384;
+constexpr size_t ME_STAGE_DOUBLES = 640;   // n <= 5: 60 + 92 + 160 + 244 doubles in 256-byte slots
+static_assert(2 * 30 <= ME_OFF_BRA - ME_OFF_PARAMS && 2 * 46 <= ME_OFF_KET - ME_OFF_BRA &&
+                  2 * 80 <= ME_OFF_OUT - ME_OFF_KET && 2 * (2 + 8 * 15) <= (int)ME_STAGE_DOUBLES - ME_OFF_OUT,
+              "staging slots too small for n = 5");
+
+int matrix_elements_run(ecg_ctx* x, const double* paramk, const double* paraml, const int* P, int gradflag,
+                        double* Hkl, double* Skl, double* Dk, double* Dl, double* dev) {
+  const int n = x->c.n, m = n * (n + 1) / 2, npt = x->npt;
+  const bool gk = (gradflag == 1 || gradflag == 3), gl = (gradflag == 2 || gradflag == 3);
+  ecg::SysConst sc = x->c.sc;
+  sc.T = 1;
+  sc.coef[0] = 1.0;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) sc.P[0][i][j] = (signed char)P[i + n * j];
+  // private 2-function basis on the device
+  ecg::Ctx c1;
+  c1.sc = sc;
+  c1.n = n;
+  c1.T = 1;
+  c1.K = 2;
+  c1.params = dev + ME_OFF_PARAMS;
+  c1.bra = dev + ME_OFF_BRA;
+  c1.ket = dev + ME_OFF_KET;
+  double* dout = dev + ME_OFF_OUT;
+  const size_t nout = 2 * (2 + 8 * (size_t)m);
+  double pp[60], hout[2 * (2 + 8 * 15)];
+  std::memcpy(pp, paramk, sizeof(double) * npt);
+  std::memcpy(pp + npt, paraml, sizeof(double) * npt);
+  HIPCHK(hipMemcpyAsync(c1.params, pp, sizeof(double) * 2 * npt, hipMemcpyHostToDevice, x->stream));
+  RCCHK(ecg::prep(c1, x->stream));
+  RCCHK(launch_one_pair(c1, sc, c1.bra, c1.ket, dout, x->stream));
+  HIPCHK(hipMemcpyAsync(hout, dout, sizeof(double) * nout, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  if (!finite_all(hout, 4)) return ECG_ERR_NONFINITE;
+  Hkl[0] = hout[0];
+  Hkl[1] = hout[1];
+  Skl[0] = hout[2];
+  Skl[1] = hout[3];
+  if (gk) std::memcpy(Dk, hout + 4, sizeof(double) * 8 * m);
+  if (gl) std::memcpy(Dl, hout + 4 + 8 * m, sizeof(double) * 8 * m);
+  return ECG_OK;
+}
+
+// Scratch of ecg_matrix_elements_noalloc: the staging of one pair, in a
+// file-scope device array (one copy per GPU, resident until the process
+// ends) so that the entry point allocates nothing.  Host code must not
+// pass `scratch` itself to a kernel or a copy: its device address comes
+// from hipGetSymbolAddress, resolved once per device (with that device
+// current) and kept in scratch_ptr.
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = 1024;   // 8 KiB
+static_assert(ME_STAGE_DOUBLES <= NOALLOC_SCRATCH_DOUBLES, "scratch smaller than one pair's staging");
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+constexpr int MAX_DEVICES = 64;
+double* scratch_ptr[MAX_DEVICES] = {};
+std::mutex scratch_mu;
+
+// Device address of `scratch` on device dev (0 when it cannot be resolved).
+double* scratch_on(int dev) {
+  if (dev < 0 || dev >= MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (scratch_ptr[dev]) return scratch_ptr[dev];
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess) return nullptr;
+  if (cur != dev && hipSetDevice(dev) != hipSuccess) return nullptr;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (cur != dev) (void)hipSetDevice(cur);
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  scratch_ptr[dev] = static_cast<double*>(p);
+  return scratch_ptr[dev];
+}
+
 }  // namespace
 
 // EnergyGB's contraction on the device (src/workproc.f90:1925-1940), for the
@@ -839,6 +920,14 @@ int ecg_ctx_create(int n, int T, const int* yhy_mats, const int* yhy_coef, const
   x->c.n = n;
   x->c.T = T;
   x->npt = n * (n + 1);
+  if (hipGetDevice(&x->dev) != hipSuccess) {
+    delete x;
+    return ECG_ERR_HIP;
+  }
+  if (!scratch_on(x->dev)) {   // resolve the no-allocation entry's scratch once per device
+    delete x;
+    return ECG_ERR_HIP;
+  }
   if (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess) {
     delete x;
     return ECG_ERR_HIP;
@@ -932,57 +1021,29 @@ int ecg_matrix_elements(ecg_ctx* x, const double* paramk, const double* paraml, 
                         double* Hkl, double* Skl, double* Dk, double* Dl) {
   (void)hipGetLastError();   // start from a clean runtime error state
   if (!x || !paramk || !paraml || !P || !Hkl || !Skl) return ECG_ERR_ARG;
-  const int n = x->c.n, m = n * (n + 1) / 2, npt = x->npt;
   const bool gk = (gradflag == 1 || gradflag == 3), gl = (gradflag == 2 || gradflag == 3);
   if ((gk && !Dk) || (gl && !Dl)) return ECG_ERR_ARG;
-  ecg::SysConst sc = x->c.sc;
-  sc.T = 1;
-  sc.coef[0] = 1.0;
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) sc.P[0][i][j] = (signed char)P[i + n * j];
-  // private 2-function basis on the device
-  ecg::Ctx c1;
-  c1.sc = sc;
-  c1.n = n;
-  c1.T = 1;
-  c1.K = 2;
-  double *dparams = nullptr, *dbra = nullptr, *dket = nullptr, *dout = nullptr;
-  const size_t nout = 2 * (2 + 8 * (size_t)m);
-  int rc = 0;
-  if (hipMalloc((void**)&dparams, sizeof(double) * 2 * npt) != hipSuccess ||
-      hipMalloc((void**)&dbra, sizeof(double) * 2 * ecg::bra_stride(n)) != hipSuccess ||
-      hipMalloc((void**)&dket, sizeof(double) * 2 * ecg::ket_stride(n)) != hipSuccess ||
-      hipMalloc((void**)&dout, sizeof(double) * nout) != hipSuccess)
-    rc = ECG_ERR_NOMEM;
-  std::vector<double> hout(nout);
-  if (!rc) {
-    std::vector<double> pp(2 * npt);
-    std::memcpy(pp.data(), paramk, sizeof(double) * npt);
-    std::memcpy(pp.data() + npt, paraml, sizeof(double) * npt);
-    c1.params = dparams;
-    c1.bra = dbra;
-    c1.ket = dket;
-    if (hipMemcpyAsync(dparams, pp.data(), sizeof(double) * 2 * npt, hipMemcpyHostToDevice, x->stream) != hipSuccess)
-      rc = ECG_ERR_HIP;
-    if (!rc) rc = ecg::prep(c1, x->stream);
-    if (!rc) rc = launch_one_pair(c1, sc, dbra, dket, dout, x->stream);
-    if (!rc && hipMemcpyAsync(hout.data(), dout, sizeof(double) * nout, hipMemcpyDeviceToHost, x->stream) != hipSuccess)
-      rc = ECG_ERR_HIP;
-    if (!rc && hipStreamSynchronize(x->stream) != hipSuccess) rc = ECG_ERR_HIP;
-  }
-  if (dparams) (void)hipFree(dparams);
-  if (dbra) (void)hipFree(dbra);
-  if (dket) (void)hipFree(dket);
-  if (dout) (void)hipFree(dout);
-  if (rc) return rc < 0 ? rc : ECG_ERR_HIP;
-  if (!finite_all(hout.data(), 4)) return ECG_ERR_NONFINITE;
-  Hkl[0] = hout[0];
-  Hkl[1] = hout[1];
-  Skl[0] = hout[2];
-  Skl[1] = hout[3];
-  if (gk) std::memcpy(Dk, hout.data() + 4, sizeof(double) * 8 * m);
-  if (gl) std::memcpy(Dl, hout.data() + 4 + 8 * m, sizeof(double) * 8 * m);
-  return ECG_OK;
+  double* dev = nullptr;
+  if (hipMalloc((void**)&dev, sizeof(double) * ME_STAGE_DOUBLES) != hipSuccess) return ECG_ERR_NOMEM;
+  const int rc = matrix_elements_run(x, paramk, paraml, P, gradflag, Hkl, Skl, Dk, Dl, dev);
+  (void)hipStreamSynchronize(x->stream);
+  (void)hipFree(dev);
+  return rc;
+}
+
+int ecg_matrix_elements_noalloc(ecg_ctx* x, const double* paramk, const double* paraml, const int* P,
+                                int gradflag, double* Hkl, double* Skl, double* Dk, double* Dl) {
+  (void)hipGetLastError();   // start from a clean runtime error state
+  if (!x || !paramk || !paraml || !P || !Hkl || !Skl) return ECG_ERR_ARG;
+  const bool gk = (gradflag == 1 || gradflag == 3), gl = (gradflag == 2 || gradflag == 3);
+  if ((gk && !Dk) || (gl && !Dl)) return ECG_ERR_ARG;
+  // the staging of one pair must fit the scratch (n <= 5 always does)
+  const int n = x->c.n, m = n * (n + 1) / 2;
+  const size_t need = (size_t)ME_OFF_OUT + 2 * (2 + 8 * (size_t)m);
+  if (n < 1 || n > 5 || need > NOALLOC_SCRATCH_DOUBLES) return ECG_ERR_ARG;
+  double* dev = scratch_on(x->dev);   // resolved at context creation: no HIP call here
+  if (!dev) return ECG_ERR_HIP;
+  return matrix_elements_run(x, paramk, paraml, P, gradflag, Hkl, Skl, Dk, Dl, dev);
 }
 
 int ecg_compute_mat_elem(ecg_ctx* x, int nmin, int nmax, char method, double approx_energy, double* H,

@@ -46,6 +46,7 @@ EXPORTS = [
     "ecg_batch_energy_grad", "ecg_trace", "ecg_trace_read", "ecg_get_params", "ecg_screen_prepare", "ecg_screen", "ecg_screen_loop", "ecg_last_timings",
     "ecg_set_overlap_penalty", "ecg_overlap_penalty", "ecg_set_params_device", "ecg_expectation_values",
     "ecg_correlation_functions", "ecg_particle_densities", "ecg_relativistic_corrections", "ecg_eigenvector",
+    "ecg_matrix_elements_noalloc",
 ]
 
 
@@ -75,6 +76,7 @@ def load(path: str = LIB_PATH):
     L.ecg_set_params.argtypes = [vp, c_int, _d]
     L.ecg_update_params.argtypes = [vp, c_int, c_int, _d]
     L.ecg_matrix_elements.argtypes = [vp, _d, _d, _i, c_int, _d, _d, _d, _d]
+    L.ecg_matrix_elements_noalloc.argtypes = [vp, _d, _d, _i, c_int, _d, _d, _d, _d]
     L.ecg_compute_mat_elem.argtypes = [vp, c_int, c_int, c_char, c_double, _d, _d, c_int, _d, _d]
     L.ecg_compute_mat_elem_and_deriv.argtypes = [vp, c_int, c_char, c_double, _d, _d, c_int, _d, _d, _d, c_int]
     L.ecg_ghepiis.argtypes = [vp, c_int, c_int, _d, c_int, _d, _d, c_int, c_double, _d, c_double, c_int, c_int,
@@ -166,8 +168,10 @@ class EcgContext:
         _check("ecg_update_params", load().ecg_update_params(self._h, first, p.shape[0], _p(p)))
 
     # --------------------------------------------------- drop-in routines
-    def MatrixElements(self, paramk, paraml, P, gradflag=0):
-        """(Hkl, Skl, Dk, Dl) of one pair and one symmetry matrix P."""
+    def MatrixElements(self, paramk, paraml, P, gradflag=0, noalloc=False):
+        """(Hkl, Skl, Dk, Dl) of one pair and one symmetry matrix P
+        (noalloc: through ecg_matrix_elements_noalloc, which stages the pair
+        in the library's fixed device scratch instead of allocating)."""
         n = self.n
         pk = np.ascontiguousarray(paramk, dtype=np.float64)
         pl = np.ascontiguousarray(paraml, dtype=np.float64)
@@ -176,7 +180,8 @@ class EcgContext:
         S = np.zeros(1, np.complex128)
         Dk = np.zeros(2 * n * (n + 1), np.complex128)
         Dl = np.zeros(2 * n * (n + 1), np.complex128)
-        _check("ecg_matrix_elements", load().ecg_matrix_elements(
+        name = "ecg_matrix_elements_noalloc" if noalloc else "ecg_matrix_elements"
+        _check(name, getattr(load(), name)(
             self._h, _p(pk), _p(pl), Pf.ctypes.data_as(_i), gradflag, _p(H.view(np.float64)),
             _p(S.view(np.float64)), _p(Dk.view(np.float64)), _p(Dl.view(np.float64))))
         return H[0], S[0], Dk, Dl

@@ -78,6 +78,16 @@ module ecg_capi
       complex(c_double_complex), intent(out) :: Hkl, Skl
       complex(c_double_complex), intent(inout) :: Dk(*), Dl(*)
     end function
+    integer(c_int) function ecg_matrix_elements_noalloc(ctx, paramk, paraml, P, gradflag, Hkl, Skl, Dk, Dl) &
+        bind(C, name="ecg_matrix_elements_noalloc")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: paramk(*), paraml(*)
+      integer(c_int), intent(in) :: P(*)
+      integer(c_int), value :: gradflag
+      complex(c_double_complex), intent(out) :: Hkl, Skl
+      complex(c_double_complex), intent(inout) :: Dk(*), Dl(*)
+    end function
     integer(c_int) function ecg_compute_mat_elem(ctx, nmin, nmax, method, approx_energy, H, S, ld, &
         diagH, diagS) bind(C, name="ecg_compute_mat_elem")
       import :: c_int, c_double, c_double_complex, c_ptr, c_char

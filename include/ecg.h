@@ -80,6 +80,19 @@ int ecg_update_params(ecg_ctx* ctx, int first, int count, const double* params);
 int ecg_matrix_elements(ecg_ctx* ctx, const double* paramk, const double* paraml, const int* P, int gradflag,
                         double* Hkl, double* Skl, double* Dk, double* Dl);
 
+/* ecg_matrix_elements without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture set-up, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results (bit for bit) as
+ * ecg_matrix_elements; the pair's staging (parameters, records, output:
+ * at most 640 doubles for n <= 5) lives in a library-owned 8 KiB device
+ * array per GPU, held until the process ends.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (from several threads,
+ * or contexts on the same device) race and must be serialised by the
+ * caller.  ECG_ERR_ARG if the staging would exceed the array. */
+int ecg_matrix_elements_noalloc(ecg_ctx* ctx, const double* paramk, const double* paraml, const int* P,
+                                int gradflag, double* Hkl, double* Skl, double* Dk, double* Dl);
+
 /* ComputeMatElem(Nmin, Nmax) + StoreHS (src/matform.f90:11-51, 119-230):
  * rows nmin..nmax, all l <= k, summed over the Y^dagger Y terms, written
  * with the reference's normalisation and layout:
