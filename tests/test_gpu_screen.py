@@ -110,3 +110,79 @@ def test_screen_loop_graph_matches_eager():
     assert abs(E3 - ecur) <= 1e-10 * abs(ecur)
     for c in (ctx, ctx2, ctx3):
         c.close()
+
+
+def test_screen_real_shape_k2000():
+    """The benchmark's screening shape: H2-like (n = 3, T = 4), K = 2000 (the
+    benchmark's basis), 256 candidates for function K, 4 inverse iterations
+    from the current eigenvector (the benchmark's setting).
+      * all 256 energies and inertias against the per-candidate path the
+        reference runs, on the device: replace function K
+        (ecg_update_params), refactor from row K and iterate at the same
+        shift (ecg_energy, the tail update of LDLHF from row K);
+      * four candidates against the oracle's per-candidate assembly +
+        GHEPIIS restatement;
+      * three hipGraph-replayed loop steps against the same steps eagerly."""
+    name, K, ncand = "h2", 2000, 256
+    say("ecg_screen real shape", name, K, ncand)
+    sy = named_system(name)
+    params = seeded_basis(sy.n, K, SEED_BASE + 3)
+    ctx = EcgContext(sy)
+    ctx.set_params(params)
+    rc, E0, _, _ = ctx.energy(1, float("nan"), 1e-12, 60)
+    sigma = E0 - 1e-6 * abs(E0)
+    rc, info = ctx.screen_prepare(sigma, 1e-12, 40)
+    rng = np.random.default_rng(SEED_BASE + 4)
+    cand = np.stack([seeded_basis(sy.n, ncand, SEED_BASE + 4 + 1000 * (s + 1)) for s in range(3)])
+    for s in range(3):   # near-copies of the current function, so that steps can improve
+        cand[s, :4] = params[-1] * (1 + 1e-4 * rng.standard_normal((4, sy.npt)))
+    E, inf = ctx.screen(cand[0], iters=4)
+    # the per-candidate tail-update path on a second context
+    say("tail-update path", ncand, "candidates")
+    ctx2 = EcgContext(sy)
+    ctx2.set_params(params)
+    rc, Ec, _, _ = ctx2.energy(1, sigma, 1e-12, 40)
+    assert abs(Ec - info[0]) <= 1e-12 * abs(Ec)
+    worst = 0.0
+    for j in range(ncand):
+        ctx2.update_params(K, cand[0, j:j + 1])
+        rc, Et, _, it = ctx2.energy(1, sigma, 1e-12, 40)
+        assert int(it[7]) == 1   # one row refactored
+        worst = max(worst, abs(E[j] - Et) / abs(Et))
+        assert abs(E[j] - Et) <= 1e-10 * abs(Et), (j, E[j], Et)
+        assert int(inf[j, 1]) == int(it[4]), (j, inf[j, 1], it[4])
+    print("worst relative difference to the tail-update path", worst)
+    ctx2.close()
+    for j in (0, 1, 100, 255):
+        say("oracle candidate", j)
+        err, lam, nneg = _oracle_candidate(sy, params, cand[0, j], sigma)
+        print(j, "E", E[j], "oracle", lam, "diff", E[j] - lam, "inertia", inf[j, 1], nneg, "err", err)
+        assert err in (0, 2)
+        assert abs(E[j] - lam) <= 1e-10 * abs(lam), j
+        assert int(inf[j, 1]) == nneg, j
+    # three graph steps against the eager steps of a fresh session
+    say("graph loop vs eager", 3, "steps")
+    ctx.set_params(params)
+    ctx.screen_prepare(sigma, 1e-12, 40)
+    Eb, ib = ctx.screen_loop(cand, iters=4)
+    ctx3 = EcgContext(sy)
+    ctx3.set_params(params)
+    rc, ecur, _, _ = ctx3.energy(1, float("nan"), 1e-12, 60)
+    rc, info3 = ctx3.screen_prepare(sigma, 1e-12, 40)
+    ecur = info3[0]
+    for s in range(3):
+        Es, _ = ctx3.screen(cand[s], iters=4)
+        j = int(np.argmin(Es))
+        acc = Es[j] < ecur
+        print("step", s, "graph", Eb[s], ib[s], "eager", Es[j], j, acc)
+        assert abs(Eb[s] - Es[j]) <= 1e-12 * abs(Es[j])
+        assert ib[s] == (j if acc else -1 - j)
+        if acc:   # the eager mirror takes the accepted candidate as function K
+            p2 = ctx3.get_params()
+            p2[-1] = cand[s, j]
+            ctx3.set_params(p2)
+            rc, info3 = ctx3.screen_prepare(sigma, 1e-12, 40)
+            ecur = info3[0]
+            assert abs(ecur - Es[j]) <= 1e-10 * abs(ecur)
+    ctx.close()
+    ctx3.close()
