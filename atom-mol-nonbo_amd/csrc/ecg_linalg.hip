@@ -10,6 +10,7 @@
 // block rows alongside the factorisation and kept as X^H in the strict
 // upper triangle, so that a solve is two GEMV passes over that triangle.
 #include "ecg_internal.hpp"
+#include "ecg_schedule.hpp"
 
 namespace ecg {
 
@@ -684,8 +685,10 @@ __global__ void __launch_bounds__(256) k_ldl_xlast(int K, int ld, zd* __restrict
 // the chain of waits ends.  A wait on a higher index has no such
 // guarantee: the waiting workgroups can fill every slot the awaited one
 // needs (a variant that let panel strips wait on sibling strips of higher
-// index ran past its time limit for this reason).  A CPU test decodes the
-// task maps of every launch and checks this rule.
+// index ran past its time limit, most likely for this reason).  The task
+// map lives in ecg_schedule.hpp, and a host-side check
+// (tests/sched_check.cpp) enumerates it for every launch of every basis
+// size and verifies this rule.
 // W is double-buffered by column parity: the tiles read column j's, the
 // strips write column j+1's.
 constexpr int NQ = 4;   // quarters of the look-ahead tile's update
@@ -698,12 +701,17 @@ __global__ void __launch_bounds__(256) k_ldl_update(int K, int ld, int j0, zd* _
   __shared__ DiagLds Ld;
   const int jb = min(NB, K - j0);
   const int t0 = (j0 + jb) / NB;
-  const int b = blockIdx.x;
-  const int nx = (int)gridDim.x - NQ - ntiles - nstrip;   // tiles of L^{-1}
-  if (b < NQ) {
+  UpdShape shape;
+  shape.ntiles = ntiles;
+  shape.nstrip = nstrip;
+  shape.nx = (int)gridDim.x - NQ - ntiles - nstrip;   // tiles of L^{-1}
+  shape.grid = (int)gridDim.x;
+  const UpdTask task = upd_task((int)blockIdx.x, K, j0, shape, qflag);
+  if (task.kind == UPD_QUARTER) {
     // rows 16b..16b+15 of the next diagonal tile: C(r, c) -= sum_p W(r, p)
     // conj(L(c, p)) for c <= r, written back to F; wave w takes columns
     // 16w..16w+15 (w <= b).  All operand fragments are loaded first.
+    const int b = task.a;
     const int T0 = t0 * NB, tb = min(NB, K - T0);
     const int R0 = b * SB, C0 = (threadIdx.x >> 6) * SB;
     if (C0 <= R0 && R0 < tb) {
@@ -735,36 +743,27 @@ __global__ void __launch_bounds__(256) k_ldl_update(int K, int ld, int j0, zd* _
         if (r < tb && c <= r) F[(size_t)(T0 + c) * ld + T0 + r] = mkz(old[v].x - acc.re[v], old[v].y - acc.im[v]);
       }
     }
-    publish_wg(&pflag[qflag + b], epoch);
+    publish_wg(&pflag[task.publish], epoch);
     return;
   }
-  if (b == NQ) {
-    // look-ahead: the next diagonal tile, finished by the four row
-    // quarters, then its factor
-    wait_wg(&pflag[qflag + 0], &pflag[qflag + 1], epoch, status);
-    wait_wg(&pflag[qflag + 2], &pflag[qflag + 3], epoch, status);
-    ldl_diag_block(K, ld, t0 * NB, F, D, Dinv, status, Ld, false, &pflag[0], epoch);
+  if (task.kind == UPD_LOOKAHEAD) {
+    // the next diagonal tile, finished by the four row quarters, then its factor
+    wait_wg(&pflag[task.wait[0]], &pflag[task.wait[1]], epoch, status);
+    wait_wg(&pflag[task.wait[2]], &pflag[task.wait[3]], epoch, status);
+    ldl_diag_block(K, ld, t0 * NB, F, D, Dinv, status, Ld, false, &pflag[task.publish], epoch);
     return;
   }
-  if (b >= NQ + ntiles && b < NQ + ntiles + nx) {
-    const int k = j0 / NB, x = b - NQ - ntiles;
-    const int i = (x < k) ? k : k + 1 + (x - k) / k;
-    const int c = (x < k) ? x : (x - k) % k;
-    xinv_tile(K, ld, k, i, c, F, Dinv, Ld);
+  if (task.kind == UPD_XINV) {
+    xinv_tile(K, ld, j0 / NB, task.a, task.b, F, Dinv, Ld);
     return;
   }
-  if (b >= NQ + ntiles) {
-    const int i0 = (t0 + 1) * NB + (b - NQ - ntiles - nx) * SB;
-    const int bi = (i0 - t0 * NB) / NB;
-    wait_wg(&pflag[0], &pflag[1 + bi], epoch, status);
-    panel_strip(K, ld, t0 * NB, i0, F, D, Dinv, Wnext);
+  if (task.kind == UPD_STRIP) {
+    wait_wg(&pflag[task.wait[0]], &pflag[task.wait[1]], epoch, status);
+    panel_strip(K, ld, t0 * NB, task.a, F, D, Dinv, Wnext);
     return;
   }
-  const int tb = b - NQ;   // lower-triangle tile index; 0 (the diagonal tile) is the look-ahead's
-  int bi = (int)((sqrt(8.0 * tb + 1.0) - 1.0) * 0.5);
-  while ((bi + 1) * (bi + 2) / 2 <= tb) ++bi;
-  while (bi * (bi + 1) / 2 > tb) --bi;
-  const int bk = tb - bi * (bi + 1) / 2;
+  // UPD_TILE: lower block pair (bi, bk) of the trailing matrix
+  const int bi = task.a, bk = task.b;
   const int I0 = (t0 + bi) * NB, K0 = (t0 + bk) * NB;
   const int ib = min(NB, K - I0), kb = min(NB, K - K0);
   Tile64 t;
@@ -783,7 +782,7 @@ __global__ void __launch_bounds__(256) k_ldl_update(int K, int ld, int j0, zd* _
           *o = mkz(old.x - t.re[a][q][v], old.y - t.im[a][q][v]);
         }
       }
-  if (bk == 0) publish_wg(&pflag[1 + bi], epoch);
+  if (task.publish >= 0) publish_wg(&pflag[task.publish], epoch);
 }
 
 __global__ void k_clear_status(int* status) {
@@ -802,20 +801,15 @@ int ldlh_factor(Factor& f, hipStream_t s) {
                        f.W);
     if (f.tr) f.tr->end(s);
   }
+  static_assert(NB == SCHED_NB && SB == SCHED_SB && NQ == UPD_NQ, "task map and kernel disagree");
   for (int j0 = 0, col = 0; j0 + NB < K; j0 += NB, ++col) {
-    const int rest = K - (j0 + NB);   // rows below block column col
-    const int nrb = (rest + NB - 1) / NB;
-    const int ntiles = nrb * (nrb + 1) / 2;
-    const int rest2 = rest - NB;      // rows below the next block column
-    const int nstrip = rest2 > 0 ? (rest2 + SB - 1) / SB : 0;
+    const UpdShape shape = upd_shape(K, col);
     zd* Wcur = f.W + (size_t)(col & 1) * f.ld * NB;
     zd* Wnext = f.W + (size_t)((col + 1) & 1) * f.ld * NB;
-    const int nb = (K + NB - 1) / NB;
-    const int nx = (col >= 1) ? col + (nb - 1 - col) * col : 0;   // tiles of X = L^{-1}
     const int epoch = ++f.epoch;
     if (f.tr) f.tr->begin(TR_LDL_UPDATE, s);
-    hipLaunchKernelGGL(k_ldl_update, dim3(NQ + ntiles + nstrip + nx), dim3(256), 0, s, K, f.ld, j0, f.F, Wcur,
-                       Wnext, f.D, f.Dinv, f.status, f.pflag, f.ld / NB + 1, epoch, ntiles, nstrip);
+    hipLaunchKernelGGL(k_ldl_update, dim3(shape.grid), dim3(256), 0, s, K, f.ld, j0, f.F, Wcur, Wnext, f.D, f.Dinv,
+                       f.status, f.pflag, f.ld / NB + 1, epoch, shape.ntiles, shape.nstrip);
     if (f.tr) f.tr->end(s);
   }
   if (K > NB) hipLaunchKernelGGL(k_ldl_xlast, dim3((K + NB - 1) / NB - 1), dim3(256), 0, s, K, f.ld, f.F, f.Dinv);

@@ -128,10 +128,12 @@ def test_c5_batch_32_problems():
         for i, (Er, gr, nd) in enumerate(ref):
             # the batch starts inverse iteration from another vector than the
             # single run; where the iteration stops at its rounding floor
-            # (measure nd), the vectors agree to about nd
-            fl = max(1e-9, 100.0 * nd)
+            # (measure nd, up to ~4e-10 on the H- bases with kappa(S) ~ 1e12)
+            # the vectors agree to about nd, and the gradient, a near-stationary
+            # sum with heavy cancellation, to a few hundred times that
+            fl = max(1e-9, 1000.0 * nd)
             assert abs(E[i] - Er) <= max(1e-11, fl * fl) * max(1.0, abs(Er)), (i, E[i], Er)
-            np.testing.assert_allclose(grads[i], gr, rtol=10 * fl, atol=fl * np.max(np.abs(gr)))
+            np.testing.assert_allclose(grads[i], gr, rtol=fl, atol=fl * np.max(np.abs(gr)))
         assert np.all(lat > 0)
     b.close()
     for c in ctxs:
