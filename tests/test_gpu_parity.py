@@ -41,6 +41,35 @@ def test_matrix_elements_single_pair(name):
     ctx.close()
 
 
+@pytest.mark.parametrize("name", ["hminus", "h2", "li"])
+def test_matrix_elements_against_reference(name):
+    """The device's pair elements against the reference's own MatrixElements
+    (src/matelem.f90:12-750, compiled into oracle/_ref), every term, all
+    four blocks, to each pair's rounding sensitivity."""
+    from oracle import ref
+    if not ref.available():
+        pytest.skip("oracle/_ref not built (reference tree absent where the tree was built)")
+    sy = named_system(name)
+    ref.setup(sy)
+    B = seeded_basis(sy.n, 8, SEED_BASE + 6500)
+    say("ecg_matrix_elements vs reference MatrixElements", name)
+    ctx = EcgContext(sy)
+    worst = 0.0
+    for t in range(sy.T):
+        P = sy.yhy_mats[t]
+        for i in range(8):
+            for j in range(i + 1):
+                _, sens, mag = pair_sensitivity(sy, B[i], B[j], 3, term=t, trials=4)
+                got = ctx.MatrixElements(B[i], B[j], P, 3)
+                want = ref.matrix_elements(sy, B[i], B[j], P, 3)
+                for q in range(4):
+                    ok, d, tol = close(got[q], want[q], mag[q], sens[q])
+                    assert ok, (name, t, i, j, q, d, tol)
+                    worst = max(worst, d / tol)
+    print("worst deviation / tolerance", worst)
+    ctx.close()
+
+
 def test_matrix_elements_gradflag_subsets():
     sy = named_system("h2")
     B = seeded_basis(sy.n, 4, SEED_BASE + 7)
