@@ -9,6 +9,19 @@
 
 namespace ecg {
 
+// p[i] if ok, else zero, with an unconditional load (of p[0], always valid,
+// when !ok) and a select of the value.  Written as `ok ? p[i] : 0` the load
+// becomes a branch, and the compiler waits for each such load at the join,
+// so a run of them costs one memory latency each instead of one in total.
+__device__ inline zd ld_or0(const zd* p, size_t i, bool ok) {
+  const zd v = p[ok ? i : 0];
+  return mkz(ok ? v.x : 0.0, ok ? v.y : 0.0);
+}
+__device__ inline double ld_or0(const double* p, size_t i, bool ok) {
+  const double v = p[ok ? i : 0];
+  return ok ? v : 0.0;
+}
+
 constexpr int TILE = 16;   // pair tile edge (TILE x TILE pairs per workgroup)
 constexpr int NB = 64;     // block size of the LDL^H factorisation
 

@@ -266,7 +266,7 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
 #pragma unroll
     for (int u = 0; u < NB * NB / 256; ++u) {
       const int e = tid + 256 * u, j = e >> 6, i = e & (NB - 1);
-      v[u] = (i < jb && j < jb && i >= j) ? F[(size_t)(j0 + j) * ld + j0 + i] : mkz(0.0, 0.0);
+      v[u] = ld_or0(F, (size_t)(j0 + j) * ld + j0 + i, i < jb && j < jb && i >= j);
     }
 #pragma unroll
     for (int u = 0; u < NB * NB / 256; ++u) {
@@ -489,9 +489,9 @@ __device__ inline void mfma_tile_abh(const zd* __restrict__ Ap, int lda, int ra,
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
       const int r = rt + a * 16 + lr;
-      fa[s][a] = (p < kd && r < ra) ? Ap[(size_t)p * lda + r] : mkz(0.0, 0.0);
+      fa[s][a] = ld_or0(Ap, (size_t)p * lda + r, p < kd && r < ra);
       const int c = ct + a * 16 + lr;
-      fb[s][a] = (p < kd && c < rb) ? Bp[(size_t)p * ldb + c] : mkz(0.0, 0.0);
+      fb[s][a] = ld_or0(Bp, (size_t)p * ldb + c, p < kd && c < rb);
     }
   }
 #pragma unroll
@@ -543,7 +543,7 @@ __device__ __attribute__((always_inline)) inline void panel_strip(int K, int ld,
 #pragma unroll
   for (int t = 0; t < NB / 4; ++t) {
     const int kk = 4 * t + lk;
-    av[t] = (i0 + lr < K) ? A[(size_t)kk * ld + lr] : mkz(0.0, 0.0);
+    av[t] = ld_or0(A, (size_t)kk * ld + lr, i0 + lr < K);
     bv[t] = zconj(X[(size_t)kk * NB + c0 + lr]);
   }
   Acc16 acc;
@@ -555,12 +555,15 @@ __device__ __attribute__((always_inline)) inline void panel_strip(int K, int ld,
     acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t].x, bv[t].y, acc.im, 0, 0, 0);
     acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t].y, bv[t].x, acc.im, 0, 0, 0);
   }
+  // the panel's block is a full one (a strip exists only below it), so
+  // D[j0 + c] is in range for every lane
+  const double rdc = 1.0 / D[j0 + c0 + acc16_col()];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int r = i0 + acc16_row(v), c = c0 + acc16_col();
     if (r < K) {
       const zd w = mkz(acc.re[v], acc.im[v]);
-      const zd l = w * (1.0 / D[j0 + c]);
+      const zd l = w * rdc;
       W[(size_t)c * ld + r] = w;
       F[(size_t)(j0 + c) * ld + r] = l;
     }
@@ -598,6 +601,19 @@ __device__ __attribute__((always_inline)) inline void xinv_tile(int K, int ld, i
   const int ldt = first ? NB : ld;
   const zd* Lb = F + (size_t)(k - 1) * NB * ld + i * NB;   // L_{i,k-1}(s, p) at Lb[p * ld + s]
   zd* out = F + (size_t)i * NB * ld + c * NB;               // block (c, i): (r, s) at out[s * ld + r]
+  // the partial sums R^H this product updates, loaded up front (they are
+  // zero on the first update)
+  zd old[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        int r, sc;
+        tile_pos(a, q, v, r, sc);
+        old[a][q][v] = ld_or0(out, (size_t)sc * ld + r, !first && sc < ib);
+      }
   Tile64 t;
   mfma_tile_abh(T, ldt, NB, Lb, ld, ib, NB, t);
   if (i > k) {
@@ -609,11 +625,8 @@ __device__ __attribute__((always_inline)) inline void xinv_tile(int K, int ld, i
         for (int v = 0; v < 4; ++v) {
           int r, sc;
           tile_pos(a, q, v, r, sc);
-          if (sc < ib) {
-            zd* o = out + (size_t)sc * ld + r;
-            const zd old = first ? mkz(0.0, 0.0) : *o;
-            *o = mkz(old.x - t.re[a][q][v], old.y - t.im[a][q][v]);
-          }
+          if (sc < ib)
+            out[(size_t)sc * ld + r] = mkz(old[a][q][v].x - t.re[a][q][v], old[a][q][v].y - t.im[a][q][v]);
         }
     return;
   }
@@ -627,12 +640,8 @@ __device__ __attribute__((always_inline)) inline void xinv_tile(int K, int ld, i
       for (int v = 0; v < 4; ++v) {
         int r, sc;
         tile_pos(a, q, v, r, sc);
-        zd m = mkz(0.0, 0.0);
-        if (sc < ib) {
-          const zd old = first ? mkz(0.0, 0.0) : out[(size_t)sc * ld + r];
-          m = mkz(old.x - t.re[a][q][v], old.y - t.im[a][q][v]);
-        }
-        Ld.A[sc][r] = m;
+        const bool in = sc < ib;
+        Ld.A[sc][r] = mkz(in ? old[a][q][v].x - t.re[a][q][v] : 0.0, in ? old[a][q][v].y - t.im[a][q][v] : 0.0);
       }
   __syncthreads();
   const zd* Xkk = Dinv + (size_t)k * 2 * NB * NB;   // X_kk(s, p) at Xkk[p * NB + s]
@@ -720,13 +729,13 @@ __global__ void __launch_bounds__(256) k_ldl_update(int K, int ld, int j0, zd* _
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int r = R0 + acc16_row(v), c = C0 + acc16_col();
-        old[v] = (r < tb && c <= r) ? F[(size_t)(T0 + c) * ld + T0 + r] : mkz(0.0, 0.0);
+        old[v] = ld_or0(F, (size_t)(T0 + c) * ld + T0 + r, r < tb && c <= r);
       }
 #pragma unroll
       for (int st = 0; st < NB / 4; ++st) {
         const int p = 4 * st + lk;
-        av[st] = (R0 + lr < tb && p < jb) ? Wcur[(size_t)p * ld + T0 + R0 + lr] : mkz(0.0, 0.0);
-        bv[st] = (C0 + lr < tb && p < jb) ? zconj(F[(size_t)(j0 + p) * ld + T0 + C0 + lr]) : mkz(0.0, 0.0);
+        av[st] = ld_or0(Wcur, (size_t)p * ld + T0 + R0 + lr, R0 + lr < tb && p < jb);
+        bv[st] = zconj(ld_or0(F, (size_t)(j0 + p) * ld + T0 + C0 + lr, C0 + lr < tb && p < jb));
       }
       Acc16 acc;
       acc16_zero(acc);
@@ -766,6 +775,18 @@ __global__ void __launch_bounds__(256) k_ldl_update(int K, int ld, int j0, zd* _
   const int bi = task.a, bk = task.b;
   const int I0 = (t0 + bi) * NB, K0 = (t0 + bk) * NB;
   const int ib = min(NB, K - I0), kb = min(NB, K - K0);
+  // the tile's old values, loaded up front (in flight with the operands)
+  zd old[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        int r, c;
+        tile_pos(a, q, v, r, c);
+        old[a][q][v] = ld_or0(F, (size_t)(K0 + c) * ld + I0 + r, r < ib && c < kb && (I0 + r) >= (K0 + c));
+      }
   Tile64 t;
   mfma_tile_abh(Wcur + I0, ld, ib, F + (size_t)j0 * ld + K0, ld, kb, jb, t);
 #pragma unroll
@@ -776,11 +797,8 @@ __global__ void __launch_bounds__(256) k_ldl_update(int K, int ld, int j0, zd* _
       for (int v = 0; v < 4; ++v) {
         int r, c;
         tile_pos(a, q, v, r, c);
-        if (r < ib && c < kb && (I0 + r) >= (K0 + c)) {
-          zd* o = F + (size_t)(K0 + c) * ld + I0 + r;
-          const zd old = *o;
-          *o = mkz(old.x - t.re[a][q][v], old.y - t.im[a][q][v]);
-        }
+        if (r < ib && c < kb && (I0 + r) >= (K0 + c))
+          F[(size_t)(K0 + c) * ld + I0 + r] = mkz(old[a][q][v].x - t.re[a][q][v], old[a][q][v].y - t.im[a][q][v]);
       }
   if (task.publish >= 0) publish_wg(&pflag[task.publish], epoch);
 }
@@ -1310,7 +1328,7 @@ __global__ void __launch_bounds__(1024) k_invit_finish_red(int K, int ld, int nc
       for (int ch0 = (j / 64) / 2; ch0 < nch; ch0 += 8) {
         zd t[8];
 #pragma unroll
-        for (int w = 0; w < 8; ++w) t[w] = (ch0 + w < nch) ? part[(size_t)(ch0 + w) * ld + j] : mkz(0.0, 0.0);
+        for (int w = 0; w < 8; ++w) t[w] = ld_or0(part, (size_t)(ch0 + w) * ld + j, ch0 + w < nch);
 #pragma unroll
         for (int w = 0; w < 8; ++w)
           if (ch0 + w < nch) a = a + t[w];
