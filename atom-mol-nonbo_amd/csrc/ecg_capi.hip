@@ -16,6 +16,7 @@
 
 #include "../../include/ecg.h"
 #include "ecg_internal.hpp"
+#include "ecg_schedule.hpp"
 
 using ecg::zd;
 
@@ -202,7 +203,7 @@ int ensure_capacity(ecg_ctx* x, int K) {
   if (!rc) rc = dev_alloc(&f.status, (size_t)4);
   if (!rc) rc = dev_alloc(&f.vec, (size_t)4 * ld);
   if (!rc) rc = dev_alloc(&f.tvec, (size_t)ld);
-  if (!rc) rc = dev_alloc(&f.pflag, (size_t)(ld / ecg::NB) + 5);
+  if (!rc) rc = dev_alloc(&f.pflag, (size_t)ecg::sched_flags(ld));
   if (!rc) rc = dev_alloc(&f.tp, (size_t)0);   // tail-update partials: allocated on first use
   if (!rc) rc = dev_alloc(&f.scal, (size_t)16);
   for (int i = 0; i < 3 && !rc; ++i) rc = dev_alloc(&x->ev[i], (size_t)ld);
@@ -215,7 +216,7 @@ int ensure_capacity(ecg_ctx* x, int K) {
   }
   f.tp_cap = 0;
   f.epoch = 0;
-  bool ok = hipMemsetAsync(f.pflag, 0, sizeof(int) * ((ld / ecg::NB) + 5), x->stream) == hipSuccess &&
+  bool ok = hipMemsetAsync(f.pflag, 0, sizeof(int) * ecg::sched_flags(ld), x->stream) == hipSuccess &&
             hipMemsetAsync(f.status, 0, sizeof(int) * 4, x->stream) == hipSuccess &&
             hipMemsetAsync(c.H, 0, sizeof(zd) * ld * ld, x->stream) == hipSuccess &&
             hipMemsetAsync(c.S, 0, sizeof(zd) * ld * ld, x->stream) == hipSuccess;

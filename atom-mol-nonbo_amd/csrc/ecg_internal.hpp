@@ -154,8 +154,7 @@ struct Factor {
   zd* vec = nullptr;    // 4 x ld work vectors
   zd* tvec = nullptr;   // ld: D^{-1} L^{-1} b between the two solve passes
   double* scal = nullptr;    // 16 scalars
-  int* pflag = nullptr;      // ld/NB + 5 ready flags of an update launch: [0] diagonal block,
-                             // [1 + i] column tile i, [ld/NB + 1 + q] quarter q of the diagonal tile
+  int* pflag = nullptr;      // sched_flags(ld) ready flags of a factorisation launch (ecg_schedule.hpp)
   int epoch = 0;             // flag value of the current factorisation launch
   zd* tp = nullptr;          // chunk partial sums of the tail update
   size_t tp_cap = 0;         // entries of tp

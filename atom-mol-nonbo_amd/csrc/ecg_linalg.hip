@@ -527,13 +527,14 @@ __device__ inline void tile_pos(int a, int q, int v, int& r, int& c) {
 // ------------------------------------------------------------ panel solve
 // 16-row strip i0..i0+15 below the diagonal block at j0: W = A X^H (so
 // that W L_JJ^H = A), L = W D^{-1}; L goes to F, W to the panel scratch
-// used by the trailing update.  Wave w
-// computes columns 16w..16w+15 as one 16 x 16 MFMA tile: all 16 k-steps'
-// operand fragments are loaded first (one memory latency), then the 64
-// MFMAs.
+// used by the trailing update (and, for a diagonal strip, to rows 0..15 of
+// wl->A).  Wave w computes columns 16w..16w+15 as one 16 x 16 MFMA tile:
+// all 16 k-steps' operand fragments are loaded first (one memory latency),
+// then the 64 MFMAs.
 __device__ __attribute__((always_inline)) inline void panel_strip(int K, int ld, int j0, int i0, zd* __restrict__ F,
                                                                   const double* __restrict__ D,
-                                                                  const zd* __restrict__ Dinv, zd* __restrict__ W) {
+                                                                  const zd* __restrict__ Dinv, zd* __restrict__ W,
+                                                                  DiagLds* wl = nullptr) {
   const int wave = threadIdx.x >> 6;
   const int c0 = SB * wave;
   const zd* X = Dinv + (size_t)(j0 / NB) * 2 * NB * NB;   // X(i, j) at X[j * NB + i]
@@ -561,8 +562,9 @@ __device__ __attribute__((always_inline)) inline void panel_strip(int K, int ld,
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int r = i0 + acc16_row(v), c = c0 + acc16_col();
+    const zd w = mkz(acc.re[v], acc.im[v]);   // zero in rows r >= K
+    if (wl) wl->A[acc16_row(v)][c] = w;
     if (r < K) {
-      const zd w = mkz(acc.re[v], acc.im[v]);
       const zd l = w * rdc;
       W[(size_t)c * ld + r] = w;
       F[(size_t)(j0 + c) * ld + r] = l;
@@ -570,12 +572,89 @@ __device__ __attribute__((always_inline)) inline void panel_strip(int K, int ld,
   }
 }
 
+// Diagonal strip q's second part: rows 16q..16q+15 of the diagonal tile of
+// the block at T1 = j0 + 64 (the rows of this strip), updated by the panel
+// at j0:  A(T1 + R, T1 + C) -= sum_p W(T1 + R, p) conj(L(T1 + C, p)),
+// C <= R.  W of the strip's rows is in rows 0..15 of Ld.A; L of rows
+// T1 + 16 w.. (strip w <= q, published) is read from F.  Wave w takes
+// columns 16w..16w+15; two accumulator chains (even and odd k-steps) halve
+// the dependent MFMA latency.
+__device__ __attribute__((always_inline)) inline void diag_quarter_update(int K, int ld, int j0, int q,
+                                                                          zd* __restrict__ F, const DiagLds& Ld) {
+  const int wave = threadIdx.x >> 6;
+  if (wave > q) return;   // those columns lie above the quarter's rows
+  const int T1 = j0 + NB;
+  const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+  const int R0 = SB * q, C0 = SB * wave;
+  zd old[4], av[NB / 4], bv[NB / 4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int rr = R0 + acc16_row(v), c = C0 + acc16_col();
+    old[v] = ld_or0(F, (size_t)(T1 + c) * ld + T1 + rr, T1 + rr < K && c <= rr);
+  }
+#pragma unroll
+  for (int st = 0; st < NB / 4; ++st) {
+    const int pp = 4 * st + lk;
+    av[st] = Ld.A[lr][pp];
+    bv[st] = zconj(ld_or0(F, (size_t)(j0 + pp) * ld + T1 + C0 + lr, T1 + C0 + lr < K));
+  }
+  Acc16 a0, a1;
+  acc16_zero(a0);
+  acc16_zero(a1);
+#pragma unroll
+  for (int st = 0; st < NB / 4; st += 2) {
+    a0.re = __builtin_amdgcn_mfma_f64_16x16x4f64(av[st].x, bv[st].x, a0.re, 0, 0, 0);
+    a0.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[st].y, bv[st].y, a0.re, 0, 0, 0);
+    a0.im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[st].x, bv[st].y, a0.im, 0, 0, 0);
+    a0.im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[st].y, bv[st].x, a0.im, 0, 0, 0);
+    a1.re = __builtin_amdgcn_mfma_f64_16x16x4f64(av[st + 1].x, bv[st + 1].x, a1.re, 0, 0, 0);
+    a1.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[st + 1].y, bv[st + 1].y, a1.re, 0, 0, 0);
+    a1.im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[st + 1].x, bv[st + 1].y, a1.im, 0, 0, 0);
+    a1.im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[st + 1].y, bv[st + 1].x, a1.im, 0, 0, 0);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int rr = R0 + acc16_row(v), c = C0 + acc16_col();
+    if (T1 + rr < K && c <= rr)
+      F[(size_t)(T1 + c) * ld + T1 + rr] = mkz(old[v].x - (a0.re[v] + a1.re[v]), old[v].y - (a0.im[v] + a1.im[v]));
+  }
+}
+
+// Waits on the flags t.wait[from..] (two per poll).
+__device__ inline void wait_list(const int* pflag, const int* wait, int from, int epoch, int* status) {
+  for (int i = from; i < UPD_WAITS && wait[i] >= 0; i += 2) {
+    const int f2 = (i + 1 < UPD_WAITS && wait[i + 1] >= 0) ? wait[i + 1] : wait[i];
+    wait_wg(&pflag[wait[i]], &pflag[f2], epoch, status);
+  }
+}
+
+// A strip of a panel (panel_task / upd_task): L and W of its rows, and for
+// a diagonal strip, once the flags it waits on are seen, its quarter of
+// the next diagonal tile.
+__device__ __attribute__((always_inline)) inline void strip_task(int K, int ld, int j0, const UpdTask& t,
+                                                                 int first_wait, zd* __restrict__ F,
+                                                                 const double* __restrict__ D,
+                                                                 const zd* __restrict__ Dinv, zd* __restrict__ W,
+                                                                 int* __restrict__ pflag, int epoch,
+                                                                 int* __restrict__ status, DiagLds& Ld) {
+  const bool dg = t.diag >= 0;
+  panel_strip(K, ld, j0, t.a, F, D, Dinv, W, dg ? &Ld : nullptr);
+  if (!dg) return;
+  publish_wg(&pflag[t.publish], epoch);   // also orders the LDS copy of W
+  wait_list(pflag, t.wait, first_wait, epoch, status);
+  diag_quarter_update(K, ld, j0, t.diag, F, Ld);
+}
+
 // Panel of the first block column (later panels are computed inside the
-// trailing update of the previous column).  Needs a full 64-wide block.
-__global__ void __launch_bounds__(256) k_ldl_panel(int K, int ld, int j0, zd* __restrict__ F,
-                                                   const double* __restrict__ D,
-                                                   const zd* __restrict__ Dinv, zd* __restrict__ W) {
-  panel_strip(K, ld, j0, j0 + NB + blockIdx.x * SB, F, D, Dinv, W);
+// trailing update of the previous column), its diagonal strips also
+// updating block 1's diagonal tile.  Needs a full 64-wide block.
+__global__ void __launch_bounds__(256) k_ldl_panel(int K, int ld, zd* __restrict__ F, const double* __restrict__ D,
+                                                   const zd* __restrict__ Dinv, zd* __restrict__ W,
+                                                   int* __restrict__ status, int* __restrict__ pflag, int qflag,
+                                                   int epoch) {
+  __shared__ DiagLds Ld;
+  const UpdTask t = panel_task((int)blockIdx.x, qflag);
+  strip_task(K, ld, 0, t, 0, F, D, Dinv, W, pflag, epoch, status, Ld);
 }
 
 // ------------------------------------------------- inverse of L by blocks
@@ -666,21 +745,22 @@ __global__ void __launch_bounds__(256) k_ldl_xlast(int K, int ld, zd* __restrict
 }
 
 // -------------------------------------------------------- trailing update
-// One launch per block column j (diagonal block at j0, trailing blocks
-// from t0 = j + 1):
-//   workgroups 0..3: the next diagonal tile's update by 16-row quarters
-//     (C_TT -= W_T L_T^H, rows 16q..16q+15, lower part), written to F
-//     and published each on its own flag.
-//   workgroup 4 (look-ahead): waits for the quarters and factors the
-//     tile.
-//   workgroups 5..ntiles+3: the other lower block pairs (I >= Kb) of the
-//     trailing matrix, C_IK -= W_I L_K^H.
-//   then block tiles of X = L^{-1} (xinv_tile): the col tiles of block
-//     row col, then rows col+1.. (no waits).  They come before the strips
-//     so that they are dispatched while the strips would only wait.
+// One launch per block column j (diagonal block at j0, next block t0 =
+// j + 1); the task map is in ecg_schedule.hpp:
+//   workgroup 0 (look-ahead): factors block t0, whose diagonal tile the
+//     previous launch's diagonal strips completed.
+//   then the other lower block pairs (I >= K) of the trailing matrix,
+//     C_IK -= W_I L_K^H; column tiles (bi, 0) and tile (1, 1) publish.
+//   then block tiles of X = L^{-1} (xinv_tile; no waits).  They come
+//     before the strips so that they are dispatched while the strips would
+//     only wait.
 //   last: 16-row strips of the next block column's panel.  Each waits
 //     until the tile holding its rows and the look-ahead factor are
-//     published, then runs the panel solve for its strip.
+//     published, then runs the panel solve for its strip; the four strips
+//     of block t0 + 1 then wait for tile (1, 1) and the lower ones' rows of
+//     L and update their quarter of block t0 + 1's diagonal tile with this
+//     panel (diag_quarter_update), so that the next launch's look-ahead
+//     starts at once.
 // Publication: the producer's threads fence at agent scope, the workgroup
 // synchronises, thread 0 stores the flag (release).  A waiting workgroup's
 // thread 0 polls the flags (relaxed, bounded in time: past the limit it
@@ -700,8 +780,6 @@ __global__ void __launch_bounds__(256) k_ldl_xlast(int K, int ld, zd* __restrict
 // size and verifies this rule.
 // W is double-buffered by column parity: the tiles read column j's, the
 // strips write column j+1's.
-constexpr int NQ = 4;   // quarters of the look-ahead tile's update
-
 __global__ void __launch_bounds__(256) k_ldl_update(int K, int ld, int j0, zd* __restrict__ F,
                                                     const zd* __restrict__ Wcur, zd* __restrict__ Wnext,
                                                     double* __restrict__ D, zd* __restrict__ Dinv,
@@ -713,52 +791,10 @@ __global__ void __launch_bounds__(256) k_ldl_update(int K, int ld, int j0, zd* _
   UpdShape shape;
   shape.ntiles = ntiles;
   shape.nstrip = nstrip;
-  shape.nx = (int)gridDim.x - NQ - ntiles - nstrip;   // tiles of L^{-1}
+  shape.nx = (int)gridDim.x - 1 - ntiles - nstrip;   // tiles of L^{-1}
   shape.grid = (int)gridDim.x;
   const UpdTask task = upd_task((int)blockIdx.x, K, j0, shape, qflag);
-  if (task.kind == UPD_QUARTER) {
-    // rows 16b..16b+15 of the next diagonal tile: C(r, c) -= sum_p W(r, p)
-    // conj(L(c, p)) for c <= r, written back to F; wave w takes columns
-    // 16w..16w+15 (w <= b).  All operand fragments are loaded first.
-    const int b = task.a;
-    const int T0 = t0 * NB, tb = min(NB, K - T0);
-    const int R0 = b * SB, C0 = (threadIdx.x >> 6) * SB;
-    if (C0 <= R0 && R0 < tb) {
-      const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
-      zd av[NB / 4], bv[NB / 4], old[4];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int r = R0 + acc16_row(v), c = C0 + acc16_col();
-        old[v] = ld_or0(F, (size_t)(T0 + c) * ld + T0 + r, r < tb && c <= r);
-      }
-#pragma unroll
-      for (int st = 0; st < NB / 4; ++st) {
-        const int p = 4 * st + lk;
-        av[st] = ld_or0(Wcur, (size_t)p * ld + T0 + R0 + lr, R0 + lr < tb && p < jb);
-        bv[st] = zconj(ld_or0(F, (size_t)(j0 + p) * ld + T0 + C0 + lr, C0 + lr < tb && p < jb));
-      }
-      Acc16 acc;
-      acc16_zero(acc);
-#pragma unroll
-      for (int st = 0; st < NB / 4; ++st) {
-        acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(av[st].x, bv[st].x, acc.re, 0, 0, 0);
-        acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[st].y, bv[st].y, acc.re, 0, 0, 0);
-        acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[st].x, bv[st].y, acc.im, 0, 0, 0);
-        acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[st].y, bv[st].x, acc.im, 0, 0, 0);
-      }
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int r = R0 + acc16_row(v), c = C0 + acc16_col();
-        if (r < tb && c <= r) F[(size_t)(T0 + c) * ld + T0 + r] = mkz(old[v].x - acc.re[v], old[v].y - acc.im[v]);
-      }
-    }
-    publish_wg(&pflag[task.publish], epoch);
-    return;
-  }
   if (task.kind == UPD_LOOKAHEAD) {
-    // the next diagonal tile, finished by the four row quarters, then its factor
-    wait_wg(&pflag[task.wait[0]], &pflag[task.wait[1]], epoch, status);
-    wait_wg(&pflag[task.wait[2]], &pflag[task.wait[3]], epoch, status);
     ldl_diag_block(K, ld, t0 * NB, F, D, Dinv, status, Ld, false, &pflag[task.publish], epoch);
     return;
   }
@@ -768,7 +804,7 @@ __global__ void __launch_bounds__(256) k_ldl_update(int K, int ld, int j0, zd* _
   }
   if (task.kind == UPD_STRIP) {
     wait_wg(&pflag[task.wait[0]], &pflag[task.wait[1]], epoch, status);
-    panel_strip(K, ld, t0 * NB, task.a, F, D, Dinv, Wnext);
+    strip_task(K, ld, t0 * NB, task, 2, F, D, Dinv, Wnext, pflag, epoch, status, Ld);
     return;
   }
   // UPD_TILE: lower block pair (bi, bk) of the trailing matrix
@@ -809,17 +845,19 @@ __global__ void k_clear_status(int* status) {
 
 int ldlh_factor(Factor& f, hipStream_t s) {
   const int K = f.K;
+  const int qflag = f.ld / NB + 1;
   hipLaunchKernelGGL(k_clear_status, dim3(1), dim3(64), 0, s, f.status);
   if (f.tr) f.tr->begin(TR_LDL_DIAG, s);
   hipLaunchKernelGGL(k_ldl_diag, dim3(1), dim3(256), 0, s, K, f.ld, 0, f.F, f.D, f.Dinv, f.status);
   if (f.tr) f.tr->end(s);
+  static_assert(NB == SCHED_NB && SB == SCHED_SB, "task map and kernel disagree");
   if (K > NB) {
+    const int epoch = ++f.epoch;
     if (f.tr) f.tr->begin(TR_LDL_PANEL, s);
-    hipLaunchKernelGGL(k_ldl_panel, dim3((K - NB + SB - 1) / SB), dim3(256), 0, s, K, f.ld, 0, f.F, f.D, f.Dinv,
-                       f.W);
+    hipLaunchKernelGGL(k_ldl_panel, dim3((K - NB + SB - 1) / SB), dim3(256), 0, s, K, f.ld, f.F, f.D, f.Dinv, f.W,
+                       f.status, f.pflag, qflag, epoch);
     if (f.tr) f.tr->end(s);
   }
-  static_assert(NB == SCHED_NB && SB == SCHED_SB && NQ == UPD_NQ, "task map and kernel disagree");
   for (int j0 = 0, col = 0; j0 + NB < K; j0 += NB, ++col) {
     const UpdShape shape = upd_shape(K, col);
     zd* Wcur = f.W + (size_t)(col & 1) * f.ld * NB;
@@ -827,7 +865,7 @@ int ldlh_factor(Factor& f, hipStream_t s) {
     const int epoch = ++f.epoch;
     if (f.tr) f.tr->begin(TR_LDL_UPDATE, s);
     hipLaunchKernelGGL(k_ldl_update, dim3(shape.grid), dim3(256), 0, s, K, f.ld, j0, f.F, Wcur, Wnext, f.D, f.Dinv,
-                       f.status, f.pflag, f.ld / NB + 1, epoch, shape.ntiles, shape.nstrip);
+                       f.status, f.pflag, qflag, epoch, shape.ntiles, shape.nstrip);
     if (f.tr) f.tr->end(s);
   }
   if (K > NB) hipLaunchKernelGGL(k_ldl_xlast, dim3((K + NB - 1) / NB - 1), dim3(256), 0, s, K, f.ld, f.F, f.Dinv);

@@ -1,13 +1,15 @@
-// Host-side check of the trailing-update launches' task maps
-// (atom-mol-nonbo_amd/csrc/ecg_schedule.hpp, the decode k_ldl_update runs):
-// for every basis size K in [1, KMAX] and every launch of the blocked
-// LDL^H, each flag a workgroup waits on must be published exactly once in
-// the same launch, by a workgroup of lower index; the tasks must cover the
-// trailing matrix's lower block pairs, the next panel's strips and the
-// L^{-1} tiles exactly once.  Prints "OK <launches checked>" or the first
-// violation and exits non-zero.  With a second argument "plant", quarter 0
-// of every launch is made to wait on the look-ahead factor (a higher
-// index): the check must then fail.
+// Host-side check of the blocked LDL^H launches' task maps
+// (atom-mol-nonbo_amd/csrc/ecg_schedule.hpp, the decode k_ldl_update and
+// k_ldl_panel run): for every basis size K in [1, KMAX] and every launch,
+// each flag a workgroup waits on must be published exactly once in the
+// same launch, by a workgroup of lower index, inside the flag array; the
+// tasks must cover the trailing matrix's lower block pairs (all but block
+// t0's diagonal tile), the next panel's strips, the diagonal strips' four
+// quarters of the next diagonal tile and the L^{-1} tiles exactly once.
+// Prints "OK <launches checked>" or the first violation and exits
+// non-zero.  With a second argument "plant", the look-ahead of every
+// update launch is made to wait on column tile (1, 0) (a higher index):
+// the check must then fail.
 #include <cstdio>
 #include <cstdlib>
 #include <map>
@@ -18,64 +20,111 @@
 
 using namespace ecg;
 
+static int check_waits(int K, int col, const std::vector<UpdTask>& tasks, int nflags) {
+  std::map<int, int> pub;   // flag -> publishing workgroup
+  for (int w = 0; w < (int)tasks.size(); ++w) {
+    const int f = tasks[w].publish;
+    if (f < 0) continue;
+    if (pub.count(f)) {
+      std::printf("K=%d col=%d: flag %d published twice\n", K, col, f);
+      return 1;
+    }
+    if (f >= nflags) {
+      std::printf("K=%d col=%d: flag %d outside the flag array\n", K, col, f);
+      return 1;
+    }
+    pub[f] = w;
+  }
+  for (int w = 0; w < (int)tasks.size(); ++w)
+    for (int i = 0; i < UPD_WAITS; ++i) {
+      const int f = tasks[w].wait[i];
+      if (f < 0) continue;
+      auto it = pub.find(f);
+      if (it == pub.end()) {
+        std::printf("K=%d col=%d: workgroup %d waits on flag %d that nobody publishes\n", K, col, w, f);
+        return 1;
+      }
+      if (it->second >= w) {
+        std::printf("K=%d col=%d: workgroup %d waits on workgroup %d (not lower)\n", K, col, w, it->second);
+        return 1;
+      }
+    }
+  return 0;
+}
+
+// the strips of a panel whose rows start at i1: one per 16 rows below, the
+// first NDIAG (block i1's rows) diagonal with quarters 0, 1, ...
+static int check_strips(int K, int col, const std::vector<UpdTask>& tasks, int i1) {
+  const int SB = SCHED_SB;
+  std::set<int> strips, quarters;
+  for (const UpdTask& t : tasks) {
+    if (t.kind != UPD_STRIP) continue;
+    strips.insert(t.a);
+    if (t.a < i1 || t.a >= K || (t.a - i1) % SB) {
+      std::printf("K=%d col=%d: bad strip %d\n", K, col, t.a);
+      return 1;
+    }
+    const bool diag = t.a < i1 + SCHED_NB;
+    if (diag != (t.diag >= 0) || (diag && t.diag != (t.a - i1) / SB)) {
+      std::printf("K=%d col=%d: strip %d has diagonal quarter %d\n", K, col, t.a, t.diag);
+      return 1;
+    }
+    if (diag) quarters.insert(t.diag);
+  }
+  const int rest = K - i1;
+  const int want = rest > 0 ? (rest + SB - 1) / SB : 0;
+  if ((int)strips.size() != want) {
+    std::printf("K=%d col=%d: strip cover wrong\n", K, col);
+    return 1;
+  }
+  const int wantq = want < NDIAG ? want : NDIAG;
+  if ((int)quarters.size() != wantq) {
+    std::printf("K=%d col=%d: diagonal quarter cover wrong\n", K, col);
+    return 1;
+  }
+  return 0;
+}
+
 int main(int argc, char** argv) {
   const int KMAX = argc > 1 ? std::atoi(argv[1]) : 4096;
   const bool plant = argc > 2;
-  const int NB = SCHED_NB, SB = SCHED_SB;
+  const int NB = SCHED_NB;
   long launches = 0;
   for (int K = 1; K <= KMAX; ++K) {
     const int nb = (K + NB - 1) / NB;
     const int ld = (K + 63) / 64 * 64;
     const int qflag = ld / NB + 1;
+    const int nflags = sched_flags(ld);
+    // the first panel's launch (strips below block 0)
+    if (K > NB) {
+      const int ns = (K - NB + SCHED_SB - 1) / SCHED_SB;
+      std::vector<UpdTask> tasks(ns);
+      for (int w = 0; w < ns; ++w) tasks[w] = panel_task(w, qflag);
+      if (check_waits(K, -1, tasks, nflags) || check_strips(K, -1, tasks, NB)) return 1;
+      ++launches;
+    }
     for (int j0 = 0, col = 0; j0 + NB < K; j0 += NB, ++col) {
       const UpdShape s = upd_shape(K, col);
-      std::map<int, int> pub;   // flag -> publishing workgroup
       std::vector<UpdTask> tasks(s.grid);
-      for (int w = 0; w < s.grid; ++w) {
-        tasks[w] = upd_task(w, K, j0, s, qflag);
-        if (plant && w == 0) tasks[w].wait[0] = 0;
-        if (tasks[w].publish >= 0) {
-          if (pub.count(tasks[w].publish)) {
-            std::printf("K=%d col=%d: flag %d published twice\n", K, col, tasks[w].publish);
-            return 1;
-          }
-          if (tasks[w].publish >= ld / NB + 5) {
-            std::printf("K=%d col=%d: flag %d outside the flag array\n", K, col, tasks[w].publish);
-            return 1;
-          }
-          pub[tasks[w].publish] = w;
-        }
-      }
+      for (int w = 0; w < s.grid; ++w) tasks[w] = upd_task(w, K, j0, s, qflag);
+      if (plant && s.ntiles >= 1) tasks[0].wait[0] = 1 + 1;
+      if (check_waits(K, col, tasks, nflags)) return 1;
       std::set<std::pair<int, int>> tiles, xtiles;
-      std::set<int> strips;
-      int quarters = 0, look = 0;
-      for (int w = 0; w < s.grid; ++w) {
-        const UpdTask& t = tasks[w];
-        for (int i = 0; i < 4; ++i) {
-          if (t.wait[i] < 0) continue;
-          auto it = pub.find(t.wait[i]);
-          if (it == pub.end()) {
-            std::printf("K=%d col=%d: workgroup %d waits on flag %d that nobody publishes\n", K, col, w, t.wait[i]);
-            return 1;
-          }
-          if (it->second >= w) {
-            std::printf("K=%d col=%d: workgroup %d waits on workgroup %d (not lower)\n", K, col, w, it->second);
-            return 1;
-          }
-        }
+      int look = 0;
+      for (const UpdTask& t : tasks) {
         switch (t.kind) {
-          case UPD_QUARTER: ++quarters; break;
-          case UPD_LOOKAHEAD: ++look; tiles.insert({0, 0}); break;
+          case UPD_LOOKAHEAD: ++look; break;
           case UPD_TILE: tiles.insert({t.a, t.b}); break;
           case UPD_XINV: xtiles.insert({t.a, t.b}); break;
-          case UPD_STRIP: strips.insert(t.a); break;
+          default: break;
         }
       }
-      // coverage: lower block pairs (bi >= bk) of the trailing matrix, the
-      // strips of the next panel, the L^{-1} tiles of block rows k.. by
-      // block columns c < k (row k: all c < k; rows i > k: all c < k)
+      // coverage: lower block pairs (bi >= bk) of the trailing matrix but
+      // (0, 0), the strips of the next panel, the L^{-1} tiles of block
+      // rows k.. by block columns c < k (row k: all c < k; rows i > k: all
+      // c < k)
       const int t0 = col + 1, nrb = nb - t0;
-      if (quarters != UPD_NQ || look != 1 || (int)tiles.size() != nrb * (nrb + 1) / 2) {
+      if (look != 1 || (int)tiles.size() != nrb * (nrb + 1) / 2 - 1 || tiles.count({0, 0})) {
         std::printf("K=%d col=%d: tile cover wrong\n", K, col);
         return 1;
       }
@@ -84,16 +133,7 @@ int main(int argc, char** argv) {
           std::printf("K=%d col=%d: bad tile (%d,%d)\n", K, col, p.first, p.second);
           return 1;
         }
-      const int rest2 = K - (t0 + 1) * NB;
-      if ((int)strips.size() != (rest2 > 0 ? (rest2 + SB - 1) / SB : 0)) {
-        std::printf("K=%d col=%d: strip cover wrong\n", K, col);
-        return 1;
-      }
-      for (int i0 : strips)
-        if (i0 < (t0 + 1) * NB || i0 >= K || (i0 - (t0 + 1) * NB) % SB) {
-          std::printf("K=%d col=%d: bad strip %d\n", K, col, i0);
-          return 1;
-        }
+      if (check_strips(K, col, tasks, (t0 + 1) * NB)) return 1;
       const int k = col;
       std::set<std::pair<int, int>> want;
       if (k >= 1)

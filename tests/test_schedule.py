@@ -1,5 +1,5 @@
 """CPU check of the wait rule of the blocked LDL^H's trailing-update
-launches: the task map the kernel decodes (csrc/ecg_schedule.hpp) is
+launches (and the first panel's launch): the task maps the kernels decode (csrc/ecg_schedule.hpp) are
 enumerated for every basis size up to 4096 and every launch; each awaited
 flag must be published in the same launch by a workgroup of lower index,
 and the tasks must cover the launch's work exactly once
