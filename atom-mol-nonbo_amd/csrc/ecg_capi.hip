@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <mutex>
 #include <thread>
@@ -50,6 +51,7 @@ struct ecg_ctx {
   hipEvent_t e[8] = {};
   double tms[8] = {};
   double* pinned = nullptr;   // pinned host staging
+  double* hrep = nullptr;     // pinned: the inverse iteration's report (k_report)
   size_t pinned_cap = 0;
   ecg::Screen scr;            // candidate screening session
   zd* dD = nullptr;           // stored derivatives (2 npt x nfo x K) of ecg_energy_grad_stored
@@ -433,6 +435,23 @@ __global__ void k_fill_ones(int K, zd* v) {
   if (i < K) v[i] = ecg::mkz(1.0, 0.0);
 }
 
+// The iteration's scalars and the factorisation status into pinned host
+// memory (one small kernel instead of two copies to pageable memory):
+// rep[0..4] = scal[0..4], rep[5..7] = status[0..2]; the Rayleigh quotient
+// E = scal[3] / scal[2] + sigma (the host's formula, same operations) goes
+// to rep[8] and to scal[8] on the device, where a contraction queued
+// before the host has read it takes it from.
+__global__ void k_report(double* scal, const int* status, double sigma, double* rep) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double E = scal[3] / scal[2] + sigma;
+  scal[8] = E;
+  for (int i = 0; i < 5; ++i) __hip_atomic_store(&rep[i], scal[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  for (int i = 0; i < 3; ++i)
+    __hip_atomic_store(&rep[5 + i], (double)status[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&rep[8], E, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __threadfence_system();
+}
+
 // Inverse iteration with the current factorisation (GHEPIIS loop,
 // src/linalg.f90:1892-1917).  Start vector in x->ev[0] unless warm.
 // Stops when the GHEPIIS measure reaches tol (tol > 0), after maxiter
@@ -447,8 +466,12 @@ __global__ void k_fill_ones(int K, zd* v) {
 // Rayleigh quotient and the normalisation are queued with a single host
 // synchronisation; further iterations only run if the measure asks for
 // them.  Result vector (normalised x^H S x = 1) in x->ev[2].
+// before_sync (optional) is run once, at the first synchronisation, after
+// the result vector and k_report's scalars are queued; *hooked reports
+// whether the result returned is the vector of that synchronisation.
 int inverse_iteration(ecg_ctx* x, double sigma, double tol, int maxiter, bool warm, InvitResult& r,
-                      bool stop_at_floor, int* fst = nullptr, int nspec = 3) {
+                      bool stop_at_floor, int* fst = nullptr, int nspec = 3,
+                      const std::function<int()>* before_sync = nullptr, bool* hooked = nullptr) {
   ecg::Ctx& c = x->c;
   const int K = c.K;
   zd* v = x->ev[0];
@@ -462,14 +485,27 @@ int inverse_iteration(ecg_ctx* x, double sigma, double tol, int maxiter, bool wa
   // tail: Rayleigh quotient, normalisation, result copy, host read of the
   // scalars and the status (one synchronisation)
   double q[5];
-  int st[4];
+  int st[4] = {0, 0, 0, 0};
+  if (hooked) *hooked = false;
+  bool hook_done = false;
+  auto read_report = [&]() {
+    for (int i = 0; i < 5; ++i) q[i] = x->hrep[i];
+    for (int i = 0; i < 3; ++i) st[i] = (int)x->hrep[5 + i];
+  };
   auto tail = [&]() -> int {
     RCCHK(ecg::rayleigh(c, x->f, v, sigma, x->stream));
     RCCHK(ecg::normalise(v, K, x->f.scal + 2, x->stream));
-    HIPCHK(hipMemcpyAsync(q, x->f.scal, sizeof(q), hipMemcpyDeviceToHost, x->stream));
-    HIPCHK(hipMemcpyAsync(st, x->f.status, sizeof(st), hipMemcpyDeviceToHost, x->stream));
+    hipLaunchKernelGGL(k_report, dim3(1), dim3(64), 0, x->stream, x->f.scal, x->f.status, sigma, x->hrep);
     HIPCHK(hipMemcpyAsync(x->ev[2], v, sizeof(zd) * K, hipMemcpyDeviceToDevice, x->stream));
+    if (before_sync && !hook_done) {
+      RCCHK((*before_sync)());
+      hook_done = true;
+      if (hooked) *hooked = true;
+    } else if (hooked) {
+      *hooked = false;   // a later result: what the hook queued is stale
+    }
     HIPCHK(hipStreamSynchronize(x->stream));
+    read_report();
     if (st[2]) return ECG_ERR_SYNC;   // a device-side wait (factor or solve) hit its limit
     if (fst)
       for (int i = 0; i < 3; ++i) fst[i] = st[i];
@@ -505,17 +541,15 @@ int inverse_iteration(ecg_ctx* x, double sigma, double tol, int maxiter, bool wa
   if (!done) {
     for (; it < maxiter; ++it) {
       RCCHK(ecg::invit_iteration(c, x->f, v, y, x->stream));
-      double sc[5];
-      int fs[3];
-      HIPCHK(hipMemcpyAsync(sc, x->f.scal, sizeof(sc), hipMemcpyDeviceToHost, x->stream));
-      if (fst) HIPCHK(hipMemcpyAsync(fs, x->f.status, sizeof(fs), hipMemcpyDeviceToHost, x->stream));
+      hipLaunchKernelGGL(k_report, dim3(1), dim3(64), 0, x->stream, x->f.scal, x->f.status, sigma, x->hrep);
       HIPCHK(hipStreamSynchronize(x->stream));
+      read_report();
       if (fst) {   // the factorisation's status, read here instead of right after it
-        for (int i = 0; i < 3; ++i) fst[i] = fs[i];
-        if (fs[2]) return ECG_ERR_SYNC;
-        if (fs[1]) return 0;   // zero pivot: the caller reports it
+        for (int i = 0; i < 3; ++i) fst[i] = st[i];
+        if (st[2]) return ECG_ERR_SYNC;
+        if (st[1]) return 0;   // zero pivot: the caller reports it
       }
-      r.nd = sc[0];
+      r.nd = q[0];
       r.iters = it + 1;
       std::swap(v, y);   // v now holds the newest iterate
       if (!std::isfinite(r.nd)) return ECG_ERR_NONFINITE;
@@ -541,8 +575,11 @@ int inverse_iteration(ecg_ctx* x, double sigma, double tol, int maxiter, bool wa
 // is located from inertia counts (number of negative pivots of
 // LDL^H(H - s S) = number of eigenvalues below s), then refined with the
 // Rayleigh quotient.
+// before_sync / hooked: as for inverse_iteration, in the fixed-shift path
+// only (*hooked false otherwise).
 int solve_root(ecg_ctx* x, int which, double sigma, double tol, int maxiter, double* E, double* info,
-               int* status) {
+               int* status, const std::function<int()>* before_sync = nullptr, bool* hooked = nullptr) {
+  if (hooked) *hooked = false;
   const double sigma_in = sigma;
   ecg::Ctx& c = x->c;
   const int K = c.K;
@@ -558,7 +595,7 @@ int solve_root(ecg_ctx* x, int which, double sigma, double tol, int maxiter, dou
     // after a tail update the previous eigenvector is close: two queued
     // iterations usually meet the criterion (more follow if not)
     const bool near = x->last_tail >= 0 && x->have_last && x->last_K == K;
-    const int rc = inverse_iteration(x, sigma, tol, maxiter, true, r, true, fst, near ? 2 : 3);
+    const int rc = inverse_iteration(x, sigma, tol, maxiter, true, r, true, fst, near ? 2 : 3, before_sync, hooked);
     if (rc == ECG_ERR_SYNC) return rc;
     if (fst[1]) {
       *status = ECG_SINGULAR;
@@ -938,6 +975,13 @@ int ecg_ctx_create(int n, int T, const int* yhy_mats, const int* yhy_coef, const
       delete x;
       return ECG_ERR_HIP;
     }
+  // coherent: the kernel's stores reach host memory without a cache in
+  // between, whatever release scope the runtime ends the kernel with
+  if (hipHostMalloc((void**)&x->hrep, 16 * sizeof(double), hipHostMallocCoherent) != hipSuccess) {
+    x->hrep = nullptr;
+    ecg_ctx_destroy(x);
+    return ECG_ERR_HIP;
+  }
   *out = x;
   return ECG_OK;
 }
@@ -952,6 +996,7 @@ int ecg_ctx_destroy(ecg_ctx* x) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (x->pinned) (void)hipHostFree(x->pinned);
+  if (x->hrep) (void)hipHostFree(x->hrep);
   free_screen(x->scr);
   if (x->dD) (void)hipFree(x->dD);
   if (x->pen) (void)hipFree(x->pen);
@@ -1250,12 +1295,22 @@ int ecg_energy_grad(ecg_ctx* x, int nfru, int which, double sigma, double tol, i
   HIPCHK(hipEventRecord(x->e[1], x->stream));
   int status = 0;
   x->vec_current = false;
-  RCCHK(solve_root(x, which, sigma, tol, maxiter, E, info, &status));
-  x->vec_current = (status == 0 || status == ECG_NOT_CONVERGED);
-  HIPCHK(hipEventRecord(x->e[2], x->stream));
+  // the gradient's contraction is queued before the solve's host
+  // synchronisation, with E read on the device (k_report); it is redone
+  // with the host's E only if the iteration went on past that point
   c.ev_adj0 = x->e[3];
   c.ev_adj1 = x->e[4];
-  RCCHK(ecg::adjoint(c, nfru, *E, x->ev[2], x->part, x->grad, x->stream));
+  const std::function<int()> queue_adjoint = [&]() -> int {
+    HIPCHK(hipEventRecord(x->e[2], x->stream));
+    return ecg::adjoint(c, nfru, 0.0, x->f.scal + 8, x->ev[2], x->part, x->grad, x->stream);
+  };
+  bool queued = false;
+  RCCHK(solve_root(x, which, sigma, tol, maxiter, E, info, &status, &queue_adjoint, &queued));
+  x->vec_current = (status == 0 || status == ECG_NOT_CONVERGED);
+  if (!queued) {
+    HIPCHK(hipEventRecord(x->e[2], x->stream));
+    RCCHK(ecg::adjoint(c, nfru, *E, nullptr, x->ev[2], x->part, x->grad, x->stream));
+  }
   x->s_ready = true;
   x->pen_total = 0.0;
   if (x->pen_on && K != 1) RCCHK(penalty_enqueue(x, x->stream, nfru, x->pen_b, x->pen_t2, x->grad));
@@ -1496,7 +1551,7 @@ int ecg_batch_energy_grad(ecg_batch* b, const double* sigma, double tol, int max
       if (!rc) rc = solve_root(x, 1, sigma[i], tol, maxiter, &E[i], info, &status);
       if (!rc) x->vec_current = (status == 0 || status == ECG_NOT_CONVERGED);
       if (!rc && !b->packed) {
-        rc = ecg::adjoint(x->c, 0, E[i], x->ev[2], x->part, x->grad, x->stream);
+        rc = ecg::adjoint(x->c, 0, E[i], nullptr, x->ev[2], x->part, x->grad, x->stream);
         x->pen_total = 0.0;
         if (!rc && x->pen_on && x->c.K != 1) rc = penalty_enqueue(x, x->stream, 0, x->pen_b, x->pen_t2, x->grad);
         if (!rc && hipMemcpyAsync(grad + goff[i], x->grad, sizeof(double) * (goff[i + 1] - goff[i]),

@@ -112,7 +112,9 @@ int assemble(const Ctx& c, int r0, int r1, hipStream_t s);   // rows r0..r1-1 (t
 size_t thin_row_entries(int cap);
 size_t thin_adjoint_entries(int n, int cap);
 int assemble_deriv(const Ctx& c, int nfru, zd* D, hipStream_t s);
-int adjoint(const Ctx& c, int nfru, double E, const zd* cvec, double* part, double* grad, hipStream_t s);
+// E is taken from *Ep (device) when Ep is not null.
+int adjoint(const Ctx& c, int nfru, double E, const double* Ep, const zd* cvec, double* part, double* grad,
+            hipStream_t s);
 size_t adjoint_partial_doubles(int n, int K);
 // Overlap penalty of the device S (pairs j > i, j >= nfru): per-function
 // partials pen[K - nfru], their fixed-order sum to *total (device), and,

@@ -446,7 +446,7 @@ __device__ __attribute__((always_inline)) inline void adjoint_tile(const SysCons
 
 template <int N>
 __global__ void __launch_bounds__(TILE* TILE, 2) k_adjoint(const SysConst sc, int K, int nfru, int tr0, int nt,
-                                                      double E,
+                                                      double E, const double* __restrict__ Ep,
                                                       const double* __restrict__ bra,
                                                       const double* __restrict__ ket,
                                                       const double* __restrict__ diagS,
@@ -455,7 +455,7 @@ __global__ void __launch_bounds__(TILE* TILE, 2) k_adjoint(const SysConst sc, in
   extern __shared__ double lds[];
   int tr, tc;
   tri_tile(blockIdx.x, tr0, tr, tc);
-  adjoint_tile<N>(sc, K, nfru, nt, tr, tc, E, bra, ket, diagS, cvec, part, lds);
+  adjoint_tile<N>(sc, K, nfru, nt, tr, tc, Ep ? *Ep : E, bra, ket, diagS, cvec, part, lds);
 }
 
 // Per function f >= nfru: sum its partial slots in fixed order and apply
@@ -1382,11 +1382,13 @@ static int launch_deriv(const Ctx& c, int nfru, zd* D, hipStream_t s) {
 // adjoint_tile would (same 16-pair sums in the same order), so
 // k_adjoint_finish is unchanged.
 template <int N, int G>
-__global__ void __launch_bounds__(256) k_adj_rows(const SysConst sc, int K, int nfru, long long npairs, double E,
+__global__ void __launch_bounds__(256) k_adj_rows(const SysConst sc, int K, int nfru, long long npairs, double Ein,
+                                                  const double* __restrict__ Ep,
                                                   const double* __restrict__ bra, const double* __restrict__ ket,
                                                   const double* __restrict__ diagS, const zd* __restrict__ cvec,
                                                   double* __restrict__ raw) {
   constexpr int m = N * (N + 1) / 2, nv = AdjAcc<N>::nvals;
+  const double E = Ep ? *Ep : Ein;
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long p = gid / G;
   const int t = (int)(gid % G);
@@ -1503,18 +1505,19 @@ __global__ void __launch_bounds__(256) k_adj_rows_slots(int K, int nfru, int nt,
 }
 
 template <int N>
-static void launch_adj_rows(const Ctx& c, int nfru, double E, const zd* cvec, double* part, hipStream_t s) {
+static void launch_adj_rows(const Ctx& c, int nfru, double E, const double* Ep, const zd* cvec, double* part,
+                            hipStream_t s) {
   const int K = c.K, nt = (K + TILE - 1) / TILE;
   const long long npairs = (long long)K * (K + 1) / 2 - (long long)nfru * (nfru + 1) / 2;
   const int T = c.sc.T;
   const int G = T <= 1 ? 1 : T <= 2 ? 2 : T <= 4 ? 4 : T <= 8 ? 8 : 16;
   const unsigned nb = (unsigned)((npairs * G + 255) / 256);
   switch (G) {
-    case 1: hipLaunchKernelGGL((k_adj_rows<N, 1>), dim3(nb), dim3(256), 0, s, c.sc, K, nfru, npairs, E, c.bra, c.ket, c.diagS, cvec, c.adjraw); break;
-    case 2: hipLaunchKernelGGL((k_adj_rows<N, 2>), dim3(nb), dim3(256), 0, s, c.sc, K, nfru, npairs, E, c.bra, c.ket, c.diagS, cvec, c.adjraw); break;
-    case 4: hipLaunchKernelGGL((k_adj_rows<N, 4>), dim3(nb), dim3(256), 0, s, c.sc, K, nfru, npairs, E, c.bra, c.ket, c.diagS, cvec, c.adjraw); break;
-    case 8: hipLaunchKernelGGL((k_adj_rows<N, 8>), dim3(nb), dim3(256), 0, s, c.sc, K, nfru, npairs, E, c.bra, c.ket, c.diagS, cvec, c.adjraw); break;
-    default: hipLaunchKernelGGL((k_adj_rows<N, 16>), dim3(nb), dim3(256), 0, s, c.sc, K, nfru, npairs, E, c.bra, c.ket, c.diagS, cvec, c.adjraw); break;
+    case 1: hipLaunchKernelGGL((k_adj_rows<N, 1>), dim3(nb), dim3(256), 0, s, c.sc, K, nfru, npairs, E, Ep, c.bra, c.ket, c.diagS, cvec, c.adjraw); break;
+    case 2: hipLaunchKernelGGL((k_adj_rows<N, 2>), dim3(nb), dim3(256), 0, s, c.sc, K, nfru, npairs, E, Ep, c.bra, c.ket, c.diagS, cvec, c.adjraw); break;
+    case 4: hipLaunchKernelGGL((k_adj_rows<N, 4>), dim3(nb), dim3(256), 0, s, c.sc, K, nfru, npairs, E, Ep, c.bra, c.ket, c.diagS, cvec, c.adjraw); break;
+    case 8: hipLaunchKernelGGL((k_adj_rows<N, 8>), dim3(nb), dim3(256), 0, s, c.sc, K, nfru, npairs, E, Ep, c.bra, c.ket, c.diagS, cvec, c.adjraw); break;
+    default: hipLaunchKernelGGL((k_adj_rows<N, 16>), dim3(nb), dim3(256), 0, s, c.sc, K, nfru, npairs, E, Ep, c.bra, c.ket, c.diagS, cvec, c.adjraw); break;
   }
   const long long nslots = (long long)(K - nfru) * (nt + 1);
   hipLaunchKernelGGL(k_adj_rows_slots<N>, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s, K, nfru, nt,
@@ -1527,8 +1530,8 @@ size_t thin_adjoint_entries(int n, int cap) {
 }
 
 template <int N>
-static int launch_adjoint(const Ctx& c, int nfru, double E, const zd* cvec, double* part, double* grad,
-                          hipStream_t s) {
+static int launch_adjoint(const Ctx& c, int nfru, double E, const double* Ep, const zd* cvec, double* part,
+                          double* grad, hipStream_t s) {
   const int K = c.K;
   const int nt = (K + TILE - 1) / TILE;
   const int tr0 = nfru / TILE;
@@ -1538,10 +1541,10 @@ static int launch_adjoint(const Ctx& c, int nfru, double E, const zd* cvec, doub
   if (c.ev_adj0) (void)hipEventRecord(c.ev_adj0, s);
   if (c.tr) c.tr->begin(TR_ADJOINT, s);
   if (K - nfru <= THIN_ROWS && c.adjraw)
-    launch_adj_rows<N>(c, nfru, E, cvec, part, s);
+    launch_adj_rows<N>(c, nfru, E, Ep, cvec, part, s);
   else
     hipLaunchKernelGGL(k_adjoint<N>, dim3((unsigned)ntiles), dim3(TILE, TILE), lds, s, c.sc, K, nfru, tr0, nt, E,
-                       c.bra, c.ket, c.diagS, cvec, part);
+                       Ep, c.bra, c.ket, c.diagS, cvec, part);
   if (c.tr) c.tr->end(s);
   if (c.ev_adj1) (void)hipEventRecord(c.ev_adj1, s);
   hipLaunchKernelGGL(k_adjoint_finish<N>, dim3((K - nfru + 3) / 4), dim3(256), 0, s, K, nfru, nt, tr0, c.bra, part,
@@ -1589,13 +1592,14 @@ int assemble_deriv(const Ctx& c, int nfru, zd* D, hipStream_t s) {
   }
   return -1;
 }
-int adjoint(const Ctx& c, int nfru, double E, const zd* cvec, double* part, double* grad, hipStream_t s) {
+int adjoint(const Ctx& c, int nfru, double E, const double* Ep, const zd* cvec, double* part, double* grad,
+            hipStream_t s) {
   switch (c.n) {
-    case 1: return launch_adjoint<1>(c, nfru, E, cvec, part, grad, s);
-    case 2: return launch_adjoint<2>(c, nfru, E, cvec, part, grad, s);
-    case 3: return launch_adjoint<3>(c, nfru, E, cvec, part, grad, s);
-    case 4: return launch_adjoint<4>(c, nfru, E, cvec, part, grad, s);
-    case 5: return launch_adjoint<5>(c, nfru, E, cvec, part, grad, s);
+    case 1: return launch_adjoint<1>(c, nfru, E, Ep, cvec, part, grad, s);
+    case 2: return launch_adjoint<2>(c, nfru, E, Ep, cvec, part, grad, s);
+    case 3: return launch_adjoint<3>(c, nfru, E, Ep, cvec, part, grad, s);
+    case 4: return launch_adjoint<4>(c, nfru, E, Ep, cvec, part, grad, s);
+    case 5: return launch_adjoint<5>(c, nfru, E, Ep, cvec, part, grad, s);
   }
   return -1;
 }

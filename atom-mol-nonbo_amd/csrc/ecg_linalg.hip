@@ -556,6 +556,9 @@ __device__ __attribute__((always_inline)) inline void panel_strip(int K, int ld,
     acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t].x, bv[t].y, acc.im, 0, 0, 0);
     acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t].y, bv[t].x, acc.im, 0, 0, 0);
   }
+  // every wave has read all 64 columns of the strip before any wave
+  // overwrites its 16 columns with L below
+  __syncthreads();
   // the panel's block is a full one (a strip exists only below it), so
   // D[j0 + c] is in range for every lane
   const double rdc = 1.0 / D[j0 + c0 + acc16_col()];
