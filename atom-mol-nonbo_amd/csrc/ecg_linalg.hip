@@ -547,14 +547,26 @@ __device__ __attribute__((always_inline)) inline void panel_strip(int K, int ld,
     av[t] = ld_or0(A, (size_t)kk * ld + lr, i0 + lr < K);
     bv[t] = zconj(X[(size_t)kk * NB + c0 + lr]);
   }
-  Acc16 acc;
+  // two accumulator chains (even and odd k-steps) halve the dependent
+  // MFMA latency of the strip, which is on the factorisation's critical path
+  Acc16 acc, ac1;
   acc16_zero(acc);
+  acc16_zero(ac1);
 #pragma unroll
-  for (int t = 0; t < NB / 4; ++t) {
+  for (int t = 0; t < NB / 4; t += 2) {
     acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t].x, bv[t].x, acc.re, 0, 0, 0);
     acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[t].y, bv[t].y, acc.re, 0, 0, 0);
     acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t].x, bv[t].y, acc.im, 0, 0, 0);
     acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t].y, bv[t].x, acc.im, 0, 0, 0);
+    ac1.re = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t + 1].x, bv[t + 1].x, ac1.re, 0, 0, 0);
+    ac1.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[t + 1].y, bv[t + 1].y, ac1.re, 0, 0, 0);
+    ac1.im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t + 1].x, bv[t + 1].y, ac1.im, 0, 0, 0);
+    ac1.im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t + 1].y, bv[t + 1].x, ac1.im, 0, 0, 0);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    acc.re[v] += ac1.re[v];
+    acc.im[v] += ac1.im[v];
   }
   // every wave has read all 64 columns of the strip before any wave
   // overwrites its 16 columns with L below
