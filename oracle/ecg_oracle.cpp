@@ -774,6 +774,510 @@ void oracle_relat_pair(int n, int T, const int* P, const int* coef, const double
   st(out, 1, o[1]);
 }
 
+// ------------------------------------ Drachmanized forms and orbit-orbit
+// Restatement of MatrixElementsForExpcVals' remaining pair values
+// (src/matelem.f90:1276-1478 and 1669-1933) and its helper integrals
+// ME_rXr_over_rij, ME_rXr_rYr_over_rij (:753-945), ME_dXd_dYd, ME_dXd and
+// ME_dXd_over_rij (:1998-2044), loop for loop, for a pair whose bra is
+// unpermuted and ket permuted by Pm (the driver's form).  Matrices are
+// A[row][column].  Inputs beyond the pair: the particle masses, the
+// charges, the mass matrix and the current energy E (Glob_CurrEnergy).
+//
+// tr[C^-1 J_ij C^-1 J_pq] is kept, as the reference keeps it, in a 4-index
+// table filled by its loops (:1275-1302).  Those loops never write the
+// entries (i, i, p, q) with i < p < q (n >= 3), and the reference reads
+// them in the 1/(r_ij r_pq) table (:1305-1343) used by the Drachmanized
+// delta.  Its build (ifort, no -auto: local arrays are static, zero at
+// start) reads zero there, which sends those entries to the uncorrelated
+// branch (4 S / pi) / sqrt(TrCJ_ii TrCJ_pq); the table below starts at
+// zero so that the same values result.
+typedef cd Mat[NN][NN];
+
+static cd clog_(const cd& z) { return {std::log(std::hypot(z.re, z.im)), std::atan2(z.im, z.re)}; }
+
+// <phi_k| r'Xr / r_ij |phi_l> (:753-818)
+static cd me_rxr_over_rij(int n, const Mat& X, int i, int j, const Mat& Ci, const cd& rm, const cd& trcj) {
+  cd trcx = {0.0, 0.0};
+  for (int m = 0; m < n; ++m)
+    for (int p = 0; p < n; ++p) trcx = trcx + Ci[m][p] * X[p][m];
+  cd cxi[NN], cxj[NN];
+  for (int m = 0; m < n; ++m) {
+    cxi[m] = {0.0, 0.0};
+    for (int p = 0; p < n; ++p) cxi[m] = cxi[m] + Ci[i][p] * X[p][m];
+  }
+  if (j != i)
+    for (int m = 0; m < n; ++m) {
+      cxj[m] = {0.0, 0.0};
+      for (int p = 0; p < n; ++p) cxj[m] = cxj[m] + Ci[j][p] * X[p][m];
+    }
+  cd t = {0.0, 0.0};
+  if (i == j)
+    for (int m = 0; m < n; ++m) t = t + cxi[m] * Ci[m][i];
+  else
+    for (int m = 0; m < n; ++m) t = t + (cxi[m] - cxj[m]) * (Ci[m][i] - Ci[m][j]);
+  return rm * (3.0 * trcx - t / trcj) / 2.0;
+}
+
+// <phi_k| (r'Xr)(r'Yr) / r_ij |phi_l> (:821-945)
+static cd me_rxr_ryr_over_rij(int n, const Mat& X, const Mat& Y, int i, int j, const Mat& Ci, const cd& rm,
+                              const cd& trcj) {
+  Mat Xs, Ys, CX, CY;
+  for (int p = 0; p < n; ++p) {
+    Ys[p][p] = Y[p][p];
+    for (int q = p + 1; q < n; ++q) Ys[q][p] = Ys[p][q] = (Y[p][q] + Y[q][p]) / 2.0;
+  }
+  for (int p = 0; p < n; ++p) {
+    Xs[p][p] = X[p][p];
+    for (int q = p + 1; q < n; ++q) Xs[q][p] = Xs[p][q] = (X[p][q] + X[q][p]) / 2.0;
+  }
+  for (int p = 0; p < n; ++p)
+    for (int q = 0; q < n; ++q) {
+      CY[p][q] = {0.0, 0.0};
+      for (int m = 0; m < n; ++m) CY[p][q] = CY[p][q] + Ci[p][m] * Ys[m][q];
+    }
+  for (int p = 0; p < n; ++p)
+    for (int q = 0; q < n; ++q) {
+      CX[p][q] = {0.0, 0.0};
+      for (int m = 0; m < n; ++m) CX[p][q] = CX[p][q] + Ci[p][m] * Xs[m][q];
+    }
+  cd ci_[NN], cj_[NN];
+  for (int m = 0; m < n; ++m) {
+    ci_[m] = {0.0, 0.0};
+    for (int p = 0; p < n; ++p) ci_[m] = ci_[m] + CX[i][p] * CY[p][m];
+  }
+  if (j != i)
+    for (int m = 0; m < n; ++m) {
+      cj_[m] = {0.0, 0.0};
+      for (int p = 0; p < n; ++p) cj_[m] = cj_[m] + CX[j][p] * CY[p][m];
+    }
+  cd trcy = {0.0, 0.0}, trcx = {0.0, 0.0}, trcxcy = {0.0, 0.0};
+  for (int m = 0; m < n; ++m) {
+    trcy = trcy + CY[m][m];
+    trcx = trcx + CX[m][m];
+  }
+  for (int m = 0; m < n; ++m)
+    for (int p = 0; p < n; ++p) trcxcy = trcxcy + CX[m][p] * CY[p][m];
+  cd tx = {0.0, 0.0}, ty = {0.0, 0.0}, txy = {0.0, 0.0};
+  if (i == j) {
+    for (int m = 0; m < n; ++m) tx = tx + CX[i][m] * Ci[m][i];
+    for (int m = 0; m < n; ++m) ty = ty + CY[i][m] * Ci[m][i];
+    for (int m = 0; m < n; ++m) txy = txy + ci_[m] * Ci[m][i];
+  } else {
+    for (int m = 0; m < n; ++m) tx = tx + (CX[i][m] - CX[j][m]) * (Ci[m][i] - Ci[m][j]);
+    for (int m = 0; m < n; ++m) ty = ty + (CY[i][m] - CY[j][m]) * (Ci[m][i] - Ci[m][j]);
+    for (int m = 0; m < n; ++m) txy = txy + (ci_[m] - cj_[m]) * (Ci[m][i] - Ci[m][j]);
+  }
+  return rm * (9.0 * trcy * trcx + 6.0 * trcxcy - (3.0 * (trcy * tx + trcx * ty) + 4.0 * txy) / trcj +
+               3.0 * tx * ty / (trcj * trcj)) / 4.0;
+}
+
+static void matmul_(int n, const Mat& A, const Mat& B, Mat& C) {
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      cd s = {0.0, 0.0};
+      for (int k = 0; k < n; ++k) s = s + A[i][k] * B[k][j];
+      C[i][j] = s;
+    }
+}
+static cd trace_(int n, const Mat& A) {
+  cd s = {0.0, 0.0};
+  for (int i = 0; i < n; ++i) s = s + A[i][i];
+  return s;
+}
+
+// ME_dXd_dYd (:1998-2016)
+static cd me_dxd_dyd(int n, const Mat& X, const Mat& Y, const Mat& Ci, const Mat& Ck, const Mat& Cl, const cd& S) {
+  Mat M, MM, Z, t1, t2;
+  matmul_(n, X, Ck, M);
+  const cd tr1 = trace_(n, M);
+  matmul_(n, Cl, Y, M);
+  const cd tr2 = trace_(n, M);
+  matmul_(n, X, Ck, t1);
+  matmul_(n, Ck, t1, t2);
+  matmul_(n, Ci, t2, M);
+  const cd tr3 = trace_(n, M);
+  matmul_(n, Y, Cl, t1);
+  matmul_(n, Cl, t1, t2);
+  matmul_(n, Ci, t2, MM);
+  const cd tr4 = trace_(n, MM);
+  matmul_(n, M, MM, Z);
+  const cd tr5 = trace_(n, Z);
+  return (36.0 * tr3 * tr4 + 24.0 * tr5 - 36.0 * tr1 * tr4 - 36.0 * tr2 * tr3 + 36.0 * tr1 * tr2) * S;
+}
+
+// ME_dXd (:2018-2030)
+static cd me_dxd(int n, const Mat& X, const Mat& Ci, const Mat& Cl, const cd& S) {
+  Mat M, t1, t2;
+  matmul_(n, Cl, X, M);
+  const cd tr1 = trace_(n, M);
+  matmul_(n, X, Cl, t1);
+  matmul_(n, Cl, t1, t2);
+  matmul_(n, Ci, t2, M);
+  const cd tr2 = trace_(n, M);
+  return 6.0 * (tr2 - tr1) * S;
+}
+
+// ME_dXd_over_rij (:2032-2044)
+static cd me_dxd_over_rij(int n, const Mat& X, int i, int j, const Mat& Ci, const Mat& Cl, const cd& rm,
+                          const cd& trcj) {
+  Mat M, Z, t1;
+  matmul_(n, X, Cl, t1);
+  matmul_(n, Cl, t1, M);
+  matmul_(n, Cl, X, Z);
+  const cd tr1 = trace_(n, Z);
+  return 4.0 * me_rxr_over_rij(n, M, i, j, Ci, rm, trcj) - 6.0 * tr1 * rm;
+}
+
+// out: the Drachmanized delta of (a, b >= a) (n(n+1)/2 values), then OO,
+// the Drachmanized mass-velocity and the Drachmanized Darwin term.
+static void drach_pair(const System<double>& sy, const double* pk, const double* pl, const int* Pm,
+                       const double* masses, double E, cd* out) {
+  cd H, S;
+  PairParts<double> pp;
+  matrix_elements<double>(sy, pk, pl, Pm, 0, H, S, nullptr, nullptr, &pp);
+  const int n = sy.n;
+  const double pi = 3.14159265358979323846264338327950, sqrtpi = 1.77245385090551602729816748334115;
+  Mat Ci, Ck, Cl, Ckl;
+  for (int a = 0; a < n; ++a)
+    for (int b = 0; b < n; ++b) {
+      Ci[a][b] = pp.Ci[a][b];
+      Ck[a][b] = pp.Ck[a][b];
+      Cl[a][b] = pp.tCl[a][b];
+      Ckl[a][b] = Ck[a][b] + Cl[a][b];
+    }
+  auto mass = [&](int part) { return masses[part]; };   // particle 1 = 0, pseudo-particle a = a + 1
+  // TrCJ, <1/r_ij> (:1236-1273)
+  Mat trcj, rm;
+  const cd cn = (2.0 / sqrtpi) * S;
+  for (int a = 0; a < n; ++a) {
+    trcj[a][a] = Ci[a][a];
+    rm[a][a] = cn / csqrt(trcj[a][a]);
+  }
+  for (int a = 0; a < n; ++a)
+    for (int b = a + 1; b < n; ++b) {
+      trcj[b][a] = trcj[a][b] = Ci[a][a] + Ci[b][b] - Ci[b][a] - Ci[b][a];
+      rm[b][a] = rm[a][b] = cn / csqrt(trcj[a][b]);
+    }
+  // tr[C^-1 J_ij C^-1 J_pq] (:1275-1302)
+  static thread_local cd tc[NN][NN][NN][NN];
+  for (int a = 0; a < NN; ++a)
+    for (int b = 0; b < NN; ++b)
+      for (int c = 0; c < NN; ++c)
+        for (int d = 0; d < NN; ++d) tc[a][b][c][d] = {0.0, 0.0};
+  for (int i = 0; i < n; ++i) {
+    tc[i][i][i][i] = Ci[i][i] * Ci[i][i];
+    for (int j = i + 1; j < n; ++j) {
+      const cd c0 = Ci[j][i] * Ci[j][i];
+      tc[j][j][i][i] = c0;
+      tc[i][i][j][j] = c0;
+      for (int p = i; p < n; ++p) {
+        const cd t = Ci[p][i] - Ci[p][j];
+        const cd c1 = t * t;
+        tc[i][j][p][p] = c1;
+        tc[j][i][p][p] = c1;
+        tc[p][p][i][j] = c1;
+        tc[p][p][j][i] = c1;
+        for (int q = p + 1; q < n; ++q) {
+          const cd u = Ci[q][i] - Ci[p][i] - Ci[q][j] + Ci[p][j];
+          const cd c2 = u * u;
+          tc[i][j][p][q] = c2;
+          tc[j][i][p][q] = c2;
+          tc[i][j][q][p] = c2;
+          tc[j][i][q][p] = c2;
+          tc[p][q][i][j] = c2;
+          tc[p][q][j][i] = c2;
+          tc[q][p][i][j] = c2;
+          tc[q][p][j][i] = c2;
+        }
+      }
+    }
+  }
+  // <1/(r_ij r_pq)> (:1305-1343)
+  static thread_local cd rr[NN][NN][NN][NN];
+  const cd c4 = (4.0 * S) / pi, c2s = 2.0 * S;
+  for (int i = 0; i < n; ++i)
+    for (int j = i; j < n; ++j)
+      for (int p = i; p < n; ++p)
+        for (int q = p; q < n; ++q) {
+          cd v;
+          if ((p == i && q == j) || (p == j && q == i)) {
+            v = c2s / trcj[i][j];
+          } else {
+            const cd ca = csqrt(tc[i][j][p][q]);
+            if (ca.re != 0.0 || ca.im != 0.0) {
+              const cd ca1 = ca / csqrt(trcj[i][j] * trcj[p][q]);
+              const cd w = cd{0.0, 1.0} * ca1 + csqrt(cd{1.0, 0.0} - ca1 * ca1);
+              const cd as = cd{0.0, -1.0} * clog_(w);
+              v = c4 * as / ca;
+            } else {
+              v = c4 / csqrt(trcj[i][j] * trcj[p][q]);
+            }
+          }
+          rr[i][j][p][q] = v;
+          rr[j][i][p][q] = v;
+          rr[i][j][q][p] = v;
+          rr[j][i][q][p] = v;
+          rr[p][q][i][j] = v;
+          rr[p][q][j][i] = v;
+          rr[q][p][i][j] = v;
+          rr[q][p][j][i] = v;
+        }
+  // Drachmanized delta(r_pq) (:1353-1392)
+  Mat W1, W2, W3, W4, W5;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      cd t = {0.0, 0.0};
+      for (int k = 0; k < n; ++k) t = t + sy.M[i][k] * Cl[k][j];
+      W1[i][j] = t;
+    }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      cd t = {0.0, 0.0};
+      for (int k = 0; k < n; ++k) t = t + Ck[i][k] * W1[k][j];
+      W2[i][j] = t;
+    }
+  int o = 0;
+  for (int p = 0; p < n; ++p)
+    for (int q = p; q < n; ++q, ++o) {
+      cd ct = {0.0, 0.0};
+      for (int i = 0; i < n; ++i) {
+        ct = ct + (sy.q0 * sy.q[i]) * rr[p][q][i][i];
+        for (int j = i + 1; j < n; ++j) ct = ct + (sy.q[i] * sy.q[j]) * rr[p][q][i][j];
+      }
+      const double ca = (p == q) ? 2.0 * pi * sy.M[p][p]
+                                 : 2.0 * pi * (sy.M[p][p] + sy.M[q][q] - sy.M[p][q] - sy.M[p][q]);
+      const cd cnm = me_rxr_over_rij(n, W2, p, q, Ci, rm[p][q], trcj[p][q]);
+      out[o] = (E * rm[p][q] - ct - 4.0 * cnm) / ca;
+    }
+  const int m = o;
+  // orbit-orbit (:1394-1477)
+  cd OO = {0.0, 0.0};
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      const cd tr1 = Cl[j][i], tr2 = tr1, tr3 = 3.0 * Cl[j][j];
+      for (int p = 0; p < n; ++p)
+        for (int q = 0; q < n; ++q) W1[q][p] = Cl[q][i] * Cl[p][j] + Cl[q][j] * Cl[p][i];
+      for (int p = 0; p < n; ++p) W1[p][j] = W1[p][j] + Ckl[p][j] * Cl[j][i];
+      for (int p = 0; p < n; ++p) W1[j][p] = W1[j][p] + Cl[i][j] * Cl[p][j] + tr3 * Cl[p][i];
+      for (int p = 0; p < n; ++p)
+        for (int q = 0; q < n; ++q) W2[p][q] = Ckl[p][j] * Cl[q][j];
+      for (int p = 0; p < n; ++p)
+        for (int q = 0; q < n; ++q) W3[p][q] = {0.0, 0.0};
+      for (int p = 0; p < n; ++p) W3[j][p] = Cl[p][i];
+      const cd cnm = me_rxr_over_rij(n, W1, j, j, Ci, rm[j][j], trcj[j][j]);
+      const cd ct = me_rxr_ryr_over_rij(n, W2, W3, j, j, Ci, rm[j][j], trcj[j][j]);
+      const cd ca = -6.0 * (tr1 + tr2) * rm[j][j] + 4.0 * cnm - 8.0 * ct;
+      OO = OO - ca * (sy.q[j] / mass(j + 1));
+    }
+  OO = OO * (sy.q0 / mass(0));
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      const cd tr1 = Cl[j][i], tr2 = tr1, tr3 = 3.0 * Cl[j][j];
+      for (int p = 0; p < n; ++p)
+        for (int q = 0; q < n; ++q) W1[q][p] = Cl[q][i] * Cl[p][j] + Cl[q][j] * Cl[p][i];
+      for (int p = 0; p < n; ++p) {
+        W1[p][j] = W1[p][j] + Ckl[p][j] * Cl[j][i];
+        W1[p][i] = W1[p][i] - Ckl[p][j] * Cl[j][i];
+      }
+      for (int p = 0; p < n; ++p) {
+        const cd cnm = Cl[i][j] * Cl[p][j] + tr3 * Cl[p][i];
+        W1[j][p] = W1[j][p] + cnm;
+        W1[i][p] = W1[i][p] - cnm;
+      }
+      for (int p = 0; p < n; ++p)
+        for (int q = 0; q < n; ++q) W2[p][q] = Ckl[p][j] * Cl[q][j];
+      for (int p = 0; p < n; ++p)
+        for (int q = 0; q < n; ++q) W3[p][q] = {0.0, 0.0};
+      for (int p = 0; p < n; ++p) {
+        W3[j][p] = Cl[p][i];
+        W3[i][p] = -Cl[p][i];
+      }
+      const cd cnm = me_rxr_over_rij(n, W1, i, j, Ci, rm[i][j], trcj[i][j]);
+      const cd ct = me_rxr_ryr_over_rij(n, W2, W3, i, j, Ci, rm[i][j], trcj[i][j]);
+      const cd ca = -6.0 * (tr1 + tr2) * rm[i][j] + 4.0 * cnm - 8.0 * ct;
+      OO = OO + ca * (sy.q[i] * sy.q[j] / (mass(i + 1) * mass(j + 1)));
+    }
+  out[m] = OO / 2.0;
+  // Drachmanized mass-velocity (:1669-1865)
+  {
+    const double c0 = 1.0 / (mass(0) * mass(0));
+    for (int p = 0; p < n; ++p) {
+      W1[p][p] = cd{c0 + 1.0 / (mass(p + 1) * mass(p + 1)), 0.0};
+      for (int q = p + 1; q < n; ++q) W1[p][q] = W1[q][p] = cd{c0, 0.0};
+    }
+  }
+  cd tr1 = {0.0, 0.0};
+  for (int p = 0; p < n; ++p) {
+    for (int q = 0; q < n; ++q) {
+      cd t = {0.0, 0.0};
+      for (int j = 0; j < n; ++j) t = t + Cl[j][q] * W1[j][p];
+      W2[q][p] = t;
+    }
+    tr1 = tr1 + W2[p][p];
+  }
+  for (int p = 0; p < n; ++p)
+    for (int q = p; q < n; ++q) {
+      cd t = {0.0, 0.0};
+      for (int j = 0; j < n; ++j) t = t + W2[q][j] * Cl[j][p];
+      W3[q][p] = W3[p][q] = t;
+    }
+  cd tr2 = {0.0, 0.0};
+  for (int p = 0; p < n; ++p)
+    for (int j = 0; j < n; ++j) tr2 = tr2 + Ci[j][p] * W3[j][p];
+  cd dmv = S * E * 6.0 * (tr2 - tr1) / 4.0;
+  {
+    cd cnm = {0.0, 0.0};
+    for (int i = 0; i < n; ++i)
+      cnm = cnm + (sy.q0 * sy.q[i]) *
+                      (4.0 * me_rxr_over_rij(n, W3, i, i, Ci, rm[i][i], trcj[i][i]) - 6.0 * tr1 * rm[i][i]);
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j)
+        cnm = cnm + (sy.q[i] * sy.q[j]) *
+                        (4.0 * me_rxr_over_rij(n, W3, i, j, Ci, rm[i][j], trcj[i][j]) - 6.0 * tr1 * rm[i][j]);
+    dmv = dmv - cnm / 4.0;
+  }
+  auto four_traces = [&](const Mat& A3, const Mat& A4, const cd& t1, const cd& t2, const cd& t3, const cd& t4) {
+    cd t5 = {0.0, 0.0};
+    for (int p = 0; p < n; ++p)
+      for (int j = 0; j < n; ++j) t5 = t5 + A3[p][j] * A4[j][p];
+    return S * (36.0 * t3 * t4 + 24.0 * t5 - 36.0 * t1 * t4 - 36.0 * t2 * t3 + 36.0 * t1 * t2) / 8.0;
+  };
+  {
+    tr1 = {0.0, 0.0};
+    for (int p = 0; p < n; ++p) {
+      cd cs = {0.0, 0.0};
+      for (int q = 0; q < n; ++q) cs = cs + Ck[q][p];
+      for (int q = 0; q < n; ++q) W1[q][p] = cs;
+      tr1 = tr1 + cs;
+    }
+    tr2 = {0.0, 0.0};
+    for (int p = 0; p < n; ++p) {
+      const double cnp = (mass(0) + mass(p + 1)) / (mass(0) * mass(0) * mass(p + 1) * mass(p + 1));
+      for (int q = 0; q < n; ++q) W2[q][p] = cnp * Cl[q][p];
+      tr2 = tr2 + W2[p][p];
+    }
+    matmul_(n, Ci, Ck, W5);
+    cd tr3 = {0.0, 0.0};
+    for (int p = 0; p < n; ++p)
+      for (int q = 0; q < n; ++q) {
+        cd t = {0.0, 0.0};
+        for (int j = 0; j < n; ++j) t = t + W5[q][j] * W1[j][p];
+        W3[q][p] = t;
+      }
+    for (int p = 0; p < n; ++p) tr3 = tr3 + W3[p][p];
+    matmul_(n, Ci, W2, W5);
+    matmul_(n, W5, Cl, W4);
+    const cd tr4 = trace_(n, W4);
+    dmv = dmv + four_traces(W3, W4, tr1, tr2, tr3, tr4);
+  }
+  for (int k = 0; k < n - 1; ++k) {
+    tr1 = Ck[k][k];
+    for (int p = 0; p <= k; ++p)
+      for (int q = 0; q < n; ++q) W2[q][p] = {0.0, 0.0};
+    tr2 = {0.0, 0.0};
+    for (int p = k + 1; p < n; ++p) {
+      const double cnp =
+          (mass(k + 1) + mass(p + 1)) / (mass(k + 1) * mass(k + 1) * mass(p + 1) * mass(p + 1));
+      for (int q = 0; q < n; ++q) W2[q][p] = cnp * Cl[q][p];
+      tr2 = tr2 + W2[p][p];
+    }
+    for (int p = 0; p < n; ++p)
+      for (int q = 0; q < n; ++q) W5[q][p] = Ck[q][k] * Ck[p][k];
+    matmul_(n, Ci, W5, W3);
+    const cd tr3 = trace_(n, W3);
+    matmul_(n, Ci, W2, W5);
+    matmul_(n, W5, Cl, W4);
+    const cd tr4 = trace_(n, W4);
+    dmv = dmv + four_traces(W3, W4, tr1, tr2, tr3, tr4);
+  }
+  out[m + 1] = dmv;
+  // Drachmanized Darwin (:1890-1933)
+  for (int a = 0; a < n; ++a)
+    for (int b = 0; b < n; ++b) W1[a][b] = cd{1.0 / (mass(0) * mass(0)), 0.0};
+  for (int i = 0; i < n; ++i) W1[i][i] = W1[i][i] + cd{1.0 / (mass(i + 1) * mass(i + 1)), 0.0};
+  cd dd = E * me_dxd(n, W1, Ci, Cl, S) / 4.0;
+  for (int i = 0; i < n; ++i)
+    dd = dd - (sy.q0 * sy.q[i]) * me_dxd_over_rij(n, W1, i, i, Ci, Cl, rm[i][i], trcj[i][i]) / 4.0;
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j)
+      dd = dd - (sy.q[i] * sy.q[j]) * me_dxd_over_rij(n, W1, i, j, Ci, Cl, rm[i][j], trcj[i][j]) / 4.0;
+  for (int a = 0; a < n; ++a)
+    for (int b = 0; b < n; ++b) {
+      W1[a][b] = {1.0, 0.0};
+      W2[a][b] = {0.0, 0.0};
+    }
+  for (int j = 0; j < n; ++j)
+    W2[j][j] = W2[j][j] + cd{(mass(0) + mass(j + 1)) / (mass(0) * mass(0) * mass(j + 1) * mass(j + 1)), 0.0};
+  dd = dd + me_dxd_dyd(n, W1, W2, Ci, Ck, Cl, S) / 8.0;
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) W1[a][b] = W2[a][b] = {0.0, 0.0};
+      W1[i][i] = {1.0, 0.0};
+      W2[j][j] = {1.0, 0.0};
+      const double cnm = (mass(i + 1) + mass(j + 1)) / (mass(i + 1) * mass(i + 1) * mass(j + 1) * mass(j + 1));
+      dd = dd + cnm * me_dxd_dyd(n, W1, W2, Ci, Ck, Cl, S) / 8.0;
+    }
+  out[m + 2] = dd;
+}
+
+void oracle_drach_pair(int n, int T, const int* P, const int* coef, const double* M, double q0, const double* q,
+                       const double* pk, const double* pl, int term, const double* masses, double E, double* out) {
+  System<double> sy = make_sys<double>(n, T, P, coef, M, q0, q);
+  const int m = n * (n + 1) / 2;
+  std::vector<cd> o(m + 3);
+  drach_pair(sy, pk, pl, P + (size_t)term * n * n, masses, E, o.data());
+  for (int a = 0; a < m + 3; ++a) st(out, a, o[a]);
+}
+
+// The Drachmanized values and OO in the vector c, with the driver's
+// contraction and rounding estimate (as oracle_expectation_values):
+// out, err: n(n+1)/2 + 3 reals each.
+int oracle_drachmanized(int n, int T, const int* P, const int* coef, const double* M, double q0, const double* q,
+                        int K, const double* params, const double* c, const double* diagS, const double* masses,
+                        double E, double* out, double* err, int nthreads) {
+  System<double> sy = make_sys<double>(n, T, P, coef, M, q0, q);
+  const int npt = n * (n + 1), nv = n * (n + 1) / 2 + 3;
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  const int nth = omp_get_max_threads();
+  std::vector<double> acc((size_t)nth * 2 * nv, 0.0);
+#pragma omp parallel
+  {
+    const int tid = omp_get_thread_num();
+    double* s_ = acc.data() + (size_t)tid * 2 * nv;
+    double* e_ = s_ + nv;
+    std::vector<cd> o(nv);
+#pragma omp for schedule(dynamic, 4)
+    for (int i = 0; i < K; ++i)
+      for (int j = 0; j <= i; ++j) {
+        const double rci = c[2 * i], ici = c[2 * i + 1], rcj = c[2 * j], icj = c[2 * j + 1];
+        const double factor = (i == j) ? 1.0 / std::sqrt(diagS[i] * diagS[i]) : 2.0 / std::sqrt(diagS[i] * diagS[j]);
+        for (int k = 0; k < T; ++k) {
+          drach_pair(sy, params + (size_t)npt * i, params + (size_t)npt * j, P + (size_t)k * n * n, masses, E,
+                     o.data());
+          for (int a = 0; a < nv; ++a) {
+            const double ro = o[a].re, io = o[a].im;
+            const double beta = (rci * rcj + ici * icj) * ro + (ici * rcj - rci * icj) * io;
+            const double mu = std::fabs(rci * (rcj * ro - icj * io)) + std::fabs(rcj * (rci * ro + ici * io)) +
+                              std::fabs(ici * (icj * ro + rcj * io)) + std::fabs(icj * (ici * ro - rci * io));
+            s_[a] += factor * coef[k] * beta;
+            e_[a] += std::fabs(factor * coef[k] * (mu + 2.0 * std::fabs(beta)));
+          }
+        }
+      }
+  }
+  for (int a = 0; a < nv; ++a) {
+    double sa = 0.0, ea = 0.0;
+    for (int t = 0; t < nth; ++t) {
+      sa += acc[(size_t)t * 2 * nv + a];
+      ea += acc[(size_t)t * 2 * nv + nv + a];
+    }
+    out[a] = sa;
+    if (err) err[a] = ea * 2.220446049250313e-16;
+  }
+  return 0;
+}
+
 // MV and Darwin in the vector c with the driver's contraction and rounding
 // estimate (as oracle_expectation_values): out[2], err[2].
 int oracle_relativistic(int n, int T, const int* P, const int* coef, const double* M, double q0, const double* q,

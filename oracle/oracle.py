@@ -81,6 +81,12 @@ def lib():
         L.oracle_relativistic.restype = ctypes.c_int
         L.oracle_relativistic.argtypes = [ctypes.c_int, ctypes.c_int, _i, _i, _d, ctypes.c_double, _d,
                                           ctypes.c_int, _d, _d, _d, _d, _d, _d, ctypes.c_int]
+        L.oracle_drach_pair.restype = None
+        L.oracle_drach_pair.argtypes = [ctypes.c_int, ctypes.c_int, _i, _i, _d, ctypes.c_double, _d, _d, _d,
+                                        ctypes.c_int, _d, ctypes.c_double, _d]
+        L.oracle_drachmanized.restype = ctypes.c_int
+        L.oracle_drachmanized.argtypes = [ctypes.c_int, ctypes.c_int, _i, _i, _d, ctypes.c_double, _d,
+                                          ctypes.c_int, _d, _d, _d, _d, ctypes.c_double, _d, _d, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -326,6 +332,39 @@ def relativistic(system, params, c, diagS, masses, nthreads=0):
                               np.ascontiguousarray(c, dtype=np.complex128).ctypes.data_as(_d),
                               _pd(np.ascontiguousarray(diagS, dtype=np.float64)),
                               _pd(np.ascontiguousarray(masses, dtype=np.float64)), _pd(out), _pd(err), nthreads)
+    return out, err
+
+
+def n_drach_values(n):
+    """Number of values of drach_pair / drachmanized for n pseudo-particles."""
+    return n * (n + 1) // 2 + 3
+
+
+def drach_pair(system, pk, pl, term, masses, E):
+    """Drachmanized delta(r) of every pair (a, b >= a), then the orbit-orbit
+    term, the Drachmanized mass-velocity and Darwin terms of one pair and
+    symmetry term, at the current energy E (complex, n_drach_values(n))."""
+    s = _Sys(system)
+    out = np.zeros(n_drach_values(system.n), np.complex128)
+    lib().oracle_drach_pair(*s.args(), _pd(np.ascontiguousarray(pk, dtype=np.float64)),
+                            _pd(np.ascontiguousarray(pl, dtype=np.float64)), int(term),
+                            _pd(np.ascontiguousarray(masses, dtype=np.float64)), float(E), out.ctypes.data_as(_d))
+    return out
+
+
+def drachmanized(system, params, c, diagS, masses, E, nthreads=0):
+    """The driver's Drachmanized values and orbit-orbit term in the vector c
+    and their rounding estimates: (values, errors), n_drach_values(n) reals."""
+    s = _Sys(system)
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    nv = n_drach_values(system.n)
+    out = np.zeros(nv)
+    err = np.zeros(nv)
+    lib().oracle_drachmanized(*s.args(), params.shape[0], _pd(params),
+                              np.ascontiguousarray(c, dtype=np.complex128).ctypes.data_as(_d),
+                              _pd(np.ascontiguousarray(diagS, dtype=np.float64)),
+                              _pd(np.ascontiguousarray(masses, dtype=np.float64)), float(E), _pd(out), _pd(err),
+                              nthreads)
     return out, err
 
 

@@ -165,3 +165,97 @@ def test_expectation_contraction_against_reference(name, K):
     dev = np.abs(got - acc)
     assert np.all(dev <= err + 1e-300), (name, int(np.argmax(dev / err)), np.max(dev / err))
     print(name, "worst deviation / tolerance", np.max(dev / err))
+
+
+# ----------------------------------------------------- Drachmanized forms
+E_CURR = -1.1   # the current energy the Drachmanized forms take (Glob_CurrEnergy)
+
+
+def _drach_sensitivity(sy, pk, pl, t, trials=4, seed=0):
+    rng = np.random.default_rng(seed)
+    o0 = oracle.drach_pair(sy, pk, pl, t, sy.masses, E_CURR)
+    s = np.zeros(o0.size)
+    for _ in range(trials):
+        a = pk * (1 + 4 * EPS * rng.standard_normal(pk.shape))
+        b = pl * (1 + 4 * EPS * rng.standard_normal(pl.shape))
+        s = np.maximum(s, np.abs(oracle.drach_pair(sy, a, b, t, sy.masses, E_CURR) - o0))
+    return o0, s
+
+
+def _ref_drach(r, n):
+    d = [r["drach_delta"][j, i] for i in range(n) for j in range(i, n)]
+    return np.array(d + [r["OO"], r["drach_MV"], r["drach_Darwin"]])
+
+
+@pytest.mark.parametrize("name", SYSTEMS)
+def test_drachmanized_pairs_against_reference(name):
+    """The oracle's Drachmanized delta, orbit-orbit, Drachmanized
+    mass-velocity and Darwin pair values (src/matelem.f90:1276-1478,
+    1669-1933) against the compiled reference, every term, to each value's
+    rounding bound.  At n = 3 this includes the entries of the 1/(r_ij r_pq)
+    table that read the reference's never-written TrCJCJ entries (zero in
+    its static storage)."""
+    sy = _system(name)
+    n = sy.n
+    ref.setup(sy, E_CURR)
+    p = seeded_basis(n, NB, SEED_BASE + 6700 + len(name))
+    ident = np.eye(n, dtype=np.int32)
+    worst = 0.0
+    for t in range(sy.T):
+        for a, b in _pairs(NB):
+            o, sens = _drach_sensitivity(sy, p[a], p[b], t)
+            want = _ref_drach(ref.expc_pair(sy, p[a], p[b], ident, sy.yhy_mats[t]), n)
+            # these values are differences of larger terms (E <1/r>, the Coulomb sums, <r'Xr/r>);
+            # the rounding of their parts reaches a few 1e-12 of the value (measured worst 5e-12)
+            tol = np.maximum(2e-11 * np.abs(want), 16 * sens)
+            dev = np.abs(o - want)
+            assert np.all(dev <= tol), (name, t, a, b, int(np.argmax(dev / tol)), np.max(dev / tol))
+            worst = max(worst, np.max(dev / tol))
+    print(name, "worst deviation / tolerance", worst)
+
+
+@pytest.mark.parametrize("name", SYSTEMS)
+def test_drachmanized_mass_velocity_equals_darwin(name):
+    """The reference computes the Drachmanized mass-velocity and Darwin pair
+    values with two different sets of formulas (:1669-1865 through
+    ME_rXr_over_rij and traces, :1890-1933 through ME_dXd, ME_dXd_dYd and
+    ME_dXd_over_rij) that give the same number: an invariant that pins the
+    two restatements against each other."""
+    sy = _system(name)
+    p = seeded_basis(sy.n, NB, SEED_BASE + 6800 + len(name))
+    m = sy.n * (sy.n + 1) // 2
+    for t in range(sy.T):
+        for a, b in _pairs(NB):
+            o, sens = _drach_sensitivity(sy, p[a], p[b], t)
+            tol = max(2e-11 * abs(o[m + 1]), 16 * (sens[m + 1] + sens[m + 2]))
+            assert abs(o[m + 1] - o[m + 2]) <= tol, (name, t, a, b, o[m + 1], o[m + 2], tol)
+
+
+@pytest.mark.parametrize("name,K", [("hminus", 5), ("h2", 4), ("li", 4)])
+def test_drachmanized_contraction_against_reference(name, K):
+    """oracle.drachmanized against the reference's per-pair values
+    contracted as the driver does (src/workproc.f90:8158-8235)."""
+    sy = _system(name)
+    n = sy.n
+    ref.setup(sy, E_CURR)
+    p = seeded_basis(n, K, SEED_BASE + 6900 + K)
+    rng = np.random.Generator(np.random.PCG64(6901))
+    c = rng.normal(size=K) + 1j * rng.normal(size=K)
+    _, _, _, dS = oracle.assemble(sy, p, "G")
+    ident = np.eye(n, dtype=np.int32)
+    nv = oracle.n_drach_values(n)
+    acc = np.zeros(nv)
+    err = np.zeros(nv)
+    for i in range(K):
+        for j in range(i + 1):
+            f = (1.0 if i == j else 2.0) / np.sqrt(dS[i] * dS[j])
+            w = np.conj(c[i]) * c[j]
+            for t in range(sy.T):
+                O = _ref_drach(ref.expc_pair(sy, p[i], p[j], ident, sy.yhy_mats[t]), n)
+                _, sens = _drach_sensitivity(sy, p[i], p[j], t, trials=2)
+                wt = f * sy.yhy_coef[t]
+                acc += wt * (w * O).real
+                err += abs(wt) * abs(w) * np.maximum(16 * sens, 2e-11 * np.abs(O))
+    vals, _ = oracle.drachmanized(sy, p, c, dS, sy.masses, E_CURR)
+    dev = np.abs(vals - acc)
+    assert np.all(dev <= err + 1e-300), (name, int(np.argmax(dev / err)), np.max(dev / err))
