@@ -1846,13 +1846,17 @@ int ecg_expectation_values(ecg_ctx* x, const double* c, double* values, double* 
 // Shared set-up of the post-processing values (correlation functions,
 // densities, relativistic terms): the partial-sum buffer, diagS and the
 // vector on the device.
-static int post_values_setup(ecg_ctx* x, const double* c, size_t nout, const zd** cv, double** dout) {
+// npart: doubles of per-tile partials the kernel needs (default: those of
+// the correlation-function and expectation kernels)
+static int post_values_setup(ecg_ctx* x, const double* c, size_t nout, const zd** cv, double** dout,
+                             size_t npart = 0) {
   ecg::Ctx& cx = x->c;
   const int K = cx.K;
   if (K < 1 || !x->params_set) return ECG_ERR_STATE;
   if (!c && !(x->vec_current && x->last_K == K)) return ECG_ERR_STATE;
   if (c && !finite_all(c, 2 * (size_t)K)) return ECG_ERR_ARG;
-  const size_t need = ecg::correlation_partial_doubles(cx.n, K) + nout;
+  if (npart == 0) npart = ecg::correlation_partial_doubles(cx.n, K);
+  const size_t need = npart + nout;
   if (need > x->xpart_cap) {
     x->xpart_cap = 0;   // stays 0 if the allocation fails
     RCCHK(dev_alloc(&x->xpart, need));
@@ -1868,7 +1872,7 @@ static int post_values_setup(ecg_ctx* x, const double* c, size_t nout, const zd*
     HIPCHK(hipMemcpyAsync(x->ev[1], c, sizeof(zd) * K, hipMemcpyHostToDevice, x->stream));
     *cv = x->ev[1];
   }
-  *dout = x->xpart + ecg::correlation_partial_doubles(cx.n, K);
+  *dout = x->xpart + npart;
   return ECG_OK;
 }
 
@@ -1902,6 +1906,25 @@ int ecg_relativistic_corrections(ecg_ctx* x, const double* c, const double* mass
     errors[1] = h[3] * 2.220446049250313e-16;
   }
   return finite_all(values, 2) ? ECG_OK : ECG_ERR_NONFINITE;
+}
+
+int ecg_drachmanized_corrections(ecg_ctx* x, const double* c, const double* masses, double E, double* values,
+                                 double* errors) {
+  (void)hipGetLastError();   // start from a clean runtime error state
+  if (!x || !values || !masses || !masses_ok(x, masses) || !std::isfinite(E)) return ECG_ERR_ARG;
+  const int n = x->c.n, nv = n * (n + 1) / 2 + 3;
+  const zd* cv = nullptr;
+  double* dout = nullptr;
+  RCCHK(post_values_setup(x, c, 2 * (size_t)nv, &cv, &dout, ecg::drachmanized_partial_doubles(n, x->c.K)));
+  RCCHK(ecg::drachmanized_values(x->c, cv, masses, E, x->xpart, dout, x->stream));
+  double h[2 * 18];
+  HIPCHK(hipMemcpyAsync(h, dout, sizeof(double) * 2 * nv, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  for (int a = 0; a < nv; ++a) {
+    values[a] = h[a];
+    if (errors) errors[a] = h[nv + a] * 2.220446049250313e-16;   // epsilon(1.0)
+  }
+  return finite_all(values, nv) ? ECG_OK : ECG_ERR_NONFINITE;
 }
 
 int ecg_correlation_functions(ecg_ctx* x, const double* c, int ngrid, const double* grid, double* values) {

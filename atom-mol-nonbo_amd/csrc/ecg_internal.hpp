@@ -143,6 +143,12 @@ int particle_densities(const Ctx& c, const zd* cvec, const double* masses, int n
 int relativistic_values(const Ctx& c, const zd* cvec, const double* masses, double* part, double* out,
                         hipStream_t s);
 size_t correlation_partial_doubles(int n, int K);
+// Drachmanized delta of every pair (a, b >= a), orbit-orbit, Drachmanized
+// mass-velocity and Darwin (n(n+1)/2 + 3 values, then as many bound sums)
+// at the current energy E; part: drachmanized_partial_doubles(n, K).
+int drachmanized_values(const Ctx& c, const zd* cvec, const double* masses, double E, double* part, double* out,
+                        hipStream_t s);
+size_t drachmanized_partial_doubles(int n, int K);
 
 // dense Hermitian linear algebra (ecg_linalg.hip)
 struct Factor {

@@ -1038,6 +1038,661 @@ int relativistic_values(const Ctx& c, const zd* cvec, const double* masses, doub
   return -1;
 }
 
+// ------------------------------- Drachmanized forms and orbit-orbit term
+// The remaining pair values of MatrixElementsForExpcVals: the Drachmanized
+// delta(r_pq) of every pair (a, b >= a) (src/matelem.f90:1353-1392), the
+// orbit-orbit term (:1394-1477), the Drachmanized mass-velocity (:1669-1865)
+// and Darwin (:1890-1933) terms, with the helper integrals
+// <r'Xr / r_ij> (ME_rXr_over_rij, :753-818), <(r'Xr)(r'Yr) / r_ij>
+// (ME_rXr_rYr_over_rij, :821-945), ME_dXd_dYd, ME_dXd, ME_dXd_over_rij
+// (:1998-2044), on full n x n matrices in registers (indices known at
+// compile time after unrolling).
+//
+// <1/(r_ij r_pq)> (:1305-1343) is evaluated per entry instead of as a table,
+// with the value the reference's table holds: 2 S / TrCJ_ij for the same
+// pair; else, with t = tr[C^-1 J_ij C^-1 J_pq] as the reference's loops
+// (:1275-1302) leave it, (4 S / pi) asin(sqrt t / sqrt(TrCJ_ij TrCJ_pq))
+// / sqrt(t) (asin z = -i log(i z + sqrt(1 - z^2))), or (4 S / pi) /
+// sqrt(TrCJ_ij TrCJ_pq) when t is zero.  Those loops leave t unwritten --
+// zero in the reference's static storage -- for a diagonal pair (x, x)
+// against an off-diagonal pair (y, z) with x < y; the other entries are
+// squares of the entry's own combination of C^-1 in the order of the loop
+// iteration that wrote it last (tc_entry).
+struct DrachConst {
+  double mass[6];   // particle masses, particle 1 first (Glob_Mass)
+  double E;         // current energy (Glob_CurrEnergy)
+};
+
+__device__ inline zd zdiv(zd a, zd b) { return a * zrcp(b); }
+__device__ inline zd zlog(zd z) { return mkz(log(hypot(z.x, z.y)), atan2(z.y, z.x)); }
+
+template <int N>
+__device__ __attribute__((always_inline)) inline zd dr_rxr_over_rij(const zd (&X)[N][N], int i, int j,
+                                                                    const zd (&Ci)[N][N], zd rm, zd trcj) {
+  zd trcx = mkz(0.0, 0.0);
+#pragma unroll
+  for (int m = 0; m < N; ++m)
+#pragma unroll
+    for (int p = 0; p < N; ++p) trcx = zfma(Ci[m][p], X[p][m], trcx);
+  zd t = mkz(0.0, 0.0);
+#pragma unroll
+  for (int m = 0; m < N; ++m) {
+    zd cxi = mkz(0.0, 0.0), cxj = mkz(0.0, 0.0);
+#pragma unroll
+    for (int p = 0; p < N; ++p) {
+      cxi = zfma(Ci[i][p], X[p][m], cxi);
+      if (j != i) cxj = zfma(Ci[j][p], X[p][m], cxj);
+    }
+    if (i == j) t = zfma(cxi, Ci[m][i], t);
+    else t = zfma(cxi - cxj, Ci[m][i] - Ci[m][j], t);
+  }
+  return (0.5 * rm) * (3.0 * trcx - zdiv(t, trcj));
+}
+
+template <int N>
+__device__ __attribute__((always_inline)) inline zd dr_rxr_ryr_over_rij(const zd (&X)[N][N], const zd (&Y)[N][N],
+                                                                        int i, int j, const zd (&Ci)[N][N], zd rm,
+                                                                        zd trcj) {
+  zd CX[N][N], CY[N][N];
+#pragma unroll
+  for (int p = 0; p < N; ++p)
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      zd ax = mkz(0.0, 0.0), ay = mkz(0.0, 0.0);
+#pragma unroll
+      for (int m = 0; m < N; ++m) {
+        const zd xs = (m == q) ? X[m][m] : 0.5 * (X[m][q] + X[q][m]);
+        const zd ys = (m == q) ? Y[m][m] : 0.5 * (Y[m][q] + Y[q][m]);
+        ax = zfma(Ci[p][m], xs, ax);
+        ay = zfma(Ci[p][m], ys, ay);
+      }
+      CX[p][q] = ax;
+      CY[p][q] = ay;
+    }
+  zd trcx = mkz(0.0, 0.0), trcy = mkz(0.0, 0.0), trcxcy = mkz(0.0, 0.0);
+#pragma unroll
+  for (int m = 0; m < N; ++m) {
+    trcy = trcy + CY[m][m];
+    trcx = trcx + CX[m][m];
+  }
+#pragma unroll
+  for (int m = 0; m < N; ++m)
+#pragma unroll
+    for (int p = 0; p < N; ++p) trcxcy = zfma(CX[m][p], CY[p][m], trcxcy);
+  zd tx = mkz(0.0, 0.0), ty = mkz(0.0, 0.0), txy = mkz(0.0, 0.0);
+#pragma unroll
+  for (int m = 0; m < N; ++m) {
+    zd xyi = mkz(0.0, 0.0), xyj = mkz(0.0, 0.0);
+#pragma unroll
+    for (int p = 0; p < N; ++p) {
+      xyi = zfma(CX[i][p], CY[p][m], xyi);
+      if (j != i) xyj = zfma(CX[j][p], CY[p][m], xyj);
+    }
+    if (i == j) {
+      tx = zfma(CX[i][m], Ci[m][i], tx);
+      ty = zfma(CY[i][m], Ci[m][i], ty);
+      txy = zfma(xyi, Ci[m][i], txy);
+    } else {
+      const zd d = Ci[m][i] - Ci[m][j];
+      tx = zfma(CX[i][m] - CX[j][m], d, tx);
+      ty = zfma(CY[i][m] - CY[j][m], d, ty);
+      txy = zfma(xyi - xyj, d, txy);
+    }
+  }
+  const zd r = zrcp(trcj);
+  return (0.25 * rm) * (9.0 * (trcy * trcx) + 6.0 * trcxcy - (3.0 * (trcy * tx + trcx * ty) + 4.0 * txy) * r +
+                        3.0 * (tx * ty) * (r * r));
+}
+
+template <int N>
+__device__ __attribute__((always_inline)) inline void dr_matmul(const zd (&A)[N][N], const zd (&B)[N][N],
+                                                                zd (&C)[N][N]) {
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      zd s = mkz(0.0, 0.0);
+#pragma unroll
+      for (int k = 0; k < N; ++k) s = zfma(A[i][k], B[k][j], s);
+      C[i][j] = s;
+    }
+}
+template <int N> __device__ __attribute__((always_inline)) inline zd dr_trace(const zd (&A)[N][N]) {
+  zd s = mkz(0.0, 0.0);
+#pragma unroll
+  for (int i = 0; i < N; ++i) s = s + A[i][i];
+  return s;
+}
+template <int N>
+__device__ __attribute__((always_inline)) inline zd dr_four_traces(const zd (&A3)[N][N], const zd (&A4)[N][N],
+                                                                   zd t1, zd t2, zd t3, zd t4, zd S) {
+  zd t5 = mkz(0.0, 0.0);
+#pragma unroll
+  for (int p = 0; p < N; ++p)
+#pragma unroll
+    for (int j = 0; j < N; ++j) t5 = zfma(A3[p][j], A4[j][p], t5);
+  return S * (36.0 * (t3 * t4) + 24.0 * t5 - 36.0 * (t1 * t4) - 36.0 * (t2 * t3) + 36.0 * (t1 * t2)) * 0.125;
+}
+
+// ME_dXd_over_rij (X real here: a mass-weight matrix)
+template <int N>
+__device__ __attribute__((always_inline)) inline zd dr_dxd_over_rij(const double (&X)[N][N], int i, int j,
+                                                                    const zd (&Ci)[N][N], const zd (&Cl)[N][N],
+                                                                    zd rm, zd trcj) {
+  zd XC[N][N], M[N][N];
+#pragma unroll
+  for (int a = 0; a < N; ++a)
+#pragma unroll
+    for (int b = 0; b < N; ++b) {
+      zd s = mkz(0.0, 0.0);
+#pragma unroll
+      for (int k = 0; k < N; ++k) s = s + X[a][k] * Cl[k][b];
+      XC[a][b] = s;
+    }
+  dr_matmul<N>(Cl, XC, M);
+  zd tr1 = mkz(0.0, 0.0);
+#pragma unroll
+  for (int a = 0; a < N; ++a)
+#pragma unroll
+    for (int k = 0; k < N; ++k) tr1 = tr1 + Cl[a][k] * X[k][a];
+  return 4.0 * dr_rxr_over_rij<N>(M, i, j, Ci, rm, trcj) - 6.0 * (tr1 * rm);
+}
+
+// ME_dXd_dYd for real X, Y
+template <int N>
+__device__ __attribute__((always_inline)) inline zd dr_dxd_dyd(const double (&X)[N][N], const double (&Y)[N][N],
+                                                               const zd (&Ci)[N][N], const zd (&Ck)[N][N],
+                                                               const zd (&Cl)[N][N], zd S) {
+  zd T1[N][N], T2[N][N], M[N][N], MM[N][N];
+  zd tr1 = mkz(0.0, 0.0), tr2 = mkz(0.0, 0.0);
+#pragma unroll
+  for (int a = 0; a < N; ++a)
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      tr1 = tr1 + X[a][k] * Ck[k][a];
+      tr2 = tr2 + Cl[a][k] * Y[k][a];
+    }
+#pragma unroll
+  for (int a = 0; a < N; ++a)
+#pragma unroll
+    for (int b = 0; b < N; ++b) {
+      zd s = mkz(0.0, 0.0);
+#pragma unroll
+      for (int k = 0; k < N; ++k) s = s + X[a][k] * Ck[k][b];
+      T1[a][b] = s;
+    }
+  dr_matmul<N>(Ck, T1, T2);
+  dr_matmul<N>(Ci, T2, M);
+  const zd tr3 = dr_trace<N>(M);
+#pragma unroll
+  for (int a = 0; a < N; ++a)
+#pragma unroll
+    for (int b = 0; b < N; ++b) {
+      zd s = mkz(0.0, 0.0);
+#pragma unroll
+      for (int k = 0; k < N; ++k) s = s + Y[a][k] * Cl[k][b];
+      T1[a][b] = s;
+    }
+  dr_matmul<N>(Cl, T1, T2);
+  dr_matmul<N>(Ci, T2, MM);
+  const zd tr4 = dr_trace<N>(MM);
+  return 8.0 * dr_four_traces<N>(M, MM, tr1, tr2, tr3, tr4, S);
+}
+
+// tr[C^-1 J_ab C^-1 J_cd] as the reference's table holds it (see above);
+// pairs (a <= b), (c <= d).
+template <int N>
+__device__ __attribute__((always_inline)) inline zd tc_entry(int a, int b, int c, int d, const zd (&Ci)[N][N]) {
+  if (a == b && c == d) {
+    const zd v = (a == c) ? Ci[a][a] : Ci[a > c ? a : c][a > c ? c : a];
+    return v * v;
+  }
+  if (a == b || c == d) {   // diagonal (x, x) against (y, z), y < z
+    const int x = (a == b) ? a : c, y = (a == b) ? c : a, z = (a == b) ? d : b;
+    if (x < y) return mkz(0.0, 0.0);   // never written
+    const zd t = Ci[x][y] - Ci[x][z];
+    return t * t;
+  }
+  // two off-diagonal pairs: the iteration (i, j, p, q) with p >= i, the
+  // last one in loop order when a == c
+  int i, j, p, q;
+  if (a < c) { i = a; j = b; p = c; q = d; }
+  else if (c < a) { i = c; j = d; p = a; q = b; }
+  else { i = a; j = b > d ? b : d; p = a; q = b > d ? d : b; }
+  const zd u = Ci[q][i] - Ci[p][i] - Ci[q][j] + Ci[p][j];
+  return u * u;
+}
+
+template <int N>
+__device__ __attribute__((always_inline)) inline zd rr_entry(int a, int b, int c, int d, const zd (&Ci)[N][N],
+                                                             const zd (&trcj)[N][N], zd S) {
+  constexpr double inv_pi = 0.318309886183790671537767526745028724;
+  if (a == c && b == d) return zdiv(2.0 * S, trcj[a][b]);
+  const zd ca = zsqrt(tc_entry<N>(a, b, c, d, Ci));
+  const zd c4 = (4.0 * inv_pi) * S;
+  const zd prod = trcj[a][b] * trcj[c][d];
+  if (ca.x != 0.0 || ca.y != 0.0) {
+    const zd ca1 = zdiv(ca, zsqrt(prod));
+    const zd w = mkz(-ca1.y, ca1.x) + zsqrt(mkz(1.0, 0.0) - ca1 * ca1);   // i ca1 + sqrt(1 - ca1^2)
+    const zd lw = zlog(w);
+    return zdiv(c4 * mkz(lw.y, -lw.x), ca);   // -i log(w) / ca
+  }
+  return zdiv(c4, zsqrt(prod));
+}
+
+// The n(n+1)/2 + 3 values of one pair and term.
+template <int N>
+__device__ __attribute__((always_inline)) inline void drach_values(const SysConst& sc, const zd* ck, const zd* ct,
+                                                                   const PairCore<N>& pc, const DrachConst& dc,
+                                                                   zd* out) {
+  constexpr double pi = 3.14159265358979323846264338327950288, sqrtpi = 1.77245385090551602729816748334115;
+  zd Ci[N][N], Ck[N][N], Cl[N][N];
+#pragma unroll
+  for (int a = 0; a < N; ++a)
+#pragma unroll
+    for (int b = 0; b < N; ++b) {
+      const int e = a >= b ? sidx(a, b) : sidx(b, a);
+      Ci[a][b] = pc.Ci[e];
+      Ck[a][b] = ck[e];
+      Cl[a][b] = ct[e];
+    }
+  const zd S = pc.S;
+  zd trcj[N][N], rm[N][N];
+  const zd cn = (2.0 / sqrtpi) * S;
+#pragma unroll
+  for (int a = 0; a < N; ++a)
+#pragma unroll
+    for (int b = 0; b < N; ++b) {
+      trcj[a][b] = (a == b) ? Ci[a][a] : Ci[a][a] + Ci[b][b] - Ci[a > b ? a : b][a > b ? b : a] -
+                                             Ci[a > b ? a : b][a > b ? b : a];
+      rm[a][b] = zdiv(cn, zsqrt(trcj[a][b]));
+    }
+  const double* mass = dc.mass;
+  // Drachmanized delta(r_pq).  The loops over particle indices stay rolled
+  // (the matrix work inside them is unrolled): unrolled, they multiply the
+  // code of these helpers by the number of index pairs.
+  {
+    zd W2[N][N];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        zd t = mkz(0.0, 0.0);
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          zd mc = mkz(0.0, 0.0);
+#pragma unroll
+          for (int r = 0; r < N; ++r) mc = mc + sc.M[k][r] * Cl[r][j];
+          t = zfma(Ck[i][k], mc, t);
+        }
+        W2[i][j] = t;
+      }
+    int o = 0;
+#pragma unroll 1
+    for (int p = 0; p < N; ++p)
+#pragma unroll 1
+      for (int q = p; q < N; ++q, ++o) {
+        zd cts = mkz(0.0, 0.0);
+#pragma unroll 1
+        for (int i = 0; i < N; ++i) {
+          const bool first_ii = p < i || (p == i && q <= i);
+          cts = cts + (sc.q0 * sc.q[i]) * (first_ii ? rr_entry<N>(p, q, i, i, Ci, trcj, S)
+                                                    : rr_entry<N>(i, i, p, q, Ci, trcj, S));
+#pragma unroll 1
+          for (int j = i + 1; j < N; ++j) cts = cts + (sc.q[i] * sc.q[j]) * rr_entry<N>(p, q, i, j, Ci, trcj, S);
+        }
+        const double ca = (p == q) ? 2.0 * pi * sc.M[p][p] : 2.0 * pi * (sc.M[p][p] + sc.M[q][q] - 2.0 * sc.M[p][q]);
+        const zd cnm = dr_rxr_over_rij<N>(W2, p, q, Ci, rm[p][q], trcj[p][q]);
+        out[o] = (1.0 / ca) * (dc.E * rm[p][q] - cts - 4.0 * cnm);
+      }
+  }
+  constexpr int m = N * (N + 1) / 2;
+  // orbit-orbit
+  {
+    zd OO = mkz(0.0, 0.0);
+#pragma unroll 1
+    for (int i = 0; i < N; ++i)
+#pragma unroll 1
+      for (int j = 0; j < N; ++j) {
+        const zd tr1 = Cl[j][i], tr3 = 3.0 * Cl[j][j];
+        zd W1[N][N], W2[N][N], W3[N][N];
+#pragma unroll
+        for (int p = 0; p < N; ++p)
+#pragma unroll
+          for (int q = 0; q < N; ++q) {
+            W1[q][p] = Cl[q][i] * Cl[p][j] + Cl[q][j] * Cl[p][i];
+            W2[p][q] = (Ck[p][j] + Cl[p][j]) * Cl[q][j];
+            W3[p][q] = mkz(0.0, 0.0);
+          }
+#pragma unroll
+        for (int p = 0; p < N; ++p) W1[p][j] = W1[p][j] + (Ck[p][j] + Cl[p][j]) * Cl[j][i];
+#pragma unroll
+        for (int p = 0; p < N; ++p) W1[j][p] = W1[j][p] + Cl[i][j] * Cl[p][j] + tr3 * Cl[p][i];
+#pragma unroll
+        for (int p = 0; p < N; ++p) W3[j][p] = Cl[p][i];
+        const zd cnm = dr_rxr_over_rij<N>(W1, j, j, Ci, rm[j][j], trcj[j][j]);
+        const zd ctt = dr_rxr_ryr_over_rij<N>(W2, W3, j, j, Ci, rm[j][j], trcj[j][j]);
+        const zd ca = -6.0 * ((tr1 + tr1) * rm[j][j]) + 4.0 * cnm - 8.0 * ctt;
+        OO = OO - (sc.q[j] / mass[j + 1]) * ca;
+      }
+    OO = (sc.q0 / mass[0]) * OO;
+#pragma unroll 1
+    for (int i = 0; i < N; ++i)
+#pragma unroll 1
+      for (int j = i + 1; j < N; ++j) {
+        const zd tr1 = Cl[j][i], tr3 = 3.0 * Cl[j][j];
+        zd W1[N][N], W2[N][N], W3[N][N];
+#pragma unroll
+        for (int p = 0; p < N; ++p)
+#pragma unroll
+          for (int q = 0; q < N; ++q) {
+            W1[q][p] = Cl[q][i] * Cl[p][j] + Cl[q][j] * Cl[p][i];
+            W2[p][q] = (Ck[p][j] + Cl[p][j]) * Cl[q][j];
+            W3[p][q] = mkz(0.0, 0.0);
+          }
+#pragma unroll
+        for (int p = 0; p < N; ++p) {
+          const zd t = (Ck[p][j] + Cl[p][j]) * Cl[j][i];
+          W1[p][j] = W1[p][j] + t;
+          W1[p][i] = W1[p][i] - t;
+        }
+#pragma unroll
+        for (int p = 0; p < N; ++p) {
+          const zd t = Cl[i][j] * Cl[p][j] + tr3 * Cl[p][i];
+          W1[j][p] = W1[j][p] + t;
+          W1[i][p] = W1[i][p] - t;
+        }
+#pragma unroll
+        for (int p = 0; p < N; ++p) {
+          W3[j][p] = Cl[p][i];
+          W3[i][p] = -Cl[p][i];
+        }
+        const zd cnm = dr_rxr_over_rij<N>(W1, i, j, Ci, rm[i][j], trcj[i][j]);
+        const zd ctt = dr_rxr_ryr_over_rij<N>(W2, W3, i, j, Ci, rm[i][j], trcj[i][j]);
+        const zd ca = -6.0 * ((tr1 + tr1) * rm[i][j]) + 4.0 * cnm - 8.0 * ctt;
+        OO = OO + (sc.q[i] * sc.q[j] / (mass[i + 1] * mass[j + 1])) * ca;
+      }
+    out[m] = 0.5 * OO;
+  }
+  // Drachmanized mass-velocity
+  {
+    double Wm[N][N];
+    const double c0 = 1.0 / (mass[0] * mass[0]);
+#pragma unroll
+    for (int p = 0; p < N; ++p)
+#pragma unroll
+      for (int q = 0; q < N; ++q) Wm[p][q] = (p == q) ? c0 + 1.0 / (mass[p + 1] * mass[p + 1]) : c0;
+    zd W2[N][N], W3[N][N];
+    zd tr1 = mkz(0.0, 0.0);
+#pragma unroll
+    for (int p = 0; p < N; ++p)
+#pragma unroll
+      for (int q = 0; q < N; ++q) {
+        zd t = mkz(0.0, 0.0);
+#pragma unroll
+        for (int j = 0; j < N; ++j) t = t + Cl[j][q] * Wm[j][p];
+        W2[q][p] = t;
+      }
+#pragma unroll
+    for (int p = 0; p < N; ++p) tr1 = tr1 + W2[p][p];
+    dr_matmul<N>(W2, Cl, W3);
+    zd tr2 = mkz(0.0, 0.0);
+#pragma unroll
+    for (int p = 0; p < N; ++p)
+#pragma unroll
+      for (int j = 0; j < N; ++j) tr2 = zfma(Ci[j][p], W3[j][p], tr2);
+    zd dmv = (6.0 * 0.25 * dc.E) * (S * (tr2 - tr1));
+    zd cnm = mkz(0.0, 0.0);
+#pragma unroll 1
+    for (int i = 0; i < N; ++i)
+      cnm = cnm + (sc.q0 * sc.q[i]) *
+                      (4.0 * dr_rxr_over_rij<N>(W3, i, i, Ci, rm[i][i], trcj[i][i]) - 6.0 * (tr1 * rm[i][i]));
+#pragma unroll 1
+    for (int i = 0; i < N; ++i)
+#pragma unroll 1
+      for (int j = i + 1; j < N; ++j)
+        cnm = cnm + (sc.q[i] * sc.q[j]) *
+                        (4.0 * dr_rxr_over_rij<N>(W3, i, j, Ci, rm[i][j], trcj[i][j]) - 6.0 * (tr1 * rm[i][j]));
+    dmv = dmv - 0.25 * cnm;
+    // third term: J Ck* and the first particle's pair weights
+    {
+      zd W4[N][N], W5[N][N], A3[N][N];
+      tr1 = mkz(0.0, 0.0);
+      zd colsum[N];
+#pragma unroll
+      for (int p = 0; p < N; ++p) {
+        zd c = mkz(0.0, 0.0);
+#pragma unroll
+        for (int q = 0; q < N; ++q) c = c + Ck[q][p];
+        colsum[p] = c;
+        tr1 = tr1 + c;
+      }
+      zd tr2b = mkz(0.0, 0.0);
+#pragma unroll
+      for (int p = 0; p < N; ++p) {
+        const double cnp = (mass[0] + mass[p + 1]) / (mass[0] * mass[0] * mass[p + 1] * mass[p + 1]);
+#pragma unroll
+        for (int q = 0; q < N; ++q) W2[q][p] = cnp * Cl[q][p];
+        tr2b = tr2b + W2[p][p];
+      }
+      dr_matmul<N>(Ci, Ck, W5);
+      zd tr3 = mkz(0.0, 0.0);
+#pragma unroll
+      for (int q = 0; q < N; ++q)
+#pragma unroll
+        for (int p = 0; p < N; ++p) {
+          zd t = mkz(0.0, 0.0);
+#pragma unroll
+          for (int j = 0; j < N; ++j) t = t + W5[q][j];
+          A3[q][p] = t * colsum[p];   // (C^-1 Ck* J Ck*)(q, p) = rowsum_q(C^-1 Ck*) colsum_p(Ck*)
+        }
+#pragma unroll
+      for (int p = 0; p < N; ++p) tr3 = tr3 + A3[p][p];
+      dr_matmul<N>(Ci, W2, W5);
+      dr_matmul<N>(W5, Cl, W4);
+      const zd tr4 = dr_trace<N>(W4);
+      dmv = dmv + dr_four_traces<N>(A3, W4, tr1, tr2b, tr3, tr4, S);
+    }
+    // fourth term: pairs of pseudo-particles
+#pragma unroll 1
+    for (int k = 0; k < N - 1; ++k) {
+      zd W4[N][N], W5[N][N], A3[N][N];
+      const zd t1 = Ck[k][k];
+      zd t2 = mkz(0.0, 0.0);
+#pragma unroll
+      for (int p = 0; p < N; ++p) {
+        const double cnp = (p > k) ? (mass[k + 1] + mass[p + 1]) /
+                                         (mass[k + 1] * mass[k + 1] * mass[p + 1] * mass[p + 1])
+                                   : 0.0;
+#pragma unroll
+        for (int q = 0; q < N; ++q) W2[q][p] = cnp * Cl[q][p];
+        if (p > k) t2 = t2 + W2[p][p];
+      }
+#pragma unroll
+      for (int p = 0; p < N; ++p)
+#pragma unroll
+        for (int q = 0; q < N; ++q) W5[q][p] = Ck[q][k] * Ck[p][k];
+      dr_matmul<N>(Ci, W5, A3);
+      const zd t3 = dr_trace<N>(A3);
+      dr_matmul<N>(Ci, W2, W5);
+      dr_matmul<N>(W5, Cl, W4);
+      const zd t4 = dr_trace<N>(W4);
+      dmv = dmv + dr_four_traces<N>(A3, W4, t1, t2, t3, t4, S);
+    }
+    out[m + 1] = dmv;
+  }
+  // Drachmanized Darwin
+  {
+    double W1[N][N], W2[N][N];
+#pragma unroll
+    for (int a = 0; a < N; ++a)
+#pragma unroll
+      for (int b = 0; b < N; ++b)
+        W1[a][b] = 1.0 / (mass[0] * mass[0]) + ((a == b) ? 1.0 / (mass[a + 1] * mass[a + 1]) : 0.0);
+    // ME_dXd
+    zd XC[N][N], T2[N][N], M[N][N];
+#pragma unroll
+    for (int a = 0; a < N; ++a)
+#pragma unroll
+      for (int b = 0; b < N; ++b) {
+        zd s = mkz(0.0, 0.0);
+#pragma unroll
+        for (int k = 0; k < N; ++k) s = s + W1[a][k] * Cl[k][b];
+        XC[a][b] = s;
+      }
+    dr_matmul<N>(Cl, XC, T2);
+    dr_matmul<N>(Ci, T2, M);
+    zd tr1 = mkz(0.0, 0.0);
+#pragma unroll
+    for (int a = 0; a < N; ++a)
+#pragma unroll
+      for (int k = 0; k < N; ++k) tr1 = tr1 + Cl[a][k] * W1[k][a];
+    const zd dxd = 6.0 * ((dr_trace<N>(M) - tr1) * S);
+    zd dd = (0.25 * dc.E) * dxd;
+#pragma unroll 1
+    for (int i = 0; i < N; ++i)
+      dd = dd - (0.25 * sc.q0 * sc.q[i]) * dr_dxd_over_rij<N>(W1, i, i, Ci, Cl, rm[i][i], trcj[i][i]);
+#pragma unroll 1
+    for (int i = 0; i < N; ++i)
+#pragma unroll 1
+      for (int j = i + 1; j < N; ++j)
+        dd = dd - (0.25 * sc.q[i] * sc.q[j]) * dr_dxd_over_rij<N>(W1, i, j, Ci, Cl, rm[i][j], trcj[i][j]);
+#pragma unroll
+    for (int a = 0; a < N; ++a)
+#pragma unroll
+      for (int b = 0; b < N; ++b) {
+        W1[a][b] = 1.0;
+        W2[a][b] = (a == b) ? (mass[0] + mass[a + 1]) / (mass[0] * mass[0] * mass[a + 1] * mass[a + 1]) : 0.0;
+      }
+    dd = dd + 0.125 * dr_dxd_dyd<N>(W1, W2, Ci, Ck, Cl, S);
+#pragma unroll 1
+    for (int i = 0; i < N; ++i)
+#pragma unroll 1
+      for (int j = i + 1; j < N; ++j) {
+#pragma unroll
+        for (int a = 0; a < N; ++a)
+#pragma unroll
+          for (int b = 0; b < N; ++b) {
+            W1[a][b] = (a == i && b == i) ? 1.0 : 0.0;
+            W2[a][b] = (a == j && b == j) ? 1.0 : 0.0;
+          }
+        const double cnm = (mass[i + 1] + mass[j + 1]) / (mass[i + 1] * mass[i + 1] * mass[j + 1] * mass[j + 1]);
+        dd = dd + (0.125 * cnm) * dr_dxd_dyd<N>(W1, W2, Ci, Ck, Cl, S);
+      }
+    out[m + 2] = dd;
+  }
+}
+
+template <int N>
+__global__ void __launch_bounds__(TILE* TILE) k_drach(const SysConst sc, int K, const double* __restrict__ bra,
+                                                    const double* __restrict__ ket,
+                                                    const double* __restrict__ diagS, const zd* __restrict__ cvec,
+                                                    const DrachConst dc, double* __restrict__ part) {
+  constexpr int NO = N * (N + 1) / 2 + 3;
+  extern __shared__ double lds[];
+  int tr, tc;
+  tri_tile(blockIdx.x, 0, tr, tc);
+  const int T = sc.T;
+  const int kb = tr * TILE, lb = tc * TILE;
+  double* sbra = lds;
+  double* sket = lds + TILE * BraRec<N>::stride;
+  const int tid = threadIdx.y * TILE + threadIdx.x;
+  for (int i = tid; i < TILE * BraRec<N>::stride; i += TILE * TILE) {
+    const int f = kb + i / BraRec<N>::stride;
+    sbra[i] = ld_or0(bra, (size_t)kb * BraRec<N>::stride + i, f < K);
+  }
+  const int kstride = T * KetRec<N>::stride;
+  for (int i = tid; i < TILE * kstride; i += TILE * TILE) {
+    const int f = lb + i / kstride;
+    sket[i] = ld_or0(ket, (size_t)lb * kstride + i, f < K);
+  }
+  __syncthreads();
+  double sum[NO], err[NO];
+#pragma unroll
+  for (int a = 0; a < NO; ++a) sum[a] = err[a] = 0.0;
+  const int k = kb + threadIdx.x, l = lb + threadIdx.y;
+  if (k < K && l <= k) {
+    const double* br = sbra + threadIdx.x * BraRec<N>::stride;
+    const double* kr0 = sket + threadIdx.y * kstride;
+    const zd* ck = reinterpret_cast<const zd*>(br + BraRec<N>::ck);
+    const double adet = br[BraRec<N>::adet] * bra[(size_t)l * BraRec<N>::stride + BraRec<N>::adet];
+    const double f = (k == l) ? 1.0 / diagS[k] : 2.0 / sqrt(diagS[k] * diagS[l]);
+    const zd ci = cvec[k], cj = cvec[l];
+    const double wr = ci.x * cj.x + ci.y * cj.y, wi = ci.x * cj.y - ci.y * cj.x;   // conj(c_k) c_l
+    for (int t = 0; t < T; ++t) {
+      const double* kr = kr0 + (size_t)t * KetRec<N>::stride;
+      const zd* ct = reinterpret_cast<const zd*>(kr + KetRec<N>::ct);
+      PairCore<N> pc;
+      pair_energy<N>(sc, ck, ct, reinterpret_cast<const zd*>(kr + KetRec<N>::nl), adet, pc);
+      zd O[NO];
+      drach_values<N>(sc, ck, ct, pc, dc, O);
+      const double fc = f * sc.coef[t];
+#pragma unroll
+      for (int a = 0; a < NO; ++a) {
+        const double ro = O[a].x, io = O[a].y;
+        const double beta = wr * ro - wi * io;
+        const double mu = fabs(ci.x * (cj.x * ro - cj.y * io)) + fabs(cj.x * (ci.x * ro + ci.y * io)) +
+                          fabs(ci.y * (cj.y * ro + cj.x * io)) + fabs(cj.y * (ci.y * ro - ci.x * io));
+        sum[a] += fc * beta;
+        err[a] += fabs(fc * (mu + 2.0 * fabs(beta)));
+      }
+    }
+  }
+  double* red = lds;
+  __syncthreads();
+  const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+  for (int a = 0; a < NO; ++a) {
+    double v = sum[a], e = err[a];
+    for (int o = 32; o > 0; o >>= 1) {
+      v += __shfl_xor(v, o);
+      e += __shfl_xor(e, o);
+    }
+    if (lane == 0) {
+      red[wave * 2 * NO + a] = v;
+      red[wave * 2 * NO + NO + a] = e;
+    }
+  }
+  __syncthreads();
+  for (int a = tid; a < 2 * NO; a += TILE * TILE)
+    part[(size_t)blockIdx.x * 2 * NO + a] = ((red[a] + red[2 * NO + a]) + red[4 * NO + a]) + red[6 * NO + a];
+}
+
+template <int N>
+static int launch_drach(const Ctx& c, const zd* cvec, const DrachConst& dc, double* part, double* out,
+                        hipStream_t s) {
+  constexpr int NO = N * (N + 1) / 2 + 3;
+  const int K = c.K;
+  const int nt = (K + TILE - 1) / TILE;
+  const long long ntiles = (long long)nt * (nt + 1) / 2;
+  size_t lds = sizeof(double) * TILE * (BraRec<N>::stride + c.T * KetRec<N>::stride);
+  lds = std::max(lds, sizeof(double) * 8 * NO);
+  hipLaunchKernelGGL(k_drach<N>, dim3((unsigned)ntiles), dim3(TILE, TILE), lds, s, c.sc, K, c.bra, c.ket, c.diagS,
+                     cvec, dc, part);
+  hipLaunchKernelGGL(k_expect_finish, dim3(1), dim3(256), 0, s, 2 * NO, ntiles, part, out);
+  return (int)hipGetLastError();
+}
+
+int drachmanized_values(const Ctx& c, const zd* cvec, const double* masses, double E, double* part, double* out,
+                        hipStream_t s) {
+  DrachConst dc{};
+  for (int i = 0; i <= c.n; ++i) dc.mass[i] = masses[i];
+  dc.E = E;
+  switch (c.n) {
+    case 1: return launch_drach<1>(c, cvec, dc, part, out, s);
+    case 2: return launch_drach<2>(c, cvec, dc, part, out, s);
+    case 3: return launch_drach<3>(c, cvec, dc, part, out, s);
+    case 4: return launch_drach<4>(c, cvec, dc, part, out, s);
+    case 5: return launch_drach<5>(c, cvec, dc, part, out, s);
+  }
+  return -1;
+}
+
+size_t drachmanized_partial_doubles(int n, int K) {
+  const int nt = (K + TILE - 1) / TILE;
+  return (size_t)nt * (nt + 1) / 2 * 2 * (n * (n + 1) / 2 + 3);
+}
+
 size_t correlation_partial_doubles(int n, int K) {
   const int nt = (K + TILE - 1) / TILE;
   const int m = std::max(n * (n + 1) / 2, n + 1);   // pair functions or particle densities

@@ -45,7 +45,7 @@ EXPORTS = [
     "ecg_energy_grad_stored", "ecg_batch_create", "ecg_batch_destroy", "ecg_batch_set_config",
     "ecg_batch_energy_grad", "ecg_trace", "ecg_trace_read", "ecg_get_params", "ecg_screen_prepare", "ecg_screen", "ecg_screen_loop", "ecg_last_timings",
     "ecg_set_overlap_penalty", "ecg_overlap_penalty", "ecg_set_params_device", "ecg_expectation_values",
-    "ecg_correlation_functions", "ecg_particle_densities", "ecg_relativistic_corrections", "ecg_eigenvector",
+    "ecg_correlation_functions", "ecg_particle_densities", "ecg_relativistic_corrections", "ecg_drachmanized_corrections", "ecg_eigenvector",
     "ecg_matrix_elements_noalloc",
 ]
 
@@ -102,6 +102,7 @@ def load(path: str = LIB_PATH):
     L.ecg_correlation_functions.argtypes = [vp, _d, c_int, _d, _d]
     L.ecg_particle_densities.argtypes = [vp, _d, _d, c_int, _d, _d]
     L.ecg_relativistic_corrections.argtypes = [vp, _d, _d, _d, _d]
+    L.ecg_drachmanized_corrections.argtypes = [vp, _d, _d, ctypes.c_double, _d, _d]
     L.ecg_eigenvector.argtypes = [vp, _d]
     for name in EXPORTS:
         getattr(L, name).restype = c_int
@@ -343,6 +344,23 @@ class EcgContext:
         cc = None if c is None else np.ascontiguousarray(c, dtype=np.complex128)
         _check("ecg_relativistic_corrections", load().ecg_relativistic_corrections(
             self._h, cc.ctypes.data_as(_d) if cc is not None else None, _p(ms), _p(vals), _p(errs)))
+        return vals, errs
+
+    def drachmanized_corrections(self, E, c=None, masses=None):
+        """ecg_drachmanized_corrections at the current energy E: (values,
+        errors), n(n+1)/2 + 3 each (Drachmanized delta of the pairs
+        (a, b >= a), orbit-orbit, Drachmanized mass-velocity and Darwin);
+        masses default to the system's."""
+        n = self.system.n
+        ms = np.ascontiguousarray(self.system.masses if masses is None else masses, dtype=np.float64).ravel()
+        if ms.size != n + 1:
+            raise ValueError("need %d masses" % (n + 1))
+        nv = n * (n + 1) // 2 + 3
+        vals = np.zeros(nv)
+        errs = np.zeros(nv)
+        cc = None if c is None else np.ascontiguousarray(c, dtype=np.complex128)
+        _check("ecg_drachmanized_corrections", load().ecg_drachmanized_corrections(
+            self._h, cc.ctypes.data_as(_d) if cc is not None else None, _p(ms), float(E), _p(vals), _p(errs)))
         return vals, errs
 
     def trace(self, enable=True):

@@ -22,6 +22,8 @@
 !                  EcgCorrelationFunctions            src/workproc.f90:8234-8243 (its correlation functions)
 !                  EcgParticleDensities               src/workproc.f90:8245-8254 (its particle densities)
 !                  EcgRelativisticCorrections         src/matelem.f90:1479-1667, 1866-1884 (MV, Darwin)
+!                  EcgDrachmanizedCorrections         src/matelem.f90:1276-1478, 1669-1933 (drach delta, OO,
+!                                                     drach MV, drach Darwin at Glob_CurrEnergy)
 !                plus EcgHostInit / EcgHostFinish (context set-up, no
 !                counterpart in the reference: ProgramDataInit ends by
 !                calling EcgHostInit).
@@ -252,6 +254,16 @@ module ecg_capi
       real(c_double), intent(out) :: values(*)
       real(c_double), intent(out) :: errors(*)
     end function
+    integer(c_int) function ecg_drachmanized_corrections(ctx, c, masses, E, values, errors) &
+        bind(C, name="ecg_drachmanized_corrections")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: c              ! K complex or c_null_ptr (last eigenvector)
+      real(c_double), intent(in) :: masses(*)
+      real(c_double), value :: E           ! Glob_CurrEnergy
+      real(c_double), intent(out) :: values(*)
+      real(c_double), intent(out) :: errors(*)
+    end function
     integer(c_int) function ecg_particle_densities(ctx, c, masses, ngrid, grid, values) &
         bind(C, name="ecg_particle_densities")
       import :: c_int, c_double, c_ptr
@@ -292,6 +304,7 @@ module ecg_globals
   real(dprec), allocatable :: Glob_diagH(:), Glob_diagS(:)
   character :: Glob_GSEPSolutionMethod = 'G'
   real(dprec) :: Glob_ApproxEnergy = 0
+  real(dprec) :: Glob_CurrEnergy = 0
   integer :: Glob_WhichEigenvalue = 1
   real(dprec) :: Glob_EigvalTol = 1.0d-12
   integer :: Glob_InvItMaxIter = 20
@@ -321,7 +334,7 @@ module ecg_hostpath
   public :: EcgHostInit, EcgHostFinish, MatrixElements, ComputeMatElem, ComputeMatElemAndDeriv, GHEPIIS, &
             EnergyGA, EnergyIA, EnergyGAM, EnergyIAM, EnergyGB, EnergyIB, ComputeOverlapPenalty, ComputeOverlapPenaltyAndAddGradient, &
             EcgExpectationValues, EcgCorrelationFunctions, EcgParticleDensities, &
-            EcgRelativisticCorrections
+            EcgRelativisticCorrections, EcgDrachmanizedCorrections
   type(c_ptr), save :: ctx = c_null_ptr
 
 contains
@@ -569,6 +582,14 @@ contains
     real(dprec) :: Mass(:), Values(:), Errors(:)
     integer :: ErrorCode
     ErrorCode = ecg_relativistic_corrections(ctx, c_null_ptr, Mass, Values, Errors)
+  end subroutine
+
+  ! Values: Glob_n*(Glob_n+1)/2 Drachmanized delta functions, then OO,
+  ! drach_MV, drach_Darwin, in the last eigenvector at Glob_CurrEnergy
+  subroutine EcgDrachmanizedCorrections(Mass, Values, Errors, ErrorCode)
+    real(dprec) :: Mass(:), Values(:), Errors(:)
+    integer :: ErrorCode
+    ErrorCode = ecg_drachmanized_corrections(ctx, c_null_ptr, Mass, Glob_CurrEnergy, Values, Errors)
   end subroutine
 
   subroutine EcgParticleDensities(Mass, NumDensGridPoints, DensGrid, Dens, ErrorCode)

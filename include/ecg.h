@@ -323,6 +323,22 @@ int ecg_particle_densities(ecg_ctx* ctx, const double* c, const double* masses, 
 int ecg_relativistic_corrections(ecg_ctx* ctx, const double* c, const double* masses, double* values,
                                  double* errors);
 
+/* Drachmanized delta functions, orbit-orbit term and the Drachmanized
+ * mass-velocity and Darwin corrections (without the alpha^2 factor): the
+ * drach_deltarkl, OOkl, drach_MVkl and drach_Darwinkl pair values of
+ * MatrixElementsForExpcVals (src/matelem.f90:1276-1478, 1669-1933)
+ * contracted with the vector (c == NULL: the last energy call's) as the
+ * driver does (src/workproc.f90:8158-8300), with its rounding estimates.
+ * masses: the n + 1 particle masses (Glob_Mass); E: the current energy
+ * (Glob_CurrEnergy), which the Drachmanized forms take.  values, errors:
+ * n(n+1)/2 + 3 each: the Drachmanized delta of the pairs (a, b >= a) in the
+ * order of ecg_expectation_values, then OO, drach MV, drach Darwin; errors
+ * nullable.  For n >= 3 the Drachmanized delta takes, as the reference's
+ * build does, zero for the tr[C^-1 J_aa C^-1 J_bc] (a < b < c) that its
+ * table leaves unwritten. */
+int ecg_drachmanized_corrections(ecg_ctx* ctx, const double* c, const double* masses, double E, double* values,
+                                 double* errors);
+
 /* The eigenvector (K complex, c^H S c = 1 for the normalised S) of the last
  * energy call on the current parameters: what EnergyGB / EnergyIB leave
  * in Glob_c (src/workproc.f90:1885-1890, 2114-2119).  ECG_ERR_STATE when

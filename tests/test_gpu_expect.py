@@ -167,3 +167,57 @@ def test_relativistic_corrections_vs_oracle(name, K):
     ctx.close()
     print("MV", vals[0], ov[0], "Darwin", vals[1], ov[1])
     _compare(vals, errs, ov, oe)
+
+
+@pytest.mark.parametrize("name,K", [("n1", 12), ("hminus", 16), ("h2", 12), ("li", 10)])
+def test_drachmanized_corrections_vs_oracle(name, K):
+    """ecg_drachmanized_corrections (Drachmanized delta of every pair,
+    orbit-orbit, Drachmanized mass-velocity and Darwin) against the oracle's
+    restatement (pinned pair by pair on the CPU against the compiled
+    reference, tests/test_ref_oracle.py).  These values are differences of
+    larger terms, so the device's rounding is compared with the size of
+    those sums: each value to 1e-10 of the driver's absolute-sum scale
+    (its rounding bound divided by epsilon) plus 10 bounds; the bounds to
+    1e-6 relative.  The Drachmanized mass-velocity and Darwin values, two
+    formulas for one quantity, agree with each other to the same bar."""
+    say("ecg_drachmanized_corrections", name, K)
+    sy = make_system([2.0, 1.0], [1.0, -1.0], "(P11)") if name == "n1" else named_system(name)
+    p = seeded_basis(sy.n, K, SEED_BASE + 68)
+    rng = np.random.Generator(np.random.PCG64(69))
+    c = rng.normal(size=K) + 1j * rng.normal(size=K)
+    E = -1.1
+    _, _, _, dS = oracle.assemble(sy, p, "G")
+    ov, oe = oracle.drachmanized(sy, p, c, dS, sy.masses, E)
+    ctx = EcgContext(sy)
+    ctx.set_params(p)
+    vals, errs = ctx.drachmanized_corrections(E, c)
+    ctx.close()
+    scale = oe / np.finfo(float).eps
+    tol = 1e-10 * scale + 10.0 * oe + 1e-300
+    dev = np.abs(vals - ov)
+    print("max dev / scale", np.max(dev / scale), "values", vals)
+    assert np.all(dev <= tol), (np.nonzero(dev > tol), vals, ov)
+    np.testing.assert_allclose(errs, oe, rtol=1e-6, atol=1e-300)
+    m = sy.n * (sy.n + 1) // 2
+    assert abs(vals[m + 1] - vals[m + 2]) <= 1e-10 * (scale[m + 1] + scale[m + 2])
+
+
+def test_drachmanized_corrections_hdplus_last_eigenvector(golden_dir):
+    """On the HD+ fixture's eigenvector at its own energy: the last energy
+    call's vector (c = NULL) gives what the explicit vector gives, and the
+    oracle agrees."""
+    say("ecg_drachmanized_corrections", "inout_1_500", "last eigenvector")
+    f = read_inout(os.path.join(golden_dir, "hdplus", "inout_1_500.txt"))
+    sigma = f.current_energy * f.invit_parameter
+    ctx = EcgContext(f.system)
+    ctx.set_params(f.params)
+    _, E, c, _ = ctx.energy(1, sigma, f.eigval_tolerance, 20, want_vector=True)
+    vals, errs = ctx.drachmanized_corrections(E)
+    vals2, _ = ctx.drachmanized_corrections(E, c)
+    ctx.close()
+    np.testing.assert_array_equal(vals, vals2)
+    _, _, _, dS = oracle.assemble(f.system, f.params, "G")
+    ov, oe = oracle.drachmanized(f.system, f.params, c, dS, f.system.masses, E)
+    scale = oe / np.finfo(float).eps
+    print("drach delta, OO, drach MV, drach Darwin", vals, "max dev / scale", np.max(np.abs(vals - ov) / scale))
+    assert np.all(np.abs(vals - ov) <= 1e-10 * scale + 10.0 * oe)
