@@ -1,4 +1,5 @@
-"""Build libecg_hip.so (the HIP kernels and the C-ABI) in-tree for gfx950.
+"""Build libecg_hip.so (the HIP kernels and the C-ABI) in-tree for gfx950,
+and the host-only libecg_drmng.so (the minimiser of the optimisers).
 
 Usage: python atom-mol-nonbo_amd/build.py   (or __graft_entry__.build())
 """
@@ -45,8 +46,24 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    build_drmng()
     build_fortran()
     return LIB
+
+
+DRMNG_LIB = os.path.join(HERE, "libecg_drmng.so")
+
+
+def build_drmng():
+    """Compile the host-only minimiser library (csrc/drmng.cpp, the C-ABI of
+    include/ecg_drmng.h).  Products are not contracted into fused
+    multiply-adds: the arithmetic follows the reference's operation order."""
+    cxx = os.environ.get("CXX", "g++")
+    tmp = DRMNG_LIB + ".tmp"
+    subprocess.run([cxx, "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-shared", "-o", tmp,
+                    os.path.join(CSRC, "drmng.cpp")], check=True)
+    os.replace(tmp, DRMNG_LIB)
+    return DRMNG_LIB
 
 
 FLANG = "/opt/rocm/lib/llvm/bin/flang"

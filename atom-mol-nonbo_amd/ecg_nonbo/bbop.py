@@ -40,6 +40,7 @@ import numpy as np
 
 from .enlarge import basis_enl
 from .inout import HistoryEntry, InoutFile, write_inout
+from .drmng import MAX_SCALED_FIRST_STEP
 from .optimise import BasisOrder, full_opt1, opt_cycle
 
 __all__ = ["run_bbop", "BbopReport"]
@@ -71,7 +72,7 @@ def _store(f: InoutFile, order: BasisOrder, hist: List[dict], energy: float) -> 
 
 
 def run_bbop(ctx, f: InoutFile, rng: np.random.Generator, path: Optional[str] = None, trials: str = "exact",
-             steps: Optional[List[int]] = None, maxiter: int = 30, max_step: float = 0.1,
+             steps: Optional[List[int]] = None, maxiter: int = 30, max_step: float = MAX_SCALED_FIRST_STEP,
              on_report: Optional[Callable[[BbopReport], None]] = None) -> List[BbopReport]:
     """Run the BBOP steps of `f` (all, or the 1-based numbers in `steps`)
     on `ctx` (a device context of f.system), updating `f` in place.

@@ -14,9 +14,11 @@ Glob_BadOverlapOrLinCoeffLim attempts), the failure limits
 in the optimisation), the history entries of the new functions and their
 numbers.
 
-What does not: random streams (numpy's PCG64 in place of random_number and
-DRNOR, so the candidates are drawn from the same distributions, not the
-same numbers) and the minimiser (the BFGS stand-in of `optimise`).
+The optimisation of the kept candidate drives the restated DRMNG
+(`drmng.minimise`) with the reference's settings and scale.  What does
+not follow the reference: random streams (numpy's PCG64 in place of
+random_number and DRNOR, so the candidates are drawn from the same
+distributions, not the same numbers).
 """
 from __future__ import annotations
 
@@ -25,7 +27,8 @@ from typing import Callable, List, Optional, Tuple
 
 import numpy as np
 
-from .optimise import BasisOrder, EnergyCalls, bfgs_minimise
+from . import drmng
+from .optimise import BasisOrder, EnergyCalls
 
 __all__ = ["generate_trial_param", "basis_enl", "EnlStepRecord", "BasisEnlError",
            "BAD_OVERLAP_OR_LIN_COEFF_LIM", "MAX_FRAC_OF_TRIAL_FAILS", "MAX_ENERGY_FAILS"]
@@ -139,7 +142,7 @@ def basis_enl(ctx, order: BasisOrder, k_stop: int, k_step: int, n_trials: int, m
               sigma: float, rng: np.random.Generator, generator_param=(0.5, 0.1, 0.3),
               overlap_threshold: float = -1.0, lin_coeff_threshold: float = -1.0, optimise: bool = True,
               history: Optional[list] = None, which: int = 1, tol: float = 1e-12, maxiter: int = 30,
-              max_step: float = 0.1, trials: str = "exact", screen_iters: int = 4,
+              max_step: float = drmng.MAX_SCALED_FIRST_STEP, trials: str = "exact", screen_iters: int = 4,
               bad_lim: int = BAD_OVERLAP_OR_LIN_COEFF_LIM,
               on_step: Optional[Callable[[EnlStepRecord, BasisOrder], None]] = None) -> List[EnlStepRecord]:
     """BasisEnlI (sigma finite: the fixed shift Glob_ApproxEnergy) or
@@ -232,34 +235,26 @@ def basis_enl(ctx, order: BasisOrder, k_stop: int, k_step: int, n_trials: int, m
             nev = 0
             x_best = best.reshape(-1).copy()
             if optimise and max_energy_eval > 0:
-                failures = [0]
-
                 def set_x(v):
                     params[nfru:K] = v.reshape(nfo, npt)
 
                 def f_only(v):
                     set_x(v)
                     rc_, e_, _, _ = ev.energy(params[:K])
-                    if rc_ != 0:
-                        failures[0] += 1
-                        if failures[0] > MAX_ENERGY_FAILS:
-                            raise BasisEnlError("BasisEnl: number of failures in energy or gradient "
-                                                "calculations during the optimisation exceeded the limit")
-                        return np.inf
-                    return e_
+                    return rc_ == 0, e_
 
                 def fg(v):
                     set_x(v)
                     rc_, e_, g_, _ = ev.energy_grad(params[:K], nfru)
-                    if rc_ != 0:
-                        failures[0] += 1
-                        if failures[0] > MAX_ENERGY_FAILS:
-                            raise BasisEnlError("BasisEnl: number of failures in energy or gradient "
-                                                "calculations during the optimisation exceeded the limit")
-                        return np.inf, np.zeros_like(v)
-                    return e_, g_
+                    return rc_ == 0, e_, g_
 
-                x_best, _, nev = bfgs_minimise(fg, f_only, x_best, max_energy_eval, max_step, np.ones(x_best.size))
+                # src/workproc.f90:3447-3562: the workspace from its initial
+                # copy, every scale the relative energy gain of the
+                # selection (1 while the basis is smaller than the group)
+                t = drmng.scale_for_enlargement(E_init, E_cur, nfru, nfo)
+                res = drmng.minimise(f_only, fg, x_best, np.full(x_best.size, t), max_energy_eval, E_cur,
+                                     lmax0=max_step, error=BasisEnlError)
+                x_best, nev = res.x_best, res.energy_evals
             params[nfru:K] = x_best.reshape(nfo, npt)
             rc, E_cur, c, _ = ev.energy(params[:K], want_vector=True)
             if rc != 0:

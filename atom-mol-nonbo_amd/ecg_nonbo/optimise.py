@@ -9,16 +9,14 @@ src/workproc.f90:2414-2487, 2731-2760, 2857-2982, 3040-3078), which brings
 each function to the end of the basis with few rows changed, the energy
 calls at a fixed shift (EnergyIA(Nmin, K) / EnergyIB, which the device
 context turns into reassembly and refactorisation from the first changed
-row), the acceptance tests on overlaps and linear coefficients, and the
-history fields (CyclesDone, InitFuncAtLastStep) with which the reference
-resumes a cycle.
-
-What does not: the minimiser.  The reference drives PORT's DRMNG (a
-reverse-communication secant method with a trust region); here a compact
-BFGS with a bounded first step and a backtracking line search stands in,
-so optimisation trajectories are not expected to match the reference's
-(they depend on the minimiser's internals), only its contract: at most
-MaxEnergyEval energy evaluations per step, the best point found is kept.
+row), the minimiser and its driver loop (PORT's DRMNG restated in
+csrc/drmng.cpp with the reference's settings: scale max(1 / (K^2 sqrt K),
+1e4 eps) on every parameter, first scaled step 0.001, at most
+MaxEnergyEval energy evaluations, the best point kept; `drmng.minimise`),
+the acceptance tests on overlaps and linear coefficients, and the history
+fields (CyclesDone, InitFuncAtLastStep) with which the reference resumes a
+cycle.  FullOpt1 (`full_opt1`) drives the same minimiser over a block of
+functions at once.
 """
 from __future__ import annotations
 
@@ -27,8 +25,10 @@ from typing import Callable, List, Optional
 
 import numpy as np
 
-__all__ = ["BasisOrder", "num_rows_to_perm_for_unit_shift", "bfgs_minimise", "lbfgs_minimise", "opt_cycle",
-           "OptStepRecord", "EnergyCalls", "full_opt1", "FullOptResult", "FAILED_STATUSES"]
+from . import drmng
+
+__all__ = ["BasisOrder", "num_rows_to_perm_for_unit_shift", "opt_cycle", "OptStepRecord", "EnergyCalls",
+           "full_opt1", "FullOptResult", "FAILED_STATUSES"]
 
 # Statuses with which an energy call counts as a failure (the reference's
 # ErrorCode /= 0): singular H - sigma S, inverse iteration not converged,
@@ -132,128 +132,21 @@ class BasisOrder:
         self.func_num[seg] = self.func_num[seg][order].copy()
 
 
-def _quasi_newton(fg, f_only, x0, max_evals, max_step, scale, memory):
-    x = np.array(x0, dtype=np.float64)
-    f, g = fg(x)
-    nev = 1
-    best = (x.copy(), f)
-    if not np.isfinite(f):
-        return best[0], best[1], nev
-    n = x.size
-    Hinv = np.eye(n) if memory is None else None
-    pairs: list = []                       # (s, y, rho) of the limited-memory form
-    gs = g / scale
-
-    def direction(q):
-        if memory is None:
-            return -(Hinv @ q)
-        # two-loop recursion (Nocedal & Wright, Algorithm 7.4)
-        q = q.copy()
-        al = []
-        for s_, y_, r_ in reversed(pairs):
-            a_ = r_ * (s_ @ q)
-            al.append(a_)
-            q -= a_ * y_
-        if pairs:
-            s_, y_, _ = pairs[-1]
-            q *= (s_ @ y_) / (y_ @ y_)
-        for (s_, y_, r_), a_ in zip(pairs, reversed(al)):
-            q += (a_ - r_ * (y_ @ q)) * s_
-        return -q
-
-    first = True
-    while nev < max_evals:
-        p = direction(gs)
-        if first:
-            nrm = np.linalg.norm(p)
-            if nrm > max_step:
-                p *= max_step / nrm
-            first = False
-        slope = gs @ p
-        if slope >= 0.0:   # not a descent direction: restart from steepest descent
-            if memory is None:
-                Hinv = np.eye(n)
-            pairs.clear()
-            p = -gs * min(1.0, max_step / max(np.linalg.norm(gs), 1e-300))
-            slope = gs @ p
-            if slope >= 0.0:
-                break
-        alpha = 1.0
-        accepted = False
-        while nev < max_evals:
-            xn = x + alpha * p / scale
-            fn = f_only(xn)
-            nev += 1
-            if np.isfinite(fn) and fn < best[1]:
-                best = (xn.copy(), fn)
-            if np.isfinite(fn) and fn <= f + 1e-4 * alpha * slope:
-                accepted = True
-                break
-            alpha *= 0.25
-            if alpha < 1e-8:
-                break
-        if not accepted or nev >= max_evals:
-            break
-        fn, gn = fg(xn)             # gradient at an accepted point: not counted
-        if not np.isfinite(fn):
-            break
-        if fn < best[1]:
-            best = (xn.copy(), fn)
-        gsn = gn / scale
-        s = alpha * p
-        yv = gsn - gs
-        sy = s @ yv
-        if sy > 1e-300:
-            rho = 1.0 / sy
-            if memory is None:
-                V = np.eye(n) - rho * np.outer(s, yv)
-                Hinv = V @ Hinv @ V.T + rho * np.outer(s, s)
-            else:
-                pairs.append((s, yv, rho))
-                if len(pairs) > memory:
-                    pairs.pop(0)
-        x, f, gs = xn, fn, gsn
-        if np.linalg.norm(gs) * np.linalg.norm(s) < 1e-15 * max(1.0, abs(f)):
-            break
-    return best[0], best[1], nev
-
-
-def bfgs_minimise(fg: Callable[[np.ndarray], tuple], f_only: Callable[[np.ndarray], float], x0: np.ndarray,
-                  max_evals: int, max_step: float, scale: np.ndarray):
-    """Minimise with at most max_evals energy evaluations: the starting
-    point and every line-search point count, the gradient calls at accepted
-    points do not, as the reference counts only DRMNG's function requests
-    (IV(1)=1) against MaxEnergyEval (src/workproc.f90:4141-4147, 4219).
-    Variables are scaled by `scale` (the role of DRMNG's D vector); the
-    first step's scaled length is at most max_step.
-    Dense inverse-Hessian update, for the few variables of one or a few
-    functions.  Returns (x_best, f_best, evaluations)."""
-    return _quasi_newton(fg, f_only, x0, max_evals, max_step, scale, None)
-
-
-def lbfgs_minimise(fg: Callable[[np.ndarray], tuple], f_only: Callable[[np.ndarray], float], x0: np.ndarray,
-                   max_evals: int, max_step: float, scale: np.ndarray, memory: int = 12):
-    """bfgs_minimise with the limited-memory inverse Hessian (the last
-    `memory` steps), for simultaneous optimisation of a whole basis, where
-    a dense matrix of (K npt)^2 entries would cost more per step than the
-    energy and gradient themselves."""
-    return _quasi_newton(fg, f_only, x0, max_evals, max_step, scale, int(memory))
-
-
 @dataclasses.dataclass
 class OptStepRecord:
     func: int                 # function number (Glob_FuncNum) optimised
     energy_before: float
     energy_after: float
-    evaluations: int
+    evaluations: int          # energy evaluations (NumOfEnergyEval)
     accepted: bool
     tail_rows: List[int]      # rows refactored by the energy calls of the step (-1: full)
+    grad_evaluations: int = 0   # gradient evaluations (NumOfGradEval)
 
 
 def opt_cycle(ctx, order: BasisOrder, func_begin: int, func_end: int, num_cycles: int, max_energy_eval: int,
               sigma: float, overlap_threshold: float = -1.0, lin_coeff_threshold: float = -1.0,
               num_func_to_opt: int = 1, num_func_to_shift: int = 1, history: Optional[dict] = None,
-              tol: float = 1e-12, max_step: float = 0.1, which: int = 1, maxiter: int = 30,
+              tol: float = 1e-12, max_step: float = drmng.MAX_SCALED_FIRST_STEP, which: int = 1, maxiter: int = 30,
               on_step: Optional[Callable[[OptStepRecord, BasisOrder], None]] = None) -> List[OptStepRecord]:
     """OptCycleI (src/workproc.f90:4902-5423) on a device context: cycles of
     optimising functions func_begin..func_end (their numbering in `order`'s
@@ -265,9 +158,12 @@ def opt_cycle(ctx, order: BasisOrder, func_begin: int, func_end: int, num_cycles
     (src/workproc.f90:4383-4898, the same loop over EnergyGA / EnergyGB:
     root `which` isolated on every call).  Returns one record per step;
     `order` ends in the input order of the functions (sorted back, as the
-    reference leaves it).  An energy call that fails inside the minimiser
-    counts as an infinite energy; failures of the calls outside it raise,
-    as the reference stops there."""
+    reference leaves it).  max_step: the minimiser's bound on its first
+    scaled step (V(35); the reference's Glob_MaxScStepAllowedInOpt by
+    default).  An energy call that fails inside the minimiser leaves the
+    last energy in place (up to Glob_MaxEnergyFailsAllowed of them, then
+    RuntimeError); failures of the calls outside it raise, as the reference
+    stops there."""
     K = len(order.func_num)
     npt = order.params.shape[1]
     if history is None:
@@ -315,12 +211,8 @@ def opt_cycle(ctx, order: BasisOrder, func_begin: int, func_end: int, num_cycles
                 j = m - nr
                 i = (j - nr + 1) if q - nfo >= nr * 2 else (K - q + 1)
                 order.permute_functions2(i, j, m)
-                rc, E_cur, _, info = energy(None)
-                if rc != 0:
-                    raise RuntimeError("OptCycle: energy after the permutation failed with status %d" % rc)
             x_init = order.params[nfru:].reshape(-1).copy()
             e_before = E_cur
-            scale = np.full(x_init.size, 1.0)
 
             def set_x(x):
                 order.params[nfru:] = x.reshape(nfo, npt)
@@ -328,17 +220,21 @@ def opt_cycle(ctx, order: BasisOrder, func_begin: int, func_end: int, num_cycles
             def f_only(x):
                 set_x(x)
                 rc_, e_, _, _ = energy(None)
-                return e_ if rc_ == 0 else np.inf
+                return rc_ == 0, e_
 
             def fg(x):
                 set_x(x)
                 rc_, e_, g_, _ = energy(nfru)
-                return (e_, g_) if rc_ == 0 else (np.inf, np.zeros_like(x))
+                return rc_ == 0, e_, g_
 
-            x_best, e_best, nev = bfgs_minimise(fg, f_only, x_init, max_energy_eval, max_step, scale)
-            set_x(x_best)
+            # src/workproc.f90:4648-4731: the workspace from its initial
+            # copy, every scale max(1 / (K^2 sqrt K), 1e4 eps), the loop
+            # ended by the energy count
+            d = np.full(x_init.size, drmng.scale_for_basis(K))
+            res = drmng.minimise(f_only, fg, x_init, d, max_energy_eval, E_cur, lmax0=max_step)
+            set_x(res.x_best)
             rc, E_new, c, info = ev.energy(order.params, want_vector=True)
-            ok = rc == 0 and E_new <= e_before
+            ok = rc == 0
             if ok and overlap_threshold > 0.0:
                 pen, _ = ctx.overlap_penalty(nfru, 1.0, overlap_threshold ** 2)
                 ok = pen <= 0.0
@@ -358,15 +254,14 @@ def opt_cycle(ctx, order: BasisOrder, func_begin: int, func_end: int, num_cycles
             else:
                 history["init_func_at_last_step"] = cur
             rec = OptStepRecord(func=int(order.func_num[K - 1]), energy_before=float(e_before),
-                                energy_after=float(E_cur), evaluations=nev, accepted=bool(ok),
-                                tail_rows=list(ev.tails[t0:]))
+                                energy_after=float(E_cur), evaluations=res.energy_evals, accepted=bool(ok),
+                                tail_rows=list(ev.tails[t0:]), grad_evaluations=res.grad_evals)
             records.append(rec)
             if on_step:
                 on_step(rec, order)
         if cycle != num_cycles:
             history["init_func_at_last_step"] = func_begin - nfs
             order.sort_decreasing(fbn, K)
-            rc, E_cur, _, info = energy(None)
     order.sort_decreasing(func_begin, K)
     order.reverse(func_begin, K)
     return records
@@ -378,13 +273,14 @@ class FullOptResult:
     energy_after: float       # energy without the penalty at the end (Glob_CurrEnergy)
     penalty: float            # Glob_TotalOverlapPenalty at the end
     evaluations: int          # energy evaluations (the reference's NumOfEnergyEval)
-    minimiser: str            # "bfgs" (dense) or "lbfgs"
+    grad_evaluations: int     # gradient evaluations (NumOfGradEval)
+    code: int                 # the minimiser's IV(1) when the loop ended
 
 
 def full_opt1(ctx, order: BasisOrder, init_func: int, final_func: int, max_energy_eval: int, sigma: float,
               overlap_threshold: float = 1.0, max_overlap_penalty: float = 1.0, history: Optional[dict] = None,
-              which: int = 1, tol: float = 1e-12, maxiter: int = 30, max_step: float = 0.1, memory: int = 12,
-              dense_limit: int = 600) -> FullOptResult:
+              which: int = 1, tol: float = 1e-12, maxiter: int = 30,
+              max_step: float = drmng.MAX_SCALED_FIRST_STEP) -> FullOptResult:
     """FullOpt1I (src/workproc.f90:5962-6422; FullOpt1G, :5428-5958, with
     sigma NaN): simultaneous optimisation of functions init_func..final_func
     (positions in `order`), E + P and its gradient from the device.
@@ -392,19 +288,20 @@ def full_opt1(ctx, order: BasisOrder, init_func: int, final_func: int, max_energ
     Steps as the reference: the overlap penalty is on when
     overlap_threshold < 1 (t2 = threshold^2, b = max_overlap_penalty,
     :6026-6032); functions init_func..final_func move to the end of the
-    basis (PermuteFunctions, :6113-6127); the minimiser runs with at most
-    max_energy_eval energy evaluations over their npt (final - init + 1)
-    parameters with the gradient of the last K - nfru functions (EnergyIB);
-    the energy without the penalty becomes the current energy (:6328-6332)
-    and history['energy'], history['num_energy_evals_full_opt'] grow as
-    Glob_History(K) does (:6356-6360); the functions that followed
-    final_func move back behind them (:6366-6378), restoring the input
-    order.  The penalty is switched off on return.
+    basis (PermuteFunctions, :6113-6127); the minimiser (DRMNG, dense, with
+    the drivers' settings and summary level -1, every scale max(1 / (K^2
+    sqrt K), 1e4 eps)) runs until it converges or max_energy_eval energy
+    evaluations are spent, a failed energy flagged to it through IV(2) = 1
+    (:6236-6262); the parameters it holds at the end (its last point, not
+    the best seen) are kept (:6303-6306); the energy without the penalty
+    becomes the current energy (:6328-6332) and history['energy'],
+    history['num_energy_evals_full_opt'] grow as Glob_History(K) does
+    (:6356-6360); the functions that followed final_func move back behind
+    them (:6366-6378), restoring the input order.  The penalty is switched
+    off on return.
 
-    Not as the reference: DRMNG (with its Hessian file) is replaced by the
-    dense BFGS stand-in up to `dense_limit` variables and by limited-memory
-    BFGS above (a dense matrix at K = 2000 would be 4.6 GB on the host), so
-    the optimisation path differs; the best point found is the one kept."""
+    Not here: the Hessian file (SaveHessian) and the periodic saves of the
+    data file during the loop."""
     K = len(order.func_num)
     if not (1 <= init_func <= final_func <= K):
         raise ValueError("FullOpt1: need 1 <= InitFunc <= FinalFunc <= K")
@@ -428,30 +325,28 @@ def full_opt1(ctx, order: BasisOrder, init_func: int, final_func: int, max_energ
         def f_only(x):
             set_x(x)
             rc_, e_, _, _ = ev.energy(order.params)
-            return e_ if rc_ == 0 else np.inf
+            return rc_ == 0, e_
 
         def fg(x):
             set_x(x)
             rc_, e_, g_, _ = ev.energy_grad(order.params, nfru)
-            return (e_, g_) if rc_ == 0 else (np.inf, np.zeros_like(x))
+            return rc_ == 0, e_, g_
 
-        dense = x0.size <= dense_limit
-        scale = np.ones(x0.size)
-        if dense:
-            x_best, _, nev = bfgs_minimise(fg, f_only, x0, max_energy_eval, max_step, scale)
-        else:
-            x_best, _, nev = lbfgs_minimise(fg, f_only, x0, max_energy_eval, max_step, scale, memory)
-        set_x(x_best)
+        d = np.full(x0.size, drmng.scale_for_basis(K))
+        res = drmng.minimise(f_only, fg, x0, d, max_energy_eval, E0, lmax0=max_step, outlev=-1,
+                             energy_failure_flag=1, grad_failure_flag=0)
+        set_x(res.x_final)
         rc, E, _, info = ev.energy(order.params)
         if rc != 0:
             raise RuntimeError("FullOpt1: energy after the optimisation failed with status %d" % rc)
         P = float(info[6]) if allowed else 0.0
+        nev = res.energy_evals
         if history is not None:
             history["energy"] = E - P
             history["num_energy_evals_full_opt"] = history.get("num_energy_evals_full_opt", 0) + nev
         if final_func != K:
             order.permute_functions(init_func, init_func + (K - final_func) - 1)
         return FullOptResult(energy_before=float(E0), energy_after=float(E - P), penalty=P, evaluations=nev,
-                             minimiser="bfgs" if dense else "lbfgs")
+                             grad_evaluations=res.grad_evals, code=res.code)
     finally:
         ctx.set_overlap_penalty(False, 1.0, 1.0, 0)

@@ -1,11 +1,13 @@
 """ctypes front end of the reference's own pair routines (oracle/_ref).
 
-TEST INFRASTRUCTURE ONLY: the library is the reference's MatrixElements and
-MatrixElementsForExpcVals (src/matelem.f90:12-750, 949-1983) compiled from
-the reference sources by `make -C oracle ref` (ref_shim.f90 sets the
-globals).  Tests use it to check the oracle's restatement and the device
-against the reference's own code; it is absent (available() is False) when
-the reference tree was not there to build it.
+TEST INFRASTRUCTURE ONLY: the libraries are the reference's MatrixElements
+and MatrixElementsForExpcVals (src/matelem.f90:12-750, 949-1983; with
+ref_shim.f90 setting the globals) and its PORT minimiser (DIVSET / DRMNG,
+src/dmng.f), compiled from the reference sources by `make -C oracle ref`.
+Tests use them to check the oracle's restatement, the device and the
+host-side minimiser against the reference's own code; they are absent
+(available() / dmng_available() are False) when the reference tree was not
+there to build them.
 """
 from __future__ import annotations
 
@@ -17,15 +19,17 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "_ref", "libref_matelem.so")
+_DMNG = os.path.join(_HERE, "_ref", "libref_dmng.so")
 REFSRC = "/root/reference/src"
 _lib = None
+_dmng = None
 _d = ctypes.POINTER(ctypes.c_double)
 _i = ctypes.POINTER(ctypes.c_int)
 
 
 def build():
-    """Build _ref/libref_matelem.so when the reference sources are present
-    (None otherwise)."""
+    """Build _ref/libref_matelem.so and _ref/libref_dmng.so when the
+    reference sources are present (None otherwise)."""
     if not os.path.isdir(REFSRC):
         return None
     subprocess.run(["make", "-s", "-C", _HERE, "ref"], check=True, stdout=subprocess.DEVNULL,
@@ -35,6 +39,41 @@ def build():
 
 def available() -> bool:
     return os.path.exists(_LIB)
+
+
+def dmng_available() -> bool:
+    return os.path.exists(_DMNG)
+
+
+def dmng_lib():
+    """The reference's DIVSET and DRMNG (Fortran symbols divset_, drmng_;
+    every argument by reference)."""
+    global _dmng
+    if _dmng is None:
+        L = ctypes.CDLL(_DMNG)
+        L.divset_.restype = None
+        L.divset_.argtypes = [_i, _i, _i, _i, _d]
+        L.drmng_.restype = None
+        L.drmng_.argtypes = [_d, _d, _d, _i, _i, _i, _i, _d, _d]
+        _dmng = L
+    return _dmng
+
+
+def ref_divset(alg, iv, v):
+    """DIVSET(ALG, IV, LIV, LV, V) on int32 iv and float64 v, in place."""
+    a, li, lv = ctypes.c_int(alg), ctypes.c_int(iv.size), ctypes.c_int(v.size)
+    dmng_lib().divset_(ctypes.byref(a), iv.ctypes.data_as(_i), ctypes.byref(li), ctypes.byref(lv),
+                       v.ctypes.data_as(_d))
+
+
+def ref_drmng(d, fx, g, iv, v, x):
+    """DRMNG(D, FX, G, IV, LIV, LV, N, V, X) in place; returns IV(1)."""
+    f = ctypes.c_double(fx)
+    li, lv, n = ctypes.c_int(iv.size), ctypes.c_int(v.size), ctypes.c_int(x.size)
+    dmng_lib().drmng_(d.ctypes.data_as(_d), ctypes.byref(f), g.ctypes.data_as(_d), iv.ctypes.data_as(_i),
+                      ctypes.byref(li), ctypes.byref(lv), ctypes.byref(n), v.ctypes.data_as(_d),
+                      x.ctypes.data_as(_d))
+    return int(iv[0])
 
 
 def lib():

@@ -171,10 +171,16 @@ def test_basis_enl_retries_after_bad_coefficients():
     t = np.array([0.3, -0.2])
     ctx = SeparableContext(t)
     ctx.drift = 1.0
-    # coefficients too large until 20 energy calls have been made (the first attempt)
-    ctx.coeff = lambda ncalls: 50.0 if ncalls < 20 else 1.0
+    # coefficients too large at the first check (the first attempt's), fine after
+    checks = []
+
+    def coeff(ncalls):
+        checks.append(ncalls)
+        return 50.0 if len(checks) == 1 else 1.0
+
+    ctx.coeff = coeff
     order, _, recs = _grow(ctx, 3, 4, 1, n_trials=6, max_energy_eval=6, lin_coeff_threshold=10.0)
-    assert recs[0].attempts >= 2 and recs[0].checks_passed
+    assert recs[0].attempts == 2 and recs[0].checks_passed
     # with the limit reached the last attempt is kept
     ctx = SeparableContext(t, coeff=50.0)
     ctx.drift = 1.0
@@ -209,8 +215,7 @@ def test_basis_enl_overlap_check():
 
 
 # ------------------------------------------------------------- FullOpt1
-@pytest.mark.parametrize("dense_limit", [600, 0])
-def test_full_opt1_subrange_order_and_penalty(dense_limit):
+def test_full_opt1_subrange_order_and_penalty():
     t = np.array([0.3, -0.2, 0.1])
     rng = _rng(11)
     K = 9
@@ -219,8 +224,8 @@ def test_full_opt1_subrange_order_and_penalty(dense_limit):
     order = BasisOrder.identity(p0)
     h = {"energy": 0.0, "cycles_done": 1, "init_func_at_last_step": 0, "num_energy_evals_full_opt": 7}
     res = full_opt1(ctx, order, 3, 6, 200, -1.0, overlap_threshold=0.5, max_overlap_penalty=1.0, history=h,
-                    max_step=0.5, dense_limit=dense_limit)
-    assert res.minimiser == ("bfgs" if dense_limit else "lbfgs")
+                    max_step=0.5)
+    assert 1 <= res.code <= 10
     # input order back; functions outside 3..6 untouched
     np.testing.assert_array_equal(order.func_num, np.arange(1, K + 1))
     np.testing.assert_array_equal(order.params[:2], p0[:2])

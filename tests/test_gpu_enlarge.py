@@ -114,12 +114,15 @@ def test_full_opt1_hdplus(golden_dir, overlap_threshold):
     order = BasisOrder.identity(f.params)
     h = {"energy": E40, "num_energy_evals_full_opt": 0}
     res = full_opt1(ctx, order, 31, 40, 15, sigma, overlap_threshold=overlap_threshold, max_overlap_penalty=1.0,
-                    history=h, tol=TOL, max_step=0.01)
+                    history=h, tol=TOL)
     ctx.close()
     print(res)
     assert res.evaluations <= 15 and h["num_energy_evals_full_opt"] == res.evaluations
     np.testing.assert_array_equal(order.params[:30], f.params[:30])
-    assert res.energy_after + res.penalty <= res.energy_before
+    # as the reference, FullOpt1 keeps the minimiser's last point: the best
+    # one only when the minimiser stopped on convergence
+    if 3 <= res.code <= 8:
+        assert res.energy_after + res.penalty <= res.energy_before
     Ef = _fresh_energy(f.system, order.params, sigma, TOL)
     assert abs(Ef - res.energy_after) <= 1e-12 * abs(Ef)
 

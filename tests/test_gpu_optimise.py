@@ -3,8 +3,9 @@ the device energy path: every step's energy is at most the one before it,
 the steps run on tail updates (a few changed rows, not a full
 refactorisation), the basis ends in its input order with only the
 optimised functions changed, and the final energy agrees with a fresh
-context's on the final parameters.  The minimiser stands in for DRMNG, so
-no trajectory is compared with the reference's."""
+context's on the final parameters.  The minimiser is the restated DRMNG
+with the reference's settings (checked call by call against the
+reference's own in test_drmng.py)."""
 from __future__ import annotations
 
 import os
@@ -37,7 +38,7 @@ def test_opt_cycle_hdplus_last_functions(golden_dir):
     # GHEPIIS's floor mode: on this truncated HD+ basis the reference's own
     # iteration levels off above the file's tolerance of 1e-12
     recs = opt_cycle(ctx, order, 185, 200, 1, max_energy_eval=10, sigma=sigma, overlap_threshold=0.99,
-                     lin_coeff_threshold=1e6, history=hist, max_step=0.05, tol=-1e-9)
+                     lin_coeff_threshold=1e6, history=hist, tol=-1e-9)
     energies = [r.energy_after for r in recs]
     print("E0", E0, "after", energies[-1], "steps", len(recs), "accepted", sum(r.accepted for r in recs))
     assert [r.func for r in recs] == list(range(185, 201))
