@@ -385,8 +385,7 @@ int copy_out_hs(ecg_ctx* x, int r0, int r1, char method, double sigma, double* H
 // the first zero pivot (0 if none).
 int factor_at(ecg_ctx* x, double sigma, int* nneg, int* zero) {
   x->fac_rows = 0;
-  RCCHK(ecg::form_shifted(x->c, x->f, sigma, x->stream));
-  RCCHK(ecg::ldlh_factor(x->f, x->stream));
+  RCCHK(ecg::ldlh_factor(x->f, x->stream, x->c.H, x->c.S, sigma));
   int st[3];
   HIPCHK(hipMemcpyAsync(st, x->f.status, sizeof(st), hipMemcpyDeviceToHost, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
@@ -416,8 +415,7 @@ int factor_queue(ecg_ctx* x, double sigma) {
     }
     RCCHK(ecg::ldlh_tail(x->c, f, sigma, m, x->stream));
   } else {
-    RCCHK(ecg::form_shifted(x->c, f, sigma, x->stream));
-    RCCHK(ecg::ldlh_factor(f, x->stream));
+    RCCHK(ecg::ldlh_factor(f, x->stream, x->c.H, x->c.S, sigma));
   }
   x->fac_sigma = sigma;
   x->last_tail = tail ? K - m : -1;

@@ -51,7 +51,7 @@ inline int ket_stride(int n) { return n * (n + 1) + 2 * n * n; }
 // after the stream has synchronised.  Off unless enabled on the context.
 enum TraceLabel {
   TR_OFFDIAG = 0, TR_ADJOINT, TR_LDL_UPDATE, TR_LDL_DIAG, TR_LDL_PANEL, TR_TRSV_FWD, TR_TRSV_BWD, TR_GEMV,
-  TR_FORM_SHIFTED, TR_LDL_TAIL, TR_COUNT
+  TR_FORM_SHIFTED, TR_LDL_TAIL, TR_COUNT   // TR_FORM_SHIFTED: unused (the factorisation forms H - sigma S itself)
 };
 struct Tracer {
   bool on = false;
@@ -168,8 +168,9 @@ struct Factor {
   size_t tp_cap = 0;         // entries of tp
   Tracer* tr = nullptr;
 };
-int form_shifted(const Ctx& c, Factor& f, double sigma, hipStream_t s);
-int ldlh_factor(Factor& f, hipStream_t s);
+// LDL^H of M = H - sigma S (f.K rows) into f; M's lower triangle is read
+// from H and S on the first pass (H null: from f.F, already holding M).
+int ldlh_factor(Factor& f, hipStream_t s, const zd* H = nullptr, const zd* S = nullptr, double sigma = 0.0);
 // Complete the factor of H - sigma S from row m (r = K - m <= NB changed
 // rows; rows < m factored at the same sigma and still in f).  Clears and
 // sets f.status like ldlh_factor.  f.tp must hold tail_partial_entries(ld).

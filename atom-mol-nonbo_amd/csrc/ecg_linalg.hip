@@ -15,22 +15,22 @@
 namespace ecg {
 
 // ---------------------------------------------------------------- shift
-__global__ void k_form_shifted(int K, int ld, const zd* __restrict__ H, const zd* __restrict__ S,
-                               double sigma, zd* __restrict__ F) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y;
-  if (i >= K || i < j) return;
-  const size_t o = (size_t)j * ld + i;
-  const zd h = H[o], s = S[o];
-  F[o] = mkz(fma(-sigma, s.x, h.x), fma(-sigma, s.y, h.y));
-}
-
-int form_shifted(const Ctx& c, Factor& f, double sigma, hipStream_t s) {
-  if (c.tr) c.tr->begin(TR_FORM_SHIFTED, s);
-  hipLaunchKernelGGL(k_form_shifted, dim3((c.K + 255) / 256, c.K), dim3(256), 0, s, c.K, c.ld, c.H, c.S,
-                     sigma, f.F);
-  if (c.tr) c.tr->end(s);
-  return (int)hipGetLastError();
+// The first pass of a factorisation reads the lower triangle of
+// M = H - sigma S straight from the assembled H and S (src.H non-null):
+// every lower element is read once there (block column 0 by the first
+// diagonal block and panel, block 1's diagonal tile by the first panel's
+// diagonal strips, every other lower tile by the first trailing update)
+// and written to F, so M is never formed as a matrix of its own.  With
+// src.H null the element comes from F (later passes, the tail update).
+struct ShiftSrc {
+  const zd* H = nullptr;
+  const zd* S = nullptr;
+  double sigma = 0.0;
+};
+__device__ inline zd ld_src(const ShiftSrc& src, const zd* __restrict__ F, size_t i, bool ok) {
+  if (!src.H) return ld_or0(F, i, ok);
+  const zd h = ld_or0(src.H, i, ok), s = ld_or0(src.S, i, ok);
+  return mkz(fma(-src.sigma, s.x, h.x), fma(-src.sigma, s.y, h.y));
 }
 
 // ------------------------------------------------ workgroup flags
@@ -258,7 +258,7 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
                                                                      zd* __restrict__ Dinv,
                                                                      int* __restrict__ status, DiagLds& L,
                                                                      bool a_in_lds, int* pub = nullptr,
-                                                                     int epoch = 0) {
+                                                                     int epoch = 0, ShiftSrc src = ShiftSrc()) {
   const int jb = min(NB, K - j0);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (!a_in_lds) {
@@ -266,7 +266,7 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
 #pragma unroll
     for (int u = 0; u < NB * NB / 256; ++u) {
       const int e = tid + 256 * u, j = e >> 6, i = e & (NB - 1);
-      v[u] = ld_or0(F, (size_t)(j0 + j) * ld + j0 + i, i < jb && j < jb && i >= j);
+      v[u] = ld_src(src, F, (size_t)(j0 + j) * ld + j0 + i, i < jb && j < jb && i >= j);
     }
 #pragma unroll
     for (int u = 0; u < NB * NB / 256; ++u) {
@@ -451,9 +451,9 @@ __device__ __attribute__((always_inline)) inline void ldl_diag_block(int K, int 
 
 __global__ void __launch_bounds__(256) k_ldl_diag(int K, int ld, int j0, zd* __restrict__ F,
                                                   double* __restrict__ D, zd* __restrict__ Dinv,
-                                                  int* __restrict__ status) {
+                                                  int* __restrict__ status, ShiftSrc src) {
   __shared__ DiagLds L;
-  ldl_diag_block(K, ld, j0, F, D, Dinv, status, L, false);
+  ldl_diag_block(K, ld, j0, F, D, Dinv, status, L, false, nullptr, 0, src);
 }
 
 // ----------------------------------------------------- MFMA 64x64 tile
@@ -534,17 +534,17 @@ __device__ inline void tile_pos(int a, int q, int v, int& r, int& c) {
 __device__ __attribute__((always_inline)) inline void panel_strip(int K, int ld, int j0, int i0, zd* __restrict__ F,
                                                                   const double* __restrict__ D,
                                                                   const zd* __restrict__ Dinv, zd* __restrict__ W,
-                                                                  DiagLds* wl = nullptr) {
+                                                                  DiagLds* wl = nullptr, ShiftSrc src = ShiftSrc()) {
   const int wave = threadIdx.x >> 6;
   const int c0 = SB * wave;
   const zd* X = Dinv + (size_t)(j0 / NB) * 2 * NB * NB;   // X(i, j) at X[j * NB + i]
-  const zd* A = F + (size_t)j0 * ld + i0;                  // A(r, k) at A[k * ld + r]
+  const size_t a0 = (size_t)j0 * ld + i0;                 // A(r, k) at F[a0 + k * ld + r]
   const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
   zd av[NB / 4], bv[NB / 4];
 #pragma unroll
   for (int t = 0; t < NB / 4; ++t) {
     const int kk = 4 * t + lk;
-    av[t] = ld_or0(A, (size_t)kk * ld + lr, i0 + lr < K);
+    av[t] = ld_src(src, F, a0 + (size_t)kk * ld + lr, i0 + lr < K);
     bv[t] = zconj(X[(size_t)kk * NB + c0 + lr]);
   }
   // two accumulator chains (even and odd k-steps) halve the dependent
@@ -595,7 +595,8 @@ __device__ __attribute__((always_inline)) inline void panel_strip(int K, int ld,
 // columns 16w..16w+15; two accumulator chains (even and odd k-steps) halve
 // the dependent MFMA latency.
 __device__ __attribute__((always_inline)) inline void diag_quarter_update(int K, int ld, int j0, int q,
-                                                                          zd* __restrict__ F, const DiagLds& Ld) {
+                                                                          zd* __restrict__ F, const DiagLds& Ld,
+                                                                          ShiftSrc src = ShiftSrc()) {
   const int wave = threadIdx.x >> 6;
   if (wave > q) return;   // those columns lie above the quarter's rows
   const int T1 = j0 + NB;
@@ -605,7 +606,7 @@ __device__ __attribute__((always_inline)) inline void diag_quarter_update(int K,
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int rr = R0 + acc16_row(v), c = C0 + acc16_col();
-    old[v] = ld_or0(F, (size_t)(T1 + c) * ld + T1 + rr, T1 + rr < K && c <= rr);
+    old[v] = ld_src(src, F, (size_t)(T1 + c) * ld + T1 + rr, T1 + rr < K && c <= rr);
   }
 #pragma unroll
   for (int st = 0; st < NB / 4; ++st) {
@@ -651,13 +652,14 @@ __device__ __attribute__((always_inline)) inline void strip_task(int K, int ld, 
                                                                  const double* __restrict__ D,
                                                                  const zd* __restrict__ Dinv, zd* __restrict__ W,
                                                                  int* __restrict__ pflag, int epoch,
-                                                                 int* __restrict__ status, DiagLds& Ld) {
+                                                                 int* __restrict__ status, DiagLds& Ld,
+                                                                 ShiftSrc src = ShiftSrc()) {
   const bool dg = t.diag >= 0;
-  panel_strip(K, ld, j0, t.a, F, D, Dinv, W, dg ? &Ld : nullptr);
+  panel_strip(K, ld, j0, t.a, F, D, Dinv, W, dg ? &Ld : nullptr, src);
   if (!dg) return;
   publish_wg(&pflag[t.publish], epoch);   // also orders the LDS copy of W
   wait_list(pflag, t.wait, first_wait, epoch, status);
-  diag_quarter_update(K, ld, j0, t.diag, F, Ld);
+  diag_quarter_update(K, ld, j0, t.diag, F, Ld, src);
 }
 
 // Panel of the first block column (later panels are computed inside the
@@ -666,10 +668,10 @@ __device__ __attribute__((always_inline)) inline void strip_task(int K, int ld, 
 __global__ void __launch_bounds__(256) k_ldl_panel(int K, int ld, zd* __restrict__ F, const double* __restrict__ D,
                                                    const zd* __restrict__ Dinv, zd* __restrict__ W,
                                                    int* __restrict__ status, int* __restrict__ pflag, int qflag,
-                                                   int epoch) {
+                                                   int epoch, ShiftSrc src) {
   __shared__ DiagLds Ld;
   const UpdTask t = panel_task((int)blockIdx.x, qflag);
-  strip_task(K, ld, 0, t, 0, F, D, Dinv, W, pflag, epoch, status, Ld);
+  strip_task(K, ld, 0, t, 0, F, D, Dinv, W, pflag, epoch, status, Ld, src);
 }
 
 // ------------------------------------------------- inverse of L by blocks
@@ -799,7 +801,8 @@ __global__ void __launch_bounds__(256) k_ldl_update(int K, int ld, int j0, zd* _
                                                     const zd* __restrict__ Wcur, zd* __restrict__ Wnext,
                                                     double* __restrict__ D, zd* __restrict__ Dinv,
                                                     int* __restrict__ status, int* __restrict__ pflag,
-                                                    int qflag, int epoch, int ntiles, int nstrip) {
+                                                    int qflag, int epoch, int ntiles, int nstrip,
+                                                    ShiftSrc src) {
   __shared__ DiagLds Ld;
   const int jb = min(NB, K - j0);
   const int t0 = (j0 + jb) / NB;
@@ -836,7 +839,7 @@ __global__ void __launch_bounds__(256) k_ldl_update(int K, int ld, int j0, zd* _
       for (int v = 0; v < 4; ++v) {
         int r, c;
         tile_pos(a, q, v, r, c);
-        old[a][q][v] = ld_or0(F, (size_t)(K0 + c) * ld + I0 + r, r < ib && c < kb && (I0 + r) >= (K0 + c));
+        old[a][q][v] = ld_src(src, F, (size_t)(K0 + c) * ld + I0 + r, r < ib && c < kb && (I0 + r) >= (K0 + c));
       }
   Tile64 t;
   mfma_tile_abh(Wcur + I0, ld, ib, F + (size_t)j0 * ld + K0, ld, kb, jb, t);
@@ -858,19 +861,24 @@ __global__ void k_clear_status(int* status) {
   if (threadIdx.x < 3) status[threadIdx.x] = 0;
 }
 
-int ldlh_factor(Factor& f, hipStream_t s) {
+int ldlh_factor(Factor& f, hipStream_t s, const zd* H, const zd* S, double sigma) {
   const int K = f.K;
   const int qflag = f.ld / NB + 1;
+  ShiftSrc src;   // first pass: H - sigma S (or F when H is null)
+  src.H = H;
+  src.S = S;
+  src.sigma = sigma;
+  const ShiftSrc none;
   hipLaunchKernelGGL(k_clear_status, dim3(1), dim3(64), 0, s, f.status);
   if (f.tr) f.tr->begin(TR_LDL_DIAG, s);
-  hipLaunchKernelGGL(k_ldl_diag, dim3(1), dim3(256), 0, s, K, f.ld, 0, f.F, f.D, f.Dinv, f.status);
+  hipLaunchKernelGGL(k_ldl_diag, dim3(1), dim3(256), 0, s, K, f.ld, 0, f.F, f.D, f.Dinv, f.status, src);
   if (f.tr) f.tr->end(s);
   static_assert(NB == SCHED_NB && SB == SCHED_SB, "task map and kernel disagree");
   if (K > NB) {
     const int epoch = ++f.epoch;
     if (f.tr) f.tr->begin(TR_LDL_PANEL, s);
     hipLaunchKernelGGL(k_ldl_panel, dim3((K - NB + SB - 1) / SB), dim3(256), 0, s, K, f.ld, f.F, f.D, f.Dinv, f.W,
-                       f.status, f.pflag, qflag, epoch);
+                       f.status, f.pflag, qflag, epoch, src);
     if (f.tr) f.tr->end(s);
   }
   for (int j0 = 0, col = 0; j0 + NB < K; j0 += NB, ++col) {
@@ -880,7 +888,7 @@ int ldlh_factor(Factor& f, hipStream_t s) {
     const int epoch = ++f.epoch;
     if (f.tr) f.tr->begin(TR_LDL_UPDATE, s);
     hipLaunchKernelGGL(k_ldl_update, dim3(shape.grid), dim3(256), 0, s, K, f.ld, j0, f.F, Wcur, Wnext, f.D, f.Dinv,
-                       f.status, f.pflag, qflag, epoch, shape.ntiles, shape.nstrip);
+                       f.status, f.pflag, qflag, epoch, shape.ntiles, shape.nstrip, col == 0 ? src : none);
     if (f.tr) f.tr->end(s);
   }
   if (K > NB) hipLaunchKernelGGL(k_ldl_xlast, dim3((K + NB - 1) / NB - 1), dim3(256), 0, s, K, f.ld, f.F, f.Dinv);
@@ -1159,7 +1167,8 @@ int ldlh_tail(const Ctx& c, Factor& f, double sigma, int m, hipStream_t s) {
       else tail_passes<8>(c, f, m, r, rp, nch, Bt, Zt, s, first);
       if (first) {
         hipLaunchKernelGGL(k_tail_schur, dim3(r * (r + 1) / 2), dim3(256), 0, s, ld, m, r, Zt, f.D, f.F, f.status);
-        if (r > 1) hipLaunchKernelGGL(k_ldl_diag, dim3(1), dim3(256), 0, s, K, ld, m, f.F, f.D, f.Dinv, f.status);
+        if (r > 1)
+          hipLaunchKernelGGL(k_ldl_diag, dim3(1), dim3(256), 0, s, K, ld, m, f.F, f.D, f.Dinv, f.status, ShiftSrc());
       }
     }
     hipLaunchKernelGGL(k_tail_xfin, dim3((m + 63) / 64), dim3(256), 0, s, ld, m, r, rp, nch, Zt, f.tp, f.F);

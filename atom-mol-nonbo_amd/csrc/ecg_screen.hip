@@ -145,8 +145,7 @@ int screen_prepare(const Ctx& c, Factor& f, Screen& sc, double sigma, hipStream_
   const int Kp = c.K - 1, ld = c.ld;
   sc.Kp = Kp;
   sc.sigma = sigma;
-  int rc = form_shifted(c, f, sigma, s);
-  if (!rc) rc = ldlh_factor(f, s);
+  int rc = ldlh_factor(f, s, c.H, c.S, sigma);
   if (rc) return rc;
   // Z = L'^{-1}: block b: Z_b = X_b R_b, then R_below -= L_below,b Z_b.
   // Only columns < b0 + bs of block rows >= b are nonzero.

@@ -134,7 +134,7 @@ def pmc_flops(kernel):
 
 ROCNAME = {"ldl_update": "k_ldl_update", "adjoint": "k_adjoint<3>", "trsv_fwd": "k_xsolve_fwd",
            "trsv_bwd": "k_xsolve_bwd", "offdiag": "k_offdiag<3>", "ldl_diag": "k_ldl_diag",
-           "ldl_panel": "k_ldl_panel", "form_shifted": "k_form_shifted"}
+           "ldl_panel": "k_ldl_panel"}
 
 
 def kernel_table(system, base, K, trace, nsteps, flops_adj):

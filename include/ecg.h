@@ -214,8 +214,9 @@ int ecg_batch_energy_grad(ecg_batch* b, const double* sigma, double tol, int max
 #define ECG_TRACE_TRSV_FWD 5      /* forward triangular solve */
 #define ECG_TRACE_TRSV_BWD 6      /* backward triangular solve */
 #define ECG_TRACE_GEMV 7          /* (reserved) */
-#define ECG_TRACE_FORM_SHIFTED 8  /* H - sigma S */
-#define ECG_TRACE_COUNT 9
+#define ECG_TRACE_FORM_SHIFTED 8  /* no launches: the factorisation reads H - sigma S in its first pass */
+#define ECG_TRACE_LDL_TAIL 9      /* incremental refactorisation from a changed row */
+#define ECG_TRACE_COUNT 10
 int ecg_trace(ecg_ctx* ctx, int enable);
 int ecg_trace_read(ecg_ctx* ctx, double* count, double* total_ms, int nmax);
 
