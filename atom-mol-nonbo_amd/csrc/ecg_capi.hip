@@ -94,8 +94,13 @@ struct ecg_batch {
     long long ntiles = 0;
   };
   std::vector<Class> classes;
-  int packed = 1;     // 1: pair phases packed across problems; 0: per problem
+  int packed = 1;     // 2: pair phases packed and factorisations batched; 1: pair phases packed; 0: per problem
   int nstreams = 4;   // problems whose eigen steps run concurrently
+  hipEvent_t ev_fac = nullptr;        // batched factorisations queued on the lead stream
+  std::vector<ecg::FacDesc> fh;       // their descriptors (host copy, alive until the upload ran)
+  ecg::FacDesc* fd = nullptr;
+  std::vector<ecg::InvDesc> ih;       // the batched inverse iterations' descriptors
+  ecg::InvDesc* id = nullptr;
 };
 
 namespace {
@@ -576,7 +581,8 @@ int inverse_iteration(ecg_ctx* x, double sigma, double tol, int maxiter, bool wa
 // before_sync / hooked: as for inverse_iteration, in the fixed-shift path
 // only (*hooked false otherwise).
 int solve_root(ecg_ctx* x, int which, double sigma, double tol, int maxiter, double* E, double* info,
-               int* status, const std::function<int()>* before_sync = nullptr, bool* hooked = nullptr) {
+               int* status, const std::function<int()>* before_sync = nullptr, bool* hooked = nullptr,
+               bool prefactored = false) {
   if (hooked) *hooked = false;
   const double sigma_in = sigma;
   ecg::Ctx& c = x->c;
@@ -586,8 +592,15 @@ int solve_root(ecg_ctx* x, int which, double sigma, double tol, int maxiter, dou
   *status = 0;
   if (std::isfinite(sigma)) {
     // factorisation queued without a host round trip; its status is read
-    // at the inverse iteration's first synchronisation
-    RCCHK(factor_queue(x, sigma));
+    // at the inverse iteration's first synchronisation (prefactored: a
+    // batched factorisation of H - sigma S already queued on this stream)
+    if (prefactored) {
+      x->fac_rows = 0;
+      x->fac_sigma = sigma;
+      x->last_tail = -1;
+    } else {
+      RCCHK(factor_queue(x, sigma));
+    }
     ++nfac;
     int fst[3] = {0, 0, 0};
     // after a tail update the previous eigenvector is close: two queued
@@ -1384,6 +1397,59 @@ int ecg_energy_grad_stored(ecg_ctx* x, int nfru, int which, double sigma, double
   return status;
 }
 
+namespace {
+// After a batched inverse iteration (ecg::invit_batch, nq iterations and
+// the tail, reported in x->hrep): the decisions of inverse_iteration's
+// energy path (stop_at_floor) and solve_root's epilogue for one problem.
+// A problem whose measure asks for more iterations continues on its own
+// stream from the normalised iterate.
+int invit_batch_finish(ecg_ctx* x, double sigma, double tol, int maxiter, int nq, double* E, int* status) {
+  const int K = x->c.K;
+  double q[5];
+  int st[3];
+  for (int i = 0; i < 5; ++i) q[i] = x->hrep[i];
+  for (int i = 0; i < 3; ++i) st[i] = (int)x->hrep[5 + i];
+  *status = 0;
+  if (st[2]) return ECG_ERR_SYNC;   // a device-side wait of the factorisation hit its limit
+  if (st[1]) {
+    *status = ECG_SINGULAR;
+    return 0;
+  }
+  InvitResult r;
+  r.iters = nq;
+  r.nd = q[0];
+  if (!std::isfinite(r.nd)) return ECG_ERR_NONFINITE;
+  x->have_last = true;
+  x->last_K = K;
+  const double p1 = (nq >= 2) ? q[4] : std::numeric_limits<double>::infinity();
+  bool done = false;
+  if (tol > 0.0 && r.nd <= tol) {
+    r.converged = true;
+    done = true;
+  } else if (nq >= 2 && r.nd < 1e-9 && r.nd > 0.5 * p1) {
+    done = true;
+  } else if (nq >= maxiter) {
+    done = true;
+  }
+  if (done) {
+    r.t1 = q[2];
+    r.lambda = q[3] / q[2] + sigma;
+  } else {
+    int fst[3] = {0, 0, 0};
+    RCCHK(inverse_iteration(x, sigma, tol, maxiter - nq, true, r, true, fst, 1));
+    if (fst[1]) {
+      *status = ECG_SINGULAR;
+      return 0;
+    }
+    r.iters += nq;
+  }
+  x->fac_rows = K;   // the factor is complete and without zero pivots
+  *E = r.lambda;
+  if (!r.converged && r.nd > std::fabs(tol)) *status = ECG_NOT_CONVERGED;
+  return 0;
+}
+}  // namespace
+
 int ecg_batch_create(int nprob, ecg_ctx** ctxs, ecg_batch** out) {
   if (!out || nprob < 1 || !ctxs) return ECG_ERR_ARG;
   *out = nullptr;
@@ -1392,7 +1458,10 @@ int ecg_batch_create(int nprob, ecg_ctx** ctxs, ecg_batch** out) {
   ecg_batch* b = new ecg_batch();
   b->ctx.assign(ctxs, ctxs + nprob);
   if (hipStreamCreateWithFlags(&b->lead, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&b->ev_asm, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&b->ev_asm, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_fac, hipEventDisableTiming) != hipSuccess ||
+      hipMalloc((void**)&b->fd, sizeof(ecg::FacDesc) * nprob) != hipSuccess ||
+      hipMalloc((void**)&b->id, sizeof(ecg::InvDesc) * nprob) != hipSuccess) {
     ecg_batch_destroy(b);
     return ECG_ERR_HIP;
   }
@@ -1431,13 +1500,16 @@ int ecg_batch_destroy(ecg_batch* b) {
   for (hipEvent_t e : b->ev_eig)
     if (e) (void)hipEventDestroy(e);
   if (b->ev_asm) (void)hipEventDestroy(b->ev_asm);
+  if (b->ev_fac) (void)hipEventDestroy(b->ev_fac);
+  if (b->fd) (void)hipFree(b->fd);
+  if (b->id) (void)hipFree(b->id);
   if (b->lead) (void)hipStreamDestroy(b->lead);
   delete b;
   return ECG_OK;
 }
 
 int ecg_batch_set_config(ecg_batch* b, int packed, int nstreams) {
-  if (!b || (packed != 0 && packed != 1) || nstreams < 1 || nstreams > 16) return ECG_ERR_ARG;
+  if (!b || packed < 0 || packed > 2 || nstreams < 1 || nstreams > 16) return ECG_ERR_ARG;
   b->packed = packed;
   b->nstreams = nstreams;
   return ECG_OK;
@@ -1525,28 +1597,120 @@ int ecg_batch_energy_grad(ecg_batch* b, const double* sigma, double tol, int max
       RCCHK(ecg::packed_pairs(cl.n, false, cl.d, (int)cl.h.size(), cl.ntiles, cl.nf, cl.maxT, b->lead));
     HIPCHK(hipEventRecord(b->ev_asm, b->lead));
   }
+  // packed == 2: the factorisations of H - sigma S of every problem with a
+  // finite shift in shared launches on the lead stream (the others locate
+  // their root by inertia on their own streams)
+  std::vector<char> batched(np, 0);
+  if (b->packed == 2) {
+    b->fh.clear();
+    int epoch = 0;
+    for (int i = 0; i < np; ++i) {
+      if (!std::isfinite(sigma[i])) continue;
+      ecg_ctx* x = b->ctx[i];
+      ecg::FacDesc d;
+      d.K = x->c.K;
+      d.ld = x->f.ld;
+      d.F = x->f.F;
+      d.D = x->f.D;
+      d.Dinv = x->f.Dinv;
+      d.W = x->f.W;
+      d.status = x->f.status;
+      d.pflag = x->f.pflag;
+      d.src.H = x->c.H;
+      d.src.S = x->c.S;
+      d.src.sigma = sigma[i];
+      b->fh.push_back(d);
+      epoch = std::max(epoch, x->f.epoch);
+      batched[i] = 1;
+    }
+    std::stable_sort(b->fh.begin(), b->fh.end(), [](const ecg::FacDesc& a, const ecg::FacDesc& c) {
+      return a.K > c.K;
+    });
+    if (!b->fh.empty()) {
+      HIPCHK(hipMemcpyAsync(b->fd, b->fh.data(), sizeof(ecg::FacDesc) * b->fh.size(), hipMemcpyHostToDevice,
+                            b->lead));
+      RCCHK(ecg::ldlh_factor_batch(b->fh, b->fd, epoch, b->lead));
+      for (int i = 0; i < np; ++i)
+        if (batched[i]) b->ctx[i]->f.epoch = epoch;
+    }
+    HIPCHK(hipEventRecord(b->ev_fac, b->lead));
+  }
+  // packed == 2: the inverse iterations of the batched factorisations in
+  // shared launches too (K <= 8192), three iterations and the tail with one
+  // synchronisation for all; the rest go to the workers below
+  std::vector<char> done(np, 0);
+  std::vector<int> st(np, 0);
+  if (b->packed == 2) {
+    b->ih.clear();
+    std::vector<int> which;
+    for (int i = 0; i < np; ++i) {
+      ecg_ctx* x = b->ctx[i];
+      if (!batched[i] || x->c.K > 8192) continue;
+      x->fac_rows = 0;
+      x->fac_sigma = sigma[i];
+      x->last_tail = -1;
+      x->vec_current = false;
+      ecg::InvDesc q;
+      q.K = x->c.K;
+      q.ld = x->c.ld;
+      q.warm = (x->have_last && x->last_K == x->c.K) ? 1 : 0;
+      q.H = x->c.H;
+      q.S = x->c.S;
+      q.sigma = sigma[i];
+      q.F = x->f.F;
+      q.D = x->f.D;
+      for (int k = 0; k < 3; ++k) q.ev[k] = x->ev[k];
+      q.w = x->f.vec;
+      q.z = x->f.tvec;
+      q.part = x->f.W;
+      q.scal = x->f.scal;
+      q.status = x->f.status;
+      q.rep = x->hrep;
+      b->ih.push_back(q);
+      which.push_back(i);
+    }
+    if (!b->ih.empty()) {
+      const int nq = std::min(maxiter, 3);
+      HIPCHK(hipMemcpyAsync(b->id, b->ih.data(), sizeof(ecg::InvDesc) * b->ih.size(), hipMemcpyHostToDevice,
+                            b->lead));
+      RCCHK(ecg::invit_batch(b->ih, b->id, nq, b->lead));
+      HIPCHK(hipStreamSynchronize(b->lead));
+      for (int i : which) {
+        ecg_ctx* x = b->ctx[i];
+        int status = 0;
+        const int rc = invit_batch_finish(x, sigma[i], tol, maxiter, nq, &E[i], &status);
+        if (!rc) x->vec_current = (status == 0 || status == ECG_NOT_CONVERGED);
+        if (!rc) HIPCHK(hipEventRecord(b->ev_eig[i], x->stream));
+        st[i] = rc ? rc : status;
+        done[i] = 1;
+      }
+    }
+  }
   // eigen steps (and, unpacked, the whole problem) on nstreams worker
   // threads, problems taken largest first
   std::vector<int> order(np);
   for (int i = 0; i < np; ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return b->ctx[a]->c.K > b->ctx[c]->c.K; });
+  order.erase(std::remove_if(order.begin(), order.end(), [&](int i) { return done[i] != 0; }), order.end());
+  const int nwork = (int)order.size();
   std::atomic<int> next(0);
-  std::vector<int> st(np, 0);
   auto worker = [&]() {
     for (;;) {
       const int q = next.fetch_add(1);
-      if (q >= np) return;
+      if (q >= nwork) return;
       const int i = order[q];
       ecg_ctx* x = b->ctx[i];
       double info[8];
       int status = 0, rc = 0;
       if (b->packed) {
-        if (hipStreamWaitEvent(x->stream, b->ev_asm, 0) != hipSuccess) rc = ECG_ERR_HIP;
+        if (hipStreamWaitEvent(x->stream, b->packed == 2 ? b->ev_fac : b->ev_asm, 0) != hipSuccess)
+          rc = ECG_ERR_HIP;
       } else {
         rc = ecg::assemble(x->c, 0, x->c.K, x->stream);
       }
       x->vec_current = false;
-      if (!rc) rc = solve_root(x, 1, sigma[i], tol, maxiter, &E[i], info, &status);
+      if (!rc)
+        rc = solve_root(x, 1, sigma[i], tol, maxiter, &E[i], info, &status, nullptr, nullptr, batched[i] != 0);
       if (!rc) x->vec_current = (status == 0 || status == ECG_NOT_CONVERGED);
       if (!rc && !b->packed) {
         rc = ecg::adjoint(x->c, 0, E[i], nullptr, x->ev[2], x->part, x->grad, x->stream);
@@ -1564,7 +1728,7 @@ int ecg_batch_energy_grad(ecg_batch* b, const double* sigma, double tol, int max
   };
   {
     std::vector<std::thread> pool;
-    const int nt = std::min(b->nstreams, np);
+    const int nt = std::min(b->nstreams, nwork);
     for (int t = 0; t < nt; ++t) pool.emplace_back(worker);
     for (auto& t : pool) t.join();
   }

@@ -171,6 +171,51 @@ struct Factor {
 // LDL^H of M = H - sigma S (f.K rows) into f; M's lower triangle is read
 // from H and S on the first pass (H null: from f.F, already holding M).
 int ldlh_factor(Factor& f, hipStream_t s, const zd* H = nullptr, const zd* S = nullptr, double sigma = 0.0);
+// Source of a factorisation's first pass: M = H - sigma S from H and S, or
+// (H null) from F itself.
+struct ShiftSrc {
+  const zd* H = nullptr;
+  const zd* S = nullptr;
+  double sigma = 0.0;
+};
+// One problem of a batched factorisation (the storage of its Factor).
+struct FacDesc {
+  int K = 0, ld = 0;
+  zd* F = nullptr;
+  double* D = nullptr;
+  zd* Dinv = nullptr;
+  zd* W = nullptr;
+  int* status = nullptr;
+  int* pflag = nullptr;
+  ShiftSrc src;
+};
+// Factorisations of independent problems in shared launches (h on the
+// host, largest K first; d its copy in device memory).  Flag values continue from epoch,
+// which must exceed every flag value the problems' flag arrays hold; on
+// return it is the last value used.
+int ldlh_factor_batch(const std::vector<FacDesc>& h, const FacDesc* d, int& epoch, hipStream_t s);
+// One problem of a batched inverse iteration at a fixed shift (the
+// storage of its Ctx, Factor and context vectors).
+struct InvDesc {
+  int K = 0, ld = 0, warm = 0;   // warm: start from ev[2], else from ones
+  const zd* H = nullptr;
+  const zd* S = nullptr;
+  double sigma = 0.0;
+  const zd* F = nullptr;
+  const double* D = nullptr;
+  zd* ev[3] = {nullptr, nullptr, nullptr};
+  zd* w = nullptr;        // S v
+  zd* z = nullptr;        // D^{-1} L^{-1} S v
+  zd* part = nullptr;     // the solve's and the quadratic forms' partial sums
+  double* scal = nullptr;
+  const int* status = nullptr;
+  double* rep = nullptr;  // pinned host report
+};
+// nq inverse iterations (x = (H - sigma S)^{-1} S v, GHEPIIS's scaling and
+// measure) of every problem, then the Rayleigh quotient, the normalisation
+// x^H S x = 1 and the report, in shared launches; the result vector ends in
+// ev[nq & 1] and ev[2].  Needs K <= 8192 (returns -1 otherwise).
+int invit_batch(const std::vector<InvDesc>& h, const InvDesc* d, int nq, hipStream_t s);
 // Complete the factor of H - sigma S from row m (r = K - m <= NB changed
 // rows; rows < m factored at the same sigma and still in f).  Clears and
 // sets f.status like ldlh_factor.  f.tp must hold tail_partial_entries(ld).

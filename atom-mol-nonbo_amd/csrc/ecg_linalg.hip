@@ -9,6 +9,8 @@
 // L overwrites the strict lower triangle of F.  X = L^{-1} is built by
 // block rows alongside the factorisation and kept as X^H in the strict
 // upper triangle, so that a solve is two GEMV passes over that triangle.
+#include <algorithm>
+
 #include "ecg_internal.hpp"
 #include "ecg_schedule.hpp"
 
@@ -22,11 +24,6 @@ namespace ecg {
 // diagonal strips, every other lower tile by the first trailing update)
 // and written to F, so M is never formed as a matrix of its own.  With
 // src.H null the element comes from F (later passes, the tail update).
-struct ShiftSrc {
-  const zd* H = nullptr;
-  const zd* S = nullptr;
-  double sigma = 0.0;
-};
 __device__ inline zd ld_src(const ShiftSrc& src, const zd* __restrict__ F, size_t i, bool ok) {
   if (!src.H) return ld_or0(F, i, ok);
   const zd h = ld_or0(src.H, i, ok), s = ld_or0(src.S, i, ok);
@@ -797,27 +794,25 @@ __global__ void __launch_bounds__(256) k_ldl_xlast(int K, int ld, zd* __restrict
 // size and verifies this rule.
 // W is double-buffered by column parity: the tiles read column j's, the
 // strips write column j+1's.
-__global__ void __launch_bounds__(256) k_ldl_update(int K, int ld, int j0, zd* __restrict__ F,
-                                                    const zd* __restrict__ Wcur, zd* __restrict__ Wnext,
-                                                    double* __restrict__ D, zd* __restrict__ Dinv,
-                                                    int* __restrict__ status, int* __restrict__ pflag,
-                                                    int qflag, int epoch, int ntiles, int nstrip,
-                                                    ShiftSrc src) {
-  __shared__ DiagLds Ld;
+// Task w of the update launch for block column col of one factorisation.
+__device__ __attribute__((always_inline)) inline void ldl_update_task(int K, int ld, int col, int w,
+                                                                      const UpdShape& shape, zd* __restrict__ F,
+                                                                      zd* __restrict__ W, double* __restrict__ D,
+                                                                      zd* __restrict__ Dinv, int* __restrict__ status,
+                                                                      int* __restrict__ pflag, int qflag, int epoch,
+                                                                      const ShiftSrc& src, DiagLds& Ld) {
+  const int j0 = col * NB;
+  const zd* Wcur = W + (size_t)(col & 1) * ld * NB;
+  zd* Wnext = W + (size_t)((col + 1) & 1) * ld * NB;
   const int jb = min(NB, K - j0);
   const int t0 = (j0 + jb) / NB;
-  UpdShape shape;
-  shape.ntiles = ntiles;
-  shape.nstrip = nstrip;
-  shape.nx = (int)gridDim.x - 1 - ntiles - nstrip;   // tiles of L^{-1}
-  shape.grid = (int)gridDim.x;
-  const UpdTask task = upd_task((int)blockIdx.x, K, j0, shape, qflag);
+  const UpdTask task = upd_task(w, K, j0, shape, qflag);
   if (task.kind == UPD_LOOKAHEAD) {
     ldl_diag_block(K, ld, t0 * NB, F, D, Dinv, status, Ld, false, &pflag[task.publish], epoch);
     return;
   }
   if (task.kind == UPD_XINV) {
-    xinv_tile(K, ld, j0 / NB, task.a, task.b, F, Dinv, Ld);
+    xinv_tile(K, ld, col, task.a, task.b, F, Dinv, Ld);
     return;
   }
   if (task.kind == UPD_STRIP) {
@@ -857,6 +852,94 @@ __global__ void __launch_bounds__(256) k_ldl_update(int K, int ld, int j0, zd* _
   if (task.publish >= 0) publish_wg(&pflag[task.publish], epoch);
 }
 
+__global__ void __launch_bounds__(256) k_ldl_update(int K, int ld, int col, zd* __restrict__ F, zd* __restrict__ W,
+                                                    double* __restrict__ D, zd* __restrict__ Dinv,
+                                                    int* __restrict__ status, int* __restrict__ pflag, int qflag,
+                                                    int epoch, ShiftSrc src) {
+  __shared__ DiagLds Ld;
+  const UpdShape shape = upd_shape(K, col);
+  ldl_update_task(K, ld, col, (int)blockIdx.x, shape, F, W, D, Dinv, status, pflag, qflag, epoch, src, Ld);
+}
+
+// ------------------------------------------------- batched factorisations
+// Independent problems of a batch factored in the same launches, on a
+// one-dimensional grid of P x (largest task count) workgroups: workgroup
+// id serves problem id % P, task id / P, so the problems' look-ahead
+// factors come first and every problem's tasks interleave.  Workgroups
+// beyond a problem's own task count return at once.  The dispatcher
+// places workgroups in index order, so a task's waits, which point to
+// lower task indices of its own problem, point to lower workgroup indices:
+// the rule of the single-problem launches carries over.
+__global__ void __launch_bounds__(64) k_clear_status_b(const FacDesc* __restrict__ d) {
+  if (threadIdx.x < 3) d[blockIdx.x].status[threadIdx.x] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_ldl_diag_b(const FacDesc* __restrict__ d) {
+  __shared__ DiagLds L;
+  const FacDesc& f = d[blockIdx.x];
+  ldl_diag_block(f.K, f.ld, 0, f.F, f.D, f.Dinv, f.status, L, false, nullptr, 0, f.src);
+}
+
+__global__ void __launch_bounds__(256) k_ldl_panel_b(const FacDesc* __restrict__ d, int P, int epoch) {
+  __shared__ DiagLds Ld;
+  const FacDesc& f = d[blockIdx.x % P];
+  const int sidx = (int)(blockIdx.x / P);
+  if (f.K <= NB || sidx >= (f.K - NB + SB - 1) / SB) return;
+  const UpdTask t = panel_task(sidx, f.ld / NB + 1);
+  strip_task(f.K, f.ld, 0, t, 0, f.F, f.D, f.Dinv, f.W, f.pflag, epoch, f.status, Ld, f.src);
+}
+
+__global__ void __launch_bounds__(256) k_ldl_update_b(const FacDesc* __restrict__ d, int P, int col, int epoch) {
+  __shared__ DiagLds Ld;
+  const FacDesc& f = d[blockIdx.x % P];
+  const int w = (int)(blockIdx.x / P);
+  if (col * NB + NB >= f.K) return;
+  const UpdShape shape = upd_shape(f.K, col);
+  if (w >= shape.grid) return;
+  const ShiftSrc none;
+  ldl_update_task(f.K, f.ld, col, w, shape, f.F, f.W, f.D, f.Dinv, f.status, f.pflag, f.ld / NB + 1, epoch,
+                  col == 0 ? f.src : none, Ld);
+}
+
+__global__ void __launch_bounds__(256) k_ldl_xlast_b(const FacDesc* __restrict__ d, int P) {
+  __shared__ DiagLds Ld;
+  const FacDesc& f = d[blockIdx.x % P];
+  const int c = (int)(blockIdx.x / P);
+  const int nb = (f.K + NB - 1) / NB;
+  if (f.K <= NB || c >= nb - 1) return;
+  xinv_tile(f.K, f.ld, nb - 1, nb - 1, c, f.F, f.Dinv, Ld);
+}
+
+int ldlh_factor_batch(const std::vector<FacDesc>& h, const FacDesc* d, int& epoch, hipStream_t s) {
+  const int P = (int)h.size();
+  if (P == 0) return 0;
+  for (int p = 1; p < P; ++p)
+    if (h[p].K > h[p - 1].K) return -1;   // problems must come largest first
+  // problems still active in a launch are a prefix of the list
+  auto active = [&](int minK) {
+    int n = 0;
+    while (n < P && h[n].K > minK) ++n;
+    return n;
+  };
+  const int maxK = h[0].K;
+  hipLaunchKernelGGL(k_clear_status_b, dim3(P), dim3(64), 0, s, d);
+  hipLaunchKernelGGL(k_ldl_diag_b, dim3(P), dim3(256), 0, s, d);
+  const int pp = active(NB);
+  if (pp > 0) {
+    ++epoch;
+    hipLaunchKernelGGL(k_ldl_panel_b, dim3(pp * ((maxK - NB + SB - 1) / SB)), dim3(256), 0, s, d, pp, epoch);
+  }
+  for (int col = 0; col * NB + NB < maxK; ++col) {
+    const int pc = active(col * NB + NB);
+    int gy = 0;
+    for (int p = 0; p < pc; ++p) gy = std::max(gy, upd_shape(h[p].K, col).grid);
+    ++epoch;
+    hipLaunchKernelGGL(k_ldl_update_b, dim3(pc * gy), dim3(256), 0, s, d, pc, col, epoch);
+  }
+  if (pp > 0) hipLaunchKernelGGL(k_ldl_xlast_b, dim3(pp * ((maxK + NB - 1) / NB - 1)), dim3(256), 0, s, d, pp);
+  return (int)hipGetLastError();
+}
+
 __global__ void k_clear_status(int* status) {
   if (threadIdx.x < 3) status[threadIdx.x] = 0;
 }
@@ -881,14 +964,12 @@ int ldlh_factor(Factor& f, hipStream_t s, const zd* H, const zd* S, double sigma
                        f.status, f.pflag, qflag, epoch, src);
     if (f.tr) f.tr->end(s);
   }
-  for (int j0 = 0, col = 0; j0 + NB < K; j0 += NB, ++col) {
+  for (int col = 0; col * NB + NB < K; ++col) {
     const UpdShape shape = upd_shape(K, col);
-    zd* Wcur = f.W + (size_t)(col & 1) * f.ld * NB;
-    zd* Wnext = f.W + (size_t)((col + 1) & 1) * f.ld * NB;
     const int epoch = ++f.epoch;
     if (f.tr) f.tr->begin(TR_LDL_UPDATE, s);
-    hipLaunchKernelGGL(k_ldl_update, dim3(shape.grid), dim3(256), 0, s, K, f.ld, j0, f.F, Wcur, Wnext, f.D, f.Dinv,
-                       f.status, f.pflag, qflag, epoch, shape.ntiles, shape.nstrip, col == 0 ? src : none);
+    hipLaunchKernelGGL(k_ldl_update, dim3(shape.grid), dim3(256), 0, s, K, f.ld, col, f.F, f.W, f.D, f.Dinv, f.status,
+                       f.pflag, qflag, epoch, col == 0 ? src : none);
     if (f.tr) f.tr->end(s);
   }
   if (K > NB) hipLaunchKernelGGL(k_ldl_xlast, dim3((K + NB - 1) / NB - 1), dim3(256), 0, s, K, f.ld, f.F, f.Dinv);
@@ -1188,10 +1269,11 @@ int ldlh_tail(const Ctx& c, Factor& f, double sigma, int m, hipStream_t s) {
 //             in a fixed order.
 constexpr int XS_CHUNK = 128;
 
-__global__ void __launch_bounds__(256) k_xsolve_fwd(int K, int ld, const zd* __restrict__ F,
-                                                    const double* __restrict__ D, const zd* __restrict__ b,
-                                                    zd* __restrict__ z) {
-  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+__device__ __attribute__((always_inline)) inline void xsolve_fwd_body(int blk, int K, int ld,
+                                                                      const zd* __restrict__ F,
+                                                                      const double* __restrict__ D,
+                                                                      const zd* __restrict__ b, zd* __restrict__ z) {
+  const int i = blk * 4 + (threadIdx.x >> 6);
   if (i >= K) return;
   const int lane = threadIdx.x & 63;
   const zd* col = F + (size_t)i * ld;
@@ -1224,10 +1306,16 @@ __global__ void __launch_bounds__(256) k_xsolve_fwd(int K, int ld, const zd* __r
     z[i] = mkz((b[i].x + ar) * inv, (b[i].y + ai) * inv);
   }
 }
+__global__ void __launch_bounds__(256) k_xsolve_fwd(int K, int ld, const zd* __restrict__ F,
+                                                    const double* __restrict__ D, const zd* __restrict__ b,
+                                                    zd* __restrict__ z) {
+  xsolve_fwd_body((int)blockIdx.x, K, ld, F, D, b, z);
+}
 
-__global__ void __launch_bounds__(256) k_xsolve_bwd(int K, int ld, const zd* __restrict__ F,
-                                                    const zd* __restrict__ z, zd* __restrict__ part) {
-  const int rb = blockIdx.x, ch = blockIdx.y;
+__device__ __attribute__((always_inline)) inline void xsolve_bwd_body(int rb, int ch, int K, int ld,
+                                                                      const zd* __restrict__ F,
+                                                                      const zd* __restrict__ z,
+                                                                      zd* __restrict__ part) {
   if (ch < rb / 2) return;   // the chunk's columns all lie left of these rows
   __shared__ zd red[4][64];
   const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
@@ -1258,6 +1346,10 @@ __global__ void __launch_bounds__(256) k_xsolve_bwd(int K, int ld, const zd* __r
   red[g][r] = acc0 + acc1;
   __syncthreads();
   if (g == 0 && j < K) part[(size_t)ch * ld + j] = red[0][r] + red[1][r] + red[2][r] + red[3][r];
+}
+__global__ void __launch_bounds__(256) k_xsolve_bwd(int K, int ld, const zd* __restrict__ F,
+                                                    const zd* __restrict__ z, zd* __restrict__ part) {
+  xsolve_bwd_body((int)blockIdx.x, (int)blockIdx.y, K, ld, F, z, part);
 }
 
 __global__ void __launch_bounds__(256) k_xsolve_reduce(int K, int ld, int nch, const zd* __restrict__ part,
@@ -1373,9 +1465,12 @@ __global__ void __launch_bounds__(1024) k_invit_finish(int K, zd* __restrict__ x
 // passes and block reductions as k_invit_finish (bitwise the same results).
 constexpr int IF_REG = 8;
 
-__global__ void __launch_bounds__(1024) k_invit_finish_red(int K, int ld, int nch, const zd* __restrict__ part,
-                                                           const zd* __restrict__ z, const zd* __restrict__ v,
-                                                           zd* __restrict__ x, double* __restrict__ out) {
+__device__ __attribute__((always_inline)) inline void invit_finish_red_body(int K, int ld, int nch,
+                                                                            const zd* __restrict__ part,
+                                                                            const zd* __restrict__ z,
+                                                                            const zd* __restrict__ v,
+                                                                            zd* __restrict__ x,
+                                                                            double* __restrict__ out) {
   __shared__ double sh[16];
   const int tid = threadIdx.x;
   zd xr[IF_REG], vr[IF_REG];
@@ -1436,13 +1531,18 @@ __global__ void __launch_bounds__(1024) k_invit_finish_red(int K, int ld, int nc
     out[1] = t1;
   }
 }
+__global__ void __launch_bounds__(1024) k_invit_finish_red(int K, int ld, int nch, const zd* __restrict__ part,
+                                                           const zd* __restrict__ z, const zd* __restrict__ v,
+                                                           zd* __restrict__ x, double* __restrict__ out) {
+  invit_finish_red_body(K, ld, nch, part, z, v, x, out);
+}
 
 // w = S^H v with MHMV's formula (src/linalg.f90:1037-1044):
 // w_i = sum_j conj(S(j, i)) v_j over the whole stored column i (contiguous),
 // one wave per column, no partial sums.
-__global__ void __launch_bounds__(256) k_mhmv(int K, int ld, const zd* __restrict__ A, const zd* __restrict__ v,
-                                              zd* __restrict__ w) {
-  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+__device__ __attribute__((always_inline)) inline void mhmv_body(int blk, int K, int ld, const zd* __restrict__ A,
+                                                                const zd* __restrict__ v, zd* __restrict__ w) {
+  const int i = blk * 4 + (threadIdx.x >> 6);
   if (i >= K) return;
   const int lane = threadIdx.x & 63;
   const zd* col = A + (size_t)i * ld;
@@ -1472,6 +1572,10 @@ __global__ void __launch_bounds__(256) k_mhmv(int K, int ld, const zd* __restric
     ai += __shfl_xor(ai, o);
   }
   if (lane == 0) w[i] = mkz(ar, ai);
+}
+__global__ void __launch_bounds__(256) k_mhmv(int K, int ld, const zd* __restrict__ A, const zd* __restrict__ v,
+                                              zd* __restrict__ w) {
+  mhmv_body((int)blockIdx.x, K, ld, A, v, w);
 }
 
 __global__ void k_scale(int K, zd* __restrict__ x, const double* __restrict__ t) {
@@ -1512,11 +1616,12 @@ int invit_iteration(const Ctx& c, Factor& f, const zd* v, zd* x, hipStream_t s) 
 // fixed order: out[2] = x^H S x, out[3] = x^H M x.
 constexpr int QF_CHUNK = 128;
 
-__global__ void __launch_bounds__(256) k_quad_part(int K, int ld, const zd* __restrict__ H,
-                                                   const zd* __restrict__ S, double sigma,
-                                                   const zd* __restrict__ x, double* __restrict__ part) {
+__device__ __attribute__((always_inline)) inline void quad_part_body(int rb, int ch, int nch, int K, int ld,
+                                                                     const zd* __restrict__ H,
+                                                                     const zd* __restrict__ S, double sigma,
+                                                                     const zd* __restrict__ x,
+                                                                     double* __restrict__ part) {
   __shared__ double red[2][4];
-  const int rb = blockIdx.x, ch = blockIdx.y, nch = gridDim.y;
   double* out = part + 2 * ((size_t)rb * nch + ch);
   const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
   const int i = rb * 64 + r;
@@ -1556,11 +1661,17 @@ __global__ void __launch_bounds__(256) k_quad_part(int K, int ld, const zd* __re
     out[1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
   }
 }
+__global__ void __launch_bounds__(256) k_quad_part(int K, int ld, const zd* __restrict__ H,
+                                                   const zd* __restrict__ S, double sigma,
+                                                   const zd* __restrict__ x, double* __restrict__ part) {
+  quad_part_body((int)blockIdx.x, (int)blockIdx.y, (int)gridDim.y, K, ld, H, S, sigma, x, part);
+}
 
-__global__ void __launch_bounds__(1024) k_quad_finish(int K, int ld, const zd* __restrict__ H,
-                                                      const zd* __restrict__ S, double sigma,
-                                                      const zd* __restrict__ x, int nparts,
-                                                      const double* __restrict__ part, double* __restrict__ out) {
+__device__ __attribute__((always_inline)) inline void quad_finish_body(int K, int ld, const zd* __restrict__ H,
+                                                                       const zd* __restrict__ S, double sigma,
+                                                                       const zd* __restrict__ x, int nparts,
+                                                                       const double* __restrict__ part,
+                                                                       double* __restrict__ out) {
   __shared__ double sh[16];
   double qs = 0.0, qm = 0.0;
   for (int i = threadIdx.x; i < K; i += blockDim.x) {
@@ -1580,6 +1691,12 @@ __global__ void __launch_bounds__(1024) k_quad_finish(int K, int ld, const zd* _
     out[3] = qm;
   }
 }
+__global__ void __launch_bounds__(1024) k_quad_finish(int K, int ld, const zd* __restrict__ H,
+                                                      const zd* __restrict__ S, double sigma,
+                                                      const zd* __restrict__ x, int nparts,
+                                                      const double* __restrict__ part, double* __restrict__ out) {
+  quad_finish_body(K, ld, H, S, sigma, x, nparts, part, out);
+}
 
 // scal[2] = x^H S x, scal[3] = x^H (H - sigma S) x; partials in f.W.
 int rayleigh(const Ctx& c, Factor& f, const zd* x, double sigma, hipStream_t s) {
@@ -1593,6 +1710,103 @@ int rayleigh(const Ctx& c, Factor& f, const zd* x, double sigma, hipStream_t s) 
 
 int normalise(zd* x, int K, const double* t, hipStream_t s) {
   hipLaunchKernelGGL(k_scale, dim3((K + 255) / 256), dim3(256), 0, s, K, x, t);
+  return (int)hipGetLastError();
+}
+
+// ---------------------------------------------- batched inverse iteration
+// The inverse iterations of a batch's problems in shared launches, each
+// kernel's per-problem work as in the single-problem launches above:
+// workgroup id of a one-dimensional grid serves problem id % P, block
+// id / P (blocks beyond a problem's own count return), and the kernels
+// of one workgroup per problem take blockIdx.x as the problem.  Iteration
+// it reads ev[it & 1] and writes ev[(it + 1) & 1].
+__global__ void __launch_bounds__(256) k_vinit_b(const InvDesc* __restrict__ d, int P) {
+  const InvDesc& q = d[blockIdx.x % P];
+  const int i = (int)(blockIdx.x / P) * 256 + threadIdx.x;
+  if (i < q.K) q.ev[0][i] = q.warm ? q.ev[2][i] : mkz(1.0, 0.0);
+}
+__global__ void __launch_bounds__(256) k_mhmv_b(const InvDesc* __restrict__ d, int P, int it) {
+  const InvDesc& q = d[blockIdx.x % P];
+  const int blk = (int)(blockIdx.x / P);
+  if (blk * 4 >= q.K) return;
+  mhmv_body(blk, q.K, q.ld, q.S, q.ev[it & 1], q.w);
+}
+__global__ void __launch_bounds__(256) k_xsolve_fwd_b(const InvDesc* __restrict__ d, int P) {
+  const InvDesc& q = d[blockIdx.x % P];
+  const int blk = (int)(blockIdx.x / P);
+  if (blk * 4 >= q.K) return;
+  xsolve_fwd_body(blk, q.K, q.ld, q.F, q.D, q.w, q.z);
+}
+__global__ void __launch_bounds__(256) k_xsolve_bwd_b(const InvDesc* __restrict__ d, int P) {
+  const InvDesc& q = d[blockIdx.x % P];
+  const int blk = (int)(blockIdx.x / P);
+  const int nrb = (q.K + 63) / 64, nch = (q.K + XS_CHUNK - 1) / XS_CHUNK;
+  if (blk >= nrb * nch) return;
+  xsolve_bwd_body(blk % nrb, blk / nrb, q.K, q.ld, q.F, q.z, q.part);
+}
+__global__ void __launch_bounds__(1024) k_invit_finish_red_b(const InvDesc* __restrict__ d, int it) {
+  const InvDesc& q = d[blockIdx.x];
+  const int nch = (q.K + XS_CHUNK - 1) / XS_CHUNK;
+  invit_finish_red_body(q.K, q.ld, nch, q.part, q.z, q.ev[it & 1], q.ev[(it + 1) & 1], q.scal);
+}
+__global__ void __launch_bounds__(256) k_quad_part_b(const InvDesc* __restrict__ d, int P, int it) {
+  const InvDesc& q = d[blockIdx.x % P];
+  const int blk = (int)(blockIdx.x / P);
+  const int nrb = (q.K + 63) / 64, nch = (q.K + QF_CHUNK - 1) / QF_CHUNK;
+  if (blk >= nrb * nch) return;
+  // the single-problem launch's grid is (nrb, nch) and its partials are
+  // stored by (rb, ch) row-major, which the finishing sum follows
+  quad_part_body(blk / nch, blk % nch, nch, q.K, q.ld, q.H, q.S, q.sigma, q.ev[it & 1],
+                 reinterpret_cast<double*>(q.part));
+}
+__global__ void __launch_bounds__(1024) k_quad_finish_b(const InvDesc* __restrict__ d, int it) {
+  const InvDesc& q = d[blockIdx.x];
+  const int nrb = (q.K + 63) / 64, nch = (q.K + QF_CHUNK - 1) / QF_CHUNK;
+  quad_finish_body(q.K, q.ld, q.H, q.S, q.sigma, q.ev[it & 1], nrb * nch, reinterpret_cast<const double*>(q.part),
+                   q.scal);
+}
+// normalise x^H S x = 1, report the scalars and the factorisation status
+// into the problem's pinned host array (the single-problem k_report's
+// layout: rep[0..4] = scal[0..4], rep[5..7] = status[0..2], rep[8] = E),
+// keep E in scal[8], and copy the vector to ev[2]
+__global__ void __launch_bounds__(256) k_invit_tail_b(const InvDesc* __restrict__ d, int P, int it) {
+  const InvDesc& q = d[blockIdx.x % P];
+  const int i = (int)(blockIdx.x / P) * 256 + threadIdx.x;
+  zd* x = q.ev[it & 1];
+  if (i < q.K) {
+    const zd v = x[i] * (1.0 / sqrt(q.scal[2]));
+    x[i] = v;
+    q.ev[2][i] = v;
+  }
+  if (blockIdx.x / P == 0 && threadIdx.x == 0) {
+    const double E = q.scal[3] / q.scal[2] + q.sigma;
+    for (int k = 0; k < 5; ++k) __hip_atomic_store(&q.rep[k], q.scal[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    for (int k = 0; k < 3; ++k)
+      __hip_atomic_store(&q.rep[5 + k], (double)q.status[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&q.rep[8], E, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __threadfence_system();
+  }
+}
+
+int invit_batch(const std::vector<InvDesc>& h, const InvDesc* d, int nq, hipStream_t s) {
+  const int P = (int)h.size();
+  if (P == 0) return 0;
+  int maxK = 0;
+  for (const InvDesc& q : h) maxK = std::max(maxK, q.K);
+  if (maxK > 1024 * IF_REG) return -1;   // the fused reduction keeps K <= 1024 IF_REG in registers
+  const int nv = (maxK + 255) / 256, nc = (maxK + 3) / 4;
+  const int nxb = ((maxK + 63) / 64) * ((maxK + XS_CHUNK - 1) / XS_CHUNK);
+  const int nqb = ((maxK + 63) / 64) * ((maxK + QF_CHUNK - 1) / QF_CHUNK);
+  hipLaunchKernelGGL(k_vinit_b, dim3(P * nv), dim3(256), 0, s, d, P);
+  for (int it = 0; it < nq; ++it) {
+    hipLaunchKernelGGL(k_mhmv_b, dim3(P * nc), dim3(256), 0, s, d, P, it);
+    hipLaunchKernelGGL(k_xsolve_fwd_b, dim3(P * nc), dim3(256), 0, s, d, P);
+    hipLaunchKernelGGL(k_xsolve_bwd_b, dim3(P * nxb), dim3(256), 0, s, d, P);
+    hipLaunchKernelGGL(k_invit_finish_red_b, dim3(P), dim3(1024), 0, s, d, it);
+  }
+  hipLaunchKernelGGL(k_quad_part_b, dim3(P * nqb), dim3(256), 0, s, d, P, nq);
+  hipLaunchKernelGGL(k_quad_finish_b, dim3(P), dim3(1024), 0, s, d, nq);
+  hipLaunchKernelGGL(k_invit_tail_b, dim3(P * nv), dim3(256), 0, s, d, P, nq);
   return (int)hipGetLastError();
 }
 

@@ -553,7 +553,8 @@ def batch_leg(nprob=32, reps=3):
     """32 independent problems (n=2 H^- and n=3 H2 / Li, round robin), K
     drawn uniformly from [200, 2000] (seeded), energy + full gradient each,
     as one packed batch.  A small autotuner times the launch structures
-    (packed or per-problem pair launches x 1/2/4 concurrent eigen streams)
+    (mode 2: packed pair launches and batched factorisations; 1: packed
+    pair launches; 0: per problem, x 1/2/4 concurrent per-problem streams)
     on this batch and keeps the fastest; the timed runs use it."""
     from ecg_nonbo.hip import EcgBatch
     rng = np.random.default_rng(SEED_BASE + 5)
@@ -570,7 +571,7 @@ def batch_leg(nprob=32, reps=3):
         pairs += K * (K + 1) // 2
     b = EcgBatch(ctxs)
     tune = []
-    for packed in (1, 0):
+    for packed in (2, 1, 0):
         for ns in (1, 2, 4):
             b.set_config(packed, ns)
             ts = []

@@ -191,8 +191,13 @@ int ecg_energy_grad_stored(ecg_ctx* ctx, int nfru, int which, double sigma, doub
  * problems run concurrently (largest first).  grad: the problems' gradients
  * concatenated in order; lat_ms (nullable): time from the call's start
  * until each problem's results were on the host.
- * ecg_batch_set_config selects packed (1) or per-problem (0) pair launches
- * and nstreams (an autotuner times the combinations; see bench.py). */
+ * ecg_batch_set_config selects the mode: 0 every problem's whole chain on
+ * its own (up to nstreams concurrently); 1 pair phases packed, eigen steps
+ * per problem; 2 pair phases packed and the factorisations of H - sigma S
+ * of all problems with a finite sigma[i] in shared launches (problem-major
+ * workgroups), then their inverse iterations per problem.  nstreams (1..16)
+ * bounds the problems whose per-problem steps run concurrently.  An
+ * autotuner times the combinations (bench.py). */
 typedef struct ecg_batch ecg_batch;
 int ecg_batch_create(int nprob, ecg_ctx** ctxs, ecg_batch** out);
 int ecg_batch_destroy(ecg_batch* b);

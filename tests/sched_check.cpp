@@ -10,10 +10,12 @@
 // non-zero.  With a second argument "plant", the look-ahead of every
 // update launch is made to wait on column tile (1, 0) (a higher index):
 // the check must then fail.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <map>
 #include <set>
+#include <string>
 #include <vector>
 
 #include "../atom-mol-nonbo_amd/csrc/ecg_schedule.hpp"
@@ -85,7 +87,91 @@ static int check_strips(int K, int col, const std::vector<UpdTask>& tasks, int i
   return 0;
 }
 
+// Batched launches (k_ldl_panel_b, k_ldl_update_b): workgroup id serves
+// problem id % P, task w = id / P, i.e. id = w * P + p.  For seeded sets of uneven
+// basis sizes, every wait must name a flag of the same problem published
+// in the same launch at a lower linear index.  With plant, the look-ahead
+// of problem 0 waits on problem 1's look-ahead (another problem's flag).
+static int check_batches(int nsets, bool plant) {
+  const int NB = SCHED_NB;
+  unsigned long long st = 20261204ull;
+  long launches = 0;
+  for (int set = 0; set < nsets; ++set) {
+    const int P = 1 + (int)(st % 40);
+    std::vector<int> Ks(P);
+    for (int p = 0; p < P; ++p) {
+      st = st * 6364136223846793005ull + 1442695040888963407ull;
+      Ks[p] = 1 + (int)((st >> 33) % 2600);
+    }
+    std::sort(Ks.begin(), Ks.end(), [](int a, int b) { return a > b; });   // the launches' order
+    const int maxK = Ks[0];
+    for (int col = -1; col * NB + NB < maxK; ++col) {
+      // (problem, task) of every workgroup, indexed by linear id
+      int gy = 0;
+      for (int K : Ks) {
+        if (col < 0 && K > NB) gy = std::max(gy, (K - NB + SCHED_SB - 1) / SCHED_SB);
+        if (col >= 0 && col * NB + NB < K) gy = std::max(gy, upd_shape(K, col).grid);
+      }
+      std::vector<std::vector<UpdTask>> tasks(P);
+      std::map<std::pair<int, int>, long> pub;   // (problem, flag) -> linear id
+      // the launch covers the active prefix of the list only
+      int PA = 0;
+      while (PA < P && Ks[PA] > (col < 0 ? NB : col * NB + NB)) ++PA;
+      for (int p = 0; p < P; ++p) {
+        const int K = Ks[p], qflag = (K + 63) / 64 * 64 / NB + 1;
+        if (col < 0) {
+          if (K > NB)
+            for (int w = 0; w < (K - NB + SCHED_SB - 1) / SCHED_SB; ++w) tasks[p].push_back(panel_task(w, qflag));
+        } else if (col * NB + NB < K) {
+          const UpdShape s = upd_shape(K, col);
+          for (int w = 0; w < s.grid; ++w) tasks[p].push_back(upd_task(w, K, col * NB, s, qflag));
+        }
+        if ((int)tasks[p].size() > gy) {
+          std::printf("set %d col %d: problem %d has more tasks than the grid\n", set, col, p);
+          return 1;
+        }
+        for (int w = 0; w < (int)tasks[p].size(); ++w)
+          if (tasks[p][w].publish >= 0) pub[{p, tasks[p][w].publish}] = (long)w * PA + p;
+      }
+      for (int p = 0; p < P; ++p)
+        for (int w = 0; w < (int)tasks[p].size(); ++w) {
+          UpdTask t = tasks[p][w];
+          int owner = p;
+          if (plant && p == 0 && w == 0 && P > 1 && !tasks[1].empty() && tasks[1][0].publish >= 0) {
+            owner = 1;
+            t.wait[0] = tasks[1][0].publish;
+          }
+          for (int i = 0; i < UPD_WAITS; ++i) {
+            const int f = t.wait[i];
+            if (f < 0) continue;
+            auto it = pub.find({owner, f});
+            if (it == pub.end()) {
+              std::printf("set %d col %d: problem %d task %d waits on an unpublished flag %d\n", set, col, p, w, f);
+              return 1;
+            }
+            if (owner != p) {
+              std::printf("set %d col %d: problem %d task %d waits on problem %d (not its own)\n", set, col, p,
+                          w, owner);
+              return 1;
+            }
+            if (p >= PA || it->second >= (long)w * PA + p) {
+              std::printf("set %d col %d: problem %d task %d waits on linear id %ld (not lower)\n", set, col, p,
+                          w, it->second);
+              return 1;
+            }
+          }
+        }
+      ++launches;
+    }
+    st = st * 6364136223846793005ull + 1442695040888963407ull;
+  }
+  std::printf("OK %ld\n", launches);
+  return 0;
+}
+
 int main(int argc, char** argv) {
+  if (argc > 1 && std::string(argv[1]) == "batch")
+    return check_batches(argc > 2 ? std::atoi(argv[2]) : 200, argc > 3);
   const int KMAX = argc > 1 ? std::atoi(argv[1]) : 4096;
   const bool plant = argc > 2;
   const int NB = SCHED_NB;

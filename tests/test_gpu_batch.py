@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 MIX = [("hminus", 70), ("h2", 130), ("li", 45), ("h2", 64), ("hminus", 200), ("li", 96), ("h2", 17)]
 
 
-@pytest.mark.parametrize("packed,nstreams", [(1, 1), (1, 3), (0, 2)])
+@pytest.mark.parametrize("packed,nstreams", [(1, 1), (1, 3), (0, 2), (2, 1), (2, 3)])
 def test_batch_matches_single_problems(packed, nstreams):
     print("[case] batch", packed, nstreams, MIX, flush=True)
     ctxs, sig, ref = [], [], []

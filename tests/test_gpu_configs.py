@@ -121,7 +121,7 @@ def test_c5_batch_32_problems():
         ref.append((E, g, info[1]))
         probs.append((sy, p))
     b = EcgBatch(ctxs)
-    for packed, ns in ((1, 4), (0, 4)):
+    for packed, ns in ((2, 4), (1, 4), (0, 4)):
         say("C5 batch", "packed", packed, "streams", ns)
         b.set_config(packed, ns)
         rc, E, grads, lat = b.energy_grad(sig, 1e-12, 30)

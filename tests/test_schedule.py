@@ -36,3 +36,19 @@ def test_planted_higher_wait_is_caught(checker):
     out = subprocess.run([checker, "200", "plant"], capture_output=True, text=True, timeout=60)
     print(out.stdout)
     assert out.returncode != 0 and "not lower" in out.stdout
+
+
+def test_batched_launches_wait_on_lower_linear_ids(checker):
+    """The batched factorisation's launches (problem-major 2-D grids, uneven
+    basis sizes): every wait names the same problem's flag at a lower
+    linear workgroup index."""
+    out = subprocess.run([checker, "batch", "300"], capture_output=True, text=True, timeout=300)
+    print(out.stdout)
+    assert out.returncode == 0 and out.stdout.startswith("OK ")
+    assert int(out.stdout.split()[1]) > 1000
+
+
+def test_batched_planted_cross_problem_wait_is_caught(checker):
+    out = subprocess.run([checker, "batch", "5", "plant"], capture_output=True, text=True, timeout=60)
+    print(out.stdout)
+    assert out.returncode != 0 and "not its own" in out.stdout
