@@ -26,9 +26,10 @@ namespace ecg {
 
 // ------------------------------------------------------------------ ZGEMM
 // C = alpha op(A) B + beta C, complex, column-major; op(A) = A (TRANS = 0)
-// or A^H (TRANS = 1).  Workgroup tile 64 x 32, k-chunks of 16 staged in
-// LDS with coalesced loads; wave w computes rows 32 (w & 1) + 16 a (a = 0,
-// 1) and columns 16 (w >> 1) as two 16 x 16 f64 MFMA tiles, four real
+// or A^H (TRANS = 1).  Workgroup tile 64 x 32, k-chunks of 32 staged in
+// LDS with coalesced loads (the next chunk's loads in flight during the
+// current chunk's MFMAs); wave w computes rows 32 (w & 1) + 16 a (a = 0,
+// 1) and columns 16 (w >> 1) as two 16 x 16 f64 MFMA tiles, three real
 // products per complex one.
 typedef double gdbl4 __attribute__((ext_vector_type(4)));
 
@@ -36,30 +37,35 @@ template <int TRANS>
 __global__ void __launch_bounds__(256) k_zgemm(int M, int N, int Kd, zd alpha, const zd* __restrict__ A, int lda,
                                                const zd* __restrict__ B, int ldb, zd beta, zd* __restrict__ Cm,
                                                int ldc) {
-  __shared__ zd As[16][64 + 1];   // As[p][r] = op(A)(r0 + r, p0 + p)
-  __shared__ zd Bs[32][16 + 1];   // Bs[c][p] = B(p0 + p, c0 + c)
+  constexpr int KC = 32;           // k-chunk
+  __shared__ zd As[KC][64 + 1];   // As[p][r] = op(A)(r0 + r, p0 + p)
+  __shared__ zd Bs[32][KC + 1];   // Bs[c][p] = B(p0 + p, c0 + c)
   const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 32;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int rt = (wave & 1) * 32, ct = (wave >> 1) * 16;
-  gdbl4 re[2], im[2];
+  // three real products per complex one: for x b, p1 += xr br, p2 += xi bi,
+  // p3 += (xr + xi)(br + bi); x b = (p1 - p2) + i (p3 - p1 - p2).  The
+  // step is bound by the MFMA rate here (a quarter fewer MFMAs than the
+  // four-product form); both forms' rounding errors are bounded by a few
+  // units of sum |x||b|.
+  gdbl4 p1[2], p2[2], p3[2];
 #pragma unroll
-  for (int a = 0; a < 2; ++a) {
-    re[a] = gdbl4{0.0, 0.0, 0.0, 0.0};
-    im[a] = gdbl4{0.0, 0.0, 0.0, 0.0};
-  }
-  for (int p0 = 0; p0 < Kd; p0 += 16) {
-    zd va[4], vb[2];
+  for (int a = 0; a < 2; ++a) p1[a] = p2[a] = p3[a] = gdbl4{0.0, 0.0, 0.0, 0.0};
+  // chunk loads into registers; the next chunk's loads are issued before
+  // the current chunk's MFMAs, so their latency overlaps the arithmetic
+  zd va[KC / 4], vb[KC / 8];
+  auto load_chunk = [&](int p0) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < KC / 4; ++u) {
       const int e = tid + 256 * u;
       int r, p;
       if (TRANS == 0) {
         r = e & 63;
         p = e >> 6;
       } else {
-        p = e & 15;
-        r = e >> 4;
+        p = e & (KC - 1);
+        r = e / KC;
       }
       const bool in = (r0 + r < M) && (p0 + p < Kd);
       // guarded loads as branches: here the branch-free form (ld_or0)
@@ -73,34 +79,38 @@ __global__ void __launch_bounds__(256) k_zgemm(int M, int N, int Kd, zd alpha, c
       va[u] = v;
     }
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const int e = tid + 256 * u, p = e & 15, c = e >> 4;
+    for (int u = 0; u < KC / 8; ++u) {
+      const int e = tid + 256 * u, p = e & (KC - 1), c = e / KC;
       vb[u] = (c0 + c < N && p0 + p < Kd) ? B[(size_t)(c0 + c) * ldb + p0 + p] : mkz(0.0, 0.0);
     }
+  };
+  load_chunk(0);
+  for (int p0 = 0; p0 < Kd; p0 += KC) {
     __syncthreads();   // the previous chunk's reads are done
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < KC / 4; ++u) {
       const int e = tid + 256 * u;
       if (TRANS == 0) As[e >> 6][e & 63] = va[u];
-      else As[e & 15][e >> 4] = va[u];
+      else As[e & (KC - 1)][e / KC] = va[u];
     }
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < KC / 8; ++u) {
       const int e = tid + 256 * u;
-      Bs[e >> 4][e & 15] = vb[u];
+      Bs[e / KC][e & (KC - 1)] = vb[u];
     }
     __syncthreads();
+    if (p0 + KC < Kd) load_chunk(p0 + KC);
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
+    for (int kk = 0; kk < KC / 4; ++kk) {
       const int p = 4 * kk + lk;
       const zd b = Bs[ct + lr][p];
+      const double bs = b.x + b.y;
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
         const zd x = As[p][rt + 16 * a + lr];
-        re[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, b.x, re[a], 0, 0, 0);
-        re[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.y, b.y, re[a], 0, 0, 0);
-        im[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, b.y, im[a], 0, 0, 0);
-        im[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, b.x, im[a], 0, 0, 0);
+        p1[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, b.x, p1[a], 0, 0, 0);
+        p2[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, b.y, p2[a], 0, 0, 0);
+        p3[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x + x.y, bs, p3[a], 0, 0, 0);
       }
     }
   }
@@ -111,7 +121,7 @@ __global__ void __launch_bounds__(256) k_zgemm(int M, int N, int Kd, zd alpha, c
     for (int v = 0; v < 4; ++v) {
       const int r = r0 + rt + 16 * a + 4 * v + lk, c = c0 + ct + lr;
       if (r < M && c < N) {
-        const zd acc = mkz(re[a][v], im[a][v]);
+        const zd acc = mkz(p1[a][v] - p2[a][v], p3[a][v] - (p1[a][v] + p2[a][v]));
         zd* o = Cm + (size_t)c * ldc + r;
         zd out = alpha * acc;
         if (!bz) out = zfma(beta, *o, out);
