@@ -1854,6 +1854,7 @@ int ecg_screen(ecg_ctx* x, int ncand, const double* cand, int iters, double* E, 
   if (!finite_all(cand, (size_t)x->npt * ncand)) return ECG_ERR_ARG;
   RCCHK(ensure_screen(x, ncand, 0));
   ecg::Screen& sc = x->scr;
+  RCCHK(ecg::screen_tune(x->c, sc, ncand, x->stream));
   HIPCHK(hipMemcpyAsync(sc.cparams, cand, sizeof(double) * x->npt * ncand, hipMemcpyHostToDevice, x->stream));
   HIPCHK(hipEventRecord(x->e[0], x->stream));
   RCCHK(ecg::screen_batch(x->c, x->f, sc, ncand, iters, false, x->npt, x->stream));
@@ -1862,6 +1863,7 @@ int ecg_screen(ecg_ctx* x, int ncand, const double* cand, int iters, double* E, 
   HIPCHK(hipMemcpyAsync(cs.data(), sc.cs, sizeof(double) * 8 * ncand, hipMemcpyDeviceToHost, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   x->tms[0] = ev_ms(x->e[0], x->e[1]);
+  x->tms[1] = sc.tm;
   bool fin = true;
   for (int j = 0; j < ncand; ++j) {
     E[j] = cs[8 * j + 0];
@@ -1887,6 +1889,7 @@ int ecg_screen_loop(ecg_ctx* x, int nsteps, int ncand, const double* cand, int i
   rows_changed(x, x->c.K - 1);
   ecg::Screen& sc = x->scr;
   const int npt = x->npt;
+  RCCHK(ecg::screen_tune(x->c, sc, ncand, x->stream));
   HIPCHK(hipMemcpyAsync(sc.call, cand, sizeof(double) * npt * ncand * nsteps, hipMemcpyHostToDevice, x->stream));
   double st1 = 0.0;
   HIPCHK(hipMemcpyAsync(sc.state + 1, &st1, sizeof(double), hipMemcpyHostToDevice, x->stream));
@@ -1925,6 +1928,7 @@ int ecg_screen_loop(ecg_ctx* x, int nsteps, int ncand, const double* cand, int i
   (void)hipGraphExecDestroy(ge);
   (void)hipGraphDestroy(g);
   x->tms[0] = ev_ms(x->e[0], x->e[1]);
+  x->tms[1] = sc.tm;
   x->have_last = false;
   x->vec_current = false;
   for (int i = 0; i < nsteps; ++i) {

@@ -278,9 +278,15 @@ struct Screen {
   zd* v0 = nullptr;      // ld start vector (current eigenvector)
   double* state = nullptr;   // loop state: [0] current E, [1] step
   double* rec = nullptr;     // 2 x scap: best E and index per step
+  int tm = 64;               // row tile of the per-step GEMMs (screen_tune)
+  int tuned_c = 0;           // candidate count tm was timed for
+  int tuned_kb = -1;         // and the basis size's 512-wide band
 };
 // 0 or a status; computes the factor of H - sigma S (whole basis), G and P.
 int screen_prepare(const Ctx& c, Factor& f, Screen& sc, double sigma, hipStream_t s);
+// Times the per-step GEMMs' row tiles for C candidates and keeps the faster
+// in sc.tm (once per C; synchronises, so never inside a graph capture).
+int screen_tune(const Ctx& c, Screen& sc, int C, hipStream_t s);
 // One batch of C candidates (parameters already in sc.cparams, or, when
 // from_loop, taken from sc.call at the step counter): rows, Schur
 // complements, `iters` inverse iterations from sc.v0, Rayleigh quotients.
@@ -292,7 +298,9 @@ int screen_batch(const Ctx& c, const Factor& f, Screen& sc, int C, int iters, bo
 // when it lowers the current energy (parameters into function K of c,
 // start vector := its eigenvector), record, advance the step counter.
 int screen_select(const Ctx& c, Screen& sc, int C, int npt, hipStream_t s);
+// C = alpha op(A) B + beta C (op = identity or A^H for trans = 1), M x N x
+// Kd, column-major; tm = rows per workgroup tile (64 or 32).
 int zgemm(int trans, int M, int N, int Kd, zd alpha, const zd* A, int lda, const zd* B, int ldb, zd beta, zd* Cm,
-          int ldc, hipStream_t s);
+          int ldc, hipStream_t s, int tm = 64);
 
 }  // namespace ecg

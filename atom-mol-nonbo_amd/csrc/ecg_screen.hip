@@ -26,43 +26,47 @@ namespace ecg {
 
 // ------------------------------------------------------------------ ZGEMM
 // C = alpha op(A) B + beta C, complex, column-major; op(A) = A (TRANS = 0)
-// or A^H (TRANS = 1).  Workgroup tile 64 x 32, k-chunks of 32 staged in
-// LDS with coalesced loads (the next chunk's loads in flight during the
-// current chunk's MFMAs); wave w computes rows 32 (w & 1) + 16 a (a = 0,
-// 1) and columns 16 (w >> 1) as two 16 x 16 f64 MFMA tiles, three real
-// products per complex one.
+// or A^H (TRANS = 1).  Workgroup tile TM x 32 (TM = 64 or 32), k-chunks
+// of 32 staged in LDS with coalesced loads (the next chunk's loads in
+// flight during the current chunk's MFMAs); wave w computes rows
+// (TM / 2)(w & 1) + 16 a (a < TM / 32) and columns 16 (w >> 1) as 16 x 16
+// f64 MFMA tiles, three real products per complex one.  TM = 32 doubles
+// the workgroups (more waves per SIMD to hide latency) at the price of
+// twice the reads of B; the screening session picks TM by timing both on
+// its own shapes (screen_tune).
 typedef double gdbl4 __attribute__((ext_vector_type(4)));
 
-template <int TRANS>
+template <int TRANS, int TM>
 __global__ void __launch_bounds__(256) k_zgemm(int M, int N, int Kd, zd alpha, const zd* __restrict__ A, int lda,
                                                const zd* __restrict__ B, int ldb, zd beta, zd* __restrict__ Cm,
                                                int ldc) {
   constexpr int KC = 32;           // k-chunk
-  __shared__ zd As[KC][64 + 1];   // As[p][r] = op(A)(r0 + r, p0 + p)
+  constexpr int NA = TM / 32;      // 16-row tiles per wave
+  __shared__ zd As[KC][TM + 1];   // As[p][r] = op(A)(r0 + r, p0 + p)
   __shared__ zd Bs[32][KC + 1];   // Bs[c][p] = B(p0 + p, c0 + c)
-  const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 32;
+  const int r0 = blockIdx.x * TM, c0 = blockIdx.y * 32;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
-  const int rt = (wave & 1) * 32, ct = (wave >> 1) * 16;
+  const int rt = (wave & 1) * (TM / 2), ct = (wave >> 1) * 16;
   // three real products per complex one: for x b, p1 += xr br, p2 += xi bi,
   // p3 += (xr + xi)(br + bi); x b = (p1 - p2) + i (p3 - p1 - p2).  The
   // step is bound by the MFMA rate here (a quarter fewer MFMAs than the
   // four-product form); both forms' rounding errors are bounded by a few
   // units of sum |x||b|.
-  gdbl4 p1[2], p2[2], p3[2];
+  gdbl4 p1[NA], p2[NA], p3[NA];
 #pragma unroll
-  for (int a = 0; a < 2; ++a) p1[a] = p2[a] = p3[a] = gdbl4{0.0, 0.0, 0.0, 0.0};
+  for (int a = 0; a < NA; ++a) p1[a] = p2[a] = p3[a] = gdbl4{0.0, 0.0, 0.0, 0.0};
   // chunk loads into registers; the next chunk's loads are issued before
   // the current chunk's MFMAs, so their latency overlaps the arithmetic
-  zd va[KC / 4], vb[KC / 8];
+  zd va[TM * KC / 256], vb[KC / 8];
   auto load_chunk = [&](int p0) {
 #pragma unroll
-    for (int u = 0; u < KC / 4; ++u) {
+    for (int u = 0; u < TM * KC / 256; ++u) {
       const int e = tid + 256 * u;
       int r, p;
       if (TRANS == 0) {
-        r = e & 63;
-        p = e >> 6;
+        r = e % TM;
+        p = e / TM;
       } else {
         p = e & (KC - 1);
         r = e / KC;
@@ -88,9 +92,9 @@ __global__ void __launch_bounds__(256) k_zgemm(int M, int N, int Kd, zd alpha, c
   for (int p0 = 0; p0 < Kd; p0 += KC) {
     __syncthreads();   // the previous chunk's reads are done
 #pragma unroll
-    for (int u = 0; u < KC / 4; ++u) {
+    for (int u = 0; u < TM * KC / 256; ++u) {
       const int e = tid + 256 * u;
-      if (TRANS == 0) As[e >> 6][e & 63] = va[u];
+      if (TRANS == 0) As[e / TM][e % TM] = va[u];
       else As[e & (KC - 1)][e / KC] = va[u];
     }
 #pragma unroll
@@ -106,7 +110,7 @@ __global__ void __launch_bounds__(256) k_zgemm(int M, int N, int Kd, zd alpha, c
       const zd b = Bs[ct + lr][p];
       const double bs = b.x + b.y;
 #pragma unroll
-      for (int a = 0; a < 2; ++a) {
+      for (int a = 0; a < NA; ++a) {
         const zd x = As[p][rt + 16 * a + lr];
         p1[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, b.x, p1[a], 0, 0, 0);
         p2[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, b.y, p2[a], 0, 0, 0);
@@ -116,7 +120,7 @@ __global__ void __launch_bounds__(256) k_zgemm(int M, int N, int Kd, zd alpha, c
   }
   const bool bz = (beta.x == 0.0 && beta.y == 0.0);
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < NA; ++a)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int r = r0 + rt + 16 * a + 4 * v + lk, c = c0 + ct + lr;
@@ -131,12 +135,57 @@ __global__ void __launch_bounds__(256) k_zgemm(int M, int N, int Kd, zd alpha, c
 }
 
 int zgemm(int trans, int M, int N, int Kd, zd alpha, const zd* A, int lda, const zd* B, int ldb, zd beta, zd* Cm,
-          int ldc, hipStream_t s) {
+          int ldc, hipStream_t s, int tm) {
   if (M <= 0 || N <= 0) return 0;
-  const dim3 grid((M + 63) / 64, (N + 31) / 32);
-  if (trans) hipLaunchKernelGGL(k_zgemm<1>, grid, dim3(256), 0, s, M, N, Kd, alpha, A, lda, B, ldb, beta, Cm, ldc);
-  else hipLaunchKernelGGL(k_zgemm<0>, grid, dim3(256), 0, s, M, N, Kd, alpha, A, lda, B, ldb, beta, Cm, ldc);
+  if (tm != 32) tm = 64;
+  const dim3 grid((M + tm - 1) / tm, (N + 31) / 32);
+  if (trans) {
+    if (tm == 32) hipLaunchKernelGGL((k_zgemm<1, 32>), grid, dim3(256), 0, s, M, N, Kd, alpha, A, lda, B, ldb, beta, Cm, ldc);
+    else hipLaunchKernelGGL((k_zgemm<1, 64>), grid, dim3(256), 0, s, M, N, Kd, alpha, A, lda, B, ldb, beta, Cm, ldc);
+  } else {
+    if (tm == 32) hipLaunchKernelGGL((k_zgemm<0, 32>), grid, dim3(256), 0, s, M, N, Kd, alpha, A, lda, B, ldb, beta, Cm, ldc);
+    else hipLaunchKernelGGL((k_zgemm<0, 64>), grid, dim3(256), 0, s, M, N, Kd, alpha, A, lda, B, ldb, beta, Cm, ldc);
+  }
   return (int)hipGetLastError();
+}
+
+// The screening GEMMs' row tile for C candidates: both variants timed on
+// the step's iteration GEMM (P V, Kp x C x Kp, into U, which the step
+// overwrites), the faster kept for this C and basis sizes within the same
+// 512-wide band.  Runs outside any graph capture (it synchronises).
+int screen_tune(const Ctx& c, Screen& sc, int C, hipStream_t s) {
+  if (sc.tuned_c == C && sc.tuned_kb == (sc.Kp >> 9)) return 0;
+  const int Kp = sc.Kp, ld = c.ld;
+  const zd one = mkz(1.0, 0.0), zero = mkz(0.0, 0.0);
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return -1;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return -1;
+  }
+  int rc = 0;
+  float best = 0.0f;
+  for (int tm : {64, 32}) {
+    float ms = 0.0f;
+    rc = zgemm(0, Kp, C, Kp, one, sc.P, ld, sc.V, ld, zero, sc.U, ld, s, tm);   // warm-up
+    if (!rc && hipEventRecord(e0, s) != hipSuccess) rc = -1;
+    for (int r = 0; r < 3 && !rc; ++r) rc = zgemm(0, Kp, C, Kp, one, sc.P, ld, sc.V, ld, zero, sc.U, ld, s, tm);
+    if (!rc && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      rc = -1;
+    if (rc) break;
+    if (tm == 64 || ms < best) {
+      best = ms;
+      sc.tm = tm;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (!rc) {
+    sc.tuned_c = C;
+    sc.tuned_kb = sc.Kp >> 9;
+  }
+  return rc;
 }
 
 // ------------------------------------------------------------ set-up
@@ -432,16 +481,16 @@ int screen_batch(const Ctx& c, const Factor& f, Screen& sc, int C, int iters, bo
   if (rc) return rc;
   hipLaunchKernelGGL(k_cand_border, dim3(C), dim3(256), 0, s, Kp, ld, C, sc.sigma, sc.Hb, sc.Bm, sc.cdiag, sc.cs,
                      sc.v0, sc.V, sc.vK);
-  rc = zgemm(0, Kp, 2 * C, Kp, one, sc.G, ld, sc.Bm, ld, zero, sc.Wm, ld, s);
+  rc = zgemm(0, Kp, 2 * C, Kp, one, sc.G, ld, sc.Bm, ld, zero, sc.Wm, ld, s, sc.tm);
   if (rc) return rc;
   hipLaunchKernelGGL(k_cand_schur, dim3(C), dim3(256), 0, s, Kp, ld, sc.Bm, sc.Wm, sc.cs);
   for (int it = 0; it < iters; ++it) {
-    rc = zgemm(0, Kp, C, Kp, one, sc.P, ld, sc.V, ld, zero, sc.U, ld, s);
+    rc = zgemm(0, Kp, C, Kp, one, sc.P, ld, sc.V, ld, zero, sc.U, ld, s, sc.tm);
     if (rc) return rc;
     hipLaunchKernelGGL(k_cand_iter, dim3(C), dim3(256), 0, s, Kp, ld, C, sc.U, sc.Wm, sc.Bm, sc.V, sc.vK, sc.cs);
   }
-  rc = zgemm(0, Kp, C, Kp, one, c.S, ld, sc.V, ld, zero, sc.Y1, ld, s);
-  if (!rc) rc = zgemm(0, Kp, C, Kp, one, c.H, ld, sc.V, ld, zero, sc.Y2, ld, s);
+  rc = zgemm(0, Kp, C, Kp, one, c.S, ld, sc.V, ld, zero, sc.Y1, ld, s, sc.tm);
+  if (!rc) rc = zgemm(0, Kp, C, Kp, one, c.H, ld, sc.V, ld, zero, sc.Y2, ld, s, sc.tm);
   if (rc) return rc;
   hipLaunchKernelGGL(k_cand_rayleigh, dim3(C), dim3(256), 0, s, Kp, ld, C, sc.V, sc.vK, sc.Y1, sc.Y2, sc.Hb, sc.Bm,
                      sc.cs);

@@ -294,7 +294,9 @@ def screening_leg(system, base, K, ncand=256, nsteps=100, iters=4, plain_sample=
     t0 = time.perf_counter()
     Eb, ib = ctx.screen_loop(cand, iters=iters)
     t_loop = time.perf_counter() - t0
-    dev_loop = ctx.timings()[0] * 1e-3
+    tl = ctx.timings()
+    dev_loop = tl[0] * 1e-3
+    gemm_tile = int(tl[1])
     # accuracy of `iters` iterations: one step's candidates at iters and at 4 x iters
     ctx.set_params(base)
     ctx.screen_prepare(sigma, 1e-12, 40)
@@ -334,6 +336,7 @@ def screening_leg(system, base, K, ncand=256, nsteps=100, iters=4, plain_sample=
         "candidates_per_s": rate,
         "ms_per_step": 1e3 * t_loop / nsteps,
         "device_ms_per_step": 1e3 * dev_loop / nsteps,
+        "gemm_row_tile": gemm_tile,
         "prepare_ms": 1e3 * t_prep,
         "per_candidate_candidates_per_s": 1.0 / t_tail,
         "per_candidate_ms": 1e3 * t_tail,

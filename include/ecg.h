@@ -355,7 +355,9 @@ int ecg_eigenvector(ecg_ctx* ctx, double* c);
  * the last ecg_energy / ecg_energy_grad call: [0] assembly, [1] eigen step
  * (all factorisations and iterations), [2] adjoint pass, [3] whole call,
  * [4] adjoint kernel alone, [5] off-diagonal assembly kernel alone.
- * After ecg_screen / ecg_screen_loop, [0] is the device time of the call.
+ * After ecg_screen / ecg_screen_loop, [0] is the device time of the call
+ * and [1] the row tile (64 or 32) its GEMMs ran with, picked by timing both
+ * on the first call for this candidate count and basis-size band.
  * Returns the number of values written. */
 int ecg_last_timings(ecg_ctx* ctx, double* ms, int nmax);
 
