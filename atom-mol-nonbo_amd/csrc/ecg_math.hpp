@@ -60,6 +60,58 @@ __host__ __device__ inline zd zsqrt(zd z) {
   return zd{fabs(z.y) * h, copysign(t, z.y)};
 }
 
+// Reciprocal and square root for the pair kernels, which are bound by
+// their VALU issue rate: an IEEE-correct division or square root costs
+// 10-12 instructions, these 5-9.  For normal arguments whose results are
+// normal (the pair algebra's range) they are within an ulp or so of the
+// correctly rounded values.
+// 1/x: hardware reciprocal, two Newton steps r += r (1 - x r).
+__device__ inline double frcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.0), r, r);
+  return fma(fma(-x, r, 1.0), r, r);
+}
+// sqrt(x) and 1/sqrt(x), x > 0: hardware reciprocal square root y, then
+// two coupled Goldschmidt steps on g = x y -> sqrt(x), h = y / 2 ->
+// 1 / (2 sqrt(x)), and a last residual correction of g.
+__device__ inline void fsqrt2(double x, double& sq, double& rsq) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  sq = fma(fma(-g, g, x), h, g);
+  rsq = 2.0 * h;
+}
+__device__ inline zd zrcp_f(zd a) {
+  const double d = frcp(fma(a.x, a.x, a.y * a.y));
+  return zd{a.x * d, -a.y * d};
+}
+// Principal square root as zsqrt (same branch cut and signed zeros); with
+// rinv, also 1 / sqrt(z) = conj(sqrt(z)) / |z|.
+__device__ inline zd zsqrt_f(zd z, zd* rinv = nullptr) {
+  if (z.x == 0.0 && z.y == 0.0) {
+    if (rinv) *rinv = zd{__builtin_inf(), 0.0};
+    return zd{0.0, z.y};
+  }
+  double r, ir;
+  fsqrt2(fma(z.x, z.x, z.y * z.y), r, ir);
+  double t, it;
+  fsqrt2(0.5 * (fabs(z.x) + r), t, it);
+  const double h = 0.5 * it;
+  const zd w = (z.x >= 0.0) ? zd{t, z.y * h} : zd{fabs(z.y) * h, copysign(t, z.y)};
+  if (rinv) *rinv = zd{w.x * ir, -w.y * ir};
+  return w;
+}
+__device__ inline zd zrsqrt_f(zd z) {
+  zd ri;
+  (void)zsqrt_f(z, &ri);
+  return ri;
+}
+
 // ------------------------------------------------------ packed symmetric
 // Lower triangle packed by rows: (0,0) (1,0) (1,1) (2,0) ...
 __host__ __device__ constexpr int sidx(int i, int j) {
@@ -91,7 +143,7 @@ template <int N> struct PairCore {
 
 // Factor C (packed symmetric), return C^-1 and the reciprocal of det C.
 template <int N>
-__host__ __device__ inline void sym_inverse(const zd* C, zd* Ci, zd& rdet) {
+__device__ inline void sym_inverse(const zd* C, zd* Ci, zd& rdet) {
   constexpr int m = N * (N + 1) / 2;
   zd l[m];   // unit lower factor (strict part at sidx(i,j), i>j)
   zd r[N];   // 1 / D_i
@@ -109,7 +161,7 @@ __host__ __device__ inline void sym_inverse(const zd* C, zd* Ci, zd& rdet) {
     zd d = C[sidx(i, i)];
 #pragma unroll
     for (int p = 0; p < i; ++p) d = zfms(u[p], l[sidx(i, p)], d);
-    r[i] = zrcp(d);
+    r[i] = zrcp_f(d);
   }
   rdet = r[0];
 #pragma unroll
@@ -144,7 +196,7 @@ __host__ __device__ inline void sym_inverse(const zd* C, zd* Ci, zd& rdet) {
 // Energy part of one (pair, term).  Ckc: Ck* packed; Ctl: tCl packed;
 // Nl: M tCl full (row-major N x N); adet = |det L_k det L_l|.
 template <int N>
-__host__ __device__ inline void pair_energy(const SysConst& sc, const zd* Ckc, const zd* Ctl,
+__device__ inline void pair_energy(const SysConst& sc, const zd* Ckc, const zd* Ctl,
                                             const zd* Nl, double adet, PairCore<N>& pc) {
   constexpr int m = N * (N + 1) / 2;
   zd C[m];
@@ -153,7 +205,7 @@ __host__ __device__ inline void pair_energy(const SysConst& sc, const zd* Ckc, c
   zd rdet;
   sym_inverse<N>(C, pc.Ci, rdet);
   const zd z = adet * rdet;
-  pc.S = (sc.two3n2 * z) * zsqrt(z);
+  pc.S = (sc.two3n2 * z) * zsqrt_f(z);
   // tr(M tCl C^-1 Ck*) = sum_ab (M tCl)_ab (C^-1 Ck*)_ba
   zd tr = mkz(0.0, 0.0);
 #pragma unroll
@@ -170,8 +222,7 @@ __host__ __device__ inline void pair_energy(const SysConst& sc, const zd* Ckc, c
   zd acc1 = mkz(0.0, 0.0), acc2 = mkz(0.0, 0.0);
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    zd s = zsqrt(pc.Ci[sidx(i, i)]);
-    zd rsi = zrcp(s);
+    const zd rsi = zrsqrt_f(pc.Ci[sidx(i, i)]);
     pc.rs[sidx(i, i)] = rsi;
     acc1 = acc1 + sc.q[i] * rsi;
   }
@@ -180,7 +231,7 @@ __host__ __device__ inline void pair_energy(const SysConst& sc, const zd* Ckc, c
 #pragma unroll
     for (int j = i + 1; j < N; ++j) {
       zd c = pc.Ci[sidx(i, i)] + pc.Ci[sidx(j, j)] - 2.0 * pc.Ci[sidx(j, i)];
-      zd rsij = zrcp(zsqrt(c));
+      const zd rsij = zrsqrt_f(c);
       pc.rs[sidx(j, i)] = rsij;
       acc2 = acc2 + (sc.q[i] * sc.q[j]) * rsij;
     }
