@@ -1067,9 +1067,15 @@ int ecg_update_params(ecg_ctx* x, int first, int count, const double* params) {
   rows_changed(x, first - 1);
   if (x->hparams.size() == (size_t)x->npt * x->c.K)
     std::memcpy(x->hparams.data() + (size_t)x->npt * (first - 1), params, sizeof(double) * x->npt * count);
+  if ((size_t)x->npt * count <= (size_t)ecg::PB_DOUBLES) {
+    // a few functions: the values travel with the launch that stores them
+    // and computes their records
+    RCCHK(ecg::prep_put(x->c, first - 1, count, params, x->stream));
+    return ECG_OK;
+  }
   HIPCHK(hipMemcpyAsync(x->c.params + (size_t)x->npt * (first - 1), params, sizeof(double) * x->npt * count,
                         hipMemcpyHostToDevice, x->stream));
-  RCCHK(ecg::prep(x->c, x->stream));
+  RCCHK(ecg::prep_range(x->c, first - 1, first - 1 + count, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   return ECG_OK;
 }

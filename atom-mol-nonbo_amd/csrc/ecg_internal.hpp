@@ -108,6 +108,17 @@ struct Ctx {
 
 // pair assembly (ecg_kernels.hip)
 int prep(const Ctx& c, hipStream_t s);
+int prep_range(const Ctx& c, int f0, int f1, hipStream_t s);   // records of functions f0..f1-1
+// Parameters small enough to travel as a kernel argument (count functions,
+// n (n + 1) doubles each, at most PB_DOUBLES in all).
+constexpr int PB_DOUBLES = 240;
+struct ParamBlock {
+  double v[PB_DOUBLES];
+};
+// Stores the host parameters hparams of functions f0..f0+count-1 and
+// computes their records, one launch and no host wait; -1 when they do not
+// fit a ParamBlock.
+int prep_put(const Ctx& c, int f0, int count, const double* hparams, hipStream_t s);
 int assemble(const Ctx& c, int r0, int r1, hipStream_t s);   // rows r0..r1-1 (thin-row kernels for few rows)
 size_t thin_row_entries(int cap);
 size_t thin_adjoint_entries(int n, int cap);

@@ -18,15 +18,13 @@
 namespace ecg {
 
 // ----------------------------------------------------------- set-up
-// One thread per function: bra record (Ck* packed, |det L|, L packed) and,
-// for every Y^dagger Y term, the ket record (P^T (A + iB) P packed, M tCl).
+// Records of function k from its parameters p (vech L, vech B): bra record
+// (Ck* packed, |det L|, L packed) and, for every Y^dagger Y term, the ket
+// record (P^T (A + iB) P packed, M tCl).
 template <int N>
-__global__ void k_prep(const SysConst sc, int K, const double* __restrict__ params, double* __restrict__ bra,
-                       double* __restrict__ ket) {
+__device__ __attribute__((always_inline)) inline void prep_one(const SysConst& sc, int k, const double* p,
+                                                               double* __restrict__ bra, double* __restrict__ ket) {
   constexpr int m = N * (N + 1) / 2;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= K) return;
-  const double* p = params + (size_t)k * 2 * m;
   double L[N][N], B[N][N], A[N][N];
 #pragma unroll
   for (int i = 0; i < N; ++i)
@@ -113,6 +111,34 @@ __global__ void k_prep(const SysConst sc, int K, const double* __restrict__ para
         kr[KetRec<N>::nl + 2 * (a * N + b) + 1] = im;
       }
   }
+}
+
+// One thread per function k in [f0, f1).
+template <int N>
+__global__ void k_prep(const SysConst sc, int f0, int f1, const double* __restrict__ params,
+                       double* __restrict__ bra, double* __restrict__ ket) {
+  constexpr int m = N * (N + 1) / 2;
+  const int k = f0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= f1) return;
+  prep_one<N>(sc, k, params + (size_t)k * 2 * m, bra, ket);
+}
+
+// The parameters of functions f0..f0+count-1 passed by value: stored into
+// params and their records computed in one launch, with no copy from host
+// memory to wait for.
+template <int N>
+__global__ void k_prep_put(const SysConst sc, int f0, int count, const ParamBlock pb, double* __restrict__ params,
+                           double* __restrict__ bra, double* __restrict__ ket) {
+  constexpr int m = N * (N + 1) / 2;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  double v[2 * m];
+#pragma unroll
+  for (int e = 0; e < 2 * m; ++e) v[e] = pb.v[i * 2 * m + e];
+  double* dst = params + (size_t)(f0 + i) * 2 * m;
+#pragma unroll
+  for (int e = 0; e < 2 * m; ++e) dst[e] = v[e];
+  prep_one<N>(sc, f0 + i, v, bra, ket);
 }
 
 // Sum over Y^dagger Y terms of one pair (energy only).
@@ -1876,7 +1902,7 @@ __global__ void __launch_bounds__(128) k_cand_rows(const SysConst sc, int Kp, in
 template <int N>
 static int launch_cand(const Ctx& c, int Kp, int C, const double* cparams, double* cbra, double* cket,
                        double* cdiag, zd* Hb, zd* Sb, int ldb, hipStream_t s) {
-  hipLaunchKernelGGL(k_prep<N>, dim3((C + 127) / 128), dim3(128), 0, s, c.sc, C, cparams, cbra, cket);
+  hipLaunchKernelGGL(k_prep<N>, dim3((C + 127) / 128), dim3(128), 0, s, c.sc, 0, C, cparams, cbra, cket);
   hipLaunchKernelGGL(k_cand_diag<N>, dim3((C + 127) / 128), dim3(128), 0, s, c.sc, C, cbra, cket, cdiag);
   hipLaunchKernelGGL(k_cand_rows<N>, dim3((Kp + 127) / 128, C), dim3(128), 0, s, c.sc, Kp, ldb, c.bra, c.ket,
                      c.diagS, cbra, cdiag, Hb, Sb);
@@ -1884,9 +1910,8 @@ static int launch_cand(const Ctx& c, int Kp, int C, const double* cparams, doubl
 }
 
 template <int N>
-static int launch_prep(const Ctx& c, hipStream_t s) {
-  const int K = c.K;
-  hipLaunchKernelGGL(k_prep<N>, dim3((K + 127) / 128), dim3(128), 0, s, c.sc, K, c.params, c.bra, c.ket);
+static int launch_prep(const Ctx& c, int f0, int f1, hipStream_t s) {
+  hipLaunchKernelGGL(k_prep<N>, dim3((f1 - f0 + 127) / 128), dim3(128), 0, s, c.sc, f0, f1, c.params, c.bra, c.ket);
   return (int)hipGetLastError();
 }
 
@@ -2217,13 +2242,36 @@ static int launch_penalty(const Ctx& c, int nfru, double pc, double t2, double* 
   return (int)hipGetLastError();
 }
 
-int prep(const Ctx& c, hipStream_t s) {
+int prep(const Ctx& c, hipStream_t s) { return prep_range(c, 0, c.K, s); }
+template <int N>
+static int launch_prep_put(const Ctx& c, int f0, int count, const ParamBlock& pb, hipStream_t s) {
+  hipLaunchKernelGGL(k_prep_put<N>, dim3((count + 63) / 64), dim3(64), 0, s, c.sc, f0, count, pb, c.params, c.bra,
+                     c.ket);
+  return (int)hipGetLastError();
+}
+int prep_put(const Ctx& c, int f0, int count, const double* hparams, hipStream_t s) {
+  const int np = c.n * (c.n + 1);
+  if (f0 < 0 || count < 1 || f0 + count > c.K || (size_t)np * count > PB_DOUBLES) return -1;
+  ParamBlock pb;
+  for (int e = 0; e < np * count; ++e) pb.v[e] = hparams[e];
   switch (c.n) {
-    case 1: return launch_prep<1>(c, s);
-    case 2: return launch_prep<2>(c, s);
-    case 3: return launch_prep<3>(c, s);
-    case 4: return launch_prep<4>(c, s);
-    case 5: return launch_prep<5>(c, s);
+    case 1: return launch_prep_put<1>(c, f0, count, pb, s);
+    case 2: return launch_prep_put<2>(c, f0, count, pb, s);
+    case 3: return launch_prep_put<3>(c, f0, count, pb, s);
+    case 4: return launch_prep_put<4>(c, f0, count, pb, s);
+    case 5: return launch_prep_put<5>(c, f0, count, pb, s);
+  }
+  return -1;
+}
+int prep_range(const Ctx& c, int f0, int f1, hipStream_t s) {
+  if (f0 < 0 || f1 > c.K) return -1;
+  if (f0 >= f1) return 0;
+  switch (c.n) {
+    case 1: return launch_prep<1>(c, f0, f1, s);
+    case 2: return launch_prep<2>(c, f0, f1, s);
+    case 3: return launch_prep<3>(c, f0, f1, s);
+    case 4: return launch_prep<4>(c, f0, f1, s);
+    case 5: return launch_prep<5>(c, f0, f1, s);
   }
   return -1;
 }

@@ -71,6 +71,12 @@ int ecg_set_params(ecg_ctx* ctx, int K, const double* params);
  * parameters surface as ECG_ERR_NONFINITE from that call.  No counterpart in
  * the reference, whose parameters live in host memory. */
 int ecg_set_params_device(ecg_ctx* ctx, int K, const double* dparams);
+/* Replace functions first..first+count-1 (1-based) of the current basis
+ * (the one-function steps of OptCycleG/I, src/workproc.f90:5202-5240):
+ * params (npt x count) is read before the call returns; for up to 240
+ * doubles (20 functions at n = 3) the values travel as a kernel argument
+ * and the call queues one launch on the context's stream without waiting,
+ * larger updates are copied and waited for. */
 int ecg_update_params(ecg_ctx* ctx, int first, int count, const double* params);
 
 /* One pair and one symmetry matrix P on the device: MatrixElements
