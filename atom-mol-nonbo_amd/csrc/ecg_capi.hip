@@ -1028,12 +1028,33 @@ int ecg_set_params(ecg_ctx* x, int K, const double* params) {
   x->s_ready = false;
   const int d = first_changed(x, K, params);
   if (d < K || K != x->c.K || !x->params_set) params_changed(x);
+  const size_t npt = (size_t)x->npt;
+  const bool resident = x->params_set && K == x->c.K && x->hparams.size() == npt * K;
   x->params_set = false;
-  RCCHK(upload_params(x, K, params));
+  if (resident) {
+    // the same basis size with its parameters resident: only functions
+    // d..e-1 (the first to the last changed one) go up, and only their
+    // records are recomputed
+    int e = K;
+    while (e > d && std::memcmp(x->hparams.data() + (e - 1) * npt, params + (e - 1) * npt, npt * sizeof(double)) == 0)
+      --e;
+    if (e > d) {
+      if (npt * (e - d) <= (size_t)ecg::PB_DOUBLES) {
+        RCCHK(ecg::prep_put(x->c, d, e - d, params + d * npt, x->stream));
+      } else {
+        HIPCHK(hipMemcpyAsync(x->c.params + d * npt, params + d * npt, sizeof(double) * npt * (e - d),
+                              hipMemcpyHostToDevice, x->stream));
+        RCCHK(ecg::prep_range(x->c, d, e, x->stream));
+        HIPCHK(hipStreamSynchronize(x->stream));
+      }
+    }
+  } else {
+    RCCHK(upload_params(x, K, params));
+    HIPCHK(hipStreamSynchronize(x->stream));
+  }
   x->params_set = true;
   rows_changed(x, d);
-  x->hparams.assign(params, params + (size_t)x->npt * K);
-  HIPCHK(hipStreamSynchronize(x->stream));
+  x->hparams.assign(params, params + npt * K);
   return ECG_OK;
 }
 

@@ -59,8 +59,11 @@ int ecg_ctx_create(int n, int T, const int* yhy_mats, const int* yhy_coef, const
 int ecg_ctx_destroy(ecg_ctx* ctx);
 
 /* Upload the nonlinear parameters of K functions (Glob_NonlinParam(npt, K),
- * src/globvars.f90:141-146).  Functions first..first+count-1 (1-based) are
- * replaced when first > 1 (the rest stay resident). */
+ * src/globvars.f90:141-146).  With the basis size unchanged, only the
+ * functions from the first to the last one that differ from the previous
+ * upload are sent (for up to 240 doubles as a kernel argument, without
+ * waiting) and their records recomputed; the next energy call reassembles
+ * the rows from the first changed function. */
 int ecg_set_params(ecg_ctx* ctx, int K, const double* params);
 /* The same from device memory (dparams: npt x K doubles in device memory of
  * the context's GPU, for callers that keep the parameters resident).  The
