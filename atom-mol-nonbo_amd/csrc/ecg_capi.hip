@@ -1024,20 +1024,23 @@ int ecg_ctx_destroy(ecg_ctx* x) {
 int ecg_set_params(ecg_ctx* x, int K, const double* params) {
   (void)hipGetLastError();   // start from a clean runtime error state
   if (!x || K < 1 || !params) return ECG_ERR_ARG;
-  if (!finite_all(params, (size_t)x->npt * K)) return ECG_ERR_ARG;
-  x->s_ready = false;
-  const int d = first_changed(x, K, params);
-  if (d < K || K != x->c.K || !x->params_set) params_changed(x);
   const size_t npt = (size_t)x->npt;
   const bool resident = x->params_set && K == x->c.K && x->hparams.size() == npt * K;
+  // functions d..e-1: the first to the last one that differs from the
+  // resident upload (all of them when nothing is resident); the others
+  // equal values already checked
+  const int d = first_changed(x, K, params);
+  int e = K;
+  if (resident)
+    while (e > d && std::memcmp(x->hparams.data() + (e - 1) * npt, params + (e - 1) * npt, npt * sizeof(double)) == 0)
+      --e;
+  if (!finite_all(params + d * npt, npt * (e - d))) return ECG_ERR_ARG;
+  x->s_ready = false;
+  if (d < K || K != x->c.K || !x->params_set) params_changed(x);
   x->params_set = false;
   if (resident) {
     // the same basis size with its parameters resident: only functions
-    // d..e-1 (the first to the last changed one) go up, and only their
-    // records are recomputed
-    int e = K;
-    while (e > d && std::memcmp(x->hparams.data() + (e - 1) * npt, params + (e - 1) * npt, npt * sizeof(double)) == 0)
-      --e;
+    // d..e-1 go up, and only their records are recomputed
     if (e > d) {
       if (npt * (e - d) <= (size_t)ecg::PB_DOUBLES) {
         RCCHK(ecg::prep_put(x->c, d, e - d, params + d * npt, x->stream));
@@ -1047,14 +1050,15 @@ int ecg_set_params(ecg_ctx* x, int K, const double* params) {
         RCCHK(ecg::prep_range(x->c, d, e, x->stream));
         HIPCHK(hipStreamSynchronize(x->stream));
       }
+      std::memcpy(x->hparams.data() + d * npt, params + d * npt, sizeof(double) * npt * (e - d));
     }
   } else {
     RCCHK(upload_params(x, K, params));
     HIPCHK(hipStreamSynchronize(x->stream));
+    x->hparams.assign(params, params + npt * K);
   }
   x->params_set = true;
   rows_changed(x, d);
-  x->hparams.assign(params, params + npt * K);
   return ECG_OK;
 }
 
