@@ -552,13 +552,14 @@ def memory_modes_leg(system, Ks=(1000, 4000), reps=3):
 BATCH_SYSTEMS = ("hminus", "h2", "li")
 
 
-def batch_leg(nprob=32, reps=3):
+def batch_leg(nprob=32, reps=3, config=None):
     """32 independent problems (n=2 H^- and n=3 H2 / Li, round robin), K
     drawn uniformly from [200, 2000] (seeded), energy + full gradient each,
     as one packed batch.  A small autotuner times the launch structures
     (mode 2: packed pair launches and batched factorisations; 1: packed
     pair launches; 0: per problem, x 1/2/4 concurrent per-problem streams)
-    on this batch and keeps the fastest; the timed runs use it."""
+    on this batch and keeps the fastest; the timed runs use it.  config =
+    (mode, streams) fixes the structure instead (no autotuning)."""
     from ecg_nonbo.hip import EcgBatch
     rng = np.random.default_rng(SEED_BASE + 5)
     Ks = rng.integers(200, 2001, nprob)
@@ -574,7 +575,7 @@ def batch_leg(nprob=32, reps=3):
         pairs += K * (K + 1) // 2
     b = EcgBatch(ctxs)
     tune = []
-    for packed in (2, 1, 0):
+    for packed in (() if config else (2, 1, 0)):
         for ns in (1, 2, 4):
             b.set_config(packed, ns)
             ts = []
@@ -583,7 +584,7 @@ def batch_leg(nprob=32, reps=3):
                 b.energy_grad(sig, 1e-12, 30)
                 ts.append(time.perf_counter() - t0)
             tune.append({"packed": packed, "streams": ns, "ms": 1e3 * min(ts)})
-    best = min(tune, key=lambda d: d["ms"])
+    best = min(tune, key=lambda d: d["ms"]) if tune else {"packed": config[0], "streams": config[1]}
     b.set_config(best["packed"], best["streams"])
     ts, lats = [], []
     for r in range(reps):
@@ -640,6 +641,8 @@ def main():
     ap.add_argument("--no-expect", action="store_true", help="skip the expectation-value leg")
     ap.add_argument("--no-cycle", action="store_true", help="skip the optimisation-cycle leg")
     ap.add_argument("--no-enlarge", action="store_true", help="skip the basis-enlargement leg")
+    ap.add_argument("--batch-config", default=None, metavar="MODE,STREAMS",
+                    help="run the batch leg with this launch structure instead of autotuning it")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first issue device copies of a known size (reference for HBM counters)")
     args = ap.parse_args()
@@ -785,7 +788,8 @@ def main():
     if not args.no_modes:
         out["memory_modes"] = memory_modes_leg(system)
     if not args.no_batch:
-        out["batch"] = batch_leg()
+        out["batch"] = batch_leg(config=tuple(int(v) for v in args.batch_config.split(",")) if args.batch_config
+                                 else None)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(system, base, K)
     ctx.close()
