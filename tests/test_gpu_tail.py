@@ -151,6 +151,34 @@ def test_tail_update_params_entry(golden_dir):
     ctx.close()
 
 
+@pytest.mark.parametrize("count", [10, 50])
+def test_update_params_few_and_many(golden_dir, count):
+    """ecg_update_params with few functions (their values travel as a kernel
+    argument: 10 x 6 doubles at n = 2) and with more than one kernel
+    argument holds (50 x 6 doubles: a copy), then ecg_set_params with an
+    interior range changed: each energy equals a fresh context's."""
+    say("ecg_update_params", "inout_1_500", count)
+    f = _fixture(golden_dir, "inout_1_500")
+    sigma = f.current_energy * f.invit_parameter
+    K = f.basis_size
+    ctx = EcgContext(f.system)
+    ctx.set_params(f.params)
+    ctx.energy(1, sigma, TOL, 20)
+    first = K - count - 2   # 1-based
+    p = _perturb(f.params, slice(first - 1, first - 1 + count), 500 + count)
+    ctx.update_params(first, p[first - 1:first - 1 + count])
+    _, E1, _, info = ctx.energy(1, sigma, TOL, 20)
+    assert info[7] == K - first + 1   # rows refactored from the first changed one
+    E2, _ = _fresh_energy(f.system, p, 1, sigma)
+    _close(E1, E2)
+    q = _perturb(p, slice(100, 100 + count), 600 + count)
+    ctx.set_params(q)
+    _, E3, _, _ = ctx.energy(1, sigma, TOL, 20)
+    E4, _ = _fresh_energy(f.system, q, 1, sigma)
+    _close(E3, E4)
+    ctx.close()
+
+
 def test_tail_basis_grows_and_shrinks(golden_dir):
     say("ecg_energy tail", "inout_1_1000", "grow/shrink")
     f = _fixture(golden_dir, "inout_1_1000")
