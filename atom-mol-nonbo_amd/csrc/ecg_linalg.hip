@@ -1267,7 +1267,8 @@ int ldlh_tail(const Ctx& c, Factor& f, double sigma, int m, hipStream_t s) {
 //   backward: x_j = z_j + sum_{i>j} F(j, i) z_i, workgroups of 64 rows x
 //             XS_CHUNK columns write partial sums that k_xsolve_reduce adds
 //             in a fixed order.
-constexpr int XS_CHUNK = 128;
+constexpr int XS_CHUNK = 256;
+static_assert(XS_CHUNK % 64 == 0, "chunks hold whole 64-row blocks");
 
 __device__ __attribute__((always_inline)) inline void xsolve_fwd_body(int blk, int K, int ld,
                                                                       const zd* __restrict__ F,
@@ -1316,7 +1317,7 @@ __device__ __attribute__((always_inline)) inline void xsolve_bwd_body(int rb, in
                                                                       const zd* __restrict__ F,
                                                                       const zd* __restrict__ z,
                                                                       zd* __restrict__ part) {
-  if (ch < rb / 2) return;   // the chunk's columns all lie left of these rows
+  if (ch < rb / (XS_CHUNK / 64)) return;   // the chunk's columns all lie left of these rows
   __shared__ zd red[4][64];
   const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
   const int j = rb * 64 + r;
@@ -1357,7 +1358,7 @@ __global__ void __launch_bounds__(256) k_xsolve_reduce(int K, int ld, int nch, c
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= K) return;
   zd s = z[j];
-  for (int ch = (j / 64) / 2; ch < nch; ++ch) s = s + part[(size_t)ch * ld + j];
+  for (int ch = j / XS_CHUNK; ch < nch; ++ch) s = s + part[(size_t)ch * ld + j];
   x[j] = s;
 }
 
@@ -1482,7 +1483,7 @@ __device__ __attribute__((always_inline)) inline void invit_finish_red_body(int 
     if (j < K) {
       zd a = z[j];
       // the partials in k_xsolve_reduce's order, eight loads in flight
-      for (int ch0 = (j / 64) / 2; ch0 < nch; ch0 += 8) {
+      for (int ch0 = j / XS_CHUNK; ch0 < nch; ch0 += 8) {
         zd t[8];
 #pragma unroll
         for (int w = 0; w < 8; ++w) t[w] = ld_or0(part, (size_t)(ch0 + w) * ld + j, ch0 + w < nch);
