@@ -54,6 +54,7 @@ struct ecg_ctx {
   double* hrep = nullptr;     // pinned: the inverse iteration's report (k_report)
   size_t pinned_cap = 0;
   ecg::Screen scr;            // candidate screening session
+  int scr_tm = 0, scr_cpb = 0;   // fixed screening shapes (0: timed by screen_tune)
   zd* dD = nullptr;           // stored derivatives (2 npt x nfo x K) of ecg_energy_grad_stored
   size_t dD_cap = 0;
   ecg::Tracer tr;             // per-kernel tracing (off by default)
@@ -1831,6 +1832,30 @@ int ecg_get_params(ecg_ctx* x, double* params) {
   return ECG_OK;
 }
 
+namespace {
+// The screening step's shapes for ncand candidates: the caller's fixed
+// ones, the others timed.
+int screen_shapes(ecg_ctx* x, int ncand) {
+  ecg::Screen& sc = x->scr;
+  if (!(x->scr_tm && x->scr_cpb)) RCCHK(ecg::screen_tune(x->c, sc, ncand, x->stream));
+  if (x->scr_tm) sc.tm = x->scr_tm;
+  if (x->scr_cpb) sc.cpb = x->scr_cpb;
+  return 0;
+}
+}  // namespace
+
+int ecg_screen_set_shapes(ecg_ctx* x, int row_tile, int cand_per_block) {
+  if (!x || (row_tile != 0 && row_tile != 32 && row_tile != 64)) return ECG_ERR_ARG;
+  if (cand_per_block > 1 &&
+      ((cand_per_block != 2 && cand_per_block != 4 && cand_per_block != 8) ||
+       ecg::cand_rows_lds(x->c.n, x->c.sc.T, cand_per_block) == 0))
+    return ECG_ERR_ARG;
+  if (cand_per_block < 0) return ECG_ERR_ARG;
+  x->scr_tm = row_tile;
+  x->scr_cpb = cand_per_block;
+  return ECG_OK;
+}
+
 int ecg_screen_prepare(ecg_ctx* x, double sigma, double tol, int maxiter, double* info) {
   (void)hipGetLastError();   // start from a clean runtime error state
   if (!x || !std::isfinite(sigma) || maxiter < 1) return ECG_ERR_ARG;
@@ -1885,8 +1910,8 @@ int ecg_screen(ecg_ctx* x, int ncand, const double* cand, int iters, double* E, 
   if (!finite_all(cand, (size_t)x->npt * ncand)) return ECG_ERR_ARG;
   RCCHK(ensure_screen(x, ncand, 0));
   ecg::Screen& sc = x->scr;
-  RCCHK(ecg::screen_tune(x->c, sc, ncand, x->stream));
   HIPCHK(hipMemcpyAsync(sc.cparams, cand, sizeof(double) * x->npt * ncand, hipMemcpyHostToDevice, x->stream));
+  RCCHK(screen_shapes(x, ncand));
   HIPCHK(hipEventRecord(x->e[0], x->stream));
   RCCHK(ecg::screen_batch(x->c, x->f, sc, ncand, iters, false, x->npt, x->stream));
   HIPCHK(hipEventRecord(x->e[1], x->stream));
@@ -1895,6 +1920,7 @@ int ecg_screen(ecg_ctx* x, int ncand, const double* cand, int iters, double* E, 
   HIPCHK(hipStreamSynchronize(x->stream));
   x->tms[0] = ev_ms(x->e[0], x->e[1]);
   x->tms[1] = sc.tm;
+  x->tms[2] = sc.cpb;
   bool fin = true;
   for (int j = 0; j < ncand; ++j) {
     E[j] = cs[8 * j + 0];
@@ -1920,8 +1946,10 @@ int ecg_screen_loop(ecg_ctx* x, int nsteps, int ncand, const double* cand, int i
   rows_changed(x, x->c.K - 1);
   ecg::Screen& sc = x->scr;
   const int npt = x->npt;
-  RCCHK(ecg::screen_tune(x->c, sc, ncand, x->stream));
   HIPCHK(hipMemcpyAsync(sc.call, cand, sizeof(double) * npt * ncand * nsteps, hipMemcpyHostToDevice, x->stream));
+  // the shapes are timed on the first step's candidates
+  HIPCHK(hipMemcpyAsync(sc.cparams, sc.call, sizeof(double) * npt * ncand, hipMemcpyDeviceToDevice, x->stream));
+  RCCHK(screen_shapes(x, ncand));
   double st1 = 0.0;
   HIPCHK(hipMemcpyAsync(sc.state + 1, &st1, sizeof(double), hipMemcpyHostToDevice, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
@@ -1960,6 +1988,7 @@ int ecg_screen_loop(ecg_ctx* x, int nsteps, int ncand, const double* cand, int i
   (void)hipGraphDestroy(g);
   x->tms[0] = ev_ms(x->e[0], x->e[1]);
   x->tms[1] = sc.tm;
+  x->tms[2] = sc.cpb;
   x->have_last = false;
   x->vec_current = false;
   for (int i = 0; i < nsteps; ++i) {

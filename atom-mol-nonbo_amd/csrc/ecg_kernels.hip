@@ -1899,13 +1899,67 @@ __global__ void __launch_bounds__(128) k_cand_rows(const SysConst sc, int Kp, in
   Sb[(size_t)j * ld + l] = zconj(S) * f;
 }
 
+// The same rows with CB candidates per workgroup: 128 / CB functions'
+// ket records (all T terms) staged in LDS once and shared by the CB
+// candidates, whose bra records are staged too, so each function's records
+// are read from L2 once per CB candidates instead of once per candidate.
+// Thread t: function l0 + t % (128 / CB), candidate j0 + t / (128 / CB).
+template <int N, int CB>
+__global__ void __launch_bounds__(128) k_cand_rows_cb(const SysConst sc, int Kp, int C, int ld,
+                                                      const double* __restrict__ bra, const double* __restrict__ ket,
+                                                      const double* __restrict__ diagS,
+                                                      const double* __restrict__ cbra,
+                                                      const double* __restrict__ cdiag, zd* __restrict__ Hb,
+                                                      zd* __restrict__ Sb) {
+  constexpr int FB = 128 / CB;
+  extern __shared__ double sm[];
+  const int kst = sc.T * KetRec<N>::stride;
+  double* sket = sm;                        // FB x T ket records
+  double* sbra = sm + (size_t)FB * kst;     // CB bra records
+  const int l0 = blockIdx.x * FB, j0 = blockIdx.y * CB;
+  for (int i = threadIdx.x; i < FB * kst; i += blockDim.x)
+    sket[i] = (l0 + i / kst < Kp) ? ket[(size_t)l0 * kst + i] : 0.0;
+  for (int i = threadIdx.x; i < CB * BraRec<N>::stride; i += blockDim.x)
+    sbra[i] = (j0 + i / BraRec<N>::stride < C) ? cbra[(size_t)j0 * BraRec<N>::stride + i] : 0.0;
+  __syncthreads();
+  const int fl = threadIdx.x % FB, cj = threadIdx.x / FB;
+  const int l = l0 + fl, j = j0 + cj;
+  if (l >= Kp || j >= C) return;
+  zd H, S;
+  pair_sum_energy<N>(sc, sbra + cj * BraRec<N>::stride, sket + (size_t)fl * kst,
+                     bra[(size_t)l * BraRec<N>::stride + BraRec<N>::adet], H, S);
+  const double f = 1.0 / sqrt(cdiag[2 * j] * diagS[l]);
+  Hb[(size_t)j * ld + l] = zconj(H) * f;
+  Sb[(size_t)j * ld + l] = zconj(S) * f;
+}
+
 template <int N>
 static int launch_cand(const Ctx& c, int Kp, int C, const double* cparams, double* cbra, double* cket,
-                       double* cdiag, zd* Hb, zd* Sb, int ldb, hipStream_t s) {
+                       double* cdiag, zd* Hb, zd* Sb, int ldb, int cpb, hipStream_t s) {
+  if (cpb != 1 && cand_rows_lds(c.n, c.sc.T, cpb) == 0) return -1;
   hipLaunchKernelGGL(k_prep<N>, dim3((C + 127) / 128), dim3(128), 0, s, c.sc, 0, C, cparams, cbra, cket);
   hipLaunchKernelGGL(k_cand_diag<N>, dim3((C + 127) / 128), dim3(128), 0, s, c.sc, C, cbra, cket, cdiag);
-  hipLaunchKernelGGL(k_cand_rows<N>, dim3((Kp + 127) / 128, C), dim3(128), 0, s, c.sc, Kp, ldb, c.bra, c.ket,
-                     c.diagS, cbra, cdiag, Hb, Sb);
+  const size_t lds = cand_rows_lds(c.n, c.sc.T, cpb);
+  switch (cpb) {
+    case 1:
+      hipLaunchKernelGGL(k_cand_rows<N>, dim3((Kp + 127) / 128, C), dim3(128), 0, s, c.sc, Kp, ldb, c.bra, c.ket,
+                         c.diagS, cbra, cdiag, Hb, Sb);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_cand_rows_cb<N, 2>), dim3((Kp + 63) / 64, (C + 1) / 2), dim3(128), lds, s, c.sc, Kp, C,
+                         ldb, c.bra, c.ket, c.diagS, cbra, cdiag, Hb, Sb);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_cand_rows_cb<N, 4>), dim3((Kp + 31) / 32, (C + 3) / 4), dim3(128), lds, s, c.sc, Kp, C,
+                         ldb, c.bra, c.ket, c.diagS, cbra, cdiag, Hb, Sb);
+      break;
+    case 8:
+      hipLaunchKernelGGL((k_cand_rows_cb<N, 8>), dim3((Kp + 15) / 16, (C + 7) / 8), dim3(128), lds, s, c.sc, Kp, C,
+                         ldb, c.bra, c.ket, c.diagS, cbra, cdiag, Hb, Sb);
+      break;
+    default:
+      return -1;
+  }
   return (int)hipGetLastError();
 }
 
@@ -2324,14 +2378,19 @@ size_t adjoint_partial_doubles(int n, int K) {
   return (size_t)K * (nt + 1) * (2 * m + 1);
 }
 
+size_t cand_rows_lds(int n, int T, int cpb) {
+  if (cpb < 1 || 128 % cpb) return 0;
+  const size_t bytes = sizeof(double) * ((size_t)(128 / cpb) * T * ket_stride(n) + (size_t)cpb * bra_stride(n));
+  return bytes <= 64 * 1024 ? bytes : 0;
+}
 int candidate_rows(const Ctx& c, int Kp, int C, const double* cparams, double* cbra, double* cket,
-                   double* cdiag, zd* Hb, zd* Sb, int ldb, hipStream_t s) {
+                   double* cdiag, zd* Hb, zd* Sb, int ldb, int cpb, hipStream_t s) {
   switch (c.n) {
-    case 1: return launch_cand<1>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, s);
-    case 2: return launch_cand<2>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, s);
-    case 3: return launch_cand<3>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, s);
-    case 4: return launch_cand<4>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, s);
-    case 5: return launch_cand<5>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, s);
+    case 1: return launch_cand<1>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, cpb, s);
+    case 2: return launch_cand<2>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, cpb, s);
+    case 3: return launch_cand<3>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, cpb, s);
+    case 4: return launch_cand<4>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, cpb, s);
+    case 5: return launch_cand<5>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, cpb, s);
   }
   return -1;
 }

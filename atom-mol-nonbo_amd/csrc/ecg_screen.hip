@@ -149,10 +149,13 @@ int zgemm(int trans, int M, int N, int Kd, zd alpha, const zd* A, int lda, const
   return (int)hipGetLastError();
 }
 
-// The screening GEMMs' row tile for C candidates: both variants timed on
-// the step's iteration GEMM (P V, Kp x C x Kp, into U, which the step
-// overwrites), the faster kept for this C and basis sizes within the same
-// 512-wide band.  Runs outside any graph capture (it synchronises).
+// The screening step's two shapes for C candidates, each the faster of
+// its variants timed on the session's own operands: the per-step GEMMs'
+// row tile (64 or 32 rows; timed on P V, Kp x C x Kp into U, which the
+// step overwrites) and the border rows' candidates per workgroup (1, 2, 4,
+// 8; timed on the candidates in cparams, writing the step's own row
+// buffers).  Kept for this C and basis sizes in the same 512-wide band;
+// runs outside any graph capture (it synchronises).
 int screen_tune(const Ctx& c, Screen& sc, int C, hipStream_t s) {
   if (sc.tuned_c == C && sc.tuned_kb == (sc.Kp >> 9)) return 0;
   const int Kp = sc.Kp, ld = c.ld;
@@ -163,20 +166,38 @@ int screen_tune(const Ctx& c, Screen& sc, int C, hipStream_t s) {
     (void)hipEventDestroy(e0);
     return -1;
   }
+  auto timed = [&](auto&& run, float& ms) -> int {
+    int rc = run();   // warm-up
+    if (!rc && hipEventRecord(e0, s) != hipSuccess) rc = -1;
+    for (int r = 0; r < 3 && !rc; ++r) rc = run();
+    if (!rc && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      rc = -1;
+    return rc;
+  };
   int rc = 0;
   float best = 0.0f;
   for (int tm : {64, 32}) {
     float ms = 0.0f;
-    rc = zgemm(0, Kp, C, Kp, one, sc.P, ld, sc.V, ld, zero, sc.U, ld, s, tm);   // warm-up
-    if (!rc && hipEventRecord(e0, s) != hipSuccess) rc = -1;
-    for (int r = 0; r < 3 && !rc; ++r) rc = zgemm(0, Kp, C, Kp, one, sc.P, ld, sc.V, ld, zero, sc.U, ld, s, tm);
-    if (!rc && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-      rc = -1;
+    rc = timed([&] { return zgemm(0, Kp, C, Kp, one, sc.P, ld, sc.V, ld, zero, sc.U, ld, s, tm); }, ms);
     if (rc) break;
     if (tm == 64 || ms < best) {
       best = ms;
       sc.tm = tm;
+    }
+  }
+  for (int cpb : {1, 2, 4, 8}) {
+    if (rc) break;
+    if (cpb > 1 && cand_rows_lds(c.n, c.sc.T, cpb) == 0) continue;
+    float ms = 0.0f;
+    rc = timed([&] {
+      return candidate_rows(c, Kp, C, sc.cparams, sc.cbra, sc.cket, sc.cdiag, sc.Hb, sc.Bm + (size_t)C * ld, ld, cpb,
+                            s);
+    }, ms);
+    if (rc) break;
+    if (cpb == 1 || ms < best) {
+      best = ms;
+      sc.cpb = cpb;
     }
   }
   (void)hipEventDestroy(e0);
@@ -477,7 +498,8 @@ int screen_batch(const Ctx& c, const Factor& f, Screen& sc, int C, int iters, bo
   if (from_loop)
     hipLaunchKernelGGL(k_cand_fetch, dim3((npt * C + 255) / 256), dim3(256), 0, s, npt * C, sc.call, sc.state,
                        sc.cparams);
-  int rc = candidate_rows(c, Kp, C, sc.cparams, sc.cbra, sc.cket, sc.cdiag, sc.Hb, sc.Bm + (size_t)C * ld, ld, s);
+  int rc = candidate_rows(c, Kp, C, sc.cparams, sc.cbra, sc.cket, sc.cdiag, sc.Hb, sc.Bm + (size_t)C * ld, ld, sc.cpb,
+                          s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_cand_border, dim3(C), dim3(256), 0, s, Kp, ld, C, sc.sigma, sc.Hb, sc.Bm, sc.cdiag, sc.cs,
                      sc.v0, sc.V, sc.vK);

@@ -46,7 +46,7 @@ EXPORTS = [
     "ecg_batch_energy_grad", "ecg_trace", "ecg_trace_read", "ecg_get_params", "ecg_screen_prepare", "ecg_screen", "ecg_screen_loop", "ecg_last_timings",
     "ecg_set_overlap_penalty", "ecg_overlap_penalty", "ecg_set_params_device", "ecg_expectation_values",
     "ecg_correlation_functions", "ecg_particle_densities", "ecg_relativistic_corrections", "ecg_drachmanized_corrections", "ecg_eigenvector",
-    "ecg_matrix_elements_noalloc",
+    "ecg_matrix_elements_noalloc", "ecg_screen_set_shapes",
 ]
 
 
@@ -84,6 +84,7 @@ def load(path: str = LIB_PATH):
     L.ecg_energy.argtypes = [vp, c_int, c_double, c_double, c_int, _d, _d, _d]
     L.ecg_energy_grad.argtypes = [vp, c_int, c_int, c_double, c_double, c_int, _d, _d, _d]
     L.ecg_energy_grad_stored.argtypes = [vp, c_int, c_int, c_double, c_double, c_int, _d, _d, _d]
+    L.ecg_screen_set_shapes.argtypes = [vp, c_int, c_int]
     L.ecg_batch_create.argtypes = [c_int, ctypes.POINTER(vp), ctypes.POINTER(vp)]
     L.ecg_batch_destroy.argtypes = [vp]
     L.ecg_batch_set_config.argtypes = [vp, c_int, c_int]
@@ -274,6 +275,11 @@ class EcgContext:
                                            ctypes.byref(E), _p(g), _p(info))
         _check("ecg_energy_grad_stored", rc, ok=(ECG_OK, ECG_NOT_CONVERGED))
         return rc, E.value, g, info
+
+    def screen_set_shapes(self, row_tile=0, cand_per_block=0):
+        """Fix the screening step's GEMM row tile (32 or 64) and border-row
+        candidates per workgroup (1, 2, 4, 8); 0 leaves a shape timed."""
+        _check("ecg_screen_set_shapes", load().ecg_screen_set_shapes(self._h, int(row_tile), int(cand_per_block)))
 
     def set_overlap_penalty(self, allowed, max_pair_penalty=1.0, threshold2=1.0, nfru=0):
         """Glob_OverlapPenaltyAllowed / Glob_MaxOverlapPenalty /

@@ -149,6 +149,12 @@ module ecg_capi
       real(c_double), intent(out) :: E
       real(c_double), intent(out) :: grad(*), info(*)
     end function
+    integer(c_int) function ecg_screen_set_shapes(ctx, row_tile, cand_per_block) &
+        bind(C, name="ecg_screen_set_shapes")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: row_tile, cand_per_block
+    end function
     integer(c_int) function ecg_batch_create(nprob, ctxs, batch) bind(C, name="ecg_batch_create")
       import :: c_int, c_ptr
       integer(c_int), value :: nprob

@@ -297,6 +297,7 @@ def screening_leg(system, base, K, ncand=256, nsteps=100, iters=4, plain_sample=
     tl = ctx.timings()
     dev_loop = tl[0] * 1e-3
     gemm_tile = int(tl[1])
+    cand_per_block = int(tl[2])
     # accuracy of `iters` iterations: one step's candidates at iters and at 4 x iters
     ctx.set_params(base)
     ctx.screen_prepare(sigma, 1e-12, 40)
@@ -337,6 +338,7 @@ def screening_leg(system, base, K, ncand=256, nsteps=100, iters=4, plain_sample=
         "ms_per_step": 1e3 * t_loop / nsteps,
         "device_ms_per_step": 1e3 * dev_loop / nsteps,
         "gemm_row_tile": gemm_tile,
+        "candidates_per_block": cand_per_block,
         "prepare_ms": 1e3 * t_prep,
         "per_candidate_candidates_per_s": 1.0 / t_tail,
         "per_candidate_ms": 1e3 * t_tail,

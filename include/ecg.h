@@ -270,6 +270,15 @@ int ecg_screen(ecg_ctx* ctx, int ncand, const double* cand, int iters, double* E
  * -1 - index of the best candidate when it was not accepted. */
 int ecg_screen_loop(ecg_ctx* ctx, int nsteps, int ncand, const double* cand, int iters, double* Ebest, int* ibest);
 
+/* The screening step's shapes, no counterpart in the reference: the
+ * per-step GEMMs' row tile (32 or 64) and the border-row kernel's
+ * candidates per workgroup (1, 2, 4 or 8); 0 leaves a shape to the timing
+ * that ecg_screen / ecg_screen_loop do on their first call for a candidate
+ * count (the default).  Results do not depend on the shapes.  ECG_ERR_ARG
+ * for other values or a candidates-per-workgroup count whose staged
+ * records exceed 64 KB of LDS for this system. */
+int ecg_screen_set_shapes(ecg_ctx* ctx, int row_tile, int cand_per_block);
+
 /* Overlap penalty settings: the reference's Glob_OverlapPenaltyAllowed,
  * Glob_MaxOverlapPenalty (b) and Glob_OverlapPenaltyThreshold2 (t2)
  * (src/globvars.f90:309-326), plus the Glob_nfru that energy-only calls
@@ -364,10 +373,11 @@ int ecg_eigenvector(ecg_ctx* ctx, double* c);
  * the last ecg_energy / ecg_energy_grad call: [0] assembly, [1] eigen step
  * (all factorisations and iterations), [2] adjoint pass, [3] whole call,
  * [4] adjoint kernel alone, [5] off-diagonal assembly kernel alone.
- * After ecg_screen / ecg_screen_loop, [0] is the device time of the call
- * and [1] the row tile (64 or 32) its GEMMs ran with, picked by timing both
- * on the first call for this candidate count and basis-size band.
- * Returns the number of values written. */
+ * After ecg_screen / ecg_screen_loop, [0] is the device time of the call,
+ * [1] the row tile (64 or 32) its GEMMs ran with and [2] the candidates per
+ * workgroup (1, 2, 4 or 8) of its border-row kernel, both picked by timing
+ * the variants on the first call for this candidate count and basis-size
+ * band.  Returns the number of values written. */
 int ecg_last_timings(ecg_ctx* ctx, double* ms, int nmax);
 
 #ifdef __cplusplus

@@ -70,6 +70,37 @@ def test_screen_candidates_vs_oracle(name, K, ncand):
     ctx.close()
 
 
+def test_screen_shapes_do_not_change_results():
+    """Every fixed screening shape (GEMM row tile 64 / 32, border rows with
+    1, 2, 4 or 8 candidates per workgroup, the latter staging the shared
+    ket records in LDS) gives the energies and inertias of the (64, 1)
+    shape bit for bit; the timed choice is reported and is one of them."""
+    say("screening shapes", "h2", 200, 40)
+    sy, params, ctx, sigma, info = _session("h2", 200, 42, 1e-4)
+    cand = seeded_basis(sy.n, 40, SEED_BASE + 43)
+    ctx.screen_set_shapes(64, 1)
+    E0, inf0 = ctx.screen(cand, iters=4)
+    for tm in (64, 32):
+        for cpb in (1, 2, 4, 8):
+            ctx.screen_set_shapes(tm, cpb)
+            E, inf = ctx.screen(cand, iters=4)
+            t = ctx.timings()
+            assert (int(t[1]), int(t[2])) == (tm, cpb)
+            print("shape", tm, cpb, "max |dE|", float(np.max(np.abs(E - E0))))
+            assert np.array_equal(E, E0), (tm, cpb)
+            assert np.array_equal(inf[:, 1], inf0[:, 1]), (tm, cpb)
+    ctx.screen_set_shapes(0, 0)
+    E, inf = ctx.screen(cand, iters=4)
+    t = ctx.timings()
+    assert int(t[1]) in (32, 64) and int(t[2]) in (1, 2, 4, 8)
+    assert np.array_equal(E, E0)
+    with pytest.raises(RuntimeError):
+        ctx.screen_set_shapes(48, 0)
+    with pytest.raises(RuntimeError):
+        ctx.screen_set_shapes(0, 3)
+    ctx.close()
+
+
 def test_screen_loop_graph_matches_eager():
     """The hipGraph-captured loop (one step per replay) against an eager
     host-driven loop with the same candidates: every replay's best energy
