@@ -6,8 +6,8 @@
 // from row K (LDLHF, src/linalg.f90:316) and runs inverse iteration
 // (GHEPIIS, src/linalg.f90:1781).  Here all candidates of a step share the
 // fixed leading block M' = H' - sigma S' (functions 1..K-1):
-//   set-up (once per session):  LDL^H of H - sigma S, Z = L'^{-1} by blocked
-//     substitution on GEMMs with the stored 64 x 64 block inverses,
+//   set-up (once per session):  LDL^H of H - sigma S, Z = L'^{-1} (the
+//     leading block of the L^{-1} the factorisation builds),
 //     G = Z^H D'^{-1} Z = M'^{-1}, P = G S';
 //   per step, for C candidates (columns of one matrix):
 //     border columns m = h - sigma s, s of each candidate (pair kernel),
@@ -210,22 +210,33 @@ int screen_tune(const Ctx& c, Screen& sc, int C, hipStream_t s) {
 }
 
 // ------------------------------------------------------------ set-up
-__global__ void k_identity(int n, int ld, zd* __restrict__ Z) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-  if (i < n) Z[(size_t)j * ld + i] = mkz(i == j ? 1.0 : 0.0, 0.0);
-}
-
-// rows r0..r0+rows-1 of Z (columns 0..ncol-1) := T (rows x ncol, leading dimension 64)
-__global__ void k_copy_rows(int rows, int ncol, int r0, int ld, const zd* __restrict__ T, zd* __restrict__ Z) {
-  const int i = threadIdx.x & 63, j = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (i < rows && j < ncol) Z[(size_t)j * ld + r0 + i] = T[(size_t)j * 64 + i];
-}
-
 // Y(i, j) = Z(i, j) / D_i
 __global__ void k_row_scale(int n, int ld, const double* __restrict__ D, const zd* __restrict__ Z,
                             zd* __restrict__ Y) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
   if (i < n) Y[(size_t)j * ld + i] = Z[(size_t)j * ld + i] * (1.0 / D[i]);
+}
+
+// Z (Kp x Kp, column-major, ld) = X = L^{-1} from X^H held in F's strict
+// upper triangle (F(r, c) = conj(X(c, r)), r < c): Z(i, j) = conj(F(j, i))
+// for i > j, 1 on the diagonal, 0 above.  A 64 x 64 tile per workgroup,
+// transposed through LDS so that both the reads of F and the writes of Z
+// are coalesced.
+__global__ void __launch_bounds__(256) k_z_from_xh(int Kp, int ld, const zd* __restrict__ F, zd* __restrict__ Z) {
+  __shared__ zd t[64][65];
+  const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;   // Z rows i0.., columns j0..
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  if (i0 + 63 > j0) {   // the tile reaches below the diagonal
+    for (int r = ty; r < 64; r += 4) {
+      const int i = i0 + r, j = j0 + tx;
+      t[r][tx] = (i < Kp && j < i) ? F[(size_t)i * ld + j] : mkz(0.0, 0.0);
+    }
+  }
+  __syncthreads();
+  for (int cc = ty; cc < 64; cc += 4) {
+    const int j = j0 + cc, i = i0 + tx;
+    if (i < Kp && j < Kp) Z[(size_t)j * ld + i] = (i > j) ? zconj(t[tx][cc]) : mkz(i == j ? 1.0 : 0.0, 0.0);
+  }
 }
 
 int screen_prepare(const Ctx& c, Factor& f, Screen& sc, double sigma, hipStream_t s) {
@@ -234,24 +245,9 @@ int screen_prepare(const Ctx& c, Factor& f, Screen& sc, double sigma, hipStream_
   sc.sigma = sigma;
   int rc = ldlh_factor(f, s, c.H, c.S, sigma);
   if (rc) return rc;
-  // Z = L'^{-1}: block b: Z_b = X_b R_b, then R_below -= L_below,b Z_b.
-  // Only columns < b0 + bs of block rows >= b are nonzero.
-  hipLaunchKernelGGL(k_identity, dim3((Kp + 255) / 256, Kp), dim3(256), 0, s, Kp, ld, sc.Z);
-  zd* T = sc.G;   // 64 x ld scratch before G is formed
-  for (int b0 = 0; b0 < Kp; b0 += NB) {
-    const int bs = (Kp - b0 < NB) ? Kp - b0 : NB;
-    const int ncol = b0 + bs;
-    const zd* Xb = f.Dinv + (size_t)(b0 / NB) * 2 * NB * NB;
-    rc = zgemm(0, bs, ncol, bs, mkz(1.0, 0.0), Xb, NB, sc.Z + b0, ld, mkz(0.0, 0.0), T, NB, s);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_copy_rows, dim3((ncol + 3) / 4), dim3(256), 0, s, bs, ncol, b0, ld, T, sc.Z);
-    const int rest = Kp - ncol;
-    if (rest > 0) {
-      rc = zgemm(0, rest, ncol, bs, mkz(-1.0, 0.0), f.F + (size_t)b0 * ld + ncol, ld, sc.Z + b0, ld,
-                 mkz(1.0, 0.0), sc.Z + ncol, ld, s);
-      if (rc) return rc;
-    }
-  }
+  // Z = L'^{-1}: the leading Kp x Kp block of the factorisation's own
+  // X = L^{-1} (X^H in F's strict upper triangle)
+  hipLaunchKernelGGL(k_z_from_xh, dim3((Kp + 63) / 64, (Kp + 63) / 64), dim3(256), 0, s, Kp, ld, f.F, sc.Z);
   // G = Z^H (D^{-1} Z), P = G S'
   hipLaunchKernelGGL(k_row_scale, dim3((Kp + 255) / 256, Kp), dim3(256), 0, s, Kp, ld, f.D, sc.Z, sc.P);
   rc = zgemm(1, Kp, Kp, Kp, mkz(1.0, 0.0), sc.Z, ld, sc.P, ld, mkz(0.0, 0.0), sc.G, ld, s);
