@@ -315,8 +315,10 @@ int screen_batch(const Ctx& c, const Factor& f, Screen& sc, int C, int iters, bo
 // start vector := its eigenvector), record, advance the step counter.
 int screen_select(const Ctx& c, Screen& sc, int C, int npt, hipStream_t s);
 // C = alpha op(A) B + beta C (op = identity or A^H for trans = 1), M x N x
-// Kd, column-major; tm = rows per workgroup tile (64 or 32).
+// Kd, column-major; tm = rows per workgroup tile (64 or 32); lower = 1 when
+// op(A)'s columns and B's rows are zero above the diagonal (A^H B with
+// both lower triangular), so each tile skips the leading terms that vanish.
 int zgemm(int trans, int M, int N, int Kd, zd alpha, const zd* A, int lda, const zd* B, int ldb, zd beta, zd* Cm,
-          int ldc, hipStream_t s, int tm = 64);
+          int ldc, hipStream_t s, int tm = 64, int lower = 0);
 
 }  // namespace ecg

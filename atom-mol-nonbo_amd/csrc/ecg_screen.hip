@@ -39,7 +39,7 @@ typedef double gdbl4 __attribute__((ext_vector_type(4)));
 template <int TRANS, int TM>
 __global__ void __launch_bounds__(256) k_zgemm(int M, int N, int Kd, zd alpha, const zd* __restrict__ A, int lda,
                                                const zd* __restrict__ B, int ldb, zd beta, zd* __restrict__ Cm,
-                                               int ldc) {
+                                               int ldc, int lower) {
   constexpr int KC = 32;           // k-chunk
   constexpr int NA = TM / 32;      // 16-row tiles per wave
   __shared__ zd As[KC][TM + 1];   // As[p][r] = op(A)(r0 + r, p0 + p)
@@ -88,8 +88,11 @@ __global__ void __launch_bounds__(256) k_zgemm(int M, int N, int Kd, zd alpha, c
       vb[u] = (c0 + c < N && p0 + p < Kd) ? B[(size_t)(c0 + c) * ldb + p0 + p] : mkz(0.0, 0.0);
     }
   };
-  load_chunk(0);
-  for (int p0 = 0; p0 < Kd; p0 += KC) {
+  // lower: A and B are zero above their diagonals, so for the tile's rows
+  // r >= r0 and columns c >= c0 the terms p < max(r0, c0) vanish
+  const int pb = lower ? (max(r0, c0) / KC) * KC : 0;
+  load_chunk(pb);
+  for (int p0 = pb; p0 < Kd; p0 += KC) {
     __syncthreads();   // the previous chunk's reads are done
 #pragma unroll
     for (int u = 0; u < TM * KC / 256; ++u) {
@@ -135,16 +138,16 @@ __global__ void __launch_bounds__(256) k_zgemm(int M, int N, int Kd, zd alpha, c
 }
 
 int zgemm(int trans, int M, int N, int Kd, zd alpha, const zd* A, int lda, const zd* B, int ldb, zd beta, zd* Cm,
-          int ldc, hipStream_t s, int tm) {
+          int ldc, hipStream_t s, int tm, int lower) {
   if (M <= 0 || N <= 0) return 0;
   if (tm != 32) tm = 64;
   const dim3 grid((M + tm - 1) / tm, (N + 31) / 32);
   if (trans) {
-    if (tm == 32) hipLaunchKernelGGL((k_zgemm<1, 32>), grid, dim3(256), 0, s, M, N, Kd, alpha, A, lda, B, ldb, beta, Cm, ldc);
-    else hipLaunchKernelGGL((k_zgemm<1, 64>), grid, dim3(256), 0, s, M, N, Kd, alpha, A, lda, B, ldb, beta, Cm, ldc);
+    if (tm == 32) hipLaunchKernelGGL((k_zgemm<1, 32>), grid, dim3(256), 0, s, M, N, Kd, alpha, A, lda, B, ldb, beta, Cm, ldc, lower);
+    else hipLaunchKernelGGL((k_zgemm<1, 64>), grid, dim3(256), 0, s, M, N, Kd, alpha, A, lda, B, ldb, beta, Cm, ldc, lower);
   } else {
-    if (tm == 32) hipLaunchKernelGGL((k_zgemm<0, 32>), grid, dim3(256), 0, s, M, N, Kd, alpha, A, lda, B, ldb, beta, Cm, ldc);
-    else hipLaunchKernelGGL((k_zgemm<0, 64>), grid, dim3(256), 0, s, M, N, Kd, alpha, A, lda, B, ldb, beta, Cm, ldc);
+    if (tm == 32) hipLaunchKernelGGL((k_zgemm<0, 32>), grid, dim3(256), 0, s, M, N, Kd, alpha, A, lda, B, ldb, beta, Cm, ldc, lower);
+    else hipLaunchKernelGGL((k_zgemm<0, 64>), grid, dim3(256), 0, s, M, N, Kd, alpha, A, lda, B, ldb, beta, Cm, ldc, lower);
   }
   return (int)hipGetLastError();
 }
@@ -250,7 +253,7 @@ int screen_prepare(const Ctx& c, Factor& f, Screen& sc, double sigma, hipStream_
   hipLaunchKernelGGL(k_z_from_xh, dim3((Kp + 63) / 64, (Kp + 63) / 64), dim3(256), 0, s, Kp, ld, f.F, sc.Z);
   // G = Z^H (D^{-1} Z), P = G S'
   hipLaunchKernelGGL(k_row_scale, dim3((Kp + 255) / 256, Kp), dim3(256), 0, s, Kp, ld, f.D, sc.Z, sc.P);
-  rc = zgemm(1, Kp, Kp, Kp, mkz(1.0, 0.0), sc.Z, ld, sc.P, ld, mkz(0.0, 0.0), sc.G, ld, s);
+  rc = zgemm(1, Kp, Kp, Kp, mkz(1.0, 0.0), sc.Z, ld, sc.P, ld, mkz(0.0, 0.0), sc.G, ld, s, 64, 1);
   if (!rc) rc = zgemm(0, Kp, Kp, Kp, mkz(1.0, 0.0), sc.G, ld, c.S, ld, mkz(0.0, 0.0), sc.P, ld, s);
   return rc;
 }
