@@ -258,11 +258,12 @@ int packed_pairs(int n, bool adjoint, const ProbDesc* dpd, int np, long long nti
                  hipStream_t s);
 
 // candidate screening (ecg_screen.hip; pair rows in ecg_kernels.hip)
-// Candidate records, diagonal pairs and border rows; cpb candidates per
-// workgroup of the row kernel (1, 2, 4 or 8; cpb > 1 stages the shared ket
-// records in LDS, cand_rows_lds bytes, 0 when they do not fit 64 KB).
+// Candidate records, diagonal pairs and border rows (rows_only: the rows
+// from records already made); cpb candidates per workgroup of the row
+// kernel (1, 2, 4 or 8; cpb > 1 stages the shared ket records in LDS,
+// cand_rows_lds bytes, 0 when they do not fit 64 KB).
 int candidate_rows(const Ctx& c, int Kp, int C, const double* cparams, double* cbra, double* cket,
-                   double* cdiag, zd* Hb, zd* Sb, int ldb, int cpb, hipStream_t s);
+                   double* cdiag, zd* Hb, zd* Sb, int ldb, int cpb, hipStream_t s, bool rows_only = false);
 size_t cand_rows_lds(int n, int T, int cpb);
 
 // Device state of a screening session: functions 1..Kp (Kp = K-1) fixed,

@@ -1935,10 +1935,12 @@ __global__ void __launch_bounds__(128) k_cand_rows_cb(const SysConst sc, int Kp,
 
 template <int N>
 static int launch_cand(const Ctx& c, int Kp, int C, const double* cparams, double* cbra, double* cket,
-                       double* cdiag, zd* Hb, zd* Sb, int ldb, int cpb, hipStream_t s) {
+                       double* cdiag, zd* Hb, zd* Sb, int ldb, int cpb, bool rows_only, hipStream_t s) {
   if (cpb != 1 && cand_rows_lds(c.n, c.sc.T, cpb) == 0) return -1;
-  hipLaunchKernelGGL(k_prep<N>, dim3((C + 127) / 128), dim3(128), 0, s, c.sc, 0, C, cparams, cbra, cket);
-  hipLaunchKernelGGL(k_cand_diag<N>, dim3((C + 127) / 128), dim3(128), 0, s, c.sc, C, cbra, cket, cdiag);
+  if (!rows_only) {
+    hipLaunchKernelGGL(k_prep<N>, dim3((C + 127) / 128), dim3(128), 0, s, c.sc, 0, C, cparams, cbra, cket);
+    hipLaunchKernelGGL(k_cand_diag<N>, dim3((C + 127) / 128), dim3(128), 0, s, c.sc, C, cbra, cket, cdiag);
+  }
   const size_t lds = cand_rows_lds(c.n, c.sc.T, cpb);
   switch (cpb) {
     case 1:
@@ -2384,13 +2386,13 @@ size_t cand_rows_lds(int n, int T, int cpb) {
   return bytes <= 64 * 1024 ? bytes : 0;
 }
 int candidate_rows(const Ctx& c, int Kp, int C, const double* cparams, double* cbra, double* cket,
-                   double* cdiag, zd* Hb, zd* Sb, int ldb, int cpb, hipStream_t s) {
+                   double* cdiag, zd* Hb, zd* Sb, int ldb, int cpb, hipStream_t s, bool rows_only) {
   switch (c.n) {
-    case 1: return launch_cand<1>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, cpb, s);
-    case 2: return launch_cand<2>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, cpb, s);
-    case 3: return launch_cand<3>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, cpb, s);
-    case 4: return launch_cand<4>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, cpb, s);
-    case 5: return launch_cand<5>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, cpb, s);
+    case 1: return launch_cand<1>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, cpb, rows_only, s);
+    case 2: return launch_cand<2>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, cpb, rows_only, s);
+    case 3: return launch_cand<3>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, cpb, rows_only, s);
+    case 4: return launch_cand<4>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, cpb, rows_only, s);
+    case 5: return launch_cand<5>(c, Kp, C, cparams, cbra, cket, cdiag, Hb, Sb, ldb, cpb, rows_only, s);
   }
   return -1;
 }
