@@ -137,9 +137,79 @@ __global__ void __launch_bounds__(256) k_zgemm(int M, int N, int Kd, zd alpha, c
     }
 }
 
+// C = alpha A B + beta C for few columns (N <= GV_N, A M x Kd column-major):
+// the MFMA tiles would leave most of each 32-column tile and most CUs idle,
+// and the product is a stream of A, so it is laid out for bandwidth. A
+// workgroup of 16 waves takes 8 rows (one 128-byte line of each column of A);
+// lane l of wave w reads row l % 8 of columns p = 8 w + l / 8 (mod 128), four
+// columns 128 apart per load batch so that each lane keeps four loads in
+// flight. The 128 partial sums of a row are added in a fixed order: a
+// butterfly over the eight column slots of a wave, then the 16 waves in
+// sequence through LDS.
+constexpr int GV_N = 8;
+constexpr int GV_ROWS = 8;
+__global__ void __launch_bounds__(1024) k_zgemv(int M, int N, int Kd, zd alpha, const zd* __restrict__ A, int lda,
+                                                const zd* __restrict__ B, int ldb, zd beta, zd* __restrict__ Cm,
+                                                int ldc) {
+  __shared__ zd red[16][GV_ROWS][GV_N + 1];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 7, ks = 8 * wave + (lane >> 3);
+  const int row = blockIdx.x * GV_ROWS + r;
+  zd acc[GV_N];
+#pragma unroll
+  for (int n = 0; n < GV_N; ++n) acc[n] = mkz(0.0, 0.0);
+  if (row < M) {
+    for (int p0 = ks; p0 < Kd; p0 += 512) {
+      zd a[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int p = p0 + 128 * u;
+        a[u] = p < Kd ? A[(size_t)p * lda + row] : mkz(0.0, 0.0);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int p = p0 + 128 * u;
+        if (p < Kd) {
+#pragma unroll
+          for (int n = 0; n < GV_N; ++n)
+            if (n < N) acc[n] = zfma(a[u], B[(size_t)n * ldb + p], acc[n]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < GV_N; ++n) {
+#pragma unroll
+    for (int off = 8; off < 64; off <<= 1) {
+      acc[n].x += __shfl_xor(acc[n].x, off);
+      acc[n].y += __shfl_xor(acc[n].y, off);
+    }
+  }
+  if (lane < GV_ROWS) {
+#pragma unroll
+    for (int n = 0; n < GV_N; ++n) red[wave][lane][n] = acc[n];
+  }
+  __syncthreads();
+  if (threadIdx.x < GV_ROWS * GV_N) {
+    const int rr = threadIdx.x % GV_ROWS, n = threadIdx.x / GV_ROWS;
+    const int i = blockIdx.x * GV_ROWS + rr;
+    if (n < N && i < M) {
+      zd t = mkz(0.0, 0.0);
+      for (int k = 0; k < 16; ++k) t = t + red[k][rr][n];
+      zd* c = Cm + (size_t)n * ldc + i;
+      const bool zb = beta.x == 0.0 && beta.y == 0.0;
+      *c = zb ? alpha * t : zfma(beta, *c, alpha * t);
+    }
+  }
+}
+
 int zgemm(int trans, int M, int N, int Kd, zd alpha, const zd* A, int lda, const zd* B, int ldb, zd beta, zd* Cm,
           int ldc, hipStream_t s, int tm, int lower) {
   if (M <= 0 || N <= 0) return 0;
+  if (!trans && !lower && N <= GV_N) {
+    hipLaunchKernelGGL(k_zgemv, dim3((M + GV_ROWS - 1) / GV_ROWS), dim3(1024), 0, s, M, N, Kd, alpha, A, lda, B, ldb, beta, Cm, ldc);
+    return (int)hipGetLastError();
+  }
   if (tm != 32) tm = 64;
   const dim3 grid((M + tm - 1) / tm, (N + 31) / 32);
   if (trans) {
