@@ -274,7 +274,8 @@ int ecg_screen_loop(ecg_ctx* ctx, int nsteps, int ncand, const double* cand, int
  * per-step GEMMs' row tile (32 or 64) and the border-row kernel's
  * candidates per workgroup (1, 2, 4 or 8); 0 leaves a shape to the timing
  * that ecg_screen / ecg_screen_loop do on their first call for a candidate
- * count (the default).  Results do not depend on the shapes.  ECG_ERR_ARG
+ * count (the default).  Results do not depend on the shapes; with at most
+ * 8 candidates the products run as a GEMV and the row tile is unused.  ECG_ERR_ARG
  * for other values or a candidates-per-workgroup count whose staged
  * records exceed 64 KB of LDS for this system. */
 int ecg_screen_set_shapes(ecg_ctx* ctx, int row_tile, int cand_per_block);
